@@ -1,0 +1,1200 @@
+/*
+ * CPU oracle: FP64 C restatement of the reference's domain-wall hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY -- the checker for the GPU path (tests/, smoke(),
+ * bench.py cpu_baseline).  Never linked into the product library.
+ *
+ * Every routine follows the reference Fortran of j-ostmeyer/thirring-rhmc
+ * line by line in *semantics* (operation order inside a site may differ) and
+ * reproduces its float32 parameter roundings: am, am3, res, the diagonal
+ * (3 - am3) + 1, zkappa and the derivs coefficient 2*real(anum).  Arrays use
+ * the reference's haloed column-major layouts (see oracle.h).
+ *
+ * Parity pins (tests/test_oracle_*.py): gamma-matrix Clifford algebra, dense
+ * brute-force operator assembly, adjointness <a,Mb> = <M^+a,b>, U(1) gauge
+ * covariance, free-field plane waves, exact finite differences of the action
+ * for derivs (M is affine in theta), multishift iterates vs. plain CG on each
+ * shifted system, dense solves for qmrherm, and the reference's own known
+ * answer itercg == 27 for congrad (tests/test_congrad.F90:47).
+ */
+#include "oracle.h"
+
+#include <complex.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef double _Complex zdp;
+typedef float _Complex zsp;
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
+/* ------------------------------------------------------------------------ */
+/* layout                                                                    */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  int lx, ly, lt, ls;     /* interior extents */
+  long S, X, Y, T;        /* haloed extents */
+  long n5, n4;            /* haloed element counts (per spinor / link field) */
+} geom;
+
+static geom mkgeom(const int d[4]) {
+  geom g;
+  g.lx = d[0]; g.ly = d[1]; g.lt = d[2]; g.ls = d[3];
+  g.S = g.ls + 2; g.X = g.lx + 2; g.Y = g.ly + 2; g.T = g.lt + 2;
+  g.n5 = g.S * g.X * g.Y * g.T * 4;
+  g.n4 = g.X * g.Y * g.T * 3;
+  return g;
+}
+
+/* 0-based spin a (0..3), halo coordinates s,x,y,t as in the Fortran bounds */
+#define I5(g, s, x, y, t, a) \
+  ((long)(s) + (g).S * ((long)(x) + (g).X * ((long)(y) + (g).Y * ((long)(t) + (g).T * (long)(a)))))
+#define I4(g, x, y, t, mu) ((long)(x) + (g).X * ((long)(y) + (g).Y * ((long)(t) + (g).T * (long)(mu))))
+/* halo-free spinor (Ls, Lx, Ly, Lt, 4), 1-based s,x,y,t */
+#define IH(g, s, x, y, t, a) \
+  ((long)((s) - 1) + (long)(g).ls * ((long)((x) - 1) + (long)(g).lx * ((long)((y) - 1) + (long)(g).ly * ((long)((t) - 1) + (long)(g).lt * (long)(a)))))
+/* halo-free real 4-D field (Lx, Ly, Lt, 3) */
+#define IF4(g, x, y, t, mu) \
+  ((long)((x) - 1) + (long)(g).lx * ((long)((y) - 1) + (long)(g).ly * ((long)((t) - 1) + (long)(g).lt * (long)(mu))))
+
+/* ------------------------------------------------------------------------ */
+/* gamma matrices (gammamatrices.F90:17-132): gamval = akappa * table         */
+/* ------------------------------------------------------------------------ */
+static const double GV_RE[6][4] = {{0, 0, 0, 0},   {-1, 1, 1, -1}, {0, 0, 0, 0},
+                                   {1, 1, -1, -1}, {-1, -1, -1, -1}, {-1, -1, 1, 1}};
+static const double GV_IM[6][4] = {{-1, -1, 1, 1}, {0, 0, 0, 0}, {-1, 1, 1, -1},
+                                   {0, 0, 0, 0},   {0, 0, 0, 0}, {0, 0, 0, 0}};
+static const int GIN[6][4] = {{4, 3, 2, 1}, {4, 3, 2, 1}, {3, 4, 1, 2},
+                              {1, 2, 3, 4}, {3, 4, 1, 2}, {3, 4, 1, 2}};
+static const double AKAPPA = 0.5; /* params.F90:20 */
+
+static zdp gamval(int mu, int a) { /* mu, a 0-based */
+  return AKAPPA * (GV_RE[mu][a] + I * GV_IM[mu][a]);
+}
+static zsp gamvalf(int mu, int a) {
+  return 0.5f * ((float)GV_RE[mu][a] + I * (float)GV_IM[mu][a]);
+}
+static int gamin(int mu, int a) { return GIN[mu][a] - 1; }
+
+/* ------------------------------------------------------------------------ */
+/* halos (periodic copies, uncomms5.f90:6-19)                                */
+/* ------------------------------------------------------------------------ */
+static void halo5_z(zdp *A, const geom *g, int nspin) {
+  const geom G = *g;
+  for (int a = 0; a < nspin; a++) {
+    /* Ls direction first, then x, y, t; each pass copies full lower-dim faces
+     * so corners come out periodic too. */
+    for (int t = 1; t <= G.lt; t++)
+      for (int y = 1; y <= G.ly; y++)
+        for (int x = 1; x <= G.lx; x++) {
+          A[I5(G, 0, x, y, t, a)] = A[I5(G, G.ls, x, y, t, a)];
+          A[I5(G, G.ls + 1, x, y, t, a)] = A[I5(G, 1, x, y, t, a)];
+        }
+    for (int t = 1; t <= G.lt; t++)
+      for (int y = 1; y <= G.ly; y++)
+        for (int s = 0; s < G.S; s++) {
+          A[I5(G, s, 0, y, t, a)] = A[I5(G, s, G.lx, y, t, a)];
+          A[I5(G, s, G.lx + 1, y, t, a)] = A[I5(G, s, 1, y, t, a)];
+        }
+    for (int t = 1; t <= G.lt; t++)
+      for (int x = 0; x < G.X; x++)
+        for (int s = 0; s < G.S; s++) {
+          A[I5(G, s, x, 0, t, a)] = A[I5(G, s, x, G.ly, t, a)];
+          A[I5(G, s, x, G.ly + 1, t, a)] = A[I5(G, s, x, 1, t, a)];
+        }
+    for (int y = 0; y < G.Y; y++)
+      for (int x = 0; x < G.X; x++)
+        for (int s = 0; s < G.S; s++) {
+          A[I5(G, s, x, y, 0, a)] = A[I5(G, s, x, y, G.lt, a)];
+          A[I5(G, s, x, y, G.lt + 1, a)] = A[I5(G, s, x, y, 1, a)];
+        }
+  }
+}
+
+static void halo5_c(zsp *A, const geom *g) {
+  const geom G = *g;
+  for (int a = 0; a < 4; a++) {
+    for (int t = 1; t <= G.lt; t++)
+      for (int y = 1; y <= G.ly; y++)
+        for (int x = 1; x <= G.lx; x++) {
+          A[I5(G, 0, x, y, t, a)] = A[I5(G, G.ls, x, y, t, a)];
+          A[I5(G, G.ls + 1, x, y, t, a)] = A[I5(G, 1, x, y, t, a)];
+        }
+    for (int t = 1; t <= G.lt; t++)
+      for (int y = 1; y <= G.ly; y++)
+        for (int s = 0; s < G.S; s++) {
+          A[I5(G, s, 0, y, t, a)] = A[I5(G, s, G.lx, y, t, a)];
+          A[I5(G, s, G.lx + 1, y, t, a)] = A[I5(G, s, 1, y, t, a)];
+        }
+    for (int t = 1; t <= G.lt; t++)
+      for (int x = 0; x < G.X; x++)
+        for (int s = 0; s < G.S; s++) {
+          A[I5(G, s, x, 0, t, a)] = A[I5(G, s, x, G.ly, t, a)];
+          A[I5(G, s, x, G.ly + 1, t, a)] = A[I5(G, s, x, 1, t, a)];
+        }
+    for (int y = 0; y < G.Y; y++)
+      for (int x = 0; x < G.X; x++)
+        for (int s = 0; s < G.S; s++) {
+          A[I5(G, s, x, y, 0, a)] = A[I5(G, s, x, y, G.lt, a)];
+          A[I5(G, s, x, y, G.lt + 1, a)] = A[I5(G, s, x, y, 1, a)];
+        }
+  }
+}
+
+static void halo4_z(zdp *U, const geom *g) {
+  const geom G = *g;
+  for (int mu = 0; mu < 3; mu++) {
+    for (int t = 1; t <= G.lt; t++)
+      for (int y = 1; y <= G.ly; y++) {
+        U[I4(G, 0, y, t, mu)] = U[I4(G, G.lx, y, t, mu)];
+        U[I4(G, G.lx + 1, y, t, mu)] = U[I4(G, 1, y, t, mu)];
+      }
+    for (int t = 1; t <= G.lt; t++)
+      for (int x = 0; x < G.X; x++) {
+        U[I4(G, x, 0, t, mu)] = U[I4(G, x, G.ly, t, mu)];
+        U[I4(G, x, G.ly + 1, t, mu)] = U[I4(G, x, 1, t, mu)];
+      }
+    for (int y = 0; y < G.Y; y++)
+      for (int x = 0; x < G.X; x++) {
+        U[I4(G, x, y, 0, mu)] = U[I4(G, x, y, G.lt, mu)];
+        U[I4(G, x, y, G.lt + 1, mu)] = U[I4(G, x, y, 1, mu)];
+      }
+  }
+}
+
+void orc_halo5(double *a, const int dims[4], int nspin) {
+  geom g = mkgeom(dims);
+  halo5_z((zdp *)a, &g, nspin);
+}
+
+void orc_halo4(double *u, const int dims[4]) {
+  geom g = mkgeom(dims);
+  halo4_z((zdp *)u, &g);
+}
+
+/* ------------------------------------------------------------------------ */
+/* coef (gauge.F90:12-43)                                                    */
+/* ------------------------------------------------------------------------ */
+void orc_coef(double *u_, const float *theta, const int dims[4], int ibound) {
+  geom g = mkgeom(dims);
+  zdp *u = (zdp *)u_;
+  for (int mu = 0; mu < 3; mu++)
+    for (int t = 1; t <= g.lt; t++)
+      for (int y = 1; y <= g.ly; y++)
+        for (int x = 1; x <= g.lx; x++) {
+          /* (1.0 + cmplx(0.0, theta)): complex(sp) arithmetic, then to dp */
+          float th = theta[IF4(g, x, y, t, mu)];
+          u[I4(g, x, y, t, mu)] = (double)1.0f + I * (double)th;
+        }
+  if (ibound == -1) /* gauge.F90:30-32, single rank: ip_t == np_t - 1 */
+    for (int y = 0; y < g.Y; y++)
+      for (int x = 0; x < g.X; x++) u[I4(g, x, y, g.lt, 2)] = -u[I4(g, x, y, g.lt, 2)];
+  halo4_z(u, &g);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Shamir operator (dirac.F90:29-179 and 303-494)                            */
+/* ------------------------------------------------------------------------ */
+static const int DIR[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}; /* kdelta(i, mu) */
+
+/* hopping (Wilson + Dirac) part, sgn = +1 for M, -1 for M^dagger
+ * (dirac.F90:49-76 and 464-491) */
+static void hop_z(zdp *Phi, const zdp *R, const zdp *u, const geom *gp, double sgn) {
+  const geom g = *gp;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int t = 1; t <= g.lt; t++)
+    for (int y = 1; y <= g.ly; y++)
+      for (int x = 1; x <= g.lx; x++)
+        for (int mu = 0; mu < 3; mu++) {
+          const int xu = x + DIR[0][mu], yu = y + DIR[1][mu], tu = t + DIR[2][mu];
+          const int xd = x - DIR[0][mu], yd = y - DIR[1][mu], td = t - DIR[2][mu];
+          const zdp uf = u[I4(g, x, y, t, mu)];
+          const zdp ub = conj(u[I4(g, xd, yd, td, mu)]);
+          for (int a = 0; a < 4; a++) {
+            const int b = gamin(mu, a);
+            const zdp gv = sgn * gamval(mu, a);
+            for (int s = 1; s <= g.ls; s++) {
+              zdp wil = uf * R[I5(g, s, xu, yu, tu, a)] + ub * R[I5(g, s, xd, yd, td, a)];
+              zdp dir = uf * R[I5(g, s, xu, yu, tu, b)] - ub * R[I5(g, s, xd, yd, td, b)];
+              long o = I5(g, s, x, y, t, a);
+              Phi[o] = Phi[o] - AKAPPA * wil + gv * dir;
+            }
+          }
+        }
+}
+
+static void hop_c(zsp *Phi, const zsp *R, const zsp *u, const geom *gp, float sgn) {
+  const geom g = *gp;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int t = 1; t <= g.lt; t++)
+    for (int y = 1; y <= g.ly; y++)
+      for (int x = 1; x <= g.lx; x++)
+        for (int mu = 0; mu < 3; mu++) {
+          const int xu = x + DIR[0][mu], yu = y + DIR[1][mu], tu = t + DIR[2][mu];
+          const int xd = x - DIR[0][mu], yd = y - DIR[1][mu], td = t - DIR[2][mu];
+          const zsp uf = u[I4(g, x, y, t, mu)];
+          const zsp ub = conjf(u[I4(g, xd, yd, td, mu)]);
+          for (int a = 0; a < 4; a++) {
+            const int b = gamin(mu, a);
+            const zsp gv = sgn * gamvalf(mu, a);
+            for (int s = 1; s <= g.ls; s++) {
+              zsp wil = uf * R[I5(g, s, xu, yu, tu, a)] + ub * R[I5(g, s, xd, yd, td, a)];
+              zsp dir = uf * R[I5(g, s, xu, yu, tu, b)] - ub * R[I5(g, s, xd, yd, td, b)];
+              long o = I5(g, s, x, y, t, a);
+              Phi[o] = Phi[o] - 0.5f * wil + gv * dir;
+            }
+          }
+        }
+}
+
+/* Ls hops + wall mass for the single-rank (np_third == 1) branch.
+ * dag = 0: dirac.F90:80-87, 119-176.  dag = 1: dirac.F90:317-324, 356-402. */
+#define LS_AND_MASS(ZT, PHI, R, g, am, imass, dag)                                                 \
+  do {                                                                                           \
+    const int lo = (dag) ? 0 : 2, hi = (dag) ? 2 : 0; /* spins hopping from s+1 / from s-1 */    \
+    for (int t = 1; t <= g.lt; t++)                                                              \
+      for (int y = 1; y <= g.ly; y++)                                                            \
+        for (int x = 1; x <= g.lx; x++) {                                                        \
+          for (int a = lo; a < lo + 2; a++)                                                      \
+            for (int s = 1; s < g.ls; s++) PHI[I5(g, s, x, y, t, a)] -= R[I5(g, s + 1, x, y, t, a)]; \
+          for (int a = hi; a < hi + 2; a++)                                                      \
+            for (int s = 2; s <= g.ls; s++) PHI[I5(g, s, x, y, t, a)] -= R[I5(g, s - 1, x, y, t, a)]; \
+          const int L = g.ls;                                                                    \
+          if ((imass) == 1) {                                                                    \
+            const ZT zk = (am);                                                                  \
+            if (!(dag)) {                                                                        \
+              for (int a = 2; a < 4; a++) PHI[I5(g, L, x, y, t, a)] += zk * R[I5(g, L + 1, x, y, t, a)]; \
+              for (int a = 0; a < 2; a++) PHI[I5(g, 1, x, y, t, a)] += zk * R[I5(g, 0, x, y, t, a)]; \
+            } else {                                                                             \
+              for (int a = 0; a < 2; a++) PHI[I5(g, L, x, y, t, a)] += zk * R[I5(g, L + 1, x, y, t, a)]; \
+              for (int a = 2; a < 4; a++) PHI[I5(g, 1, x, y, t, a)] += zk * R[I5(g, 0, x, y, t, a)]; \
+            }                                                                                    \
+          } else if ((imass) == 3) {                                                             \
+            if (!(dag)) {                                                                        \
+              const ZT zk = -I * (am); /* cmplx(0.0, -am) */                                     \
+              for (int a = 2; a < 4; a++) PHI[I5(g, L, x, y, t, a)] -= zk * R[I5(g, L + 1, x, y, t, a)]; \
+              for (int a = 0; a < 2; a++) PHI[I5(g, 1, x, y, t, a)] += zk * R[I5(g, 0, x, y, t, a)]; \
+            } else {                                                                             \
+              const ZT zk = I * (am); /* cmplx(0.0, am) */                                       \
+              for (int a = 0; a < 2; a++) PHI[I5(g, L, x, y, t, a)] += zk * R[I5(g, L + 1, x, y, t, a)]; \
+              for (int a = 2; a < 4; a++) PHI[I5(g, 1, x, y, t, a)] -= zk * R[I5(g, 0, x, y, t, a)]; \
+            }                                                                                    \
+          } else if ((imass) == 5) {                                                             \
+            if (!(dag)) {                                                                        \
+              const ZT zk = -I * (am);                                                           \
+              for (int a = 2; a < 4; a++) PHI[I5(g, L, x, y, t, a)] -= zk * R[I5(g, L, x, y, t, a - 2)]; \
+              for (int a = 0; a < 2; a++) PHI[I5(g, 1, x, y, t, a)] -= zk * R[I5(g, 1, x, y, t, a + 2)]; \
+            } else {                                                                             \
+              const ZT zk = I * (am);                                                            \
+              for (int a = 0; a < 2; a++) PHI[I5(g, L, x, y, t, a)] -= zk * R[I5(g, L, x, y, t, a + 2)]; \
+              for (int a = 2; a < 4; a++) PHI[I5(g, 1, x, y, t, a)] -= zk * R[I5(g, 1, x, y, t, a - 2)]; \
+            }                                                                                    \
+          }                                                                                      \
+        }                                                                                        \
+  } while (0)
+
+static void dslash_shamir_z(zdp *Phi, const zdp *R, const zdp *u, const geom *gp, float am,
+                            int imass, float am3, int dag) {
+  const geom g = *gp;
+  const float diag = (3.0f - am3) + 1.0f; /* dirac.F90:45 (default real) */
+  if (!dag) {
+    /* Phi = diag*R over the whole haloed array (dirac.F90:46) */
+    for (long i = 0; i < g.n5; i++) Phi[i] = (double)diag * R[i];
+  } else {
+    /* interior only (dirac.F90:312-313) */
+    for (int a = 0; a < 4; a++)
+      for (int t = 1; t <= g.lt; t++)
+        for (int y = 1; y <= g.ly; y++)
+          for (int x = 1; x <= g.lx; x++)
+            for (int s = 1; s <= g.ls; s++) {
+              long o = I5(g, s, x, y, t, a);
+              Phi[o] = (double)diag * R[o];
+            }
+  }
+  if (!dag) {
+    hop_z(Phi, R, u, gp, +1.0);
+    LS_AND_MASS(zdp, Phi, R, g, (double)am, imass, 0);
+  } else {
+    LS_AND_MASS(zdp, Phi, R, g, (double)am, imass, 1);
+    hop_z(Phi, R, u, gp, -1.0);
+  }
+}
+
+static void dslash_shamir_c(zsp *Phi, const zsp *R, const zsp *u, const geom *gp, float am,
+                            int imass, float am3, int dag) {
+  const geom g = *gp;
+  const float diag = (3.0f - am3) + 1.0f;
+  if (!dag) {
+    for (long i = 0; i < g.n5; i++) Phi[i] = diag * R[i];
+  } else {
+    for (int a = 0; a < 4; a++)
+      for (int t = 1; t <= g.lt; t++)
+        for (int y = 1; y <= g.ly; y++)
+          for (int x = 1; x <= g.lx; x++)
+            for (int s = 1; s <= g.ls; s++) {
+              long o = I5(g, s, x, y, t, a);
+              Phi[o] = diag * R[o];
+            }
+  }
+  if (!dag) {
+    hop_c(Phi, R, u, gp, +1.0f);
+    LS_AND_MASS(zsp, Phi, R, g, am, imass, 0);
+  } else {
+    LS_AND_MASS(zsp, Phi, R, g, am, imass, 1);
+    hop_c(Phi, R, u, gp, -1.0f);
+  }
+}
+
+void orc_dslash_shamir(double *phi, const double *r, const double *u, const int dims[4], float am,
+                       int imass, float am3) {
+  geom g = mkgeom(dims);
+  dslash_shamir_z((zdp *)phi, (const zdp *)r, (const zdp *)u, &g, am, imass, am3, 0);
+}
+
+void orc_dslashd_shamir(double *phi, const double *r, const double *u, const int dims[4], float am,
+                        int imass, float am3) {
+  geom g = mkgeom(dims);
+  dslash_shamir_z((zdp *)phi, (const zdp *)r, (const zdp *)u, &g, am, imass, am3, 1);
+}
+
+void orc_dslash_shamir_sp(float *phi, const float *r, const float *u, const int dims[4], float am,
+                          int imass, float am3) {
+  geom g = mkgeom(dims);
+  dslash_shamir_c((zsp *)phi, (const zsp *)r, (const zsp *)u, &g, am, imass, am3, 0);
+}
+
+void orc_dslashd_shamir_sp(float *phi, const float *r, const float *u, const int dims[4], float am,
+                           int imass, float am3) {
+  geom g = mkgeom(dims);
+  dslash_shamir_c((zsp *)phi, (const zsp *)r, (const zsp *)u, &g, am, imass, am3, 1);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Wilson-kernel operator (dirac.F90:181-299, 496-625)                       */
+/* ------------------------------------------------------------------------ */
+/* 4-D Wilson operator on one haloed slice (X,Y,T,4); dag selects DWilsonD.
+ * Phi = diag*R over the whole slice, then the interior hopping terms. */
+static void dwilson(zdp *Phi, const zdp *R, const zdp *u, const geom *gp, float mass, int dag) {
+  const geom g = *gp;
+  const long X = g.X, Y = g.Y, T = g.T;
+#define I3(x, y, t, a) ((long)(x) + X * ((long)(y) + Y * ((long)(t) + T * (long)(a))))
+  const double diag = (double)(3.0f + mass); /* dirac.F90:191, 505 */
+  const long n = X * Y * T * 4;
+  for (long i = 0; i < n; i++) Phi[i] = diag * R[i];
+  const double sgn = dag ? -1.0 : 1.0;
+  for (int mu = 0; mu < 3; mu++)
+    for (int a = 0; a < 4; a++) {
+      const int b = gamin(mu, a);
+      const zdp gv = sgn * gamval(mu, a);
+      for (int t = 1; t <= g.lt; t++)
+        for (int y = 1; y <= g.ly; y++)
+          for (int x = 1; x <= g.lx; x++) {
+            const int xu = x + DIR[0][mu], yu = y + DIR[1][mu], tu = t + DIR[2][mu];
+            const int xd = x - DIR[0][mu], yd = y - DIR[1][mu], td = t - DIR[2][mu];
+            const zdp uf = u[I4(g, x, y, t, mu)];
+            const zdp ub = conj(u[I4(g, xd, yd, td, mu)]);
+            zdp wil = uf * R[I3(xu, yu, tu, a)] + ub * R[I3(xd, yd, td, a)];
+            zdp dir = uf * R[I3(xu, yu, tu, b)] - ub * R[I3(xd, yd, td, b)];
+            long o = I3(x, y, t, a);
+            Phi[o] = Phi[o] - AKAPPA * wil + gv * dir;
+          }
+    }
+#undef I3
+}
+
+/* copy spins [a0,a1) of slice s of R (whole x,y,t halo range) into a slice */
+static void get_slice(zdp *sl, const zdp *R, const geom *gp, int s, int a0, int a1) {
+  const geom g = *gp;
+  const long n3 = g.X * g.Y * g.T;
+  memset(sl, 0, sizeof(zdp) * n3 * 4);
+  for (int a = a0; a < a1; a++)
+    for (int t = 0; t < g.T; t++)
+      for (int y = 0; y < g.Y; y++)
+        for (int x = 0; x < g.X; x++)
+          sl[x + g.X * (y + g.Y * (t + g.T * (long)a))] = R[I5(g, s, x, y, t, a)];
+}
+
+static void dslash_wilson_z(zdp *Phi, const zdp *R, const zdp *u, const geom *gp, float am,
+                            int imass, float am3) {
+  const geom g = *gp;
+  const long n3 = g.X * g.Y * g.T, n4s = n3 * 4;
+  zdp *Rs = malloc(sizeof(zdp) * n4s), *Ps = malloc(sizeof(zdp) * n4s),
+      *Ms = malloc(sizeof(zdp) * n4s);
+#define SL(x, y, t, a) ((long)(x) + g.X * ((long)(y) + g.Y * ((long)(t) + g.T * (long)(a))))
+  for (long i = 0; i < g.n5; i++) Phi[i] = 0;
+  for (int pass = 0; pass < 2; pass++) {
+    /* pass 0: P+ component (spins 1:2), lower diagonal; pass 1: P-, upper */
+    for (int il = 1; il <= g.ls; il++) {
+      get_slice(Rs, R, gp, il, pass ? 2 : 0, pass ? 4 : 2);
+      dwilson(Ps, Rs, u, gp, -am3, 0);
+      for (int a = 0; a < 4; a++)
+        for (int t = 0; t < g.T; t++)
+          for (int y = 0; y < g.Y; y++)
+            for (int x = 0; x < g.X; x++) {
+              long q = SL(x, y, t, a);
+              Phi[I5(g, il, x, y, t, a)] += Ps[q] + Rs[q];
+              if (pass == 0 && il < g.ls) Phi[I5(g, il + 1, x, y, t, a)] += Ps[q] - Rs[q];
+              if (pass == 1 && il > 1) Phi[I5(g, il - 1, x, y, t, a)] += Ps[q] - Rs[q];
+            }
+    }
+  }
+  /* mass term, top right (dirac.F90:271-281) */
+  get_slice(Rs, R, gp, g.ls, 0, 2);
+  dwilson(Ps, Rs, u, gp, -am3, 0);
+  zdp c = (imass == 1) ? -(double)am : -I * (double)am;
+  for (long q = 0; q < n4s; q++) Ms[q] = c * (Ps[q] - Rs[q]);
+  for (int a = 0; a < 4; a++)
+    for (int t = 1; t <= g.lt; t++)
+      for (int y = 1; y <= g.ly; y++)
+        for (int x = 1; x <= g.lx; x++) Phi[I5(g, 1, x, y, t, a)] += Ms[SL(x, y, t, a)];
+  /* mass term, bottom left (dirac.F90:284-294) */
+  get_slice(Rs, R, gp, 1, 2, 4);
+  dwilson(Ps, Rs, u, gp, -am3, 0);
+  c = (imass == 1) ? -(double)am : I * (double)am;
+  for (long q = 0; q < n4s; q++) Ms[q] = c * (Ps[q] - Rs[q]);
+  for (int a = 0; a < 4; a++)
+    for (int t = 1; t <= g.lt; t++)
+      for (int y = 1; y <= g.ly; y++)
+        for (int x = 1; x <= g.lx; x++) Phi[I5(g, g.ls, x, y, t, a)] += Ms[SL(x, y, t, a)];
+  free(Rs);
+  free(Ps);
+  free(Ms);
+}
+
+static void dslashd_wilson_z(zdp *Phi, const zdp *R, const zdp *u, const geom *gp, float am,
+                             int imass, float am3) {
+  const geom g = *gp;
+  const long n3 = g.X * g.Y * g.T, n4s = n3 * 4;
+  zdp *Rs = malloc(sizeof(zdp) * n4s), *Ps = malloc(sizeof(zdp) * n4s);
+  for (long i = 0; i < g.n5; i++) Phi[i] = 0;
+  /* identity and Ls hops (dirac.F90:565-579) */
+  for (int il = 1; il <= g.ls; il++)
+    for (int a = 0; a < 4; a++)
+      for (int t = 0; t < g.T; t++)
+        for (int y = 0; y < g.Y; y++)
+          for (int x = 0; x < g.X; x++) {
+            zdp v = R[I5(g, il, x, y, t, a)];
+            Phi[I5(g, il, x, y, t, a)] += v;
+            if (il < g.ls && a >= 2) Phi[I5(g, il + 1, x, y, t, a)] -= v;
+            if (il > 1 && a < 2) Phi[I5(g, il - 1, x, y, t, a)] -= v;
+          }
+  /* Wilson slices (dirac.F90:582-596) */
+  for (int il = 1; il <= g.ls; il++) {
+    get_slice(Rs, R, gp, il, 0, 4);
+    dwilson(Ps, Rs, u, gp, -am3, 1);
+    for (int a = 0; a < 4; a++)
+      for (int t = 0; t < g.T; t++)
+        for (int y = 0; y < g.Y; y++)
+          for (int x = 0; x < g.X; x++) {
+            zdp v = Ps[SL(x, y, t, a)];
+            Phi[I5(g, il, x, y, t, a)] += v;
+            if (il < g.ls && a >= 2) Phi[I5(g, il + 1, x, y, t, a)] += v;
+            if (il > 1 && a < 2) Phi[I5(g, il - 1, x, y, t, a)] += v;
+          }
+  }
+  /* upper right mass (dirac.F90:599-609) */
+  get_slice(Rs, R, gp, g.ls, 0, 4);
+  dwilson(Ps, Rs, u, gp, -am3, 1);
+  zdp zk = (imass == 1) ? -(double)am : -I * (double)am;
+  for (int a = 2; a < 4; a++)
+    for (int t = 1; t <= g.lt; t++)
+      for (int y = 1; y <= g.ly; y++)
+        for (int x = 1; x <= g.lx; x++) {
+          long q = SL(x, y, t, a);
+          Phi[I5(g, 1, x, y, t, a)] += zk * (Ps[q] - Rs[q]);
+        }
+  /* lower left mass (dirac.F90:612-622) */
+  get_slice(Rs, R, gp, 1, 0, 4);
+  dwilson(Ps, Rs, u, gp, -am3, 1);
+  zk = (imass == 1) ? -(double)am : I * (double)am;
+  for (int a = 0; a < 2; a++)
+    for (int t = 1; t <= g.lt; t++)
+      for (int y = 1; y <= g.ly; y++)
+        for (int x = 1; x <= g.lx; x++) {
+          long q = SL(x, y, t, a);
+          Phi[I5(g, g.ls, x, y, t, a)] += zk * (Ps[q] - Rs[q]);
+        }
+#undef SL
+  free(Rs);
+  free(Ps);
+}
+
+void orc_dslash_wilson(double *phi, const double *r, const double *u, const int dims[4], float am,
+                       int imass, float am3) {
+  geom g = mkgeom(dims);
+  dslash_wilson_z((zdp *)phi, (const zdp *)r, (const zdp *)u, &g, am, imass, am3);
+}
+
+void orc_dslashd_wilson(double *phi, const double *r, const double *u, const int dims[4], float am,
+                        int imass, float am3) {
+  geom g = mkgeom(dims);
+  dslashd_wilson_z((zdp *)phi, (const zdp *)r, (const zdp *)u, &g, am, imass, am3);
+}
+
+/* kernel dispatch: the reference picks one at compile time
+ * (GENERATE_WITH_SHAMIR / GENERATE_WITH_WILSON, dirac.F90:19-25) */
+static void op_M(zdp *Phi, const zdp *R, const zdp *u, const geom *g, float am, int imass,
+                 float am3, int kernel) {
+  if (kernel == 1)
+    dslash_wilson_z(Phi, R, u, g, am, imass, am3);
+  else
+    dslash_shamir_z(Phi, R, u, g, am, imass, am3, 0);
+}
+static void op_Md(zdp *Phi, const zdp *R, const zdp *u, const geom *g, float am, int imass,
+                  float am3, int kernel) {
+  if (kernel == 1)
+    dslashd_wilson_z(Phi, R, u, g, am, imass, am3);
+  else
+    dslash_shamir_z(Phi, R, u, g, am, imass, am3, 1);
+}
+
+void orc_dirac_op_shifted(double *xout_, const double *xin_, const double *u_, const int dims[4],
+                          float am, int imass, float am3, double shift, int kernel) {
+  geom g = mkgeom(dims);
+  zdp *xout = (zdp *)xout_;
+  const zdp *xin = (const zdp *)xin_, *u = (const zdp *)u_;
+  zdp *tmp = calloc(g.n5, sizeof(zdp));
+  op_M(tmp, xin, u, &g, am, imass, am3, kernel);
+  halo5_z(tmp, &g, 4);
+  op_Md(xout, tmp, u, &g, am, imass, am3, kernel);
+  halo5_z(xout, &g, 4);
+  if (shift != 0.0)
+    for (long i = 0; i < g.n5; i++) xout[i] = xout[i] + shift * xin[i];
+  free(tmp);
+}
+
+/* ------------------------------------------------------------------------ */
+/* reductions over the interior                                              */
+/* ------------------------------------------------------------------------ */
+static double norm2_int(const zdp *A, const geom *gp) {
+  const geom g = *gp;
+  double acc = 0.0;
+#pragma omp parallel for collapse(2) reduction(+ : acc) schedule(static)
+  for (int a = 0; a < 4; a++)
+    for (int t = 1; t <= g.lt; t++)
+      for (int y = 1; y <= g.ly; y++)
+        for (int x = 1; x <= g.lx; x++)
+          for (int s = 1; s <= g.ls; s++) {
+            zdp v = A[I5(g, s, x, y, t, a)];
+            acc += creal(v) * creal(v) + cimag(v) * cimag(v);
+          }
+  return acc;
+}
+
+static double redot_int(const zdp *A, const zdp *B, const geom *gp) {
+  const geom g = *gp;
+  double acc = 0.0;
+#pragma omp parallel for collapse(2) reduction(+ : acc) schedule(static)
+  for (int a = 0; a < 4; a++)
+    for (int t = 1; t <= g.lt; t++)
+      for (int y = 1; y <= g.ly; y++)
+        for (int x = 1; x <= g.lx; x++)
+          for (int s = 1; s <= g.ls; s++) {
+            long o = I5(g, s, x, y, t, a);
+            acc += creal(conj(A[o]) * B[o]);
+          }
+  return acc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* congrad (measure_module.F90:9-140)                                        */
+/* ------------------------------------------------------------------------ */
+void orc_congrad(const double *phi_, double *x_, const double *u_, const int dims[4], float res,
+                 float am, int imass, float am3, int niterc, int kernel, int *itercg,
+                 int *iterations) {
+  geom g = mkgeom(dims);
+  const zdp *Phi = (const zdp *)phi_, *u = (const zdp *)u_;
+  zdp *x = (zdp *)x_;
+  zdp *x1 = calloc(g.n5, sizeof(zdp)), *x2 = calloc(g.n5, sizeof(zdp)),
+      *p = malloc(sizeof(zdp) * g.n5), *r = malloc(sizeof(zdp) * g.n5);
+  /* resid = 4*ksize*ksize*ksizet*kthird*res*res: integer * real -> real */
+  const long nsite4 = 4L * g.lx * g.ly * g.lt * g.ls;
+  const float resid = ((float)nsite4 * res) * res;
+  int it = 0, nx;
+  double alphan = 0.0, betacgd = 1.0, alpha = 1.0, betacg, betacgn, alphad;
+  memcpy(p, x, sizeof(zdp) * g.n5);
+  memcpy(r, Phi, sizeof(zdp) * g.n5);
+  for (nx = 1; nx <= niterc; nx++) {
+    it++;
+    op_M(x1, p, u, &g, am, imass, am3, kernel);
+    if (nx != 1) {
+      alphad = norm2_int(x1, &g);
+      alpha = alphan / alphad;
+#pragma omp parallel for schedule(static)
+      for (long i = 0; i < g.n5; i++) x[i] = x[i] + alpha * p[i];
+    }
+    halo5_z(x1, &g, 4);
+    op_Md(x2, x1, u, &g, am, imass, am3, kernel);
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < g.n5; i++) r[i] = r[i] - alpha * x2[i];
+    betacgn = norm2_int(r, &g);
+    betacg = betacgn / betacgd;
+    betacgd = betacgn;
+    alphan = betacgn;
+    if (nx == 1) betacg = 0.0;
+    halo5_z(r, &g, 4);
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < g.n5; i++) p[i] = r[i] + betacg * p[i];
+    if (betacgn < (double)resid) break;
+  }
+  *itercg = it;
+  if (iterations) *iterations = nx;
+  free(x1);
+  free(x2);
+  free(p);
+  free(r);
+}
+
+/* ------------------------------------------------------------------------ */
+/* multishift CG (multishift_module.F90:13-284)                              */
+/* ------------------------------------------------------------------------ */
+void orc_multishift(const double *u_, const int dims[4], float am, int imass, float am3,
+                    int ndiagq, const double *aden, const double *anum, double *output_,
+                    const double *input_, float res, int maxcg, int kernel, int *cg_return,
+                    int *cg_returns) {
+  geom g = mkgeom(dims);
+  const zdp *u = (const zdp *)u_, *input = (const zdp *)input_;
+  zdp *output = (zdp *)output_;
+  const long nh = (long)g.ls * g.lx * g.ly * g.lt * 4; /* halo-free length */
+  zdp *r = malloc(sizeof(zdp) * g.n5), *h = calloc(g.n5, sizeof(zdp)),
+      *s = calloc(g.n5, sizeof(zdp)), *p = malloc(sizeof(zdp) * g.n5);
+  zdp *shiftferm = malloc(sizeof(zdp) * nh * ndiagq);
+  double *zeta_i = calloc(ndiagq, sizeof(double)), *zeta_ii = calloc(ndiagq, sizeof(double)),
+         *zeta_iii = calloc(ndiagq, sizeof(double)), *omegas = calloc(ndiagq, sizeof(double)),
+         *gammas = calloc(ndiagq, sizeof(double)), *corr = calloc(ndiagq, sizeof(double));
+  int *flags = calloc(ndiagq, sizeof(int));
+  memcpy(r, input, sizeof(zdp) * g.n5);
+  memcpy(p, r, sizeof(zdp) * g.n5);
+  double delta = norm2_int(r, &g);
+  double omega = 1.0, alpha = 0.0, omega_save, lambda, gammag;
+  memset(output, 0, sizeof(zdp) * nh * ndiagq);
+  double lsum = 0.0;
+  for (int i = 0; i < ndiagq; i++) {
+    corr[i] = fabs(anum[i] / aden[i]);
+    lsum += log(corr[i]);
+  }
+  const double gm = exp(lsum / ndiagq);
+  for (int i = 0; i < ndiagq; i++) corr[i] = corr[i] / gm;
+  for (int i = 0; i < ndiagq; i++) {
+    flags[i] = 1;
+    zdp *sf = shiftferm + (long)i * nh;
+    for (int a = 0; a < 4; a++)
+      for (int t = 1; t <= g.lt; t++)
+        for (int y = 1; y <= g.ly; y++)
+          for (int x = 1; x <= g.lx; x++)
+            for (int ss = 1; ss <= g.ls; ss++) sf[IH(g, ss, x, y, t, a)] = input[I5(g, ss, x, y, t, a)];
+    zeta_i[i] = 1.0;
+    zeta_ii[i] = 1.0;
+    gammas[i] = 0.0;
+  }
+  gammag = 0.0;
+  int cgr = 0, maxi = ndiagq, mini = 1; /* 1-based window as in the reference */
+  while (maxi > mini - 1 && cgr < maxcg) {
+    cgr++;
+    op_M(h, p, u, &g, am, imass, am3, kernel);
+    halo5_z(h, &g, 4);
+    op_Md(s, h, u, &g, am, imass, am3, kernel);
+    alpha = redot_int(p, s, &g);
+    omega_save = omega;
+    omega = -delta / alpha;
+    for (int i = mini - 1; i < maxi; i++) {
+      zeta_iii[i] = (zeta_i[i] * zeta_ii[i] * omega_save) /
+                    (omega * gammag * (zeta_i[i] - zeta_ii[i]) +
+                     zeta_i[i] * omega_save * (1.0 - aden[i] * omega));
+      omegas[i] = omega * zeta_iii[i] / zeta_ii[i];
+    }
+#pragma omp parallel for schedule(static)
+    for (long k = 0; k < g.n5; k++) r[k] = r[k] + omega * s[k];
+    lambda = norm2_int(r, &g);
+    gammag = lambda / delta;
+#pragma omp parallel for schedule(static)
+    for (long k = 0; k < g.n5; k++) p[k] = r[k] + gammag * p[k];
+    halo5_z(p, &g, 4);
+    for (int i = mini - 1; i < maxi; i++)
+      gammas[i] = gammag * zeta_iii[i] * omegas[i] / (zeta_ii[i] * omega);
+    for (int i = mini - 1; i < maxi; i++) {
+      zdp *sf = shiftferm + (long)i * nh, *out = output + (long)i * nh;
+      const double om = omegas[i], ga = gammas[i], ze = zeta_iii[i];
+#pragma omp parallel for collapse(2) schedule(static)
+      for (int a = 0; a < 4; a++)
+        for (int t = 1; t <= g.lt; t++)
+          for (int y = 1; y <= g.ly; y++)
+            for (int x = 1; x <= g.lx; x++)
+              for (int ss = 1; ss <= g.ls; ss++) {
+                long q = IH(g, ss, x, y, t, a);
+                out[q] = out[q] - sf[q] * om;
+                sf[q] = sf[q] * ga + r[I5(g, ss, x, y, t, a)] * ze;
+              }
+    }
+    if (cg_returns)
+      for (int i = mini - 1; i < maxi; i++) cg_returns[i] = cgr;
+    int nmax = 0, nmin = ndiagq + 1;
+    for (int i = 0; i < ndiagq; i++) {
+      if (flags[i]) {
+        if (sqrt(delta * zeta_ii[i] * zeta_ii[i]) * corr[i] < (double)res)
+          flags[i] = 0;
+        else {
+          if (i + 1 > nmax) nmax = i + 1;
+          if (i + 1 < nmin) nmin = i + 1;
+        }
+      }
+    }
+    maxi = nmax;
+    mini = nmin;
+    for (int i = mini - 1; i < maxi; i++) {
+      zeta_i[i] = zeta_ii[i];
+      zeta_ii[i] = zeta_iii[i];
+    }
+    delta = lambda;
+  }
+  *cg_return = cgr;
+  free(r); free(h); free(s); free(p); free(shiftferm);
+  free(zeta_i); free(zeta_ii); free(zeta_iii); free(omegas); free(gammas); free(corr); free(flags);
+}
+
+/* FP32 vectors, FP64 scalars (multishift_module.F90:289-534) */
+static double reduce_sp_norm2(const zsp *A, const geom *gp) {
+  /* reduce_real_dp5d(abs(r)**2): float32 terms summed in float64 (reductions.F90:6-32) */
+  const geom g = *gp;
+  double acc = 0.0;
+#pragma omp parallel for collapse(2) reduction(+ : acc) schedule(static)
+  for (int a = 0; a < 4; a++)
+    for (int t = 1; t <= g.lt; t++)
+      for (int y = 1; y <= g.ly; y++)
+        for (int x = 1; x <= g.lx; x++)
+          for (int s = 1; s <= g.ls; s++) {
+            zsp v = A[I5(g, s, x, y, t, a)];
+            float term = crealf(v) * crealf(v) + cimagf(v) * cimagf(v);
+            acc += (double)term;
+          }
+  return acc;
+}
+
+static double reduce_sp_redot(const zsp *A, const zsp *B, const geom *gp) {
+  const geom g = *gp;
+  double acc = 0.0;
+#pragma omp parallel for collapse(2) reduction(+ : acc) schedule(static)
+  for (int a = 0; a < 4; a++)
+    for (int t = 1; t <= g.lt; t++)
+      for (int y = 1; y <= g.ly; y++)
+        for (int x = 1; x <= g.lx; x++)
+          for (int s = 1; s <= g.ls; s++) {
+            long o = I5(g, s, x, y, t, a);
+            float term = crealf(conjf(A[o]) * B[o]);
+            acc += (double)term;
+          }
+  return acc;
+}
+
+void orc_multishift_sp(const double *udp_, const int dims[4], float am, int imass, float am3,
+                       int ndiagq, const double *aden, const double *anum, double *outputdp_,
+                       const double *inputdp_, float res, int maxcg, int *cg_return,
+                       int *cg_returns) {
+  geom g = mkgeom(dims);
+  const zdp *udp = (const zdp *)udp_, *inputdp = (const zdp *)inputdp_;
+  zdp *outputdp = (zdp *)outputdp_;
+  const long nh = (long)g.ls * g.lx * g.ly * g.lt * 4;
+  zsp *u = malloc(sizeof(zsp) * g.n4);
+  zsp *r = malloc(sizeof(zsp) * g.n5), *h = calloc(g.n5, sizeof(zsp)),
+      *s = calloc(g.n5, sizeof(zsp)), *p = malloc(sizeof(zsp) * g.n5);
+  zsp *output = calloc(nh * ndiagq, sizeof(zsp)), *shiftferm = malloc(sizeof(zsp) * nh * ndiagq);
+  double *zeta_i = calloc(ndiagq, sizeof(double)), *zeta_ii = calloc(ndiagq, sizeof(double)),
+         *zeta_iii = calloc(ndiagq, sizeof(double)), *omegas = calloc(ndiagq, sizeof(double)),
+         *gammas = calloc(ndiagq, sizeof(double)), *corr = calloc(ndiagq, sizeof(double));
+  int *flags = calloc(ndiagq, sizeof(int));
+  for (long k = 0; k < g.n4; k++) u[k] = (float)creal(udp[k]) + I * (float)cimag(udp[k]);
+  for (long k = 0; k < g.n5; k++) r[k] = (float)creal(inputdp[k]) + I * (float)cimag(inputdp[k]);
+  memcpy(p, r, sizeof(zsp) * g.n5);
+  double delta = reduce_sp_norm2(r, &g);
+  double omega = 1.0, alpha, omega_save, lambda, gammag;
+  double lsum = 0.0;
+  for (int i = 0; i < ndiagq; i++) {
+    corr[i] = fabs(anum[i] / aden[i]);
+    lsum += log(corr[i]);
+  }
+  const double gm = exp(lsum / ndiagq);
+  for (int i = 0; i < ndiagq; i++) corr[i] = corr[i] / gm;
+  for (int i = 0; i < ndiagq; i++) {
+    flags[i] = 1;
+    zsp *sf = shiftferm + (long)i * nh;
+    for (int a = 0; a < 4; a++)
+      for (int t = 1; t <= g.lt; t++)
+        for (int y = 1; y <= g.ly; y++)
+          for (int x = 1; x <= g.lx; x++)
+            for (int ss = 1; ss <= g.ls; ss++) sf[IH(g, ss, x, y, t, a)] = r[I5(g, ss, x, y, t, a)];
+    zeta_i[i] = 1.0;
+    zeta_ii[i] = 1.0;
+    gammas[i] = 0.0;
+  }
+  gammag = 0.0;
+  int cgr = 0, maxi = ndiagq, mini = 1;
+  if (cg_returns)
+    for (int i = 0; i < ndiagq; i++) cg_returns[i] = 0;
+  while (maxi > mini - 1 && cgr < maxcg) {
+    cgr++;
+    dslash_shamir_c(h, p, u, &g, am, imass, am3, 0);
+    halo5_c(h, &g);
+    dslash_shamir_c(s, h, u, &g, am, imass, am3, 1);
+    alpha = reduce_sp_redot(p, s, &g);
+    omega_save = omega;
+    omega = -delta / alpha;
+    for (int i = mini - 1; i < maxi; i++) {
+      zeta_iii[i] = (zeta_i[i] * zeta_ii[i] * omega_save) /
+                    (omega * gammag * (zeta_i[i] - zeta_ii[i]) +
+                     zeta_i[i] * omega_save * (1.0 - aden[i] * omega));
+      omegas[i] = omega * zeta_iii[i] / zeta_ii[i];
+    }
+    const float omf = (float)omega;
+#pragma omp parallel for schedule(static)
+    for (long k = 0; k < g.n5; k++) r[k] = r[k] + omf * s[k];
+    lambda = reduce_sp_norm2(r, &g);
+    gammag = lambda / delta;
+    const float gmf = (float)gammag;
+#pragma omp parallel for schedule(static)
+    for (long k = 0; k < g.n5; k++) p[k] = r[k] + gmf * p[k];
+    halo5_c(p, &g);
+    for (int i = mini - 1; i < maxi; i++)
+      gammas[i] = gammag * zeta_iii[i] * omegas[i] / (zeta_ii[i] * omega);
+    for (int i = mini - 1; i < maxi; i++) {
+      zsp *sf = shiftferm + (long)i * nh, *out = output + (long)i * nh;
+      const float om = (float)omegas[i], ga = (float)gammas[i], ze = (float)zeta_iii[i];
+#pragma omp parallel for collapse(2) schedule(static)
+      for (int a = 0; a < 4; a++)
+        for (int t = 1; t <= g.lt; t++)
+          for (int y = 1; y <= g.ly; y++)
+            for (int x = 1; x <= g.lx; x++)
+              for (int ss = 1; ss <= g.ls; ss++) {
+                long q = IH(g, ss, x, y, t, a);
+                out[q] = out[q] - sf[q] * om;
+                sf[q] = sf[q] * ga + r[I5(g, ss, x, y, t, a)] * ze;
+              }
+    }
+    if (cg_returns)
+      for (int i = mini - 1; i < maxi; i++) cg_returns[i] = cgr;
+    int nmax = 0, nmin = ndiagq + 1;
+    for (int i = 0; i < ndiagq; i++) {
+      if (flags[i]) {
+        if (sqrt(delta * zeta_ii[i] * zeta_ii[i]) * corr[i] < (double)res)
+          flags[i] = 0;
+        else {
+          if (i + 1 > nmax) nmax = i + 1;
+          if (i + 1 < nmin) nmin = i + 1;
+        }
+      }
+    }
+    maxi = nmax;
+    mini = nmin;
+    for (int i = mini - 1; i < maxi; i++) {
+      zeta_i[i] = zeta_ii[i];
+      zeta_ii[i] = zeta_iii[i];
+    }
+    delta = lambda;
+  }
+  for (long k = 0; k < nh * ndiagq; k++) outputdp[k] = (double)crealf(output[k]) + I * (double)cimagf(output[k]);
+  *cg_return = cgr;
+  free(u); free(r); free(h); free(s); free(p); free(output); free(shiftferm);
+  free(zeta_i); free(zeta_ii); free(zeta_iii); free(omegas); free(gammas); free(corr); free(flags);
+}
+
+/* ------------------------------------------------------------------------ */
+/* derivs (derivs_module.F90:99-196)                                         */
+/* ------------------------------------------------------------------------ */
+void orc_derivs_shamir(const double *r_, const double *x2_, double anum, int iflag,
+                       const int dims[4], double *dsdpi) {
+  geom g = mkgeom(dims);
+  const zdp *R = (const zdp *)r_, *X2 = (const zdp *)x2_;
+  const float tzi_real = 2.0f * (float)anum;         /* real :: tzi_real = 2*real(anum) */
+  const double c1 = (double)(tzi_real * 0.5f);       /* tzi_real*real(akappa) */
+  const double c2 = (double)tzi_real * (iflag == 0 ? 1.0 : -1.0);
+  const long n4 = (long)g.lx * g.ly * g.lt;
+  double *tmp = calloc(n4 * 3, sizeof(double));
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int mu = 0; mu < 3; mu++)
+    for (int t = 1; t <= g.lt; t++)
+      for (int y = 1; y <= g.ly; y++)
+        for (int x = 1; x <= g.lx; x++) {
+          const int xu = x + DIR[0][mu], yu = y + DIR[1][mu], tu = t + DIR[2][mu];
+          double acc = 0.0;
+          for (int a = 0; a < 4; a++) {
+            double sw = 0.0, sd = 0.0;
+            const int b = gamin(mu, a);
+            const zdp gv = gamval(mu, a);
+            for (int s = 1; s <= g.ls; s++) {
+              sw += cimag(conj(R[I5(g, s, x, y, t, a)]) * X2[I5(g, s, xu, yu, tu, a)]) -
+                    cimag(conj(R[I5(g, s, xu, yu, tu, a)]) * X2[I5(g, s, x, y, t, a)]);
+              sd += cimag(gv * (conj(R[I5(g, s, x, y, t, a)]) * X2[I5(g, s, xu, yu, tu, b)] +
+                                conj(R[I5(g, s, xu, yu, tu, a)]) * X2[I5(g, s, x, y, t, b)]));
+            }
+            acc = acc + c1 * sw;
+            acc = acc + c2 * sd;
+          }
+          tmp[IF4(g, x, y, t, mu)] = acc;
+        }
+  for (long k = 0; k < n4 * 3; k++) dsdpi[k] += tmp[k];
+  free(tmp);
+}
+
+/* derivSlice (derivs_module.F90:348-410): accumulates into dsdpi */
+static void deriv_slice(const zdp *sL, const zdp *sR, double anum, int iflag, zdp cmult,
+                        const geom *gp, double *dsdpi) {
+  const geom g = *gp;
+  const long X = g.X, Y = g.Y, T = g.T;
+#define I3(x, y, t, a) ((long)(x) + X * ((long)(y) + Y * ((long)(t) + T * (long)(a))))
+  const float tzi_real = 2.0f * (float)anum;
+  const double c1 = (double)(tzi_real * 0.5f);
+  const float dmult = (iflag == 1) ? -1.0f : 1.0f;
+  const double c2 = (double)(tzi_real * dmult);
+  for (int mu = 0; mu < 3; mu++)
+    for (int a = 0; a < 4; a++) {
+      for (int t = 1; t <= g.lt; t++)
+        for (int y = 1; y <= g.ly; y++)
+          for (int x = 1; x <= g.lx; x++) {
+            const int xu = x + DIR[0][mu], yu = y + DIR[1][mu], tu = t + DIR[2][mu];
+            double v = cimag(cmult * conj(sL[I3(x, y, t, a)]) * sR[I3(xu, yu, tu, a)]) -
+                       cimag(cmult * conj(sL[I3(xu, yu, tu, a)]) * sR[I3(x, y, t, a)]);
+            dsdpi[IF4(g, x, y, t, mu)] += c1 * v;
+          }
+      const int b = gamin(mu, a);
+      const zdp gv = gamval(mu, a);
+      for (int t = 1; t <= g.lt; t++)
+        for (int y = 1; y <= g.ly; y++)
+          for (int x = 1; x <= g.lx; x++) {
+            const int xu = x + DIR[0][mu], yu = y + DIR[1][mu], tu = t + DIR[2][mu];
+            double v = cimag(cmult * gv *
+                             (conj(sL[I3(x, y, t, a)]) * sR[I3(xu, yu, tu, b)] +
+                              conj(sL[I3(xu, yu, tu, a)]) * sR[I3(x, y, t, b)]));
+            dsdpi[IF4(g, x, y, t, mu)] += c2 * v;
+          }
+    }
+#undef I3
+}
+
+/* spin-masked slice pair for derivs{DP,DM,PD,MD}Test (derivs_module.F90:252-345) */
+static void deriv_test(const zdp *R, const zdp *X2, double anum, int iflag, int idxL, int idxR,
+                       int maskL0, int maskL1, int maskR0, int maskR1, zdp cmult, const geom *gp,
+                       double *dsdpi, zdp *sL, zdp *sR) {
+  get_slice(sL, R, gp, idxL, maskL0, maskL1);
+  get_slice(sR, X2, gp, idxR, maskR0, maskR1);
+  deriv_slice(sL, sR, anum, iflag, cmult, gp, dsdpi);
+}
+
+void orc_derivs_wilson(const double *r_, const double *x2_, double anum, int iflag,
+                       const int dims[4], float am, int imass, float am3, double *dsdpi) {
+  (void)am3;
+  geom g = mkgeom(dims);
+  const zdp *R = (const zdp *)r_, *X2 = (const zdp *)x2_;
+  orc_derivs_shamir(r_, x2_, anum, iflag, dims, dsdpi);
+  const long n4s = g.X * g.Y * g.T * 4;
+  zdp *sL = malloc(sizeof(zdp) * n4s), *sR = malloc(sizeof(zdp) * n4s);
+  /* derivsWilsonMass (derivs_module.F90:198-249) */
+  zdp cm = (imass == 1) ? -(double)am : -I * (double)am; /* cmplx(-am,0) / cmplx(0,-am) */
+  if (iflag == 1) /* DPTest: L all spins, R spins 1:2 */
+    deriv_test(R, X2, anum, iflag, 1, g.ls, 0, 4, 0, 2, cm, &g, dsdpi, sL, sR);
+  else if (iflag == 0) /* MDTest: L spins 3:4, R all */
+    deriv_test(R, X2, anum, iflag, 1, g.ls, 2, 4, 0, 4, cm, &g, dsdpi, sL, sR);
+  cm = (imass == 1) ? -(double)am : I * (double)am;
+  if (iflag == 1) /* DMTest: L all, R spins 3:4 */
+    deriv_test(R, X2, anum, iflag, g.ls, 1, 0, 4, 2, 4, cm, &g, dsdpi, sL, sR);
+  else if (iflag == 0) /* PDTest: L spins 1:2, R all */
+    deriv_test(R, X2, anum, iflag, g.ls, 1, 0, 2, 0, 4, cm, &g, dsdpi, sL, sR);
+  /* Ls-direction couplings (derivs_module.F90:71-81) */
+  const zdp one = 1.0;
+  for (int il = 1; il <= g.ls - 1; il++) {
+    if (iflag == 1) {
+      deriv_test(R, X2, anum, iflag, il, il + 1, 0, 4, 2, 4, one, &g, dsdpi, sL, sR); /* DM */
+      deriv_test(R, X2, anum, iflag, il + 1, il, 0, 4, 0, 2, one, &g, dsdpi, sL, sR); /* DP */
+    } else if (iflag == 0) {
+      deriv_test(R, X2, anum, iflag, il, il + 1, 0, 2, 0, 4, one, &g, dsdpi, sL, sR); /* PD */
+      deriv_test(R, X2, anum, iflag, il + 1, il, 2, 4, 0, 4, one, &g, dsdpi, sL, sR); /* MD */
+    }
+  }
+  free(sL);
+  free(sR);
+}
+
+/* ------------------------------------------------------------------------ */
+/* qmrherm (qmrherm_module.F90:20-194)                                       */
+/* ------------------------------------------------------------------------ */
+void orc_qmrherm(const double *phi_, double *x_, float res, int *itercg, float am, int imass,
+                 float am3, const double *anum, const double *aden, int ndiagq, int iflag,
+                 int use_sp, const double *u_, const int dims[4], int max_qmr_iters, int kernel,
+                 double *x1_, double *phi0_, double *dsdpi, int *cg_returns) {
+  geom g = mkgeom(dims);
+  const zdp *Phi = (const zdp *)phi_, *u = (const zdp *)u_;
+  zdp *x = (zdp *)x_, *x1 = (zdp *)x1_, *Phi0 = (zdp *)phi0_;
+  const long nh = (long)g.ls * g.lx * g.ly * g.lt * 4;
+  for (long k = 0; k < g.n5; k++) x[k] = anum[0] * Phi[k];
+  int sp = use_sp && kernel == 0; /* SP is not available for the Wilson kernel (54-63) */
+  if (sp)
+    orc_multishift_sp(u_, dims, am, imass, am3, ndiagq, aden, anum + 1, x1_, phi_, res,
+                      max_qmr_iters, itercg, cg_returns);
+  else
+    orc_multishift(u_, dims, am, imass, am3, ndiagq, aden, anum + 1, x1_, phi_, res,
+                   max_qmr_iters, kernel, itercg, cg_returns);
+  if (iflag < 2) {
+    for (int d = 0; d < ndiagq; d++) {
+      const zdp *xd = x1 + (long)d * nh;
+      for (int a = 0; a < 4; a++)
+        for (int t = 1; t <= g.lt; t++)
+          for (int y = 1; y <= g.ly; y++)
+            for (int xx = 1; xx <= g.lx; xx++)
+              for (int s = 1; s <= g.ls; s++) {
+                long o = I5(g, s, xx, y, t, a);
+                x[o] = x[o] + anum[d + 1] * xd[IH(g, s, xx, y, t, a)];
+              }
+    }
+    halo5_z(x, &g, 4);
+    if (iflag == 1) {
+      for (int d = 0; d < ndiagq; d++) {
+        zdp *p0 = Phi0 + (long)d * g.n5;
+        const zdp *xd = x1 + (long)d * nh;
+        for (int a = 0; a < 4; a++)
+          for (int t = 1; t <= g.lt; t++)
+            for (int y = 1; y <= g.ly; y++)
+              for (int xx = 1; xx <= g.lx; xx++)
+                for (int s = 1; s <= g.ls; s++) p0[I5(g, s, xx, y, t, a)] = xd[IH(g, s, xx, y, t, a)];
+        halo5_z(p0, &g, 4);
+      }
+    }
+  } else {
+    zdp *R = calloc(g.n5, sizeof(zdp)), *x2 = calloc(g.n5, sizeof(zdp));
+    for (int d = 0; d < ndiagq; d++) {
+      const zdp *xd = x1 + (long)d * nh;
+      for (int a = 0; a < 4; a++)
+        for (int t = 1; t <= g.lt; t++)
+          for (int y = 1; y <= g.ly; y++)
+            for (int xx = 1; xx <= g.lx; xx++)
+              for (int s = 1; s <= g.ls; s++) R[I5(g, s, xx, y, t, a)] = xd[IH(g, s, xx, y, t, a)];
+      halo5_z(R, &g, 4);
+      op_M(x2, R, u, &g, am, imass, am3, kernel);
+      halo5_z(x2, &g, 4);
+      if (iflag == 2) {
+        if (kernel == 1)
+          orc_derivs_wilson((double *)R, (double *)x2, anum[d + 1], 0, dims, am, imass, am3, dsdpi);
+        else
+          orc_derivs_shamir((double *)R, (double *)x2, anum[d + 1], 0, dims, dsdpi);
+      } else {
+        const double coeff = -anum[d + 1];
+        memcpy(R, Phi0 + (long)d * g.n5, sizeof(zdp) * g.n5);
+        if (kernel == 1)
+          orc_derivs_wilson((double *)R, (double *)x2, coeff, 0, dims, am, imass, am3, dsdpi);
+        else
+          orc_derivs_shamir((double *)R, (double *)x2, coeff, 0, dims, dsdpi);
+        op_M(x2, R, u, &g, am, imass, am3, kernel);
+        halo5_z(x2, &g, 4);
+        for (int a = 0; a < 4; a++)
+          for (int t = 1; t <= g.lt; t++)
+            for (int y = 1; y <= g.ly; y++)
+              for (int xx = 1; xx <= g.lx; xx++)
+                for (int s = 1; s <= g.ls; s++) R[I5(g, s, xx, y, t, a)] = xd[IH(g, s, xx, y, t, a)];
+        halo5_z(R, &g, 4);
+        if (kernel == 1)
+          orc_derivs_wilson((double *)x2, (double *)R, coeff, 1, dims, am, imass, am3, dsdpi);
+        else
+          orc_derivs_shamir((double *)x2, (double *)R, coeff, 1, dims, dsdpi);
+      }
+    }
+    free(R);
+    free(x2);
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* force (force_module.F90:9-113), rational (non-HMC) branch, Nf = 1          */
+/* ------------------------------------------------------------------------ */
+void orc_force(const double *phi, float res1, float am, int imass, float am3, float beta,
+               int ibound, const float *theta, const double *u, const int dims[4],
+               const double *anum4g, const double *aden4g, const double *bnum2g,
+               const double *bden2g, const double *anum2g, const double *aden2g, int ndiagg,
+               int use_sp, int max_qmr_iters, double *dsdpi) {
+  geom g = mkgeom(dims);
+  const long nh = (long)g.ls * g.lx * g.ly * g.lt * 4;
+  const long n4 = (long)g.lx * g.ly * g.lt;
+  double *X2 = malloc(sizeof(zdp) * g.n5), *Xr = malloc(sizeof(zdp) * g.n5);
+  double *x1 = calloc(nh * ndiagg * 2, sizeof(double));
+  double *phi0 = calloc(g.n5 * 25 * 2, sizeof(double));
+  int it;
+  for (long k = 0; k < n4 * 3; k++) dsdpi[k] = 0.0;
+  const float One = 1.0f;
+  memcpy(X2, phi, sizeof(zdp) * g.n5);
+  orc_qmrherm(X2, Xr, res1, &it, One, 1, am3, anum4g, aden4g, ndiagg, 1, use_sp, u, dims,
+              max_qmr_iters, 0, x1, phi0, dsdpi, NULL);
+  memcpy(X2, Xr, sizeof(zdp) * g.n5);
+  orc_qmrherm(X2, Xr, res1, &it, am, imass, am3, bnum2g, bden2g, ndiagg, 0, use_sp, u, dims,
+              max_qmr_iters, 0, x1, phi0, dsdpi, NULL);
+  memcpy(X2, Xr, sizeof(zdp) * g.n5);
+  orc_qmrherm(X2, Xr, res1, &it, am, imass, am3, anum2g, aden2g, ndiagg, 2, use_sp, u, dims,
+              max_qmr_iters, 0, x1, phi0, dsdpi, NULL);
+  orc_qmrherm(X2, Xr, res1, &it, One, 1, am3, anum4g, aden4g, ndiagg, 3, use_sp, u, dims,
+              max_qmr_iters, 0, x1, phi0, dsdpi, NULL);
+  if (ibound == -1)
+    for (int y = 1; y <= g.ly; y++)
+      for (int x = 1; x <= g.lx; x++) dsdpi[IF4(g, x, y, g.lt, 2)] = -dsdpi[IF4(g, x, y, g.lt, 2)];
+  /* dSdpi = dSdpi + beta*Nf*theta: real(sp) beta*Nf*theta, then added in dp */
+  for (long k = 0; k < n4 * 3; k++) dsdpi[k] = dsdpi[k] + (double)(beta * 1.0f * theta[k]);
+  free(X2);
+  free(Xr);
+  free(x1);
+  free(phi0);
+}
+
+/* ------------------------------------------------------------------------ */
+/* hamilton (dwf3d_lib.F90:511-616), rational branch, Nf = 1                  */
+/* ------------------------------------------------------------------------ */
+void orc_hamilton(const double *phi, float res2, float am, int imass, float am3, float beta,
+                  const float *theta, const float *pp, const double *u, const int dims[4],
+                  const double *anum4, const double *aden4, const double *bnum2,
+                  const double *bden2, int ndiag, int max_qmr_iters, double out[5],
+                  int itercg_out[2]) {
+  geom g = mkgeom(dims);
+  const long nh = (long)g.ls * g.lx * g.ly * g.lt * 4;
+  const long n4 = (long)g.lx * g.ly * g.lt;
+  /* hp = 0.5*sum(pp**2): the sum of a real(sp) array is real(sp) */
+  float spp = 0.0f, sth = 0.0f;
+  for (long k = 0; k < n4 * 3; k++) spp += pp[k] * pp[k];
+  for (long k = 0; k < n4 * 3; k++) sth += theta[k] * theta[k];
+  double hp = (double)(0.5f * spp);
+  double hg = (double)(0.5f * 1.0f * beta * sth);
+  double *R = malloc(sizeof(zdp) * g.n5), *Xr = malloc(sizeof(zdp) * g.n5);
+  double *x1 = calloc(nh * ndiag * 2, sizeof(double));
+  double *phi0 = calloc(g.n5 * 25 * 2, sizeof(double));
+  double dummy_force[1];
+  (void)dummy_force;
+  int it1, it2;
+  memcpy(R, phi, sizeof(zdp) * g.n5);
+  orc_qmrherm(R, Xr, res2, &it1, 1.0f, 1, am3, anum4, aden4, ndiag, 0, 0, u, dims, max_qmr_iters,
+              0, x1, phi0, NULL, NULL);
+  memcpy(R, Xr, sizeof(zdp) * g.n5);
+  orc_qmrherm(R, Xr, res2, &it2, am, imass, am3, bnum2, bden2, ndiag, 0, 0, u, dims,
+              max_qmr_iters, 0, x1, phi0, NULL, NULL);
+  double hf = 0.0 + redot_int((const zdp *)R, (const zdp *)Xr, &g);
+  out[0] = hg + hp + hf;
+  out[1] = hg;
+  out[2] = hp;
+  out[3] = hg + hf;
+  out[4] = hf;
+  itercg_out[0] = it1;
+  itercg_out[1] = it2;
+  free(R);
+  free(Xr);
+  free(x1);
+  free(phi0);
+}

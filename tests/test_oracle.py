@@ -1,0 +1,404 @@
+"""Pins the CPU oracle (oracle/oracle.c) without the reference binary.
+
+The reference Fortran cannot be built here under the project's rules (its
+MPI build pulls in sed-generated *_sp modules and a hand-made `mpi` module),
+so every oracle routine is pinned by what the mathematics fixes, chosen so a
+plausible slip (sign, index, conjugation, projector, coefficient) fails:
+
+  * reference known answer: congrad itercg == 27 (tests/test_congrad.F90:47)
+  * Clifford algebra of the gamma tables (gammamatrices.F90:27-125)
+  * dense brute-force assembly of M from those tables vs. the oracle
+  * adjointness <a, M b> = <M^+ a, b> (Shamir and Wilson, imass 1/3/5)
+  * U(1) gauge covariance (link direction and conjugation)
+  * Wilson-kernel DWF == Shamir DWF dressed with (1 - D_W) on every s-coupling
+  * derivs == exact finite difference of Re<R, M^(+) X2> (M is affine in u)
+  * multishift iterate k == plain CG iterate k of each shifted system; dense solves
+  * qmrherm: rational function via dense solves; iflag 2/3 forces vs. finite
+    differences of the pseudofermion action
+  * remez4 block 2 approximates x^(-1/4) on [1e-4, 64] to < 5e-10
+"""
+import numpy as np
+import pytest
+
+import dense as D
+import oracle as O
+import refstate
+
+
+def ip(a, b):
+    return np.vdot(D.interior_vec(a), D.interior_vec(b))
+
+
+TINY = (2, 2, 2, 4)
+SMALL = (3, 2, 4, 4)
+
+
+def test_congrad_reference_known_answer():
+    """tests/test_congrad.F90:30-47: 12^3 x 24, res 0.1, am 0.05, imass 3 -> 27."""
+    dims = (12, 12, 12, 24)
+    st = refstate.starting_state(dims)
+    it, nx = O.congrad(st["phi"], st["x"], st["u"], dims, res=0.1, am=0.05, imass=3)
+    assert it == 27 and nx == 27
+    # the stopping rule is absolute: |r|^2 < 4*V5*res^2 (measure_module.F90:41)
+    r = st["phi"].copy(order="F")
+    mx = O.dirac_op_shifted(st["x"], st["u"], dims, 0.05, 3)
+    resid2 = np.sum(np.abs(D.interior_vec(r) - D.interior_vec(mx)) ** 2)
+    assert resid2 < 4 * 12 * 12 * 12 * 24 * np.float32(0.1) ** 2
+
+
+def test_gamma_clifford():
+    G = D.gamma_tables()
+    for m in range(4):
+        assert np.allclose(G[m], G[m].conj().T)
+        for n in range(4):
+            assert np.allclose(G[m] @ G[n] + G[n] @ G[m], 2 * np.eye(4) * (m == n))
+    assert np.allclose(G[4], G[0] @ G[1] @ G[2] @ G[3])
+    assert np.allclose(G[5], G[3] @ G[4])
+
+
+def _shift_matrix(dims, mu, u):
+    """T+ (psi)(x) = U(x,mu) psi(x+mu) on the 4-D interior, periodic."""
+    lx, ly, lt, ls = dims
+    n4 = lx * ly * lt
+    T = np.zeros((n4, n4), dtype=np.complex128)
+    step = [(1, 0, 0), (0, 1, 0), (0, 0, 1)][mu]
+    for t in range(lt):
+        for y in range(ly):
+            for x in range(lx):
+                i = x + lx * (y + ly * t)
+                xn, yn, tn = (x + step[0]) % lx, (y + step[1]) % ly, (t + step[2]) % lt
+                j = xn + lx * (yn + ly * tn)
+                T[i, j] += u[x + 1, y + 1, t + 1, mu]
+    return T
+
+
+def _dense_shamir(dims, u, am, imass, am3=1.0, dag=False):
+    """Independent dense assembly (Kronecker form, ordering s fastest, then x,y,t, spin)."""
+    lx, ly, lt, ls = dims
+    G = D.gamma_tables()
+    kap = 0.5
+    n4 = lx * ly * lt
+    I4s, Is, Ix = np.eye(4), np.eye(ls), np.eye(n4)
+    diag = float(np.float32(np.float32(3.0) - np.float32(am3)) + np.float32(1.0))
+    M = diag * np.kron(I4s, np.kron(Ix, Is)).astype(np.complex128)
+    sgn = -1.0 if dag else 1.0
+    for mu in range(3):
+        Tp = _shift_matrix(dims, mu, u)
+        Tm = Tp.conj().T
+        M += np.kron(I4s, np.kron(-kap * (Tp + Tm), Is))
+        M += sgn * np.kron(kap * G[mu], np.kron(Tp - Tm, Is))
+    Pp = np.diag([1.0, 1.0, 0, 0])
+    Pm = np.diag([0, 0, 1.0, 1.0])
+    up = np.eye(ls, k=1)   # (S+ v)(s) = v(s+1)
+    dn = np.eye(ls, k=-1)  # (S- v)(s) = v(s-1)
+    e1Ls = np.zeros((ls, ls)); e1Ls[0, ls - 1] = 1.0   # s=1 <- s=Ls
+    eLs1 = np.zeros((ls, ls)); eLs1[ls - 1, 0] = 1.0   # s=Ls <- s=1
+    e11 = np.zeros((ls, ls)); e11[0, 0] = 1.0
+    eLL = np.zeros((ls, ls)); eLL[ls - 1, ls - 1] = 1.0
+    m = float(np.float32(am))
+    P_hi_in, P_lo_in = (Pp, Pm) if not dag else (Pm, Pp)
+    M -= np.kron(P_lo_in, np.kron(Ix, up)) + np.kron(P_hi_in, np.kron(Ix, dn))
+    swap = np.zeros((4, 4)); swap[0, 2] = swap[1, 3] = 1.0   # spins 1:2 <- 3:4
+    if imass == 1:
+        M += m * (np.kron(P_lo_in, np.kron(Ix, eLs1)) + np.kron(P_hi_in, np.kron(Ix, e1Ls)))
+    elif imass == 3:
+        c = 1j * m if not dag else 1j * m
+        if not dag:
+            M += c * np.kron(Pm, np.kron(Ix, eLs1)) - c * np.kron(Pp, np.kron(Ix, e1Ls))
+        else:
+            M += c * np.kron(Pp, np.kron(Ix, eLs1)) - c * np.kron(Pm, np.kron(Ix, e1Ls))
+    elif imass == 5:
+        c = 1j * m
+        if not dag:
+            M += c * (np.kron(swap.T, np.kron(Ix, eLL)) + np.kron(swap, np.kron(Ix, e11)))
+        else:
+            M -= c * (np.kron(swap, np.kron(Ix, eLL)) + np.kron(swap.T, np.kron(Ix, e11)))
+    return M
+
+
+@pytest.mark.parametrize("imass", [1, 3, 5])
+def test_dense_assembly(imass):
+    rng = np.random.default_rng(10 + imass)
+    dims = TINY
+    u = D.rand_links(dims, rng)
+    for dag, f in ((False, O.dslash), (True, O.dslashd)):
+        Mo = D.dense(lambda v: f(v, u, dims, 0.3, imass), dims)
+        Md = _dense_shamir(dims, u, 0.3, imass, dag=dag)
+        assert np.abs(Mo - Md).max() < 1e-14
+    # M^dagger is the conjugate transpose of M
+    Mo = D.dense(lambda v: O.dslash(v, u, dims, 0.3, imass), dims)
+    Md = D.dense(lambda v: O.dslashd(v, u, dims, 0.3, imass), dims)
+    assert np.abs(Md - Mo.conj().T).max() < 1e-14
+
+
+@pytest.mark.parametrize("kernel,imass", [("shamir", 1), ("shamir", 3), ("shamir", 5),
+                                          ("wilson", 1), ("wilson", 3)])
+def test_adjoint(kernel, imass):
+    rng = np.random.default_rng(imass)
+    dims = SMALL
+    u = D.rand_links(dims, rng)
+    a, b = D.rand_spinor(dims, rng), D.rand_spinor(dims, rng)
+    lhs = ip(a, O.dslash(b, u, dims, 0.3, imass, kernel=kernel))
+    rhs = ip(O.dslashd(a, u, dims, 0.3, imass, kernel=kernel), b)
+    assert abs(lhs - rhs) < 1e-13 * abs(lhs)
+
+
+@pytest.mark.parametrize("kernel", ["shamir", "wilson"])
+def test_gauge_covariance(kernel):
+    rng = np.random.default_rng(5)
+    dims = SMALL
+    lx, ly, lt, ls = dims
+    u = D.rand_links(dims, rng)
+    b = D.rand_spinor(dims, rng)
+    g = np.exp(1j * rng.uniform(0, 2 * np.pi, (lx, ly, lt)))
+    up = u.copy(order="F")
+    steps = [(1, 0, 0), (0, 1, 0), (0, 0, 1)]
+    for mu in range(3):
+        gn = np.roll(g, tuple(-s for s in steps[mu]), axis=(0, 1, 2))
+        up[1:-1, 1:-1, 1:-1, mu] = g * u[1:-1, 1:-1, 1:-1, mu] * np.conj(gn)
+    refstate.halo4(up)
+    gb = b.copy(order="F")
+    gb[1:-1, 1:-1, 1:-1, 1:-1, :] *= g[None, :, :, :, None]
+    refstate.halo5(gb)
+    for f in (O.dslash, O.dslashd):
+        lhs = O.interior(f(gb, up, dims, 0.3, 3, kernel=kernel))
+        rhs = O.interior(f(b, u, dims, 0.3, 3, kernel=kernel)) * g[None, :, :, :, None]
+        assert np.abs(lhs - rhs).max() < 1e-13
+
+
+def test_wilson_kernel_identity():
+    """dirac.F90:226-299 == Shamir with every s-coupling N replaced by (1 - D_W) N.
+
+    D_W is taken from the Shamir operator on one s-slice (Ls = 1, am = 0).  The
+    reference's imass=3 wall term carries the opposite sign in the Wilson
+    kernel, so it matches Shamir at -am."""
+    rng = np.random.default_rng(7)
+    dims = SMALL
+    lx, ly, lt, ls = dims
+    u = D.rand_links(dims, rng)
+    b = D.rand_spinor(dims, rng)
+
+    def DW(v):
+        out = np.zeros_like(v)
+        d1 = (lx, ly, lt, 1)
+        for s in range(1, ls + 1):
+            sl = np.zeros((3, lx + 2, ly + 2, lt + 2, 4), complex, order="F")
+            sl[1] = v[s]
+            refstate.halo5(sl)
+            out[s] = O.dslash(sl, u, d1, 0.0, 1)[1] - v[s]
+        return refstate.halo5(out)
+
+    for imass, sgn in ((1, 1.0), (3, -1.0)):
+        dwb = DW(b)
+        N = refstate.halo5(O.dslash(b, u, dims, sgn * 0.3, imass) - dwb - b)
+        rhs = dwb + b + N - DW(N)
+        lhs = O.dslash(b, u, dims, 0.3, imass, kernel="wilson")
+        assert np.abs(O.interior(lhs) - O.interior(rhs)).max() < 1e-13
+
+
+@pytest.mark.parametrize("kernel", ["shamir", "wilson"])
+@pytest.mark.parametrize("iflag", [0, 1])
+def test_derivs_exact_finite_difference(kernel, iflag):
+    """derivs(R,X2,a,0) = fl32(2a) d/du_imag Re<R, M^+ X2>; iflag 1 uses M."""
+    rng = np.random.default_rng(20 + iflag)
+    dims = (2, 3, 2, 4)
+    lx, ly, lt, ls = dims
+    u = D.rand_links(dims, rng)
+    R, X2 = D.rand_spinor(dims, rng), D.rand_spinor(dims, rng)
+    anum = 0.37
+    ds = np.full((lx, ly, lt, 3), 0.25, order="F")  # accumulates on top
+    O.derivs(R, X2, anum, iflag, dims, ds, kernel=kernel, am=0.3, imass=3)
+    c = float(np.float32(2 * np.float32(anum)))
+    f = O.dslashd if iflag == 0 else O.dslash
+    base = ip(R, f(X2, u, dims, 0.3, 3, kernel=kernel)).real
+    fd = np.zeros_like(ds)
+    for mu in range(3):
+        for x in range(lx):
+            for y in range(ly):
+                for t in range(lt):
+                    up = u.copy(order="F")
+                    up[x + 1, y + 1, t + 1, mu] += 1j
+                    refstate.halo4(up)
+                    fd[x, y, t, mu] = c * (ip(R, f(X2, up, dims, 0.3, 3, kernel=kernel)).real - base)
+    assert np.abs(ds - 0.25 - fd).max() < 1e-12 * np.abs(fd).max()
+
+
+def test_coef_closed_form():
+    rng = np.random.default_rng(3)
+    dims = (3, 4, 5, 4)
+    lx, ly, lt, ls = dims
+    th = rng.standard_normal((lx, ly, lt, 3)).astype(np.float32)
+    u = O.coef(th, dims, ibound=-1)
+    ref = (1.0 + 1j * th.astype(np.float64))
+    ref[:, :, lt - 1, 2] *= -1
+    assert np.array_equal(u[1:-1, 1:-1, 1:-1, :], ref)
+    assert np.array_equal(u[0, 1:-1, 1:-1, :], ref[lx - 1])
+    assert np.array_equal(u[1:-1, 1:-1, 0, :], ref[:, :, lt - 1])
+    u2 = O.coef(th, dims, ibound=1)
+    assert np.array_equal(u2[1:-1, 1:-1, lt, 2], 1.0 + 1j * th[:, :, lt - 1, 2].astype(np.float64))
+
+
+def _dense_MdM(dims, u, am, imass, kernel="shamir"):
+    M = D.dense(lambda v: O.dslash(v, u, dims, am, imass, kernel=kernel), dims)
+    return M.conj().T @ M
+
+
+@pytest.mark.parametrize("kernel", ["shamir", "wilson"])
+def test_multishift_iterates_equal_shifted_cg(kernel):
+    """After k iterations x_i equals the k-th CG iterate of (A + sigma_i) x = b."""
+    rng = np.random.default_rng(30)
+    dims = TINY
+    u = D.rand_links(dims, rng, kind="theta")
+    b = D.rand_spinor(dims, rng)
+    A = _dense_MdM(dims, u, 0.2, 3, kernel)
+    bv = D.interior_vec(b)
+    sig = np.array([0.05, 0.3, 1.7, 6.0])
+    anum = np.array([0.1, 0.2, 0.3, 0.4])
+    for k in (1, 3, 6):
+        out, cgr, _ = O.multishift(u, dims, 0.2, 3, sig, anum, b, res=1e-30, maxcg=k, kernel=kernel)
+        assert cgr == k
+        for i, s in enumerate(sig):
+            As = A + s * np.eye(A.shape[0])
+            x = np.zeros_like(bv)
+            r = bv.copy()
+            p = r.copy()
+            for _ in range(k):
+                Ap = As @ p
+                al = np.vdot(r, r).real / np.vdot(p, Ap).real
+                x = x + al * p
+                rn = r - al * Ap
+                p = rn + (np.vdot(rn, rn).real / np.vdot(r, r).real) * p
+                r = rn
+            xi = out[..., i].reshape(-1, order="F")
+            assert np.linalg.norm(xi - x) < 1e-10 * np.linalg.norm(x)
+
+
+def test_multishift_converged_and_sp():
+    rng = np.random.default_rng(31)
+    dims = TINY
+    u = D.rand_links(dims, rng, kind="theta")
+    b = D.rand_spinor(dims, rng)
+    A = _dense_MdM(dims, u, 0.2, 3)
+    bv = D.interior_vec(b)
+    sig = np.array([0.01, 0.3, 2.0])
+    anum = np.array([0.5, 0.5, 0.5])
+    out, cgr, cgrs = O.multishift(u, dims, 0.2, 3, sig, anum, b, res=1e-9, maxcg=5000)
+    for i, s in enumerate(sig):
+        x = np.linalg.solve(A + s * np.eye(A.shape[0]), bv)
+        xi = out[..., i].reshape(-1, order="F")
+        assert np.linalg.norm(xi - x) < 1e-7 * np.linalg.norm(x)
+    # a large shift converges first and stops being updated
+    assert cgrs[2] < cgrs[0] == cgr
+    outs, cgrsp, _ = O.multishift(u, dims, 0.2, 3, sig, anum, b, res=1e-5, maxcg=5000, sp=True)
+    for i in range(3):
+        x = np.linalg.solve(A + sig[i] * np.eye(A.shape[0]), bv)
+        xi = outs[..., i].reshape(-1, order="F")
+        assert np.linalg.norm(xi - x) < 1e-4 * np.linalg.norm(x)
+
+
+def _remez(path, block):
+    with open(path) as fh:
+        rows = [l.replace(",", " ").split() for l in fh if l.strip()]
+    n = (len(rows) - 2) // 2
+    blk = rows[block * (n + 1):(block + 1) * (n + 1)]
+    a0 = float(blk[0][0])
+    a = np.array([float(r[0]) for r in blk[1:]])
+    s = np.array([float(r[1]) for r in blk[1:]])
+    return a0, a, s
+
+
+def test_remez4_block2_accuracy():
+    import os
+    path = os.path.join(os.path.dirname(__file__), "..", "thirring-rhmc_amd", "data", "remez4")
+    a0, a, s = _remez(path, 1)
+    assert len(a) == 25 and np.all(a > 0) and np.all(s > 0)
+    x = np.logspace(-4, np.log10(64.0), 4001)
+    r = a0 + np.sum(a[None, :] / (x[:, None] + s[None, :]), axis=1)
+    assert np.max(np.abs(r * x ** 0.25 - 1.0)) < 5e-10
+
+
+@pytest.mark.parametrize("iflag", [0, 1])
+def test_qmrherm_rational_function(iflag):
+    rng = np.random.default_rng(40)
+    dims = TINY
+    u = D.rand_links(dims, rng, kind="theta")
+    phi = D.rand_spinor(dims, rng)
+    A = _dense_MdM(dims, u, 0.2, 3)
+    sig = np.array([0.02, 0.4, 3.0])
+    anum = np.array([0.7, 0.25, 0.5, 1.5])
+    x, it, x1, phi0, _, _ = O.qmrherm(phi, 1e-10, 0.2, 3, anum, sig, iflag, u, dims)
+    pv = D.interior_vec(phi)
+    ref = anum[0] * pv
+    for i, s in enumerate(sig):
+        xi = np.linalg.solve(A + s * np.eye(A.shape[0]), pv)
+        ref = ref + anum[i + 1] * xi
+        if iflag == 1:
+            assert np.linalg.norm(D.interior_vec(phi0[..., i]) - xi) < 1e-8 * np.linalg.norm(xi)
+    assert np.linalg.norm(D.interior_vec(x) - ref) < 1e-8 * np.linalg.norm(ref)
+    # x comes back with periodic halos
+    assert np.array_equal(x[0], x[-2]) and np.array_equal(x[:, :, :, 0], x[:, :, :, -2])
+
+
+def _theta_links(th, dims):
+    return O.coef(th, dims, ibound=1)
+
+
+def test_qmrherm_forces_vs_action():
+    """iflag 2 == -dS/dtheta and iflag 3 == 2 dF/dtheta by central differences.
+
+    S(theta) = Re sum_i a_i <X, (A+s_i)^-1 X>, F = Re sum_i a_i <Phi, (A(theta)+s_i)^-1 X>,
+    A = M^+ M with u = 1 + i theta (periodic, so du/dtheta = i on every link)."""
+    rng = np.random.default_rng(41)
+    dims = TINY
+    lx, ly, lt, ls = dims
+    th = (0.4 * rng.standard_normal((lx, ly, lt, 3))).astype(np.float32)
+    u = _theta_links(th, dims)
+    X = D.rand_spinor(dims, rng)
+    Phi = D.rand_spinor(dims, rng)
+    sig = np.array([0.05, 0.6])
+    anum = np.array([0.0, 0.375, 0.5])  # exact in float32 after doubling
+    Xv, Pv = D.interior_vec(X), D.interior_vec(Phi)
+
+    def S_F(thd):
+        uu = O.coef(thd.astype(np.float32), dims, ibound=1) if thd.dtype == np.float32 else None
+        return uu
+
+    def dense_A(thd):
+        uu = np.zeros((lx + 2, ly + 2, lt + 2, 3), complex, order="F")
+        uu[1:-1, 1:-1, 1:-1, :] = 1.0 + 1j * thd
+        refstate.halo4(uu)
+        return _dense_MdM(dims, uu, 0.2, 3)
+
+    # iflag 1 to fill Phi0 from Phi (as force_module.F90:80 does), then iflag 2 and 3 on X
+    _, _, x1, phi0, _, _ = O.qmrherm(Phi, 1e-11, 0.2, 3, anum, sig, 1, u, dims)
+    _, _, _, _, ds2, _ = O.qmrherm(X, 1e-11, 0.2, 3, anum, sig, 2, u, dims)
+    _, _, _, _, ds3, _ = O.qmrherm(X, 1e-11, 0.2, 3, anum, sig, 3, u, dims, x1=x1, phi0=phi0)
+    h = 1e-5
+    thd = th.astype(np.float64)
+    for (x, y, t, mu) in [(0, 0, 0, 0), (1, 0, 1, 1), (0, 1, 1, 2), (1, 1, 0, 2)]:
+        vals = []
+        for sgn in (1, -1):
+            tp = thd.copy()
+            tp[x, y, t, mu] += sgn * h
+            A = dense_A(tp)
+            S = F = 0.0
+            for i, s in enumerate(sig):
+                As = A + s * np.eye(A.shape[0])
+                S += anum[i + 1] * np.vdot(Xv, np.linalg.solve(As, Xv)).real
+                F += anum[i + 1] * np.vdot(Pv, np.linalg.solve(As, Xv)).real
+            vals.append((S, F))
+        dS = (vals[0][0] - vals[1][0]) / (2 * h)
+        dF = (vals[0][1] - vals[1][1]) / (2 * h)
+        assert abs(ds2[x, y, t, mu] + dS) < 1e-6 * max(1.0, abs(dS))
+        assert abs(ds3[x, y, t, mu] - 2 * dF) < 1e-6 * max(1.0, abs(dF))
+
+
+def test_congrad_wilson_true_residual():
+    rng = np.random.default_rng(50)
+    dims = (4, 4, 4, 4)
+    u = D.rand_links(dims, rng, kind="theta")
+    phi = D.rand_spinor(dims, rng)
+    x = np.zeros_like(phi)
+    it, _ = O.congrad(phi, x, u, dims, res=1e-6, am=0.1, imass=1, kernel="wilson")
+    r = D.interior_vec(phi) - D.interior_vec(O.dirac_op_shifted(x, u, dims, 0.1, 1, kernel="wilson"))
+    assert np.sum(np.abs(r) ** 2) < 4 * 4 ** 4 * np.float32(1e-6) ** 2
