@@ -1,0 +1,134 @@
+/*
+ * thirring_hip.h -- C ABI of the MI355X (gfx950) domain-wall hot path.
+ *
+ * Drop-in boundary for j-ostmeyer/thirring-rhmc's Fortran subroutine API.
+ * Every entry point returns 0 (THR_OK) or a THR_ERR_* code; the message of
+ * the last failure is available from thr_last_error().  Plain pointers and
+ * sizes only.  `dims` is always {Lx, Ly, Lt, Ls} (ksizex_l, ksizey_l,
+ * ksizet_l, kthird_l); `kernel` selects the generation kernel: 0 Shamir
+ * (GENERATE_WITH_SHAMIR), 1 Wilson kernel (GENERATE_WITH_WILSON).
+ *
+ * Two families of entry points:
+ *
+ *  (1) host-array entry points taking the reference's own haloed
+ *      column-major complex(dp) arrays (interleaved re/im doubles):
+ *        spinor  Phi(0:Ls+1, 0:Lx+1, 0:Ly+1, 0:Lt+1, 4)
+ *        links   u(0:Lx+1, 0:Ly+1, 0:Lt+1, 3)
+ *        shifts  x1(Ls, Lx, Ly, Lt, 4, n)        (halo-free)
+ *        force   dSdpi(Lx, Ly, Lt, 3)            real(dp)
+ *      Output spinors come back with periodic halos filled (what the
+ *      reference's callers do with update_halo_5 after every call).
+ *      These are what the Fortran shim modules (dirac, measure_module,
+ *      multishift_module, qmrherm_module, derivs_module, gauge) bind.
+ *
+ *  (2) device-resident entry points (suffix _dev) on halo-free device
+ *      buffers in the layout [rhs][spin][t][y][x][s] (s fastest), links
+ *      [mu][t][y][x]; `stream` is a hipStream_t (NULL: the library's own).
+ *
+ * Scalars the reference declares as default real (am, am3, res, theta) are
+ * float here; coefficient arrays are double.
+ */
+#ifndef THIRRING_HIP_H
+#define THIRRING_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define THR_OK 0
+#define THR_ERR_ARG 1
+#define THR_ERR_HIP 2
+#define THR_ERR_NOCONV 3
+#define THR_ERR_SIZE 4
+#define THR_ERR_STATE 5
+
+/* precisions of device spinors */
+#define THR_F64 0
+#define THR_F32 1
+#define THR_F16 2 /* half2 per complex + one float scale per 5-D site */
+
+/* ---- lifetime ------------------------------------------------------------ */
+/* select the device and create the library stream; replaces init_MPI and
+ * init_gammas (comms.F90:38-95, gammamatrices.F90:17-132) on one GPU */
+int thr_init(int device);
+int thr_finalize(void);
+const char *thr_last_error(void);
+
+/* ---- (1) host-array drop-in entry points -------------------------------- */
+
+/* gauge.F90:12-43  coef(u, theta): u = 1 + i theta, u(:,:,Lt,3) *= -1 when
+ * ibound == -1, periodic halo */
+int thr_coef(double *u, const float *theta, const int dims[4], int ibound);
+
+/* dirac.F90:11-27, 29-179  dslash / dslash_shamir / dslash_wilson (Phi = M R) */
+int thr_dslash(double *phi, const double *r, const double *u, const int dims[4], float am,
+               int imass, float am3, int kernel);
+
+/* dirac.F90:409-494, 538-625  dslashd / dslashd_shamir / dslashd_wilson
+ * (Phi = M^dagger R); the optional MPI request argument has no meaning here */
+int thr_dslashd(double *phi, const double *r, const double *u, const int dims[4], float am,
+                int imass, float am3, int kernel);
+
+/* inverter_utils.F90:9-54  dirac_op_shifted / dirac_operator:
+ * xout = M^dagger M xin + shift xin */
+int thr_dirac_op_shifted(double *xout, const double *xin, const double *u, const int dims[4],
+                         float am, int imass, float am3, double shift, int kernel);
+
+/* measure_module.F90:9-140  congrad(Phi, res, itercg, am, imass, iterations):
+ * x is vector::X (initial guess in, solution out); niterc as params.F90:57 */
+int thr_congrad(const double *phi, double *x, const double *u, const int dims[4], float res,
+                float am, int imass, float am3, int niterc, int kernel, int *itercg,
+                int *iterations);
+
+/* multishift_module.F90:13-284 (use_sp = 0) and 289-534 (use_sp = 1)
+ * multishift_solver(u, am, imass, ndiagq, aden, anum, output, input, res,
+ *                   maxcg, cg_return, cg_returns) */
+int thr_multishift(const double *u, const int dims[4], float am, int imass, float am3,
+                   int ndiagq, const double *aden, const double *anum, double *output,
+                   const double *input, float res, int maxcg, int use_sp, int kernel,
+                   int *cg_return, int *cg_returns);
+
+/* derivs_module.F90:8-26, 99-196  derivs(R, X2, anum, iflag, am, imass):
+ * accumulates into dsdpi (gforce::dSdpi) */
+int thr_derivs(const double *r, const double *x2, double anum, int iflag, const int dims[4],
+               float am, int imass, float am3, int kernel, double *dsdpi);
+
+/* qmrherm_module.F90:20-194  qmrherm(Phi, x, res, itercg, am, imass, anum,
+ * aden, ndiagq, iflag, use_sp, cg_returns).  x1 (Ls,Lx,Ly,Lt,4,ndiagq) and
+ * phi0 (haloed, 25 slots) are the module arrays; dsdpi accumulates for
+ * iflag 2/3; max_qmr_iters as params.F90:56. */
+int thr_qmrherm(const double *phi, double *x, float res, int *itercg, float am, int imass,
+                float am3, const double *anum, const double *aden, int ndiagq, int iflag,
+                int use_sp, const double *u, const int dims[4], int max_qmr_iters, int kernel,
+                double *x1, double *phi0, double *dsdpi, int *cg_returns);
+
+/* ---- (2) device-resident entry points ------------------------------------ */
+
+/* links from theta on device (gauge.F90:12-43); links64 [3][Lt][Ly][Lx] complex128 */
+int thr_coef_dev(void *links64, const float *theta, const int dims[4], int ibound, void *stream);
+
+/* convert FP64 device links to the FP32 copy used by the FP32/FP16 operators */
+int thr_links_to_f32_dev(void *links32, const void *links64, const int dims[4], void *stream);
+
+/* out = M in (dagger = 0) or M^dagger in (dagger = 1) for nrhs right-hand
+ * sides with per-rhs masses am[nrhs]; prec THR_F64/F32/F16 (links must be
+ * complex128 for F64, complex64 otherwise; norms only for F16) */
+int thr_dslash_dev(void *out, float *out_norm, const void *in, const float *in_norm,
+                   const void *links, const int dims[4], int nrhs, const float *am, int imass,
+                   float am3, int dagger, int prec, void *stream);
+
+/* reference-faithful FP64 CG on device fields (phi, x: 4*V5 complex128) */
+int thr_congrad_dev(const void *phi, void *x, const void *links64, const int dims[4], float res,
+                    float am, int imass, float am3, int niterc, int kernel, int *itercg,
+                    int *iterations, void *stream);
+
+/* reference-faithful multishift on device fields; out [ndiagq][4*V5] complex128 */
+int thr_multishift_dev(const void *links64, const int dims[4], float am, int imass, float am3,
+                       int ndiagq, const double *aden, const double *anum, void *out,
+                       const void *input, float res, int maxcg, int use_sp, int kernel,
+                       int *cg_return, int *cg_returns, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,451 @@
+// C ABI entry points (declared in include/thirring_hip.h).
+//
+// Host-array entry points stage the reference's haloed complex128 arrays
+// through device buffers: one H2D copy of the haloed array, a pack kernel
+// that strips the halo into the device layout, the operator or solver, an
+// unpack kernel that writes the interior and the periodic halo, one D2H copy.
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/thirring_hip.h"
+#include "solvers.h"
+
+namespace thr {
+
+namespace {
+thread_local std::string g_err;
+int g_device = -1;
+}  // namespace
+
+void set_error(const std::string &msg) { g_err = msg; }
+const char *last_error() { return g_err.c_str(); }
+void release_workspace();
+
+static hipStream_t pick(void *stream) {
+  return stream ? (hipStream_t)stream : default_stream();
+}
+
+static int check_dims(const int dims[4]) {
+  THR_CHECK(dims, THR_ERR_ARG, "dims is NULL");
+  for (int k = 0; k < 4; k++) THR_CHECK(dims[k] > 0, THR_ERR_ARG, "dims must be positive");
+  const long v5 = (long)dims[0] * dims[1] * dims[2] * dims[3];
+  THR_CHECK(v5 < (1L << 31) / 4, THR_ERR_SIZE, "lattice too large for 32-bit site indexing");
+  return THR_OK;
+}
+
+static Dims mkdims(const int dims[4]) { return Dims{dims[0], dims[1], dims[2], dims[3]}; }
+
+static long haloed5(const Dims &d) { return (long)(d.ls + 2) * (d.lx + 2) * (d.ly + 2) * (d.lt + 2) * 4; }
+static long haloed4(const Dims &d) { return (long)(d.lx + 2) * (d.ly + 2) * (d.lt + 2) * 3; }
+
+// workspace slots of the host staging path
+enum {
+  W_H0 = 100, W_H1, W_H2, W_L, W_LH, W_A, W_B, W_C, W_D, W_X1, W_F, W_T
+};
+
+static int stage_links(const Dims &d, const double *u, double2 **u64, hipStream_t s) {
+  double2 *uh = (double2 *)workspace(W_LH, sizeof(double2) * haloed4(d));
+  *u64 = (double2 *)workspace(W_L, sizeof(double2) * d.v4() * 3);
+  THR_CHECK(uh && *u64, THR_ERR_HIP, "workspace allocation failed");
+  THR_HIP(hipMemcpyAsync(uh, u, sizeof(double2) * haloed4(d), hipMemcpyHostToDevice, s));
+  return launch_pack_links(d, uh, *u64, nullptr, s);
+}
+
+static int stage_in(const Dims &d, const double *h, int slot_h, double2 *dev, hipStream_t s) {
+  double2 *hb = (double2 *)workspace(slot_h, sizeof(double2) * haloed5(d));
+  THR_CHECK(hb, THR_ERR_HIP, "workspace allocation failed");
+  THR_HIP(hipMemcpyAsync(hb, h, sizeof(double2) * haloed5(d), hipMemcpyHostToDevice, s));
+  return launch_pack_spinor(d, hb, dev, nullptr, P64, s);
+}
+
+static int stage_out(const Dims &d, const double2 *dev, int slot_h, double *h, hipStream_t s) {
+  double2 *hb = (double2 *)workspace(slot_h, sizeof(double2) * haloed5(d));
+  THR_CHECK(hb, THR_ERR_HIP, "workspace allocation failed");
+  int rc = launch_unpack_spinor(d, dev, nullptr, P64, hb, 1, s);
+  if (rc) return rc;
+  THR_HIP(hipMemcpyAsync(h, hb, sizeof(double2) * haloed5(d), hipMemcpyDeviceToHost, s));
+  return THR_OK;
+}
+
+static OpDesc mkop(const Dims &d, int kernel, int imass, float am, float am3, const double2 *u64) {
+  OpDesc op{};
+  op.d = d;
+  op.kernel = kernel;
+  op.imass = imass;
+  op.am = am;
+  op.am3 = am3;
+  op.u64 = u64;
+  return op;
+}
+
+static int check_op(int kernel, int imass) {
+  THR_CHECK(kernel == 0 || kernel == 1, THR_ERR_ARG, "kernel must be 0 (Shamir) or 1 (Wilson)");
+  THR_CHECK(imass == 1 || imass == 3 || imass == 5, THR_ERR_ARG, "imass must be 1, 3 or 5");
+  THR_CHECK(kernel == 0 || imass != 5, THR_ERR_ARG, "the Wilson kernel supports imass 1 and 3");
+  return THR_OK;
+}
+
+#define TRY(x)              \
+  do {                      \
+    int _rc = (x);          \
+    if (_rc) return _rc;    \
+  } while (0)
+
+static int host_op(double *phi, const double *r, const double *u, const int dims[4], float am,
+                   int imass, float am3, int kernel, int dagger) {
+  TRY(check_dims(dims));
+  TRY(check_op(kernel, imass));
+  THR_CHECK(phi && r && u, THR_ERR_ARG, "NULL array");
+  const Dims d = mkdims(dims);
+  hipStream_t s = default_stream();
+  double2 *u64;
+  TRY(stage_links(d, u, &u64, s));
+  double2 *a = (double2 *)workspace(W_A, sizeof(double2) * d.nspinor());
+  double2 *b = (double2 *)workspace(W_B, sizeof(double2) * d.nspinor());
+  THR_CHECK(a && b, THR_ERR_HIP, "workspace allocation failed");
+  TRY(stage_in(d, r, W_H0, a, s));
+  TRY(apply_op(mkop(d, kernel, imass, am, am3, u64), dagger, a, b, s));
+  TRY(stage_out(d, b, W_H1, phi, s));
+  THR_HIP(hipStreamSynchronize(s));
+  return THR_OK;
+}
+
+}  // namespace thr
+
+using namespace thr;
+
+extern "C" {
+
+int thr_init(int device) {
+  int n = 0;
+  THR_HIP(hipGetDeviceCount(&n));
+  THR_CHECK(n > 0, THR_ERR_HIP, "no HIP device visible");
+  THR_CHECK(device >= 0 && device < n, THR_ERR_ARG, "device index out of range");
+  THR_HIP(hipSetDevice(device));
+  g_device = device;
+  THR_CHECK(default_stream(), THR_ERR_HIP, "stream creation failed");
+  return THR_OK;
+}
+
+int thr_finalize(void) {
+  THR_HIP(hipDeviceSynchronize());
+  release_workspace();
+  return THR_OK;
+}
+
+const char *thr_last_error(void) { return last_error(); }
+
+int thr_coef(double *u, const float *theta, const int dims[4], int ibound) {
+  TRY(check_dims(dims));
+  THR_CHECK(u && theta, THR_ERR_ARG, "NULL array");
+  const Dims d = mkdims(dims);
+  hipStream_t s = default_stream();
+  float *th = (float *)workspace(W_F, sizeof(float) * d.v4() * 3);
+  double2 *u64 = (double2 *)workspace(W_L, sizeof(double2) * d.v4() * 3);
+  THR_CHECK(th && u64, THR_ERR_HIP, "workspace allocation failed");
+  THR_HIP(hipMemcpyAsync(th, theta, sizeof(float) * d.v4() * 3, hipMemcpyHostToDevice, s));
+  TRY(launch_coef(d, th, ibound, u64, s));
+  // host halo of the link array: periodic copy of the interior
+  std::vector<double2> h(d.v4() * 3);
+  THR_HIP(hipMemcpyAsync(h.data(), u64, sizeof(double2) * h.size(), hipMemcpyDeviceToHost, s));
+  THR_HIP(hipStreamSynchronize(s));
+  const long X = d.lx + 2, Y = d.ly + 2, T = d.lt + 2;
+  for (int mu = 0; mu < 3; mu++)
+    for (int t = 0; t < T; t++)
+      for (int y = 0; y < Y; y++)
+        for (int x = 0; x < X; x++) {
+          const int xi = x == 0 ? d.lx - 1 : (x == X - 1 ? 0 : x - 1);
+          const int yi = y == 0 ? d.ly - 1 : (y == Y - 1 ? 0 : y - 1);
+          const int ti = t == 0 ? d.lt - 1 : (t == T - 1 ? 0 : t - 1);
+          const double2 v = h[xi + (long)d.lx * (yi + (long)d.ly * (ti + (long)d.lt * mu))];
+          const long o = 2 * (x + X * (y + Y * (t + T * (long)mu)));
+          u[o] = v.x;
+          u[o + 1] = v.y;
+        }
+  return THR_OK;
+}
+
+int thr_dslash(double *phi, const double *r, const double *u, const int dims[4], float am,
+               int imass, float am3, int kernel) {
+  return host_op(phi, r, u, dims, am, imass, am3, kernel, 0);
+}
+
+int thr_dslashd(double *phi, const double *r, const double *u, const int dims[4], float am,
+                int imass, float am3, int kernel) {
+  return host_op(phi, r, u, dims, am, imass, am3, kernel, 1);
+}
+
+int thr_dirac_op_shifted(double *xout, const double *xin, const double *u, const int dims[4],
+                         float am, int imass, float am3, double shift, int kernel) {
+  TRY(check_dims(dims));
+  TRY(check_op(kernel, imass));
+  THR_CHECK(xout && xin && u, THR_ERR_ARG, "NULL array");
+  const Dims d = mkdims(dims);
+  hipStream_t s = default_stream();
+  double2 *u64;
+  TRY(stage_links(d, u, &u64, s));
+  const long n = d.nspinor();
+  double2 *a = (double2 *)workspace(W_A, sizeof(double2) * n);
+  double2 *b = (double2 *)workspace(W_B, sizeof(double2) * n);
+  double2 *c = (double2 *)workspace(W_C, sizeof(double2) * n);
+  THR_CHECK(a && b && c, THR_ERR_HIP, "workspace allocation failed");
+  TRY(stage_in(d, xin, W_H0, a, s));
+  const OpDesc op = mkop(d, kernel, imass, am, am3, u64);
+  TRY(apply_op(op, 0, a, b, s));
+  TRY(apply_op(op, 1, b, c, s));
+  double2 *hb = (double2 *)workspace(W_H1, sizeof(double2) * haloed5(d));
+  TRY(launch_unpack_spinor(d, c, nullptr, P64, hb, 1, s));
+  THR_HIP(hipMemcpyAsync(xout, hb, sizeof(double2) * haloed5(d), hipMemcpyDeviceToHost, s));
+  THR_HIP(hipStreamSynchronize(s));
+  if (shift != 0.0) {
+    const long nh = haloed5(d) * 2;
+    for (long k = 0; k < nh; k++) xout[k] = xout[k] + shift * xin[k];
+  }
+  return THR_OK;
+}
+
+int thr_congrad(const double *phi, double *x, const double *u, const int dims[4], float res,
+                float am, int imass, float am3, int niterc, int kernel, int *itercg,
+                int *iterations) {
+  TRY(check_dims(dims));
+  TRY(check_op(kernel, imass));
+  THR_CHECK(phi && x && u && itercg, THR_ERR_ARG, "NULL argument");
+  const Dims d = mkdims(dims);
+  hipStream_t s = default_stream();
+  double2 *u64;
+  TRY(stage_links(d, u, &u64, s));
+  const long n = d.nspinor();
+  double2 *a = (double2 *)workspace(W_A, sizeof(double2) * n);
+  double2 *b = (double2 *)workspace(W_B, sizeof(double2) * n);
+  THR_CHECK(a && b, THR_ERR_HIP, "workspace allocation failed");
+  TRY(stage_in(d, phi, W_H0, a, s));
+  TRY(stage_in(d, x, W_H1, b, s));
+  TRY(cg_reference(mkop(d, kernel, imass, am, am3, u64), a, b, res, niterc, itercg, iterations, s));
+  TRY(stage_out(d, b, W_H1, x, s));
+  THR_HIP(hipStreamSynchronize(s));
+  return THR_OK;
+}
+
+int thr_multishift(const double *u, const int dims[4], float am, int imass, float am3,
+                   int ndiagq, const double *aden, const double *anum, double *output,
+                   const double *input, float res, int maxcg, int use_sp, int kernel,
+                   int *cg_return, int *cg_returns) {
+  TRY(check_dims(dims));
+  TRY(check_op(kernel, imass));
+  THR_CHECK(u && aden && anum && output && input && cg_return, THR_ERR_ARG, "NULL argument");
+  THR_CHECK(ndiagq >= 1 && ndiagq <= MAXSHIFT, THR_ERR_ARG, "ndiagq out of range");
+  const Dims d = mkdims(dims);
+  hipStream_t s = default_stream();
+  double2 *u64;
+  TRY(stage_links(d, u, &u64, s));
+  const long n = d.nspinor();
+  double2 *a = (double2 *)workspace(W_A, sizeof(double2) * n);
+  double2 *x1 = (double2 *)workspace(W_X1, sizeof(double2) * n * ndiagq);
+  THR_CHECK(a && x1, THR_ERR_HIP, "workspace allocation failed");
+  TRY(stage_in(d, input, W_H0, a, s));
+  const OpDesc op = mkop(d, kernel, imass, am, am3, u64);
+  if (use_sp)
+    TRY(multishift_reference_sp(op, ndiagq, aden, anum, a, x1, res, maxcg, cg_return, cg_returns, s));
+  else
+    TRY(multishift_reference(op, ndiagq, aden, anum, a, x1, res, maxcg, cg_return, cg_returns, s));
+  // the device shift layout equals the halo-free Fortran output(Ls,Lx,Ly,Lt,4,n)
+  THR_HIP(hipMemcpyAsync(output, x1, sizeof(double2) * n * ndiagq, hipMemcpyDeviceToHost, s));
+  THR_HIP(hipStreamSynchronize(s));
+  return THR_OK;
+}
+
+int thr_derivs(const double *r, const double *x2, double anum, int iflag, const int dims[4],
+               float am, int imass, float am3, int kernel, double *dsdpi) {
+  (void)am;
+  (void)imass;
+  (void)am3;
+  TRY(check_dims(dims));
+  THR_CHECK(kernel == 0, THR_ERR_ARG, "device derivs: Shamir kernel only");
+  THR_CHECK(r && x2 && dsdpi, THR_ERR_ARG, "NULL argument");
+  const Dims d = mkdims(dims);
+  hipStream_t s = default_stream();
+  const long n = d.nspinor();
+  double2 *a = (double2 *)workspace(W_A, sizeof(double2) * n);
+  double2 *b = (double2 *)workspace(W_B, sizeof(double2) * n);
+  double *f = (double *)workspace(W_T, sizeof(double) * d.v4() * 3);
+  THR_CHECK(a && b && f, THR_ERR_HIP, "workspace allocation failed");
+  TRY(stage_in(d, r, W_H0, a, s));
+  TRY(stage_in(d, x2, W_H1, b, s));
+  THR_HIP(hipMemcpyAsync(f, dsdpi, sizeof(double) * d.v4() * 3, hipMemcpyHostToDevice, s));
+  TRY(launch_derivs(d, a, b, anum, iflag, f, s));
+  THR_HIP(hipMemcpyAsync(dsdpi, f, sizeof(double) * d.v4() * 3, hipMemcpyDeviceToHost, s));
+  THR_HIP(hipStreamSynchronize(s));
+  return THR_OK;
+}
+
+int thr_qmrherm(const double *phi, double *x, float res, int *itercg, float am, int imass,
+                float am3, const double *anum, const double *aden, int ndiagq, int iflag,
+                int use_sp, const double *u, const int dims[4], int max_qmr_iters, int kernel,
+                double *x1h, double *phi0, double *dsdpi, int *cg_returns) {
+  TRY(check_dims(dims));
+  TRY(check_op(kernel, imass));
+  THR_CHECK(phi && x && itercg && anum && aden && u && x1h, THR_ERR_ARG, "NULL argument");
+  THR_CHECK(ndiagq >= 1 && ndiagq <= 25, THR_ERR_ARG,
+            "qmrherm requires ndiagq <= ndiag = 25 (qmrherm_module.F90:66-73)");
+  THR_CHECK(dims[3] % 4 == 0, THR_ERR_ARG,
+            "QMRHERM: kthird_l must be divisible by 4 (qmrherm_module.F90:75-81)");
+  THR_CHECK(iflag >= 0 && iflag <= 3, THR_ERR_ARG, "iflag must be 0..3");
+  THR_CHECK(iflag != 1 || phi0, THR_ERR_ARG, "iflag 1 needs the Phi0 register");
+  THR_CHECK(iflag < 2 || dsdpi, THR_ERR_ARG, "iflag 2/3 need dSdpi");
+  THR_CHECK(iflag != 3 || (phi0 && kernel == 0), THR_ERR_ARG, "iflag 3 needs Phi0 (Shamir)");
+  THR_CHECK(iflag != 2 || kernel == 0, THR_ERR_ARG, "device derivs: Shamir kernel only");
+  const Dims d = mkdims(dims);
+  hipStream_t s = default_stream();
+  double2 *u64;
+  TRY(stage_links(d, u, &u64, s));
+  const long n = d.nspinor();
+  double2 *a = (double2 *)workspace(W_A, sizeof(double2) * n);
+  double2 *b = (double2 *)workspace(W_B, sizeof(double2) * n);
+  double2 *c = (double2 *)workspace(W_C, sizeof(double2) * n);
+  double2 *x1 = (double2 *)workspace(W_X1, sizeof(double2) * n * ndiagq);
+  THR_CHECK(a && b && c && x1, THR_ERR_HIP, "workspace allocation failed");
+  TRY(stage_in(d, phi, W_H0, a, s));
+  const OpDesc op = mkop(d, kernel, imass, am, am3, u64);
+  // single precision is not available with the Wilson kernel (qmrherm_module.F90:54-63)
+  const int sp = use_sp && kernel == 0;
+  if (sp)
+    TRY(multishift_reference_sp(op, ndiagq, aden, anum + 1, a, x1, res, max_qmr_iters, itercg,
+                                cg_returns, s));
+  else
+    TRY(multishift_reference(op, ndiagq, aden, anum + 1, a, x1, res, max_qmr_iters, itercg,
+                             cg_returns, s));
+  THR_HIP(hipMemcpyAsync(x1h, x1, sizeof(double2) * n * ndiagq, hipMemcpyDeviceToHost, s));
+  THR_HIP(hipStreamSynchronize(s));
+  // x = anum(0)*Phi over the whole haloed array (qmrherm_module.F90:83)
+  const long nh = haloed5(d);
+  for (long k = 0; k < 2 * nh; k++) x[k] = anum[0] * phi[k];
+  if (iflag < 2) {
+    // x += sum anum(i) x1(i) on the interior, then the periodic halo (96-108)
+    std::vector<double> xi(2 * n);
+    // interior of x in halo-free order
+    const long S = d.ls + 2, X = d.lx + 2, Y = d.ly + 2;
+    auto hidx = [&](int sidx, int xx, int yy, int tt, int aa) {
+      return (sidx + S * (xx + X * (yy + Y * (tt + (long)(d.lt + 2) * aa))));
+    };
+    for (int aa = 0; aa < 4; aa++)
+      for (int tt = 0; tt < d.lt; tt++)
+        for (int yy = 0; yy < d.ly; yy++)
+          for (int xx = 0; xx < d.lx; xx++)
+            for (int ss = 0; ss < d.ls; ss++) {
+              const long q = ss + d.ls * (xx + (long)d.lx * (yy + (long)d.ly * (tt + (long)d.lt * aa)));
+              const long o = hidx(ss + 1, xx + 1, yy + 1, tt + 1, aa);
+              double re = x[2 * o], im = x[2 * o + 1];
+              for (int k = 0; k < ndiagq; k++) {
+                re = re + anum[k + 1] * x1h[2 * (q + n * k)];
+                im = im + anum[k + 1] * x1h[2 * (q + n * k) + 1];
+              }
+              xi[2 * q] = re;
+              xi[2 * q + 1] = im;
+            }
+    THR_HIP(hipMemcpyAsync(b, xi.data(), sizeof(double2) * n, hipMemcpyHostToDevice, s));
+    TRY(stage_out(d, b, W_H1, x, s));
+    if (iflag == 1) {
+      for (int k = 0; k < ndiagq; k++) {
+        double2 *hb = (double2 *)workspace(W_H2, sizeof(double2) * nh);
+        THR_CHECK(hb, THR_ERR_HIP, "workspace allocation failed");
+        TRY(launch_unpack_spinor(d, x1 + n * k, nullptr, P64, hb, 1, s));
+        THR_HIP(hipMemcpyAsync(phi0 + 2 * nh * k, hb, sizeof(double2) * nh, hipMemcpyDeviceToHost, s));
+      }
+    }
+    THR_HIP(hipStreamSynchronize(s));
+    return THR_OK;
+  }
+  double *f = (double *)workspace(W_T, sizeof(double) * d.v4() * 3);
+  THR_CHECK(f, THR_ERR_HIP, "workspace allocation failed");
+  THR_HIP(hipMemcpyAsync(f, dsdpi, sizeof(double) * d.v4() * 3, hipMemcpyHostToDevice, s));
+  for (int k = 0; k < ndiagq; k++) {
+    const double2 *xk = x1 + n * k;
+    TRY(apply_op(op, 0, xk, b, s));  // x2 = M x1(k)  (143)
+    if (iflag == 2) {
+      TRY(launch_derivs(d, xk, b, anum[k + 1], 0, f, s));  // (150-155)
+    } else {
+      const double coeff = -anum[k + 1];
+      TRY(stage_in(d, phi0 + 2 * nh * k, W_H0, c, s));       // R = Phi0(k) (159)
+      TRY(launch_derivs(d, c, b, coeff, 0, f, s));            // (163)
+      TRY(apply_op(op, 0, c, b, s));                          // x2 = M R (166)
+      TRY(launch_derivs(d, b, xk, coeff, 1, f, s));           // (181)
+    }
+  }
+  THR_HIP(hipMemcpyAsync(dsdpi, f, sizeof(double) * d.v4() * 3, hipMemcpyDeviceToHost, s));
+  THR_HIP(hipStreamSynchronize(s));
+  return THR_OK;
+}
+
+// ---- device-resident ------------------------------------------------------
+
+int thr_coef_dev(void *links64, const float *theta, const int dims[4], int ibound, void *stream) {
+  TRY(check_dims(dims));
+  THR_CHECK(links64 && theta, THR_ERR_ARG, "NULL argument");
+  return launch_coef(mkdims(dims), theta, ibound, (double2 *)links64, pick(stream));
+}
+
+int thr_links_to_f32_dev(void *links32, const void *links64, const int dims[4], void *stream) {
+  TRY(check_dims(dims));
+  THR_CHECK(links32 && links64, THR_ERR_ARG, "NULL argument");
+  const Dims d = mkdims(dims);
+  return convert_d2f(d.v4() * 3, (const double2 *)links64, (float2 *)links32, pick(stream));
+}
+
+int thr_dslash_dev(void *out, float *out_norm, const void *in, const float *in_norm,
+                   const void *links, const int dims[4], int nrhs, const float *am, int imass,
+                   float am3, int dagger, int prec, void *stream) {
+  TRY(check_dims(dims));
+  THR_CHECK(out && in && links && am, THR_ERR_ARG, "NULL argument");
+  THR_CHECK(prec >= 0 && prec <= 2, THR_ERR_ARG, "prec must be THR_F64, THR_F32 or THR_F16");
+  THR_CHECK(prec != 2 || (in_norm && out_norm), THR_ERR_ARG, "FP16 spinors need norm arrays");
+  THR_CHECK(nrhs >= 1 && nrhs <= MAXRHS, THR_ERR_ARG, "nrhs out of range (1..32)");
+  DslashLaunch L{};
+  L.d = mkdims(dims);
+  L.nrhs = nrhs;
+  L.prec = prec;
+  L.dagger = dagger ? 1 : 0;
+  L.imass = imass;
+  L.am3 = am3;
+  for (int k = 0; k < nrhs; k++) L.am[k] = am[k];
+  L.in = in;
+  L.in_norm = in_norm;
+  L.out = out;
+  L.out_norm = out_norm;
+  L.links = links;
+  L.mode = EPI_STORE;
+  L.sc_stride = 1;
+  L.done_stride = 1;
+  return launch_dslash(L, pick(stream));
+}
+
+int thr_congrad_dev(const void *phi, void *x, const void *links64, const int dims[4], float res,
+                    float am, int imass, float am3, int niterc, int kernel, int *itercg,
+                    int *iterations, void *stream) {
+  TRY(check_dims(dims));
+  TRY(check_op(kernel, imass));
+  THR_CHECK(phi && x && links64 && itercg, THR_ERR_ARG, "NULL argument");
+  const Dims d = mkdims(dims);
+  return cg_reference(mkop(d, kernel, imass, am, am3, (const double2 *)links64),
+                      (const double2 *)phi, (double2 *)x, res, niterc, itercg, iterations,
+                      pick(stream));
+}
+
+int thr_multishift_dev(const void *links64, const int dims[4], float am, int imass, float am3,
+                       int ndiagq, const double *aden, const double *anum, void *out,
+                       const void *input, float res, int maxcg, int use_sp, int kernel,
+                       int *cg_return, int *cg_returns, void *stream) {
+  TRY(check_dims(dims));
+  TRY(check_op(kernel, imass));
+  THR_CHECK(links64 && aden && anum && out && input && cg_return, THR_ERR_ARG, "NULL argument");
+  THR_CHECK(ndiagq >= 1 && ndiagq <= MAXSHIFT, THR_ERR_ARG, "ndiagq out of range");
+  const OpDesc op = mkop(mkdims(dims), kernel, imass, am, am3, (const double2 *)links64);
+  if (use_sp)
+    return multishift_reference_sp(op, ndiagq, aden, anum, (const double2 *)input, (double2 *)out,
+                                   res, maxcg, cg_return, cg_returns, pick(stream));
+  return multishift_reference(op, ndiagq, aden, anum, (const double2 *)input, (double2 *)out, res,
+                              maxcg, cg_return, cg_returns, pick(stream));
+}
+
+}  // extern "C"

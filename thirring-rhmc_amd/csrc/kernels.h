@@ -1,0 +1,70 @@
+// Kernel launch interfaces (device code lives in dslash.hip / blas.hip).
+#pragma once
+#include "common.h"
+
+namespace thr {
+
+constexpr int MAXRHS = 32;
+
+// epilogue modes of the fused Dslash kernel
+enum EpiMode {
+  EPI_STORE = 0,     // out = M r
+  EPI_NORM = 1,      // out = M r ; partial += |out|^2
+  EPI_DOTW = 2,      // out = M r ; partial += Re <w, out>
+  EPI_CG = 3,        // (CG dagger step) r -= a*out ; x += b*p ; partial += |r|^2 ; out not stored
+};
+
+struct DslashLaunch {
+  Dims d;
+  int nrhs;
+  int prec;        // Prec of in/out (and of w/r/x/p in the fused modes)
+  int dagger;
+  int imass;
+  float am3;
+  float am[MAXRHS];  // per right-hand side (float32, as the reference's `real :: am`)
+  const void *in;
+  const float *in_norm;
+  void *out;
+  float *out_norm;
+  const void *links;  // double2 for P64, float2 otherwise
+  // epilogue
+  int mode;
+  const void *w;       // EPI_DOTW
+  void *r, *x;         // EPI_CG
+  const void *p;       // EPI_CG
+  const double *a_ptr; // EPI_CG: r -= a_ptr[rhs*sc_stride] * out (device scalar)
+  const double *b_ptr; // EPI_CG: x += b_ptr[rhs*sc_stride] * p  (null: no x update)
+  int sc_stride;       // stride (in doubles) between the per-rhs scalars
+  double *partial;     // [nrhs][nblocks]
+  const int *done;     // optional early-exit flag, done[rhs*done_stride] (device)
+  int done_stride;
+  int wilson4d;        // 1: 4-D Wilson slice operator only (diag 3 - am3, no s hops, no mass)
+};
+
+int launch_dslash(const DslashLaunch &L, hipStream_t s);
+long dslash_blocks(const Dims &d);
+
+// Wilson-kernel operator on device (FP64): slice-composed, imass 1 or 3
+int launch_dslash_wilson(const Dims &d, int nrhs, int dagger, int imass, float am, float am3,
+                         const double2 *in, double2 *out, const double2 *links, double2 *tmp,
+                         hipStream_t s);
+
+// layout conversion between the reference's haloed complex128 arrays and the
+// device layout; prec selects the device storage
+int launch_pack_spinor(const Dims &d, const double2 *haloed, void *dst, float *dst_norm, int prec,
+                       hipStream_t s);
+int launch_unpack_spinor(const Dims &d, const void *src, const float *src_norm, int prec,
+                         double2 *haloed, int fill_halo, hipStream_t s);
+int launch_pack_links(const Dims &d, const double2 *haloed, double2 *dst64, float2 *dst32,
+                      hipStream_t s);
+int convert_spinors(const Dims &d, int nrhs, const void *src, const float *snrm, int sprec,
+                    void *dst, float *dnrm, int dprec, hipStream_t s);
+
+// u = 1 + i theta with antiperiodic t-links (gauge.F90:12-43), device layout
+int launch_coef(const Dims &d, const float *theta, int ibound, double2 *links, hipStream_t s);
+
+// fermion force (derivs_module.F90:99-196), FP64, accumulates into dsdpi [mu][t][y][x]
+int launch_derivs(const Dims &d, const double2 *R, const double2 *X2, double anum, int iflag,
+                  double *dsdpi, hipStream_t s);
+
+}  // namespace thr

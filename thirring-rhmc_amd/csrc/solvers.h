@@ -1,0 +1,47 @@
+// Krylov solvers on device-resident fields.
+#pragma once
+#include "kernels.h"
+
+namespace thr {
+
+constexpr int MAXSHIFT = 32;
+
+// One operator configuration: which DWF kernel, wall mass, links.
+struct OpDesc {
+  Dims d;
+  int kernel;         // 0 Shamir, 1 Wilson kernel
+  int imass;
+  float am3;
+  float am;           // single-rhs mass (multi-rhs masses live in the launch)
+  const double2 *u64; // device links [mu][t][y][x]
+  const float2 *u32;  // FP32 copy (may be null when only FP64 is used)
+};
+
+// growable device workspace, keyed by slot
+void *workspace(int slot, size_t bytes);
+double *partial_buffer(int nrhs_blocks);
+hipStream_t default_stream();
+
+// reference-faithful CG on M^+M (measure_module.F90:9-140); x in/out (FP64 device)
+int cg_reference(const OpDesc &op, const double2 *phi, double2 *x, float res, int niterc,
+                 int *itercg, int *iterations, hipStream_t s);
+
+// reference-faithful multishift CG (multishift_module.F90:13-284), FP64 vectors.
+// out: [n][4*V5] shift solutions.
+int multishift_reference(const OpDesc &op, int n, const double *aden, const double *anum,
+                         const double2 *input, double2 *out, float res, int maxcg,
+                         int *cg_return, int *cg_returns, hipStream_t s);
+
+// FP32-vector multishift with FP64 scalars (multishift_module.F90:289-534);
+// input/output FP64 device fields.
+int multishift_reference_sp(const OpDesc &op, int n, const double *aden, const double *anum,
+                            const double2 *input, double2 *out, float res, int maxcg,
+                            int *cg_return, int *cg_returns, hipStream_t s);
+
+// element-wise FP64 -> FP32 complex conversion
+int convert_d2f(long n, const double2 *a, float2 *b, hipStream_t s);
+
+// apply M or M^+ (FP64) into out
+int apply_op(const OpDesc &op, int dagger, const double2 *in, double2 *out, hipStream_t s);
+
+}  // namespace thr

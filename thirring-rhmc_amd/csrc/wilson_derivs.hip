@@ -1,0 +1,181 @@
+// Wilson-kernel domain-wall operator (secondary generation kernel) and the
+// Shamir fermion force.
+//
+// Wilson kernel, dirac.F90:226-299 (M) and 538-625 (M^dagger).  The reference
+// builds it from 4-D Wilson applications on s-slices; with D_W that slice
+// operator and the s-couplings collected into one field it reads
+//   M  R = D_W (R + Q) + (R - Q),
+//        Q(s) = P+ R(s-1) + P- R(s+1)       (interior s)
+//             + c1 P+ R(Ls) at s=1, c2 P- R(1) at s=Ls
+//   M^+R = T + R + P- cA (T - R)(s_A) + P+ cB (T - R)(s_B),  T = D_W^+ R
+// so each application is one 4-D stencil pass plus two local passes.
+//
+// Force, derivs_module.F90:99-196: for every link (x, mu) the sum over s and
+// spin of the Wilson and Dirac parts of d/dtheta Re<R, M^(+) X2>, with the
+// float32 coefficient tzi_real = 2*real(anum).
+#include "kernels.h"
+
+namespace thr {
+
+namespace {
+
+// Y = R + Q  (M) ; coefficients c1, c2 multiply the wall terms
+__global__ __launch_bounds__(BLK) void k_wk_pre(const Dims d, int nrhs, const double2 *R,
+                                                double2 *Y, double2 c1, double2 c2) {
+  const long v5 = d.v5();
+  const long i = (long)blockIdx.x * BLK + threadIdx.x;
+  if (i >= v5 * nrhs) return;
+  const long rhs = i / v5, site = i % v5;
+  const int s = (int)(site % d.ls);
+  const long col = site - s;
+  const double2 *r = R + rhs * 4 * v5;
+  double2 *y = Y + rhs * 4 * v5;
+  for (int a = 0; a < 4; a++) {
+    cx<double> v = ldc(r, a * v5 + site);
+    if (a < 2) {  // P+ from s-1
+      if (s > 0) v = v + ldc(r, a * v5 + site - 1);
+      if (s == 0) v = v + cmul(cx<double>{c1.x, c1.y}, ldc(r, a * v5 + col + d.ls - 1));
+    } else {  // P- from s+1
+      if (s + 1 < d.ls) v = v + ldc(r, a * v5 + site + 1);
+      if (s == d.ls - 1) v = v + cmul(cx<double>{c2.x, c2.y}, ldc(r, a * v5 + col));
+    }
+    st(y, a * v5 + site, v);
+  }
+}
+
+// Phi += 2R - Y
+__global__ __launch_bounds__(BLK) void k_wk_post(long n, const double2 *R, const double2 *Y,
+                                                 double2 *Phi) {
+  const long i = (long)blockIdx.x * BLK + threadIdx.x;
+  if (i >= n) return;
+  const double2 r = R[i], y = Y[i], p = Phi[i];
+  Phi[i] = make_double2(p.x + (2.0 * r.x - y.x), p.y + (2.0 * r.y - y.y));
+}
+
+// Phi = T + R + P- cA (T - R)(sA) + P+ cB (T - R)(sB)   (M^dagger)
+__global__ __launch_bounds__(BLK) void k_wkd_post(const Dims d, int nrhs, const double2 *R,
+                                                  const double2 *T, double2 *Phi, double2 z1,
+                                                  double2 z2) {
+  const long v5 = d.v5();
+  const long i = (long)blockIdx.x * BLK + threadIdx.x;
+  if (i >= v5 * nrhs) return;
+  const long rhs = i / v5, site = i % v5;
+  const int s = (int)(site % d.ls);
+  const long col = site - s;
+  const double2 *r = R + rhs * 4 * v5;
+  const double2 *tt = T + rhs * 4 * v5;
+  double2 *ph = Phi + rhs * 4 * v5;
+  for (int a = 0; a < 4; a++) {
+    cx<double> v = ldc(tt, a * v5 + site) + ldc(r, a * v5 + site);
+    if (a >= 2) {  // P-: from s-1, wall s=1 <- s=Ls with z1
+      if (s > 0) v = v + (ldc(tt, a * v5 + site - 1) - ldc(r, a * v5 + site - 1));
+      if (s == 0) {
+        const long q = a * v5 + col + d.ls - 1;
+        v = v + cmul(cx<double>{z1.x, z1.y}, ldc(tt, q) - ldc(r, q));
+      }
+    } else {  // P+: from s+1, wall s=Ls <- s=1 with z2
+      if (s + 1 < d.ls) v = v + (ldc(tt, a * v5 + site + 1) - ldc(r, a * v5 + site + 1));
+      if (s == d.ls - 1) {
+        const long q = a * v5 + col;
+        v = v + cmul(cx<double>{z2.x, z2.y}, ldc(tt, q) - ldc(r, q));
+      }
+    }
+    st(ph, a * v5 + site, v);
+  }
+}
+
+// gamval(mu,a)/akappa and gamin(mu,a)-1 for mu = 1..3 (gammamatrices.F90:27-75)
+__constant__ double kGre[3][4] = {{0, 0, 0, 0}, {-1, 1, 1, -1}, {0, 0, 0, 0}};
+__constant__ double kGim[3][4] = {{-1, -1, 1, 1}, {0, 0, 0, 0}, {-1, 1, 1, -1}};
+__constant__ int kGin[3][4] = {{3, 2, 1, 0}, {3, 2, 1, 0}, {2, 3, 0, 1}};
+
+__global__ __launch_bounds__(BLK) void k_derivs(const Dims d, const double2 *__restrict__ R,
+                                                const double2 *__restrict__ X2, float tzi,
+                                                int iflag, double *ds) {
+  const long v4 = d.v4(), v5 = d.v5();
+  const long i = (long)blockIdx.x * BLK + threadIdx.x;
+  if (i >= v4 * 3) return;
+  const int mu = (int)(i / v4);
+  const long s4 = i % v4;
+  const int x = (int)(s4 % d.lx), y = (int)((s4 / d.lx) % d.ly), t = (int)(s4 / ((long)d.lx * d.ly));
+  int xf = x, yf = y, tf = t;
+  if (mu == 0) xf = (x + 1 == d.lx) ? 0 : x + 1;
+  if (mu == 1) yf = (y + 1 == d.ly) ? 0 : y + 1;
+  if (mu == 2) tf = (t + 1 == d.lt) ? 0 : t + 1;
+  const long f4 = xf + (long)d.lx * (yf + (long)d.ly * tf);
+  const long c0 = s4 * d.ls, c1 = f4 * d.ls;
+  const double w1 = (double)(tzi * 0.5f);  // tzi_real*real(akappa)
+  const double w2 = (double)tzi * (iflag == 0 ? 1.0 : -1.0);
+  double acc = 0.0;
+  for (int a = 0; a < 4; a++) {
+    const int b = kGin[mu][a];
+    const cx<double> g{0.5 * kGre[mu][a], 0.5 * kGim[mu][a]};
+    double sw = 0.0, sd = 0.0;
+    for (int s = 0; s < d.ls; s++) {
+      const cx<double> r0 = ldc(R, a * v5 + c0 + s), r1 = ldc(R, a * v5 + c1 + s);
+      const cx<double> xa0 = ldc(X2, a * v5 + c0 + s), xa1 = ldc(X2, a * v5 + c1 + s);
+      const cx<double> xb0 = ldc(X2, b * v5 + c0 + s), xb1 = ldc(X2, b * v5 + c1 + s);
+      sw += cjmul(r0, xa1).i - cjmul(r1, xa0).i;
+      sd += cmul(g, cjmul(r0, xb1) + cjmul(r1, xb0)).i;
+    }
+    acc = acc + w1 * sw;
+    acc = acc + w2 * sd;
+  }
+  ds[i] += acc;
+}
+
+}  // namespace
+
+int launch_dslash_wilson(const Dims &d, int nrhs, int dagger, int imass, float am, float am3,
+                         const double2 *in, double2 *out, const double2 *links, double2 *tmp,
+                         hipStream_t s) {
+  THR_CHECK(imass == 1 || imass == 3, THR_ERR_ARG, "Wilson kernel supports imass 1 and 3");
+  const long n = d.v5() * nrhs;
+  const unsigned nb = (unsigned)((n + BLK - 1) / BLK);
+  DslashLaunch L{};
+  L.d = d;
+  L.nrhs = nrhs;
+  L.prec = P64;
+  L.dagger = dagger;
+  L.imass = imass;
+  L.am3 = am3;
+  L.links = links;
+  L.mode = EPI_STORE;
+  L.wilson4d = 1;
+  const double m = (double)am;
+  if (!dagger) {
+    // c1 = -am | -i am (dirac.F90:274-278), c2 = -am | +i am (287-291)
+    const double2 c1 = imass == 1 ? make_double2(-m, 0.0) : make_double2(0.0, -m);
+    const double2 c2 = imass == 1 ? make_double2(-m, 0.0) : make_double2(0.0, m);
+    hipLaunchKernelGGL(k_wk_pre, dim3(nb), dim3(BLK), 0, s, d, nrhs, in, tmp, c1, c2);
+    THR_HIP(hipGetLastError());
+    L.in = tmp;
+    L.out = out;
+    int rc = launch_dslash(L, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_wk_post, dim3((unsigned)((n * 4 + BLK - 1) / BLK)), dim3(BLK), 0, s,
+                       n * 4, in, tmp, out);
+  } else {
+    L.in = in;
+    L.out = tmp;
+    int rc = launch_dslash(L, s);
+    if (rc) return rc;
+    // zkappa: -am | -i am (dirac.F90:602-606), -am | +i am (615-619)
+    const double2 z1 = imass == 1 ? make_double2(-m, 0.0) : make_double2(0.0, -m);
+    const double2 z2 = imass == 1 ? make_double2(-m, 0.0) : make_double2(0.0, m);
+    hipLaunchKernelGGL(k_wkd_post, dim3(nb), dim3(BLK), 0, s, d, nrhs, in, tmp, out, z1, z2);
+  }
+  THR_HIP(hipGetLastError());
+  return THR_OK;
+}
+
+int launch_derivs(const Dims &d, const double2 *R, const double2 *X2, double anum, int iflag,
+                  double *dsdpi, hipStream_t s) {
+  const float tzi = 2.0f * (float)anum;  // real :: tzi_real = 2*real(anum)
+  const unsigned nb = (unsigned)((d.v4() * 3 + BLK - 1) / BLK);
+  hipLaunchKernelGGL(k_derivs, dim3(nb), dim3(BLK), 0, s, d, R, X2, tzi, iflag, dsdpi);
+  THR_HIP(hipGetLastError());
+  return THR_OK;
+}
+
+}  // namespace thr

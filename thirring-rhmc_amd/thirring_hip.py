@@ -1,0 +1,259 @@
+"""Python front end of libthirring_hip.so (the gfx950 domain-wall hot path).
+
+Mirrors the reference's Fortran subroutine API (dirac.F90, measure_module.F90
+congrad, multishift_module.F90, qmrherm_module.F90, derivs_module.F90,
+gauge.F90 coef) on the reference's own haloed column-major arrays, held as
+numpy arrays in Fortran order, and exposes the device-resident entry points
+for callers that keep fields on the GPU.
+
+There is no CPU fallback: if the shared library is missing or fails to load,
+every call raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "libthirring_hip.so")
+
+F32 = ctypes.c_float
+I32 = ctypes.c_int
+F64 = ctypes.c_double
+P = ctypes.c_void_p
+IP = ctypes.POINTER(ctypes.c_int)
+
+PREC = {"f64": 0, "f32": 1, "f16": 2}
+KERNEL = {"shamir": 0, "wilson": 1}
+
+_SIGS = {
+    "thr_init": ([I32], I32),
+    "thr_finalize": ([], I32),
+    "thr_last_error": ([], ctypes.c_char_p),
+    "thr_coef": ([P, P, P, I32], I32),
+    "thr_dslash": ([P, P, P, P, F32, I32, F32, I32], I32),
+    "thr_dslashd": ([P, P, P, P, F32, I32, F32, I32], I32),
+    "thr_dirac_op_shifted": ([P, P, P, P, F32, I32, F32, F64, I32], I32),
+    "thr_congrad": ([P, P, P, P, F32, F32, I32, F32, I32, I32, IP, IP], I32),
+    "thr_multishift": ([P, P, F32, I32, F32, I32, P, P, P, P, F32, I32, I32, I32, IP, P], I32),
+    "thr_derivs": ([P, P, F64, I32, P, F32, I32, F32, I32, P], I32),
+    "thr_qmrherm": ([P, P, F32, IP, F32, I32, F32, P, P, I32, I32, I32, P, P, I32, I32, P, P, P,
+                     P], I32),
+    "thr_coef_dev": ([P, P, P, I32, P], I32),
+    "thr_links_to_f32_dev": ([P, P, P, P], I32),
+    "thr_dslash_dev": ([P, P, P, P, P, P, I32, P, I32, F32, I32, I32, P], I32),
+    "thr_congrad_dev": ([P, P, P, P, F32, F32, I32, F32, I32, I32, IP, IP, P], I32),
+    "thr_multishift_dev": ([P, P, F32, I32, F32, I32, P, P, P, P, F32, I32, I32, I32, IP, P, P],
+                           I32),
+}
+
+_lib = None
+
+
+class ThrError(RuntimeError):
+    pass
+
+
+def lib():
+    """Loads the HIP library (raises if it is missing: there is no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise ThrError(f"{LIBPATH} is missing: build it with __graft_entry__.build()")
+        L = ctypes.CDLL(LIBPATH)
+        for name, (args, res) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+_initialised = False
+
+
+def init(device=0):
+    global _initialised
+    if not _initialised:
+        _check(lib().thr_init(int(device)), "thr_init")
+        _initialised = True
+
+
+def _check(rc, what):
+    if rc != 0:
+        msg = lib().thr_last_error()
+        raise ThrError(f"{what} failed (status {rc}): {msg.decode() if msg else ''}")
+
+
+def _p(a):
+    if a is None:
+        return None
+    if isinstance(a, int):
+        return ctypes.c_void_p(a)
+    if hasattr(a, "data_ptr"):
+        return ctypes.c_void_p(a.data_ptr())
+    assert a.flags["F_CONTIGUOUS"] or a.ndim == 1, "arrays must be Fortran-contiguous"
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _dims(dims):
+    return (ctypes.c_int * 4)(*[int(d) for d in dims])
+
+
+# ---- host-array drop-in API (reference layouts) ---------------------------
+
+def coef(theta, dims, ibound=-1):
+    init()
+    lx, ly, lt, ls = dims
+    u = np.zeros((lx + 2, ly + 2, lt + 2, 3), dtype=np.complex128, order="F")
+    th = np.asfortranarray(theta, dtype=np.float32)
+    _check(lib().thr_coef(_p(u), _p(th), _dims(dims), int(ibound)), "coef")
+    return u
+
+
+def dslash(r, u, dims, am, imass, am3=1.0, kernel="shamir"):
+    init()
+    phi = np.zeros_like(r)
+    _check(lib().thr_dslash(_p(phi), _p(r), _p(u), _dims(dims), am, int(imass), am3,
+                            KERNEL[kernel]), "dslash")
+    return phi
+
+
+def dslashd(r, u, dims, am, imass, am3=1.0, kernel="shamir"):
+    init()
+    phi = np.zeros_like(r)
+    _check(lib().thr_dslashd(_p(phi), _p(r), _p(u), _dims(dims), am, int(imass), am3,
+                             KERNEL[kernel]), "dslashd")
+    return phi
+
+
+def dirac_op_shifted(xin, u, dims, am, imass, shift=0.0, am3=1.0, kernel="shamir"):
+    init()
+    xout = np.zeros_like(xin)
+    _check(lib().thr_dirac_op_shifted(_p(xout), _p(xin), _p(u), _dims(dims), am, int(imass), am3,
+                                      float(shift), KERNEL[kernel]), "dirac_op_shifted")
+    return xout
+
+
+def congrad(phi, x, u, dims, res, am, imass, am3=1.0, niterc=None, kernel="shamir"):
+    """In place on x (vector::X); returns (itercg, iterations)."""
+    init()
+    lx, ly, lt, ls = dims
+    if niterc is None:
+        niterc = ls * lx * ly * lt
+    it, nx = ctypes.c_int(), ctypes.c_int()
+    _check(lib().thr_congrad(_p(phi), _p(x), _p(u), _dims(dims), res, am, int(imass), am3,
+                             int(niterc), KERNEL[kernel], ctypes.byref(it), ctypes.byref(nx)),
+           "congrad")
+    return it.value, nx.value
+
+
+def multishift(u, dims, am, imass, aden, anum, inp, res, maxcg, am3=1.0, sp=False,
+               kernel="shamir"):
+    init()
+    lx, ly, lt, ls = dims
+    n = len(aden)
+    out = np.zeros((ls, lx, ly, lt, 4, n), dtype=np.complex128, order="F")
+    aden = np.ascontiguousarray(aden, dtype=np.float64)
+    anum = np.ascontiguousarray(anum, dtype=np.float64)
+    cgr = ctypes.c_int()
+    cgrs = np.zeros(n, dtype=np.int32)
+    _check(lib().thr_multishift(_p(u), _dims(dims), am, int(imass), am3, n, _p(aden), _p(anum),
+                                _p(out), _p(inp), res, int(maxcg), int(bool(sp)), KERNEL[kernel],
+                                ctypes.byref(cgr), _p(cgrs)), "multishift")
+    return out, cgr.value, cgrs
+
+
+def derivs(r, x2, anum, iflag, dims, dsdpi, kernel="shamir", am=0.0, imass=3, am3=1.0):
+    init()
+    assert dsdpi.flags["F_CONTIGUOUS"] and dsdpi.dtype == np.float64
+    _check(lib().thr_derivs(_p(r), _p(x2), float(anum), int(iflag), _dims(dims), am, int(imass),
+                            am3, KERNEL[kernel], _p(dsdpi)), "derivs")
+    return dsdpi
+
+
+def qmrherm(phi, res, am, imass, anum, aden, iflag, u, dims, x1=None, phi0=None, dsdpi=None,
+            use_sp=False, max_qmr_iters=50000, am3=1.0, kernel="shamir"):
+    """Returns (x, itercg, x1, phi0, dsdpi, cg_returns) like the oracle."""
+    init()
+    lx, ly, lt, ls = dims
+    n = len(aden)
+    x = np.zeros_like(phi)
+    if x1 is None:
+        x1 = np.zeros((ls, lx, ly, lt, 4, n), dtype=np.complex128, order="F")
+    if phi0 is None:
+        phi0 = np.zeros((ls + 2, lx + 2, ly + 2, lt + 2, 4, 25), dtype=np.complex128, order="F")
+    if dsdpi is None:
+        dsdpi = np.zeros((lx, ly, lt, 3), dtype=np.float64, order="F")
+    anum = np.ascontiguousarray(anum, dtype=np.float64)
+    aden = np.ascontiguousarray(aden, dtype=np.float64)
+    it = ctypes.c_int()
+    cgrs = np.zeros(n, dtype=np.int32)
+    _check(lib().thr_qmrherm(_p(phi), _p(x), res, ctypes.byref(it), am, int(imass), am3,
+                             _p(anum), _p(aden), n, int(iflag), int(bool(use_sp)), _p(u),
+                             _dims(dims), int(max_qmr_iters), KERNEL[kernel], _p(x1), _p(phi0),
+                             _p(dsdpi), _p(cgrs)), "qmrherm")
+    return x, it.value, x1, phi0, dsdpi, cgrs
+
+
+# ---- device-resident API --------------------------------------------------
+
+def _stream(stream):
+    if stream is None:
+        return None
+    if isinstance(stream, int):
+        return ctypes.c_void_p(stream)
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+def coef_dev(links64, theta_dev, dims, ibound=-1, stream=None):
+    init()
+    _check(lib().thr_coef_dev(_p(links64), _p(theta_dev), _dims(dims), int(ibound),
+                              _stream(stream)), "coef_dev")
+
+
+def links_to_f32_dev(links32, links64, dims, stream=None):
+    init()
+    _check(lib().thr_links_to_f32_dev(_p(links32), _p(links64), _dims(dims), _stream(stream)),
+           "links_to_f32_dev")
+
+
+def dslash_dev(out, inp, links, dims, am, imass=3, am3=1.0, dagger=False, prec="f64",
+               in_norm=None, out_norm=None, nrhs=1, stream=None):
+    init()
+    ams = np.ascontiguousarray(np.broadcast_to(np.asarray(am, dtype=np.float32), (nrhs,)))
+    _check(lib().thr_dslash_dev(_p(out), _p(out_norm), _p(inp), _p(in_norm), _p(links),
+                                _dims(dims), int(nrhs), _p(ams), int(imass), am3,
+                                int(bool(dagger)), PREC[prec], _stream(stream)), "dslash_dev")
+
+
+def congrad_dev(phi, x, links64, dims, res, am, imass, am3=1.0, niterc=None, kernel="shamir",
+                stream=None):
+    init()
+    lx, ly, lt, ls = dims
+    if niterc is None:
+        niterc = ls * lx * ly * lt
+    it, nx = ctypes.c_int(), ctypes.c_int()
+    _check(lib().thr_congrad_dev(_p(phi), _p(x), _p(links64), _dims(dims), res, am, int(imass),
+                                 am3, int(niterc), KERNEL[kernel], ctypes.byref(it),
+                                 ctypes.byref(nx), _stream(stream)), "congrad_dev")
+    return it.value, nx.value
+
+
+def multishift_dev(links64, dims, am, imass, aden, anum, out, inp, res, maxcg, am3=1.0, sp=False,
+                   kernel="shamir", stream=None):
+    init()
+    n = len(aden)
+    aden = np.ascontiguousarray(aden, dtype=np.float64)
+    anum = np.ascontiguousarray(anum, dtype=np.float64)
+    cgr = ctypes.c_int()
+    cgrs = np.zeros(n, dtype=np.int32)
+    _check(lib().thr_multishift_dev(_p(links64), _dims(dims), am, int(imass), am3, n, _p(aden),
+                                    _p(anum), _p(out), _p(inp), res, int(maxcg), int(bool(sp)),
+                                    KERNEL[kernel], ctypes.byref(cgr), _p(cgrs), _stream(stream)),
+           "multishift_dev")
+    return cgr.value, cgrs
