@@ -209,7 +209,8 @@ __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L) {
     if (L.mode == EPI_CG) {
       const C a = (C)L.a_ptr[(long)rhs * L.sc_stride];
       T *r = (T *)L.r + roff;
-      Acc<PREC> rr{r, nullptr, v5};
+      T *ro = L.r_out ? (T *)L.r_out + roff : r;
+      Acc<PREC> rr{r, L.r_norm ? L.r_norm + (long)rhs * v5 : nullptr, v5};
       cx<C> rv[4];
       rr.load4(i, rv);
 #pragma unroll
@@ -217,8 +218,8 @@ __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L) {
         rv[k] = rv[k] - a * o[k];
         red += (double)norm2(rv[k]);
       }
-      Out<PREC> ro{r, nullptr, v5};
-      ro.store4(i, rv);
+      Out<PREC> rout{ro, L.r_out_norm ? L.r_out_norm + (long)rhs * v5 : nullptr, v5};
+      rout.store4(i, rv);
       if (L.b_ptr) {
         const C b = (C)L.b_ptr[(long)rhs * L.sc_stride];
         Acc<PREC> pp{(const T *)L.p + roff, nullptr, v5};
@@ -261,7 +262,10 @@ int launch_dslash(const DslashLaunch &L, hipStream_t s) {
   THR_CHECK(L.d.lx > 0 && L.d.ly > 0 && L.d.lt > 0 && L.d.ls > 0, THR_ERR_ARG, "bad dims");
   THR_CHECK(L.imass == 1 || L.imass == 3 || L.imass == 5, THR_ERR_ARG, "imass must be 1, 3 or 5");
   THR_CHECK(L.mode == EPI_STORE || L.partial, THR_ERR_ARG, "partial buffer required");
-  THR_CHECK(L.prec != P16 || L.mode != EPI_CG, THR_ERR_ARG, "fp16 CG epilogue unsupported");
+  THR_CHECK(L.prec != P16 || L.mode != EPI_CG || (L.r_norm && !L.b_ptr &&
+                                                   (L.r_out ? L.r_out_norm != nullptr : true)),
+            THR_ERR_ARG, "fp16 CG epilogue needs r scales and no x update");
+  THR_CHECK(L.prec != P16 || L.mode != EPI_DOTW, THR_ERR_ARG, "fp16 dot epilogue unsupported");
   dim3 grid((unsigned)dslash_blocks(L.d), (unsigned)L.nrhs);
   dim3 block(BLK);
 #define THR_LAUNCH(P, D) hipLaunchKernelGGL((k_dslash<P, D>), grid, block, 0, s, L)

@@ -30,8 +30,11 @@ struct DslashLaunch {
   // epilogue
   int mode;
   const void *w;       // EPI_DOTW
-  void *r, *x;         // EPI_CG
+  void *r, *x;         // EPI_CG: r read (and written unless r_out is set)
   const void *p;       // EPI_CG
+  void *r_out;         // EPI_CG: optional separate destination of the updated r
+  const float *r_norm; // EPI_CG with FP16 storage: per-site scales of r / r_out
+  float *r_out_norm;
   const double *a_ptr; // EPI_CG: r -= a_ptr[rhs*sc_stride] * out (device scalar)
   const double *b_ptr; // EPI_CG: x += b_ptr[rhs*sc_stride] * p  (null: no x update)
   int sc_stride;       // stride (in doubles) between the per-rhs scalars
