@@ -1,19 +1,25 @@
 """Benchmark of the domain-wall hot path on one MI355X.
 
-Headline workload: the Shamir Dslash (imass 3, am3 = 1) at 32^3 x Ls=64,
-FP64 (the mixed-precision production shape), on synthetic U(1) links
-u = 1 + i theta, theta ~ N(0, 1/sqrt(beta)) with beta = 0.3, and unit-variance
-complex Gaussian sources.  One step = one application of M (alternating with
-M^dagger) to a fresh source; inputs rotate over enough buffer sets to exceed
-the 256 MB Infinity Cache, so the figure is HBM bandwidth, not cache.
+Headline (`value`): Shamir Dslash bandwidth, FP64, at 32^3 x Ls=64 (imass 3,
+am3 = 1, am = 0.005) on synthetic U(1) links u = 1 + i theta with
+theta ~ N(0, 1/sqrt(0.3)) and unit-variance complex Gaussian sources.  One
+step = one application of M or M^dagger (alternating) to a fresh source;
+inputs rotate over enough buffer sets to exceed the 256 MB Infinity Cache.
 
-value = algorithmic bytes per application / average kernel duration, with
-algorithmic bytes = 2 * V5 * 4 * 16 + 3 * V4 * 16 (each input read once, the
-output written once).  Kernel durations come from HIP events recorded on the
-stream the kernels are launched on.
+    value = algorithmic bytes per application / mean kernel duration
+    algorithmic bytes = 2 * V5 * 4 * 16 + 3 * V4 * 16   (inputs read once,
+                                                          output written once)
 
-Also reported: FP32 and FP16 operator bandwidth, the multishift solve rates
-(plain FP64 vs the mixed-precision path) and the oracle CPU baseline.
+Kernel durations come from HIP events recorded around every launch on the
+stream the kernels run on.
+
+Also reported in the same JSON line:
+  * FP32 / FP16 operator bandwidth;
+  * multishift solves/s to FP64 true residual 1e-10 for all 25 poles of the
+    x^(-1/4) approximation (remez4 block 2): the plain FP64 GPU path (the
+    reference's recurrences) against the batched, blocked-update,
+    mixed-precision path, with the true residuals of both;
+  * the CPU baseline: the FP64 C oracle's Dslash on the host cores.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -32,47 +38,57 @@ for sub in ("thirring-rhmc_amd", "oracle", "tests"):
 HBM_PEAK_GBS = 8000.0
 
 
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def dslash_bytes(v4, v5, prec, nrhs=1):
     b_c = {"f64": 16, "f32": 8, "f16": 5}[prec]
     b_link = 16 if prec == "f64" else 8
     return nrhs * 2 * v5 * 4 * b_c + 3 * v4 * b_link
 
 
-def make_fields(dims, beta, seed, torch, dev):
-    import thirring_hip as G
+def remez_block(name, block):
+    """read_remez_file (remez_common_subroutines.F90:20-35): a0, then (a_i, s_i)."""
+    path = os.path.join(ROOT, "thirring-rhmc_amd", "data", name)
+    rows = [l.replace(",", " ").split() for l in open(path) if l.strip()]
+    n = (len(rows) - 2) // 2
+    blk = rows[block * (n + 1):(block + 1) * (n + 1)]
+    return (float(blk[0][0]), np.array([float(r[0]) for r in blk[1:]]),
+            np.array([float(r[1]) for r in blk[1:]]))
+
+
+def make_fields(G, DA, dims, beta, seed):
     lx, ly, lt, ls = dims
     rng = np.random.default_rng(seed)
     th = (rng.standard_normal((lx, ly, lt, 3)) / np.sqrt(beta)).astype(np.float32)
-    theta = torch.from_numpy(th.reshape(-1, order="F")).to(dev)
-    l64 = torch.empty(lx * ly * lt * 3, 2, dtype=torch.float64, device=dev)
-    l32 = torch.empty(lx * ly * lt * 3, 2, dtype=torch.float32, device=dev)
+    theta = DA.from_host(th.reshape(-1, order="F"))
+    l64 = DA((lx * ly * lt * 3, 2), np.float64)
+    l32 = DA((lx * ly * lt * 3, 2), np.float32)
     G.coef_dev(l64, theta, dims)
     G.links_to_f32_dev(l32, l64, dims)
     return th, l64, l32
 
 
-def time_dslash(G, torch, dims, links, prec, steps, warmup, nbuf, stream):
+def time_dslash(G, hr, dims, links, prec, steps, warmup, nbuf, stream):
     lx, ly, lt, ls = dims
     v5 = lx * ly * lt * ls
-    nsp = 4 * v5
-    gen = torch.Generator(device="cuda").manual_seed(1)
-    if prec == "f64":
-        mk = lambda: torch.randn(nsp, 2, dtype=torch.float64, device="cuda", generator=gen) / np.sqrt(2)
-    else:
-        mk = lambda: (torch.randn(nsp, 2, dtype=torch.float32, device="cuda", generator=gen) / np.sqrt(2))
+    rng = np.random.default_rng(1)
     ins, outs, inn, outn = [], [], [], []
     for _ in range(nbuf):
-        x = mk()
+        x = (rng.standard_normal((4, v5, 2)) / np.sqrt(2)).astype(np.float32)
         if prec == "f16":
-            sc = x.view(4, v5, 2).abs().amax(dim=(0, 2)).contiguous()
-            x = (x.view(4, v5, 2) / sc[None, :, None]).half().contiguous().view(nsp, 2)
-            inn.append(sc)
-            outn.append(torch.empty_like(sc))
+            sc = np.abs(x).max(axis=(0, 2)).astype(np.float32)
+            ins.append(hr.DeviceArray.from_host((x / sc[None, :, None]).astype(np.float16)))
+            inn.append(hr.DeviceArray.from_host(sc))
+            outn.append(hr.DeviceArray((v5,), np.float32))
+            outs.append(hr.DeviceArray((4, v5, 2), np.float16))
         else:
+            dt = np.float64 if prec == "f64" else np.float32
+            ins.append(hr.DeviceArray.from_host(x.astype(dt)))
+            outs.append(hr.DeviceArray((4, v5, 2), dt))
             inn.append(None)
             outn.append(None)
-        ins.append(x)
-        outs.append(torch.empty_like(x))
 
     def launch(k):
         j = k % nbuf
@@ -81,21 +97,62 @@ def time_dslash(G, torch, dims, links, prec, steps, warmup, nbuf, stream):
 
     for k in range(warmup):
         launch(k)
-    torch.cuda.synchronize()
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-          for _ in range(steps)]
-    t_all0 = torch.cuda.Event(enable_timing=True)
-    t_all1 = torch.cuda.Event(enable_timing=True)
-    t_all0.record(stream)
+    stream.synchronize()
+    ev = [(hr.Event(), hr.Event()) for _ in range(steps)]
+    t0, t1 = hr.Event(), hr.Event()
+    t0.record(stream)
     for k in range(steps):
         ev[k][0].record(stream)
         launch(k)
         ev[k][1].record(stream)
-    t_all1.record(stream)
-    torch.cuda.synchronize()
-    per = np.array([a.elapsed_time(b) for a, b in ev])  # ms
-    region = t_all0.elapsed_time(t_all1) / steps
-    return float(np.mean(per)), float(region)
+    t1.record(stream)
+    stream.synchronize()
+    per = np.array([a.elapsed_ms(b) for a, b in ev])
+    return float(np.mean(per)), float(t0.elapsed_ms(t1) / steps)
+
+
+def bench_multishift(G, hr, dims, l64, l32, am, nsrc, stream, tol=5e-11, maxcg=100000):
+    """Plain FP64 path (one source at a time) vs the batched mixed path."""
+    lx, ly, lt, ls = dims
+    nsp = 4 * lx * ly * lt * ls
+    a0, a, sig = remez_block("remez4", 1)
+    n = len(sig)
+    rng = np.random.default_rng(7)
+    bh = rng.standard_normal((nsrc, nsp, 2)) / np.sqrt(2)
+    b = hr.DeviceArray.from_host(bh)
+    # plain FP64: the reference's recurrences, one source
+    b0 = hr.DeviceArray.from_host(bh[:1])
+    xp = hr.DeviceArray((1, n, nsp, 2), np.float64)
+    progress(f"multishift plain FP64 {dims} am={am}")
+    hr.synchronize()
+    t = time.perf_counter()
+    cg_plain, _ = G.multishift_plain_dev(l64, dims, am, 3, sig, b0, xp, tol=tol, maxcg=maxcg,
+                                         stream=stream)
+    hr.synchronize()
+    t_plain = time.perf_counter() - t
+    rel_plain = G.true_residuals_dev(l64, dims, am, 3, sig, b0, xp, stream=stream)
+    del xp
+    out = {"plain_fp64": {"solves_per_s": round(1.0 / t_plain, 4), "seconds": round(t_plain, 3),
+                          "iterations": int(cg_plain),
+                          "max_true_residual": float(rel_plain.max())}}
+    for label, th32, th16 in (("blocked_fp64", 0.0, 0.0), ("blocked_mixed", 1e-4, 0.0)):
+        x = hr.DeviceArray((nsrc, n, nsp, 2), np.float64)
+        progress(f"multishift {label} x{nsrc} sources")
+        hr.synchronize()
+        t = time.perf_counter()
+        it, cgr, _ = G.multishift_fast_dev(l64, l32, dims, am, 3, sig, b, x, tol=tol, maxcg=maxcg,
+                                           nsrc=nsrc, th32=th32, th16=th16,
+                                           true_residuals=False, stream=stream)
+        hr.synchronize()
+        dt = time.perf_counter() - t
+        rel = G.true_residuals_dev(l64, dims, am, 3, sig, b, x, nsrc=nsrc, stream=stream)
+        out[label] = {"solves_per_s": round(nsrc / dt, 4), "seconds": round(dt, 3),
+                      "sources": nsrc, "iterations": int(it),
+                      "max_true_residual": float(rel.max())}
+        del x
+    out["speedup_mixed_vs_plain_fp64"] = round(out["blocked_mixed"]["solves_per_s"] /
+                                               out["plain_fp64"]["solves_per_s"], 3)
+    return out
 
 
 def cpu_baseline(dims, seconds=12.0):
@@ -130,6 +187,10 @@ def main():
     ap.add_argument("--lattice", default="32,32,32,64")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-solver", action="store_true")
+    ap.add_argument("--solver-lattice", default="16,16,16,48")
+    ap.add_argument("--solver-am", type=float, default=0.01)
+    ap.add_argument("--solver-sources", type=int, default=4)
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run N device copies of 1 GiB (scales the HBM counters)")
     args = ap.parse_args()
@@ -138,35 +199,43 @@ def main():
               "the single-GPU kernel at its real shape", file=sys.stderr)
         sys.exit(2)
 
-    import torch
-    if args.copy_probe:
-        a = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)
-        b = torch.empty_like(a)
-        for _ in range(args.copy_probe):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_probe_bytes_read": a.numel() * 8, "copies": args.copy_probe}))
-        return
+    import hiprt as hr
     import thirring_hip as G
     G.init(0)
-    dev = torch.device("cuda:0")
-    stream = torch.cuda.current_stream()
+    if args.copy_probe:
+        a = hr.DeviceArray((1 << 27,), np.float64)
+        a.zero()
+        b = hr.DeviceArray((1 << 27,), np.float64)
+        for _ in range(args.copy_probe):
+            b.copy_from(a)
+        hr.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": a.nbytes, "copies": args.copy_probe}))
+        return
+    stream = hr.Stream()
     dims = tuple(int(v) for v in args.lattice.split(","))
     lx, ly, lt, ls = dims
     v4, v5 = lx * ly * lt, lx * ly * lt * ls
-    th, l64, l32 = make_fields(dims, 0.3, 1234, torch, dev)
+    th, l64, l32 = make_fields(G, hr.DeviceArray, dims, 0.3, 1234)
 
     t_wall0 = time.time()
-    # enough rotating buffer sets to exceed the 256 MB Infinity Cache
     nbuf64 = max(2, int(np.ceil(4 * 256e6 / (2 * 16 * 4 * v5))))
-    ms64, region64 = time_dslash(G, torch, dims, l64, "f64", args.steps, args.warmup, nbuf64, stream)
-    b64 = dslash_bytes(v4, v5, "f64")
-    gbs64 = b64 / (ms64 * 1e-3) / 1e9
+    progress(f"dslash f64 {dims} x{args.steps}")
+    ms64, region64 = time_dslash(G, hr, dims, l64, "f64", args.steps, args.warmup, nbuf64, stream)
+    gbs64 = dslash_bytes(v4, v5, "f64") / (ms64 * 1e-3) / 1e9
     extra = {}
-    for prec, links in (("f32", l32), ("f16", l32)):
+    for prec in ("f32", "f16"):
         nb = max(2, int(np.ceil(4 * 256e6 / (2 * {"f32": 8, "f16": 5}[prec] * 4 * v5))))
-        ms, reg = time_dslash(G, torch, dims, links, prec, args.steps, args.warmup, nb, stream)
-        extra[prec] = {"ms": round(ms, 5), "GB_s": round(dslash_bytes(v4, v5, prec) / (ms * 1e-3) / 1e9, 1)}
+        progress(f"dslash {prec}")
+        ms, _ = time_dslash(G, hr, dims, l32, prec, args.steps, args.warmup, nb, stream)
+        extra[prec] = {"ms": round(ms, 5),
+                       "GB_s": round(dslash_bytes(v4, v5, prec) / (ms * 1e-3) / 1e9, 1)}
+    solver = None
+    if not args.no_solver:
+        sd = tuple(int(v) for v in args.solver_lattice.split(","))
+        _, s64, s32 = make_fields(G, hr.DeviceArray, sd, 0.3, 4321)
+        solver = bench_multishift(G, hr, sd, s64, s32, args.solver_am, args.solver_sources, stream)
+        solver["lattice"] = list(sd)
+        solver["am"] = args.solver_am
     wall = time.time() - t_wall0
 
     traffic = None
@@ -175,6 +244,7 @@ def main():
         with open(tpath) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_launch")
 
+    progress("cpu baseline")
     cpu = None if args.no_cpu else cpu_baseline(dims, args.cpu_seconds)
 
     line = {
@@ -199,6 +269,7 @@ def main():
                      "unit": "GB/s", "frac": round(gbs64 / HBM_PEAK_GBS, 4), "traffic": traffic},
         "cpu_baseline": cpu,
         "dslash_other_precisions": extra,
+        "multishift": solver,
         "bench_wall_s": round(wall, 2),
     }
     print(json.dumps(line))

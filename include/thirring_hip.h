@@ -128,6 +128,34 @@ int thr_multishift_dev(const void *links64, const int dims[4], float am, int ima
                        const void *input, float res, int maxcg, int use_sp, int kernel,
                        int *cg_return, int *cg_returns, void *stream);
 
+/* Plain FP64 GPU path for comparisons: the reference's multishift recurrences
+ * (multishift_module.F90:13-284) with corr_i = 1 and every shift stopped once
+ * its shifted residual |zeta_i| |r| falls below tol |b|; res is that tol. */
+int thr_multishift_plain_dev(const void *links64, const int dims[4], float am, int imass,
+                             float am3, int ndiagq, const double *aden, const void *input,
+                             void *out, float tol, int maxcg, int *cg_return, int *cg_returns,
+                             void *stream);
+
+/* Batched mixed-precision multishift with blocked shift updates: nsrc sources
+ * b[nsrc][4 V5] (complex128) with masses am[nsrc], the same nshift shifts for
+ * all; solutions x[nsrc][nshift][4 V5] (complex128).  Each shift stops when
+ * |b - (M^+M + aden_i) x_i| (iterated) <= tol |b|; Krylov storage drops to FP32
+ * below |r|/|b| < th32 and to FP16 below th16 (<= 0 disables).  iterations:
+ * iterations run; cg_return[nsrc]: per-source iterations; relres (optional,
+ * nsrc*nshift): true FP64 residuals |b - (M^+M + sigma_i) x_i| / |b|.
+ * Same recurrences as multishift_module.F90:103-282 (see thr_multishift). */
+int thr_multishift_fast_dev(const void *links64, const void *links32, const int dims[4],
+                            int nsrc, const float *am, int imass, float am3, int nshift,
+                            const double *aden, const void *b, void *x, double tol, int maxcg,
+                            double th32, double th16, int *iterations, int *cg_return,
+                            double *relres, void *stream);
+
+/* True FP64 residuals relres[src*nshift + i] = |b - (M^+M + aden_i) x_i| / |b|
+ * for solutions laid out as thr_multishift_fast_dev's x (check of any solver) */
+int thr_true_residuals_dev(const void *links64, const int dims[4], int nsrc, const float *am,
+                           int imass, float am3, int nshift, const double *aden, const void *b,
+                           const void *x, double *relres, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -204,39 +204,39 @@ def test_coef(G):
 
 def test_device_precisions(G):
     """FP32 and FP16 (+per-site scale) operators against FP64 on device buffers."""
-    import torch
+    from hiprt import DeviceArray
     dims = (8, 8, 8, 16)
     lx, ly, lt, ls = dims
+    v5 = lx * ly * lt * ls
     rng = np.random.default_rng(12)
     th = (rng.standard_normal((lx, ly, lt, 3)) / np.sqrt(0.3)).astype(np.float32)
-    dev = torch.device("cuda:0")
-    theta = torch.from_numpy(th.reshape(-1, order="F")).to(dev)
-    l64 = torch.empty(lx * ly * lt * 3, 2, dtype=torch.float64, device=dev)
-    l32 = torch.empty(lx * ly * lt * 3, 2, dtype=torch.float32, device=dev)
+    theta = DeviceArray.from_host(th.reshape(-1, order="F"))
+    l64 = DeviceArray((lx * ly * lt * 3, 2), np.float64)
+    l32 = DeviceArray((lx * ly * lt * 3, 2), np.float32)
     G.coef_dev(l64, theta, dims)
     G.links_to_f32_dev(l32, l64, dims)
-    nsp = 4 * lx * ly * lt * ls
-    x64 = torch.randn(nsp, 2, dtype=torch.float64, device=dev)
-    ref = torch.empty_like(x64)
+    xh = rng.standard_normal((4 * v5, 2))
+    x64 = DeviceArray.from_host(xh)
+    ref = DeviceArray((4 * v5, 2), np.float64)
     G.dslash_dev(ref, x64, l64, dims, 0.05, imass=3)
-    # links and source through the oracle too (haloed layout)
+    rc = ref.to_host()
+    # the same operator through the oracle on the haloed layout
     u = O.coef(th, dims)
-    r = np.zeros((ls + 2, lx + 2, ly + 2, lt + 2, 4), complex, order="F")
-    xc = x64.cpu().numpy()
-    r[1:-1, 1:-1, 1:-1, 1:-1, :] = (xc[:, 0] + 1j * xc[:, 1]).reshape((ls, lx, ly, lt, 4), order="F")
-    refstate.halo5(r)
+    r = D.from_vec(xh[:, 0] + 1j * xh[:, 1], dims)
     o = O.interior(O.dslash(r, u, dims, 0.05, 3)).reshape(-1, order="F")
-    rc = ref.cpu().numpy()
     assert rel(rc[:, 0] + 1j * rc[:, 1], o) <= 1e-13
-    x32 = x64.float()
-    out32 = torch.empty_like(x32)
+    x32 = DeviceArray.from_host(xh.astype(np.float32))
+    out32 = DeviceArray((4 * v5, 2), np.float32)
     G.dslash_dev(out32, x32, l32, dims, 0.05, imass=3, prec="f32")
-    assert (torch.linalg.norm(out32.double() - ref) / torch.linalg.norm(ref)).item() <= 2e-6
-    # fp16: quantise the source the same way the kernel stores it
-    scale = x32.view(4, -1, 2).abs().amax(dim=(0, 2))
-    xh = (x32.view(4, -1, 2) / scale[None, :, None]).half().contiguous()
-    outh = torch.empty_like(xh)
-    on = torch.empty_like(scale)
-    G.dslash_dev(outh, xh, l32, dims, 0.05, imass=3, prec="f16", in_norm=scale, out_norm=on)
-    got = (outh.float() * on[None, :, None]).reshape(-1, 2).double()
-    assert (torch.linalg.norm(got - ref) / torch.linalg.norm(ref)).item() <= 4e-3
+    assert rel(out32.to_host().astype(np.float64), rc) <= 2e-6
+    # fp16 + per-site scale: quantise the source the way the kernel stores it
+    xs = xh.astype(np.float32).reshape(4, v5, 2)
+    scale = np.abs(xs).max(axis=(0, 2)).astype(np.float32)
+    xq = (xs / scale[None, :, None]).astype(np.float16)
+    xhd = DeviceArray.from_host(xq)
+    sd = DeviceArray.from_host(scale)
+    outh = DeviceArray((4, v5, 2), np.float16)
+    on = DeviceArray((v5,), np.float32)
+    G.dslash_dev(outh, xhd, l32, dims, 0.05, imass=3, prec="f16", in_norm=sd, out_norm=on)
+    got = (outh.to_host().astype(np.float32) * on.to_host()[None, :, None]).reshape(-1, 2)
+    assert rel(got.astype(np.float64), rc) <= 4e-3

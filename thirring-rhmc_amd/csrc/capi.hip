@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/thirring_hip.h"
+#include "msfast.h"
 #include "solvers.h"
 
 namespace thr {
@@ -446,6 +447,77 @@ int thr_multishift_dev(const void *links64, const int dims[4], float am, int ima
                                    res, maxcg, cg_return, cg_returns, pick(stream));
   return multishift_reference(op, ndiagq, aden, anum, (const double2 *)input, (double2 *)out, res,
                               maxcg, cg_return, cg_returns, pick(stream));
+}
+
+int thr_multishift_plain_dev(const void *links64, const int dims[4], float am, int imass,
+                             float am3, int ndiagq, const double *aden, const void *input,
+                             void *out, float tol, int maxcg, int *cg_return, int *cg_returns,
+                             void *stream) {
+  TRY(check_dims(dims));
+  TRY(check_op(0, imass));
+  THR_CHECK(links64 && aden && out && input && cg_return, THR_ERR_ARG, "NULL argument");
+  THR_CHECK(ndiagq >= 1 && ndiagq <= MAXSHIFT, THR_ERR_ARG, "ndiagq out of range");
+  std::vector<double> ones(ndiagq, 1.0);
+  const OpDesc op = mkop(mkdims(dims), 0, imass, am, am3, (const double2 *)links64);
+  return multishift_reference(op, ndiagq, aden, ones.data(), (const double2 *)input,
+                              (double2 *)out, tol, maxcg, cg_return, cg_returns, pick(stream), 1);
+}
+
+int thr_multishift_fast_dev(const void *links64, const void *links32, const int dims[4],
+                            int nsrc, const float *am, int imass, float am3, int nshift,
+                            const double *aden, const void *b, void *x, double tol, int maxcg,
+                            double th32, double th16, int *iterations, int *cg_return,
+                            double *relres, void *stream) {
+  TRY(check_dims(dims));
+  TRY(check_op(0, imass));
+  THR_CHECK(links64 && links32 && am && aden && b && x, THR_ERR_ARG, "NULL argument");
+  MsfArgs A{};
+  A.d = mkdims(dims);
+  A.nsrc = nsrc;
+  A.nshift = nshift;
+  A.aden = aden;
+  A.am = am;
+  A.imass = imass;
+  A.am3 = am3;
+  A.u64 = (const double2 *)links64;
+  A.u32 = (const float2 *)links32;
+  A.b = (const double2 *)b;
+  A.x = (double2 *)x;
+  A.tol = tol;
+  A.maxcg = maxcg;
+  A.th32 = th32;
+  A.th16 = th16;
+  MsfResult R{};
+  hipStream_t s = pick(stream);
+  TRY(msfast_solve(A, &R, s));
+  if (iterations) *iterations = R.iterations;
+  if (cg_return)
+    for (int k = 0; k < nsrc; k++) cg_return[k] = R.cg_return[k];
+  if (relres) TRY(msfast_true_residuals(A, relres, s));
+  THR_HIP(hipStreamSynchronize(s));
+  return THR_OK;
+}
+
+int thr_true_residuals_dev(const void *links64, const int dims[4], int nsrc, const float *am,
+                           int imass, float am3, int nshift, const double *aden, const void *b,
+                           const void *x, double *relres, void *stream) {
+  TRY(check_dims(dims));
+  TRY(check_op(0, imass));
+  THR_CHECK(links64 && am && aden && b && x && relres, THR_ERR_ARG, "NULL argument");
+  THR_CHECK(nsrc >= 1 && nsrc <= MAXRHS && nshift >= 1 && nshift <= MAXSHIFT, THR_ERR_ARG,
+            "nsrc or nshift out of range");
+  MsfArgs A{};
+  A.d = mkdims(dims);
+  A.nsrc = nsrc;
+  A.nshift = nshift;
+  A.aden = aden;
+  A.am = am;
+  A.imass = imass;
+  A.am3 = am3;
+  A.u64 = (const double2 *)links64;
+  A.b = (const double2 *)b;
+  A.x = (double2 *)x;
+  return msfast_true_residuals(A, relres, pick(stream));
 }
 
 }  // extern "C"

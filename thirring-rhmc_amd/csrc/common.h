@@ -33,6 +33,12 @@ const char *last_error();
     }                                                                                   \
   } while (0)
 
+#define TRY_RC(x)            \
+  do {                       \
+    int _rc = (x);           \
+    if (_rc) return _rc;     \
+  } while (0)
+
 #define THR_CHECK(cond, code, msg) \
   do {                             \
     if (!(cond)) {                 \

@@ -1,0 +1,47 @@
+// Batched mixed-precision multishift with blocked shift updates.
+#pragma once
+#include "solvers.h"
+
+namespace thr {
+
+constexpr int MSB = 16;  // iterations per shift-update block
+
+struct MsfState {
+  double delta, omega, omega_save, minus_omega, gammag, alpha, lambda, bnorm2, tol2;
+  double zeta_i[MAXSHIFT], zeta_ii[MAXSHIFT], zeta_iii[MAXSHIFT], omegas[MAXSHIFT],
+      gammas[MAXSHIFT];
+  double ca[MAXSHIFT], cc[MAXSHIFT];           // p_i = ca P_i + sum cb R ; x_i = X_i + cc P_i + sum cd R
+  double cb[MAXSHIFT][MSB], cd[MAXSHIFT][MSB];
+  int active[MAXSHIFT], touched[MAXSHIFT], cg_returns[MAXSHIFT];
+  int n, mini, maxi, jcount, cg_return, maxcg, stop, first_block;
+};
+
+struct MsfArgs {
+  Dims d;
+  int nsrc, nshift;
+  const double *aden;  // host, nshift
+  const float *am;     // host, nsrc
+  int imass;
+  float am3;
+  const double2 *u64;  // device links FP64
+  const float2 *u32;   // device links FP32
+  const double2 *b;    // device sources [nsrc][4 V5]
+  double2 *x;          // device solutions [nsrc][nshift][4 V5]
+  double tol;          // per-shift |r_i| <= tol |b|
+  int maxcg;
+  double th32, th16;   // switch Krylov storage to FP32 / FP16 below these |r|/|b| (<= 0: never)
+  int start_prec;      // 0 FP64 (default), 1 FP32, 2 FP16
+};
+
+struct MsfResult {
+  int iterations, blocks;
+  int cg_return[MAXRHS];
+  int cg_returns[MAXRHS * MAXSHIFT];
+  int switches[3];
+};
+
+int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s);
+// relres[src * nshift + i] = |b - (M^+M + sigma_i) x_i| / |b| in FP64
+int msfast_true_residuals(const MsfArgs &A, double *relres, hipStream_t s);
+
+}  // namespace thr

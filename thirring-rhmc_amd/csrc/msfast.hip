@@ -1,0 +1,603 @@
+// Batched, mixed-precision multishift CG with blocked shift updates.
+//
+// Same Krylov recurrences as the reference's Jegerlehner solver
+// (multishift_module.F90:103-282), reorganised for HBM bandwidth:
+//
+//  * alpha = Re<p, M^+M p> = |M p|^2 is taken from the M pass, so omega is
+//    known before M^+ runs and the M^+ pass updates r in its epilogue
+//    (s = M^+ h is never stored).
+//  * Shift vectors are not touched every iteration.  Every p_i and x_i is a
+//    linear combination of its value at the start of a block and the block's
+//    residuals R_0..R_{B-1}; the per-iteration recurrences
+//        x_i -= omega_i p_i ;  p_i = gamma_i p_i + zeta_i R_j
+//    act on the (B+1) FP64 coefficients only, and one flush pass per block
+//    applies them, reading each residual once for all shifts.  Shift traffic
+//    per iteration drops from 4 streams per active shift to (B + 4 k)/B.
+//  * Krylov vectors (residual ring, p, h) are stored in FP64, FP32 or FP16
+//    (half2 + per-site scale); precision is lowered as the residual falls
+//    (matrix-vector errors may grow as |r_k| shrinks), while all scalars,
+//    dot products and the shift solutions x_i, p_i stay FP64.
+//  * Up to 32 sources with per-source masses run in one launch each step.
+//
+// Per-shift convergence: |r_i| = |zeta_i| |r| <= tol |b|.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "msfast.h"
+
+namespace thr {
+
+namespace {
+
+inline unsigned nblk(long n) { return (unsigned)((n + BLK - 1) / BLK); }
+
+template <int P> using TT = typename Store<P>::T;
+template <int P> using CC = typename Store<P>::C;
+
+__device__ inline double fin_sum(const double *partial, int n) {
+  double v = 0.0;
+  for (int k = threadIdx.x; k < n; k += BLK) v += partial[k];
+  return block_sum(v);
+}
+
+template <int P> struct VAcc {
+  const TT<P> *p;
+  const float *nrm;
+  long v5;
+  __device__ inline cx<CC<P>> ld(long e) const {
+    if constexpr (P == P16) {
+      const float2 f = __half22float2(p[e]);
+      const float sc = nrm[e % v5];
+      return {f.x * sc, f.y * sc};
+    } else {
+      return ldc(p, e);
+    }
+  }
+};
+
+__global__ __launch_bounds__(BLK) void k_msf_init(MsfState *st, int n, const double *partial,
+                                                  int nb, double tol, int maxcg) {
+  MsfState *S = st + blockIdx.x;
+  const double b2 = fin_sum(partial + (long)blockIdx.x * nb, nb);
+  if (threadIdx.x == 0) {
+    S->delta = b2;
+    S->bnorm2 = b2;
+    S->lambda = b2;
+    S->tol2 = tol * tol * b2;
+    S->omega = 1.0;
+    S->omega_save = 1.0;
+    S->minus_omega = -1.0;
+    S->gammag = 0.0;
+    S->n = n;
+    for (int i = 0; i < n; i++) {
+      S->zeta_i[i] = 1.0;
+      S->zeta_ii[i] = 1.0;
+      S->zeta_iii[i] = 1.0;
+      S->active[i] = 1;
+      S->touched[i] = 0;
+      S->cg_returns[i] = 0;
+      S->ca[i] = 1.0;
+      S->cc[i] = 0.0;
+      for (int j = 0; j < MSB; j++) S->cb[i][j] = S->cd[i][j] = 0.0;
+    }
+    S->mini = 1;
+    S->maxi = n;
+    S->jcount = 0;
+    S->cg_return = 0;
+    S->maxcg = maxcg;
+    S->first_block = 1;
+    S->stop = (maxcg <= 0 || b2 == 0.0) ? 1 : 0;
+  }
+}
+
+// alpha = |h|^2, omega, zeta''' and omega_i (multishift_module.F90:118-135)
+__global__ __launch_bounds__(BLK) void k_msf_fin1(MsfState *st, const double *aden,
+                                                  const double *partial, int nb) {
+  MsfState *S = st + blockIdx.x;
+  if (S->stop) return;
+  const double al = fin_sum(partial + (long)blockIdx.x * nb, nb);
+  if (threadIdx.x == 0) {
+    S->cg_return += 1;
+    S->alpha = al;
+    S->omega_save = S->omega;
+    S->omega = -S->delta / al;
+    S->minus_omega = -S->omega;
+    const double om = S->omega, os = S->omega_save, gg = S->gammag;
+    for (int i = S->mini - 1; i < S->maxi; i++) {
+      if (!S->active[i]) continue;
+      S->zeta_iii[i] = (S->zeta_i[i] * S->zeta_ii[i] * os) /
+                       (om * gg * (S->zeta_i[i] - S->zeta_ii[i]) +
+                        S->zeta_i[i] * os * (1.0 - aden[i] * om));
+      S->omegas[i] = om * S->zeta_iii[i] / S->zeta_ii[i];
+    }
+  }
+}
+
+// lambda, gamma, gamma_i; block coefficients; convergence (136-176, 230-275)
+__global__ __launch_bounds__(BLK) void k_msf_fin2(MsfState *st, const double *partial, int nb) {
+  MsfState *S = st + blockIdx.x;
+  if (S->stop) return;
+  const double lam = fin_sum(partial + (long)blockIdx.x * nb, nb);
+  if (threadIdx.x == 0) {
+    S->lambda = lam;
+    const double gam = lam / S->delta;
+    S->gammag = gam;
+    const int j = S->jcount;
+    int nmax = 0, nmin = S->n + 1;
+    for (int i = S->mini - 1; i < S->maxi; i++) {
+      if (!S->active[i]) continue;
+      const double zt = S->zeta_iii[i], om = S->omegas[i];
+      const double gi = gam * zt * om / (S->zeta_ii[i] * S->omega);
+      S->gammas[i] = gi;
+      // x_i -= omega_i p_i  with p_i = ca P + sum cb R
+      S->cc[i] -= om * S->ca[i];
+      for (int q = 0; q < j; q++) S->cd[i][q] -= om * S->cb[i][q];
+      // p_i = gamma_i p_i + zeta_i''' R_j
+      S->ca[i] *= gi;
+      for (int q = 0; q < j; q++) S->cb[i][q] *= gi;
+      S->cb[i][j] = zt;
+      S->cd[i][j] = 0.0;
+      S->touched[i] = 1;
+      S->cg_returns[i] = S->cg_return;
+      // shifted residual of the new iterate: |zeta'''| |r_new|
+      if (zt * zt * lam <= S->tol2) {
+        S->active[i] = 0;
+      } else {
+        nmax = max(nmax, i + 1);
+        nmin = min(nmin, i + 1);
+      }
+    }
+    for (int i = nmin - 1; i < nmax; i++) {
+      if (!S->active[i]) continue;
+      S->zeta_i[i] = S->zeta_ii[i];
+      S->zeta_ii[i] = S->zeta_iii[i];
+    }
+    S->mini = nmin;
+    S->maxi = nmax;
+    S->delta = lam;
+    S->jcount = j + 1;
+    if (nmax == 0 || S->cg_return >= S->maxcg) S->stop = 1;
+  }
+}
+
+// p = r_new + gamma p  (per source)
+template <int P>
+__global__ __launch_bounds__(BLK) void k_msf_xpay(long v5, const TT<P> *r, const float *rn,
+                                                  TT<P> *p, float *pn, const MsfState *sts) {
+  const int src = blockIdx.y;
+  const MsfState *S = sts + src;
+  if (S->stop) return;
+  const long i = (long)blockIdx.x * BLK + threadIdx.x;
+  if (i >= v5) return;
+  const long off = (long)src * 4 * v5;
+  using C = CC<P>;
+  const C g = (C)S->gammag;
+  cx<C> v[4];
+  if constexpr (P == P16) {
+    const float sr = rn[(long)src * v5 + i], sp = pn[(long)src * v5 + i];
+    float m = 0.f;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const float2 a = __half22float2(r[off + k * v5 + i]);
+      const float2 b = __half22float2(p[off + k * v5 + i]);
+      v[k] = {a.x * sr + g * b.x * sp, a.y * sr + g * b.y * sp};
+      m = fmaxf(m, fmaxf(fabsf(v[k].r), fabsf(v[k].i)));
+    }
+    const float inv = m > 0.f ? 1.f / m : 0.f;
+    pn[(long)src * v5 + i] = m;
+#pragma unroll
+    for (int k = 0; k < 4; k++) p[off + k * v5 + i] = __floats2half2_rn(v[k].r * inv, v[k].i * inv);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const long e = off + k * v5 + i;
+      st(p, e, ldc(r, e) + g * ldc(p, e));
+    }
+  }
+}
+
+// apply the block's coefficients to x_i and p_i (FP64), reading each residual once
+template <int P>
+__global__ __launch_bounds__(BLK) void k_msf_flush(long v5, int n, TT<P> *const *ring,
+                                                   float *const *ring_n, const double2 *b,
+                                                   double2 *X, double2 *Pv,
+                                                   const MsfState *sts, int final_flush) {
+  const int src = blockIdx.y;
+  const MsfState *S = sts + src;
+  const int jc = S->jcount;
+  if (jc == 0) return;
+  __shared__ double sa[MAXSHIFT], sc[MAXSHIFT], sb[MAXSHIFT][MSB], sd[MAXSHIFT][MSB];
+  __shared__ int stouch[MAXSHIFT], sact[MAXSHIFT];
+  for (int k = threadIdx.x; k < n; k += BLK) {
+    sa[k] = S->ca[k];
+    sc[k] = S->cc[k];
+    stouch[k] = S->touched[k];
+    sact[k] = S->active[k] && !S->stop && !final_flush;
+    for (int q = 0; q < MSB; q++) {
+      sb[k][q] = S->cb[k][q];
+      sd[k][q] = S->cd[k][q];
+    }
+  }
+  __syncthreads();
+  const long ns = 4 * v5;
+  const long e = (long)blockIdx.x * BLK + threadIdx.x;
+  if (e >= ns) return;
+  const long off = (long)src * ns;
+  cx<double> R[MSB];
+#pragma unroll
+  for (int q = 0; q < MSB; q++) {
+    if (q < jc) {
+      VAcc<P> a{ring[q] + off, ring_n ? ring_n[q] + (long)src * v5 : nullptr, v5};
+      const cx<CC<P>> v = a.ld(e);
+      R[q] = {(double)v.r, (double)v.i};
+    } else {
+      R[q] = {0.0, 0.0};
+    }
+  }
+  const int first = S->first_block;
+  const cx<double> bv = ldc(b, off + e);
+  for (int i = 0; i < n; i++) {
+    if (!stouch[i]) continue;
+    const long xo = ((long)src * n + i) * ns + e;
+    const cx<double> base = first ? bv : ldc(Pv, xo);
+    cx<double> xs = first ? cx<double>{0.0, 0.0} : ldc(X, xo);
+    xs = xs + sc[i] * base;
+#pragma unroll
+    for (int q = 0; q < MSB; q++)
+      if (q < jc) xs = xs + sd[i][q] * R[q];
+    st(X, xo, xs);
+    if (sact[i]) {
+      cx<double> ps = sa[i] * base;
+#pragma unroll
+      for (int q = 0; q < MSB; q++)
+        if (q < jc) ps = ps + sb[i][q] * R[q];
+      st(Pv, xo, ps);
+    }
+  }
+}
+
+__global__ void k_msf_reset(MsfState *st) {
+  MsfState *S = st + blockIdx.x;
+  if (threadIdx.x != 0) return;
+  if (S->jcount == 0) return;
+  for (int i = 0; i < S->n; i++) {
+    S->ca[i] = 1.0;
+    S->cc[i] = 0.0;
+    S->touched[i] = 0;
+    for (int q = 0; q < MSB; q++) S->cb[i][q] = S->cd[i][q] = 0.0;
+  }
+  S->jcount = 0;
+  S->first_block = 0;
+}
+
+// partial |b - (M^+M + sigma) x|^2 per source: t holds M^+M x
+__global__ __launch_bounds__(BLK) void k_true_res(long v5, const double2 *b, const double2 *t,
+                                                  const double2 *x, double sigma, long xstride,
+                                                  double *partial) {
+  const int src = blockIdx.y;
+  const long i = (long)blockIdx.x * BLK + threadIdx.x;
+  double red = 0.0;
+  if (i < v5) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const long e = (long)src * 4 * v5 + k * v5 + i;
+      const cx<double> r = ldc(b, e) - ldc(t, e) - sigma * ldc(x, (long)src * xstride + k * v5 + i);
+      red += norm2(r);
+    }
+  }
+  const double tot = block_sum(red);
+  if (threadIdx.x == 0) partial[(long)src * gridDim.x + blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(BLK) void k_sum_rows(const double *partial, int nb, double *out) {
+  const double v = fin_sum(partial + (long)blockIdx.x * nb, nb);
+  if (threadIdx.x == 0) out[blockIdx.x] = v;
+}
+
+// precision conversion of nsrc batched spinors
+template <int SP, int DP>
+__global__ __launch_bounds__(BLK) void k_conv(long v5, int nsrc, const TT<SP> *a, const float *an,
+                                              TT<DP> *b, float *bn) {
+  const long i = (long)blockIdx.x * BLK + threadIdx.x;
+  const int src = blockIdx.y;
+  if (i >= v5) return;
+  const long off = (long)src * 4 * v5;
+  cx<float> vf[4];
+  cx<double> vd[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    if constexpr (SP == P16) {
+      const float2 f = __half22float2(a[off + k * v5 + i]);
+      const float s = an[(long)src * v5 + i];
+      vd[k] = {(double)(f.x * s), (double)(f.y * s)};
+    } else {
+      const auto v = ldc(a, off + k * v5 + i);
+      vd[k] = {(double)v.r, (double)v.i};
+    }
+    vf[k] = {(float)vd[k].r, (float)vd[k].i};
+  }
+  if constexpr (DP == P16) {
+    float m = 0.f;
+#pragma unroll
+    for (int k = 0; k < 4; k++) m = fmaxf(m, fmaxf(fabsf(vf[k].r), fabsf(vf[k].i)));
+    const float inv = m > 0.f ? 1.f / m : 0.f;
+    bn[(long)src * v5 + i] = m;
+#pragma unroll
+    for (int k = 0; k < 4; k++) b[off + k * v5 + i] = __floats2half2_rn(vf[k].r * inv, vf[k].i * inv);
+  } else if constexpr (DP == P32) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) st(b, off + k * v5 + i, vf[k]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; k++) st(b, off + k * v5 + i, vd[k]);
+  }
+}
+
+__global__ __launch_bounds__(BLK) void k_norm_src(long v5, const double2 *x, double *partial) {
+  const int src = blockIdx.y;
+  const long i = (long)blockIdx.x * BLK + threadIdx.x;
+  double red = 0.0;
+  if (i < v5) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) red += norm2(ldc(x, (long)src * 4 * v5 + k * v5 + i));
+  }
+  const double t = block_sum(red);
+  if (threadIdx.x == 0) partial[(long)src * gridDim.x + blockIdx.x] = t;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host driver
+// ---------------------------------------------------------------------------
+static int conv(const Dims &d, int nsrc, int sp, const void *a, const float *an, int dp, void *b,
+                float *bn, hipStream_t s) {
+  dim3 g(nblk(d.v5()), nsrc);
+  const long v5 = d.v5();
+#define CV(A, B) hipLaunchKernelGGL((k_conv<A, B>), g, dim3(BLK), 0, s, v5, nsrc, (const TT<A> *)a, an, (TT<B> *)b, bn)
+  if (sp == P64 && dp == P64) CV(P64, P64);
+  else if (sp == P64 && dp == P32) CV(P64, P32);
+  else if (sp == P64 && dp == P16) CV(P64, P16);
+  else if (sp == P32 && dp == P64) CV(P32, P64);
+  else if (sp == P32 && dp == P32) CV(P32, P32);
+  else if (sp == P32 && dp == P16) CV(P32, P16);
+  else if (sp == P16 && dp == P64) CV(P16, P64);
+  else if (sp == P16 && dp == P32) CV(P16, P32);
+  else CV(P16, P16);
+#undef CV
+  THR_HIP(hipGetLastError());
+  return THR_OK;
+}
+
+int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
+  const Dims &d = A.d;
+  const int n = A.nshift, ns_ = A.nsrc;
+  THR_CHECK(n >= 1 && n <= MAXSHIFT, THR_ERR_ARG, "number of shifts out of range (1..32)");
+  THR_CHECK(ns_ >= 1 && ns_ <= MAXRHS, THR_ERR_ARG, "number of sources out of range (1..32)");
+  THR_CHECK(A.imass == 1 || A.imass == 3 || A.imass == 5, THR_ERR_ARG, "imass must be 1, 3 or 5");
+  THR_CHECK(A.u64 && A.u32 && A.b && A.x, THR_ERR_ARG, "NULL argument");
+  const long v5 = d.v5(), nsp = 4 * v5, nsb = nsp * ns_;
+  const int nb = (int)dslash_blocks(d);
+  // buffers (workspace slots 200+)
+  size_t bytes_p[3] = {16, 8, 4};
+  void *ring[3][MSB];
+  float *ringn[MSB];
+  void *pbuf[3], *hbuf[3], *r0[3];
+  float *pn = nullptr, *hn = nullptr, *r0n = nullptr;
+  for (int pr = 0; pr < 3; pr++) {
+    const bool need = (pr == P64) || (pr == P32 && A.th32 > 0) || (pr == P16 && A.th16 > 0);
+    for (int q = 0; q < MSB; q++)
+      ring[pr][q] = need ? workspace(200 + pr * 40 + q, bytes_p[pr] * nsb) : nullptr;
+    pbuf[pr] = need ? workspace(200 + pr * 40 + 20, bytes_p[pr] * nsb) : nullptr;
+    hbuf[pr] = need ? workspace(200 + pr * 40 + 21, bytes_p[pr] * nsb) : nullptr;
+    r0[pr] = need ? workspace(200 + pr * 40 + 22, bytes_p[pr] * nsb) : nullptr;
+    if (need)
+      THR_CHECK(pbuf[pr] && hbuf[pr] && r0[pr], THR_ERR_HIP, "workspace allocation failed");
+  }
+  if (A.th16 > 0) {
+    for (int q = 0; q < MSB; q++) ringn[q] = (float *)workspace(340 + q, sizeof(float) * v5 * ns_);
+    pn = (float *)workspace(360, sizeof(float) * v5 * ns_);
+    hn = (float *)workspace(361, sizeof(float) * v5 * ns_);
+    r0n = (float *)workspace(362, sizeof(float) * v5 * ns_);
+  }
+  double2 *Pv = (double2 *)workspace(363, sizeof(double2) * nsp * ns_ * n);
+  MsfState *st = (MsfState *)workspace(364, sizeof(MsfState) * ns_);
+  double *aden = (double *)workspace(365, sizeof(double) * MAXSHIFT);
+  double *partial = (double *)workspace(366, sizeof(double) * nb * ns_);
+  void **ringptr = (void **)workspace(367, sizeof(void *) * 3 * MSB + sizeof(float *) * MSB);
+  double *sums = (double *)workspace(368, sizeof(double) * ns_);
+  THR_CHECK(Pv && st && aden && partial && ringptr && sums, THR_ERR_HIP, "workspace allocation failed");
+  {
+    std::vector<void *> hp(3 * MSB + MSB);
+    for (int pr = 0; pr < 3; pr++)
+      for (int q = 0; q < MSB; q++) hp[pr * MSB + q] = ring[pr][q];
+    for (int q = 0; q < MSB; q++) hp[3 * MSB + q] = A.th16 > 0 ? (void *)ringn[q] : nullptr;
+    THR_HIP(hipMemcpyAsync(ringptr, hp.data(), sizeof(void *) * hp.size(), hipMemcpyHostToDevice, s));
+    double ad[MAXSHIFT] = {0};
+    for (int i = 0; i < n; i++) ad[i] = A.aden[i];
+    THR_HIP(hipMemcpyAsync(aden, ad, sizeof(ad), hipMemcpyHostToDevice, s));
+    THR_HIP(hipStreamSynchronize(s));
+  }
+  // |b|^2 per source, state init
+  hipLaunchKernelGGL(k_norm_src, dim3(nb, ns_), dim3(BLK), 0, s, v5, A.b, partial);
+  hipLaunchKernelGGL(k_msf_init, dim3(ns_), dim3(BLK), 0, s, st, n, partial, nb, A.tol, A.maxcg);
+  THR_HIP(hipGetLastError());
+
+  THR_HIP(hipMemsetAsync(A.x, 0, sizeof(double2) * nsp * ns_ * n, s));
+  int prec = P64;
+  if (A.start_prec > 0) prec = A.start_prec;
+  THR_CHECK(prec == P64 || (prec == P32 && A.th32 > 0) || (prec == P16 && A.th16 > 0), THR_ERR_ARG,
+            "start precision needs its buffers (set th32/th16)");
+  // r0 and p in the starting precision
+  TRY_RC(conv(d, ns_, P64, A.b, nullptr, prec, r0[prec], prec == P16 ? r0n : nullptr, s));
+  TRY_RC(conv(d, ns_, P64, A.b, nullptr, prec, pbuf[prec], prec == P16 ? pn : nullptr, s));
+
+  DslashLaunch L{};
+  L.d = d;
+  L.nrhs = ns_;
+  L.imass = A.imass;
+  L.am3 = A.am3;
+  for (int k = 0; k < ns_; k++) L.am[k] = A.am[k];
+  L.sc_stride = (int)(sizeof(MsfState) / sizeof(double));
+  L.done = &st->stop;
+  L.done_stride = (int)(sizeof(MsfState) / sizeof(int));
+  L.partial = partial;
+
+  std::vector<MsfState> hs(ns_);
+  long t = 0;  // completed iterations
+  int rc = THR_OK;
+  int blocks = 0;
+  for (;;) {
+    for (int j = 0; j < MSB; j++, t++) {
+      void *rin = (t == 0) ? r0[prec] : ring[prec][(t - 1) % MSB];
+      float *rinn = (prec == P16) ? ((t == 0) ? r0n : ringn[(t - 1) % MSB]) : nullptr;
+      void *rout = ring[prec][t % MSB];
+      float *routn = (prec == P16) ? ringn[t % MSB] : nullptr;
+      // h = M p ; alpha = |h|^2
+      DslashLaunch a = L;
+      a.prec = prec;
+      a.links = prec == P64 ? (const void *)A.u64 : (const void *)A.u32;
+      a.dagger = 0;
+      a.in = pbuf[prec];
+      a.in_norm = prec == P16 ? pn : nullptr;
+      a.out = hbuf[prec];
+      a.out_norm = prec == P16 ? hn : nullptr;
+      a.mode = EPI_NORM;
+      TRY_RC(launch_dslash(a, s));
+      hipLaunchKernelGGL(k_msf_fin1, dim3(ns_), dim3(BLK), 0, s, st, aden, partial, nb);
+      // r_new = r + omega M^+ h ; lambda = |r_new|^2
+      DslashLaunch b = L;
+      b.prec = prec;
+      b.links = a.links;
+      b.dagger = 1;
+      b.in = hbuf[prec];
+      b.in_norm = a.out_norm;
+      b.mode = EPI_CG;
+      b.r = rin;
+      b.r_norm = rinn;
+      b.r_out = rout;
+      b.r_out_norm = routn;
+      b.a_ptr = &st->minus_omega;
+      b.b_ptr = nullptr;
+      TRY_RC(launch_dslash(b, s));
+      hipLaunchKernelGGL(k_msf_fin2, dim3(ns_), dim3(BLK), 0, s, st, partial, nb);
+      dim3 gx(nblk(v5), ns_);
+      if (prec == P64)
+        hipLaunchKernelGGL(k_msf_xpay<P64>, gx, dim3(BLK), 0, s, v5, (const double2 *)rout, nullptr,
+                           (double2 *)pbuf[P64], nullptr, st);
+      else if (prec == P32)
+        hipLaunchKernelGGL(k_msf_xpay<P32>, gx, dim3(BLK), 0, s, v5, (const float2 *)rout, nullptr,
+                           (float2 *)pbuf[P32], nullptr, st);
+      else
+        hipLaunchKernelGGL(k_msf_xpay<P16>, gx, dim3(BLK), 0, s, v5, (const __half2 *)rout, routn,
+                           (__half2 *)pbuf[P16], pn, st);
+      THR_HIP(hipGetLastError());
+    }
+    // flush the block into x_i, p_i
+    dim3 gf(nblk(nsp), ns_);
+    void *const *rp = (void *const *)ringptr + prec * MSB;
+    float *const *rnp = (float *const *)((void **)ringptr + 3 * MSB);
+    if (prec == P64)
+      hipLaunchKernelGGL(k_msf_flush<P64>, gf, dim3(BLK), 0, s, v5, n, (double2 *const *)rp,
+                         nullptr, A.b, A.x, Pv, st, 0);
+    else if (prec == P32)
+      hipLaunchKernelGGL(k_msf_flush<P32>, gf, dim3(BLK), 0, s, v5, n, (float2 *const *)rp,
+                         nullptr, A.b, A.x, Pv, st, 0);
+    else
+      hipLaunchKernelGGL(k_msf_flush<P16>, gf, dim3(BLK), 0, s, v5, n, (__half2 *const *)rp, rnp,
+                         A.b, A.x, Pv, st, 0);
+    hipLaunchKernelGGL(k_msf_reset, dim3(ns_), dim3(1), 0, s, st);
+    THR_HIP(hipGetLastError());
+    blocks++;
+    THR_HIP(hipMemcpyAsync(hs.data(), st, sizeof(MsfState) * ns_, hipMemcpyDeviceToHost, s));
+    THR_HIP(hipStreamSynchronize(s));
+    bool all = true;
+    double worst = 0.0;
+    for (int k = 0; k < ns_; k++) {
+      all = all && hs[k].stop;
+      if (!hs[k].stop) worst = std::max(worst, std::sqrt(hs[k].lambda / hs[k].bnorm2));
+    }
+    if (all) break;
+    if (t > (long)A.maxcg + 2 * MSB) {
+      set_error("multishift: stop flag not raised");
+      return THR_ERR_STATE;
+    }
+    // lower the Krylov precision as the residual falls
+    int want = prec;
+    if (A.th32 > 0 && worst < A.th32 && prec == P64) want = P32;
+    if (A.th16 > 0 && worst < A.th16 && prec != P16) want = P16;
+    if (want != prec) {
+      void *rl = ring[prec][(t - 1) % MSB];
+      float *rln = prec == P16 ? ringn[(t - 1) % MSB] : nullptr;
+      TRY_RC(conv(d, ns_, prec, rl, rln, want, ring[want][(t - 1) % MSB],
+                  want == P16 ? ringn[(t - 1) % MSB] : nullptr, s));
+      TRY_RC(conv(d, ns_, prec, pbuf[prec], prec == P16 ? pn : nullptr, want, pbuf[want],
+                  want == P16 ? pn : nullptr, s));
+      prec = want;
+      if (res) res->switches[prec] = (int)t;
+    }
+  }
+  if (res) {
+    res->iterations = (int)t;
+    res->blocks = blocks;
+    for (int k = 0; k < ns_; k++) {
+      res->cg_return[k] = hs[k].cg_return;
+      for (int i = 0; i < n; i++) res->cg_returns[k * MAXSHIFT + i] = hs[k].cg_returns[i];
+    }
+  }
+  return rc;
+}
+
+int msfast_true_residuals(const MsfArgs &A, double *relres, hipStream_t s) {
+  const Dims &d = A.d;
+  const int n = A.nshift, ns_ = A.nsrc;
+  const long v5 = d.v5(), nsp = 4 * v5;
+  const int nb = (int)dslash_blocks(d);
+  double2 *t1 = (double2 *)workspace(370, sizeof(double2) * nsp * ns_);
+  double2 *t2 = (double2 *)workspace(371, sizeof(double2) * nsp * ns_);
+  double *partial = (double *)workspace(366, sizeof(double) * nb * ns_);
+  double *sums = (double *)workspace(368, sizeof(double) * ns_);
+  double2 *xs = (double2 *)workspace(372, sizeof(double2) * nsp * ns_);
+  THR_CHECK(t1 && t2 && partial && sums && xs, THR_ERR_HIP, "workspace allocation failed");
+  std::vector<double> bn(ns_), hsum(ns_);
+  hipLaunchKernelGGL(k_norm_src, dim3(nb, ns_), dim3(BLK), 0, s, v5, A.b, partial);
+  hipLaunchKernelGGL(k_sum_rows, dim3(ns_), dim3(BLK), 0, s, partial, nb, sums);
+  THR_HIP(hipMemcpyAsync(bn.data(), sums, sizeof(double) * ns_, hipMemcpyDeviceToHost, s));
+  THR_HIP(hipStreamSynchronize(s));
+  DslashLaunch L{};
+  L.d = d;
+  L.nrhs = ns_;
+  L.prec = P64;
+  L.imass = A.imass;
+  L.am3 = A.am3;
+  for (int k = 0; k < ns_; k++) L.am[k] = A.am[k];
+  L.links = A.u64;
+  L.mode = EPI_STORE;
+  L.sc_stride = 1;
+  L.done_stride = 1;
+  for (int i = 0; i < n; i++) {
+    // gather shift i of every source into one batch
+    for (int k = 0; k < ns_; k++)
+      THR_HIP(hipMemcpyAsync(xs + k * nsp, A.x + ((long)k * n + i) * nsp, sizeof(double2) * nsp,
+                             hipMemcpyDeviceToDevice, s));
+    DslashLaunch a = L;
+    a.in = xs;
+    a.out = t1;
+    TRY_RC(launch_dslash(a, s));
+    DslashLaunch b = L;
+    b.dagger = 1;
+    b.in = t1;
+    b.out = t2;
+    TRY_RC(launch_dslash(b, s));
+    hipLaunchKernelGGL(k_true_res, dim3(nb, ns_), dim3(BLK), 0, s, v5, A.b, t2, xs, A.aden[i],
+                       nsp, partial);
+    hipLaunchKernelGGL(k_sum_rows, dim3(ns_), dim3(BLK), 0, s, partial, nb, sums);
+    THR_HIP(hipMemcpyAsync(hsum.data(), sums, sizeof(double) * ns_, hipMemcpyDeviceToHost, s));
+    THR_HIP(hipStreamSynchronize(s));
+    for (int k = 0; k < ns_; k++) relres[k * n + i] = std::sqrt(hsum[k] / bn[k]);
+  }
+  return THR_OK;
+}
+
+}  // namespace thr

@@ -28,9 +28,11 @@ int cg_reference(const OpDesc &op, const double2 *phi, double2 *x, float res, in
 
 // reference-faithful multishift CG (multishift_module.F90:13-284), FP64 vectors.
 // out: [n][4*V5] shift solutions.
+// plain = 1: corr_i = 1 and the threshold is res*|b| (stop when the shifted
+// residual |zeta_i||r| < res |b|), the rule used for the FP64 comparison path
 int multishift_reference(const OpDesc &op, int n, const double *aden, const double *anum,
                          const double2 *input, double2 *out, float res, int maxcg,
-                         int *cg_return, int *cg_returns, hipStream_t s);
+                         int *cg_return, int *cg_returns, hipStream_t s, int plain = 0);
 
 // FP32-vector multishift with FP64 scalars (multishift_module.F90:289-534);
 // input/output FP64 device fields.

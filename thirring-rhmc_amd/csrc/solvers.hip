@@ -231,8 +231,9 @@ struct MsState {
   double zeta_i[MAXSHIFT], zeta_ii[MAXSHIFT], zeta_iii[MAXSHIFT], omegas[MAXSHIFT],
       gammas[MAXSHIFT], corr[MAXSHIFT], aden[MAXSHIFT];
   int flags[MAXSHIFT], cg_returns[MAXSHIFT];
-  int n, mini, maxi, umin, umax, cg_return, maxcg, stop, done;
+  int n, mini, maxi, umin, umax, cg_return, maxcg, stop, done, plain;
   float res;
+  double resd;  // threshold used by the convergence test
 };
 
 __global__ __launch_bounds__(BLK) void k_ms_init(MsState *st, const double *partial, int nb) {
@@ -256,6 +257,9 @@ __global__ __launch_bounds__(BLK) void k_ms_init(MsState *st, const double *part
     st->umax = 0;
     st->stop = (st->maxcg <= 0 || st->n <= 0) ? 1 : 0;
     st->done = st->stop;
+    // reference rule: sqrt(delta zeta^2) corr_i < res; plain rule (corr_i = 1):
+    // shifted residual below res |b|
+    st->resd = st->plain ? (double)st->res * sqrt(d) : (double)st->res;
   }
 }
 
@@ -295,7 +299,7 @@ __global__ void k_ms_lambda(MsState *st, const double *partial, int nb) {
     int nmax = 0, nmin = st->n + 1;
     for (int i = 0; i < st->n; i++) {
       if (st->flags[i]) {
-        if (sqrt(st->delta * st->zeta_ii[i] * st->zeta_ii[i]) * st->corr[i] < (double)st->res) {
+        if (sqrt(st->delta * st->zeta_ii[i] * st->zeta_ii[i]) * st->corr[i] < st->resd) {
           st->flags[i] = 0;
         } else {
           nmax = max(nmax, i + 1);
@@ -504,7 +508,7 @@ int cg_reference(const OpDesc &op, const double2 *phi, double2 *x, float res, in
 }
 
 static int ms_setup(MsState *st, int n, const double *aden, const double *anum, float res,
-                    int maxcg, hipStream_t s) {
+                    int maxcg, hipStream_t s, int plain = 0) {
   THR_CHECK(n >= 1 && n <= MAXSHIFT, THR_ERR_ARG, "number of shifts out of range");
   MsState h;
   std::memset(&h, 0, sizeof(h));
@@ -519,9 +523,10 @@ static int ms_setup(MsState *st, int n, const double *aden, const double *anum, 
   }
   const double gm = std::exp(lsum / n);
   for (int i = 0; i < n; i++) {
-    h.corr[i] = h.corr[i] / gm;
+    h.corr[i] = plain ? 1.0 : h.corr[i] / gm;
     h.aden[i] = aden[i];
   }
+  h.plain = plain;
   THR_HIP(hipMemcpyAsync(st, &h, sizeof(h), hipMemcpyHostToDevice, s));
   THR_HIP(hipStreamSynchronize(s));  // h is a stack object
   return THR_OK;
@@ -529,7 +534,7 @@ static int ms_setup(MsState *st, int n, const double *aden, const double *anum, 
 
 int multishift_reference(const OpDesc &op, int n, const double *aden, const double *anum,
                          const double2 *input, double2 *out, float res, int maxcg,
-                         int *cg_return, int *cg_returns, hipStream_t s) {
+                         int *cg_return, int *cg_returns, hipStream_t s, int plain) {
   const Dims &d = op.d;
   const long ns = d.nspinor();
   const int nb = (int)dslash_blocks(d);
@@ -541,7 +546,7 @@ int multishift_reference(const OpDesc &op, int n, const double *aden, const doub
   MsState *st = (MsState *)workspace(25, sizeof(MsState));
   double *partial = partial_buffer(nb);
   THR_CHECK(r && h && sv && p && ps && st && partial, THR_ERR_HIP, "workspace allocation failed");
-  int rc = ms_setup(st, n, aden, anum, res, maxcg, s);
+  int rc = ms_setup(st, n, aden, anum, res, maxcg, s, plain);
   if (rc) return rc;
   THR_HIP(hipMemcpyAsync(r, input, sizeof(double2) * ns, hipMemcpyDeviceToDevice, s));
   THR_HIP(hipMemcpyAsync(p, input, sizeof(double2) * ns, hipMemcpyDeviceToDevice, s));

@@ -1,0 +1,143 @@
+"""Minimal ctypes binding to the HIP runtime: device buffers, copies, streams
+and events for the device-resident entry points of libthirring_hip.so.
+
+It loads the same libamdhip64 that libthirring_hip.so links against, so one
+HIP runtime serves the whole process.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_hip = None
+
+
+def hip():
+    global _hip
+    if _hip is None:
+        import thirring_hip
+        thirring_hip.lib()  # the library's own runtime dependency loads first
+        for cand in ("libamdhip64.so.7", "/opt/rocm/lib/libamdhip64.so"):
+            try:
+                _hip = ctypes.CDLL(cand)
+                break
+            except OSError:
+                continue
+        if _hip is None:
+            raise RuntimeError("libamdhip64 not found")
+        V = ctypes.c_void_p
+        S = ctypes.c_size_t
+        sig = {
+            "hipMalloc": ([ctypes.POINTER(V), S], ctypes.c_int),
+            "hipFree": ([V], ctypes.c_int),
+            "hipMemcpy": ([V, V, S, ctypes.c_int], ctypes.c_int),
+            "hipMemset": ([V, ctypes.c_int, S], ctypes.c_int),
+            "hipDeviceSynchronize": ([], ctypes.c_int),
+            "hipStreamCreate": ([ctypes.POINTER(V)], ctypes.c_int),
+            "hipStreamSynchronize": ([V], ctypes.c_int),
+            "hipEventCreate": ([ctypes.POINTER(V)], ctypes.c_int),
+            "hipEventRecord": ([V, V], ctypes.c_int),
+            "hipEventSynchronize": ([V], ctypes.c_int),
+            "hipEventElapsedTime": ([ctypes.POINTER(ctypes.c_float), V, V], ctypes.c_int),
+            "hipEventDestroy": ([V], ctypes.c_int),
+            "hipGetErrorString": ([ctypes.c_int], ctypes.c_char_p),
+        }
+        for k, (a, r) in sig.items():
+            f = getattr(_hip, k)
+            f.argtypes = a
+            f.restype = r
+    return _hip
+
+
+def check(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"{what}: {hip().hipGetErrorString(rc).decode()}")
+
+
+H2D, D2H, D2D = 1, 2, 3
+
+
+class DeviceArray:
+    """A device allocation with a numpy dtype/shape view for copies."""
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(int(s) for s in np.atleast_1d(shape))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        check(hip().hipMalloc(ctypes.byref(p), max(self.nbytes, 1)), "hipMalloc")
+        self.ptr = p.value
+
+    def data_ptr(self):
+        return self.ptr
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        check(hip().hipMemcpy(ctypes.c_void_p(d.ptr), a.ctypes.data_as(ctypes.c_void_p),
+                              d.nbytes, H2D), "hipMemcpy H2D")
+        return d
+
+    def to_host(self):
+        a = np.empty(self.shape, dtype=self.dtype)
+        check(hip().hipMemcpy(a.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(self.ptr),
+                              self.nbytes, D2H), "hipMemcpy D2H")
+        return a
+
+    def copy_from(self, other):
+        assert other.nbytes == self.nbytes
+        check(hip().hipMemcpy(ctypes.c_void_p(self.ptr), ctypes.c_void_p(other.ptr), self.nbytes,
+                              D2D), "hipMemcpy D2D")
+
+    def zero(self):
+        check(hip().hipMemset(ctypes.c_void_p(self.ptr), 0, self.nbytes), "hipMemset")
+
+    def free(self):
+        if self.ptr:
+            hip().hipFree(ctypes.c_void_p(self.ptr))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def synchronize():
+    check(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+
+class Stream:
+    def __init__(self):
+        p = ctypes.c_void_p()
+        check(hip().hipStreamCreate(ctypes.byref(p)), "hipStreamCreate")
+        self.ptr = p.value
+
+    def synchronize(self):
+        check(hip().hipStreamSynchronize(ctypes.c_void_p(self.ptr)), "hipStreamSynchronize")
+
+
+class Event:
+    def __init__(self):
+        p = ctypes.c_void_p()
+        check(hip().hipEventCreate(ctypes.byref(p)), "hipEventCreate")
+        self.ptr = p.value
+
+    def record(self, stream=None):
+        s = ctypes.c_void_p(stream.ptr if stream is not None else None)
+        check(hip().hipEventRecord(ctypes.c_void_p(self.ptr), s), "hipEventRecord")
+
+    def elapsed_ms(self, end):
+        check(hip().hipEventSynchronize(ctypes.c_void_p(end.ptr)), "hipEventSynchronize")
+        ms = ctypes.c_float()
+        check(hip().hipEventElapsedTime(ctypes.byref(ms), ctypes.c_void_p(self.ptr),
+                                        ctypes.c_void_p(end.ptr)), "hipEventElapsedTime")
+        return float(ms.value)
+
+    def __del__(self):
+        try:
+            hip().hipEventDestroy(ctypes.c_void_p(self.ptr))
+        except Exception:
+            pass

@@ -45,6 +45,10 @@ _SIGS = {
     "thr_congrad_dev": ([P, P, P, P, F32, F32, I32, F32, I32, I32, IP, IP, P], I32),
     "thr_multishift_dev": ([P, P, F32, I32, F32, I32, P, P, P, P, F32, I32, I32, I32, IP, P, P],
                            I32),
+    "thr_true_residuals_dev": ([P, P, I32, P, I32, F32, I32, P, P, P, P, P], I32),
+    "thr_multishift_plain_dev": ([P, P, F32, I32, F32, I32, P, P, P, F32, I32, IP, P, P], I32),
+    "thr_multishift_fast_dev": ([P, P, P, I32, P, I32, F32, I32, P, P, P, F64, I32, F64, F64, IP,
+                                 P, P, P], I32),
 }
 
 _lib = None
@@ -207,7 +211,7 @@ def _stream(stream):
         return None
     if isinstance(stream, int):
         return ctypes.c_void_p(stream)
-    return ctypes.c_void_p(stream.cuda_stream)
+    return ctypes.c_void_p(stream.ptr)
 
 
 def coef_dev(links64, theta_dev, dims, ibound=-1, stream=None):
@@ -257,3 +261,49 @@ def multishift_dev(links64, dims, am, imass, aden, anum, out, inp, res, maxcg, a
                                     KERNEL[kernel], ctypes.byref(cgr), _p(cgrs), _stream(stream)),
            "multishift_dev")
     return cgr.value, cgrs
+
+
+def multishift_plain_dev(links64, dims, am, imass, aden, inp, out, tol, maxcg, am3=1.0,
+                         stream=None):
+    """Plain FP64 path: reference recurrences, stop when |zeta_i||r| < tol |b|."""
+    init()
+    n = len(aden)
+    aden = np.ascontiguousarray(aden, dtype=np.float64)
+    cgr = ctypes.c_int()
+    cgrs = np.zeros(n, dtype=np.int32)
+    _check(lib().thr_multishift_plain_dev(_p(links64), _dims(dims), am, int(imass), am3, n,
+                                          _p(aden), _p(inp), _p(out), tol, int(maxcg),
+                                          ctypes.byref(cgr), _p(cgrs), _stream(stream)),
+           "multishift_plain_dev")
+    return cgr.value, cgrs
+
+
+def multishift_fast_dev(links64, links32, dims, am, imass, aden, b, x, tol, maxcg, nsrc=1,
+                        th32=0.0, th16=0.0, am3=1.0, true_residuals=True, stream=None):
+    """Batched blocked-update multishift; returns (iterations, cg_return[nsrc], relres)."""
+    init()
+    n = len(aden)
+    aden = np.ascontiguousarray(aden, dtype=np.float64)
+    ams = np.ascontiguousarray(np.broadcast_to(np.asarray(am, dtype=np.float32), (nsrc,)))
+    it = ctypes.c_int()
+    cgr = np.zeros(nsrc, dtype=np.int32)
+    rel = np.zeros(nsrc * n, dtype=np.float64) if true_residuals else None
+    _check(lib().thr_multishift_fast_dev(_p(links64), _p(links32), _dims(dims), int(nsrc),
+                                         _p(ams), int(imass), am3, n, _p(aden), _p(b), _p(x),
+                                         float(tol), int(maxcg), float(th32), float(th16),
+                                         ctypes.byref(it), _p(cgr), _p(rel), _stream(stream)),
+           "multishift_fast_dev")
+    return it.value, cgr, (rel.reshape(nsrc, n) if rel is not None else None)
+
+
+def true_residuals_dev(links64, dims, am, imass, aden, b, x, nsrc=1, am3=1.0, stream=None):
+    """|b - (M^+M + sigma_i) x_i| / |b| in FP64 for x laid out [nsrc][nshift][4 V5]."""
+    init()
+    n = len(aden)
+    aden = np.ascontiguousarray(aden, dtype=np.float64)
+    ams = np.ascontiguousarray(np.broadcast_to(np.asarray(am, dtype=np.float32), (nsrc,)))
+    rel = np.zeros(nsrc * n, dtype=np.float64)
+    _check(lib().thr_true_residuals_dev(_p(links64), _dims(dims), int(nsrc), _p(ams), int(imass),
+                                        am3, n, _p(aden), _p(b), _p(x), _p(rel), _stream(stream)),
+           "true_residuals_dev")
+    return rel.reshape(nsrc, n)
