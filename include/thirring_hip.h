@@ -54,6 +54,17 @@ int thr_init(int device);
 int thr_finalize(void);
 const char *thr_last_error(void);
 
+/* Solver contexts: each owns its device scratch buffers and its stream.  A
+ * thread that binds a context (NULL: the default one) runs every later call
+ * in it, so concurrent solves on different threads share no scratch memory.
+ * No counterpart in the reference (one MPI rank per process there); used by
+ * the serving loop to overlap independent requests on one GPU. */
+int thr_context_create(void **ctx);
+int thr_context_destroy(void *ctx);
+int thr_context_bind(void *ctx);
+/* device bytes currently held by the bound context's scratch buffers */
+int thr_context_bytes(unsigned long long *bytes);
+
 /* ---- (1) host-array drop-in entry points -------------------------------- */
 
 /* gauge.F90:12-43  coef(u, theta): u = 1 + i theta, u(:,:,Lt,3) *= -1 when

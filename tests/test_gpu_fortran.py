@@ -1,0 +1,95 @@
+"""The Fortran drop-in boundary end to end: reference-style Fortran programs
+(tests/fortran/*.F90, built against thirring-rhmc_amd/fortran/shims.F90)
+call dslash/dslashd, congrad, qmrherm and derivs with the reference's
+subroutine signatures and haloed arrays; their outputs are compared with the
+oracle on the same closed-form inputs (tests/test_utils.F90:35-151).
+
+Tolerances: dslash relative L2 <= 1e-13; congrad itercg == 27 and X relative
+L2 <= 1e-10; qmrherm itercg == 2 (max_qmr_iters = 2) and x, Phi0, dSdpi
+relative L2 <= 1e-10; derivs <= 1e-12.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+import refstate
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "thirring-rhmc_amd", "fortran", "_build")
+DIMS = (12, 12, 12, 24)
+TAU = 8.0 * np.arctan(1.0)
+
+
+def run(name, outdir):
+    exe = os.path.join(BIN, name)
+    assert os.path.exists(exe), f"{exe} not built (run __graft_entry__.build())"
+    p = subprocess.run([exe, str(outdir)], capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stdout + p.stderr
+    return p.stdout
+
+
+def load(path, shape, dtype=np.complex128):
+    return np.fromfile(path, dtype=dtype).reshape(shape, order="F")
+
+
+def rel(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def check(a, b, tol):
+    e = rel(a, b)
+    if not e <= tol:
+        raise AssertionError(f"relative L2 error {e:.3e} > {tol:.1e}")
+
+
+def test_fortran_dslash(tmp_path):
+    run("test_dslash_drop_in", tmp_path)
+    st = refstate.starting_state(DIMS, want=("r", "u"))
+    shape = (24, 12, 12, 12, 4)
+    for imass in (1, 3, 5):
+        g = load(tmp_path / f"dslash_M_{imass}.bin", shape)
+        o = O.interior(O.dslash(st["r"], st["u"], DIMS, 0.05, imass))
+        check(g, o, 1e-13)
+        g = load(tmp_path / f"dslash_Md_{imass}.bin", shape)
+        o = O.interior(O.dslashd(st["r"], st["u"], DIMS, 0.05, imass))
+        check(g, o, 1e-13)
+def test_fortran_congrad(tmp_path):
+    out = run("test_congrad_drop_in", tmp_path)
+    assert "PASS itercg" in out and "27" in out, out
+    # the oracle on the Fortran program's own inputs (its exp() may differ from
+    # numpy's in the last bit, which CG amplifies)
+    phi = load(tmp_path / "congrad_phi.bin", (26, 14, 14, 14, 4))
+    x = load(tmp_path / "congrad_x0.bin", (26, 14, 14, 14, 4))
+    u = load(tmp_path / "congrad_u.bin", (14, 14, 14, 3))
+    it, _ = O.congrad(phi, x, u, DIMS, res=0.1, am=0.05, imass=3)
+    assert it == 27
+    g = load(tmp_path / "congrad_x.bin", (24, 12, 12, 12, 4))
+    check(g, O.interior(x), 1e-10)
+def test_fortran_qmrherm_and_derivs(tmp_path):
+    out = run("test_qmrherm_drop_in", tmp_path)
+    assert out.count("PASS itercg") == 4, out
+    st = refstate.starting_state(DIMS, want=("phi", "r", "x", "u", "phi0", "dsdpi"))
+    i = np.arange(1, 26)
+    anum = np.concatenate([[0.5], np.cos(i * TAU / 25)])
+    aden = np.cos(0.5 * i * TAU / 25)
+    for iflag in range(4):
+        p0 = st["phi0"].copy(order="F")
+        ds = np.asfortranarray(st["dsdpi"].copy())
+        x, it, x1, p0, ds, _ = O.qmrherm(st["phi"], 0.1, 0.05, 3, anum, aden, iflag, st["u"], DIMS,
+                                         phi0=p0, dsdpi=ds, max_qmr_iters=2)
+        assert it == 2
+        gx = load(tmp_path / f"qmrherm_x_{iflag}.bin", (24, 12, 12, 12, 4))
+        check(gx, O.interior(x), 1e-10)
+        gp = load(tmp_path / f"qmrherm_phi0_{iflag}.bin", (24, 12, 12, 12, 4, 25))
+        check(gp, p0[1:-1, 1:-1, 1:-1, 1:-1], 1e-10)
+        gd = load(tmp_path / f"qmrherm_dsdpi_{iflag}.bin", (12, 12, 12, 3), np.float64)
+        check(gd, ds, 1e-10)
+    ds = np.asfortranarray(st["dsdpi"].copy())
+    O.derivs(st["r"], st["x"], TAU, 0, DIMS, ds)
+    gd = load(tmp_path / "derivs_shamir.bin", (12, 12, 12, 3), np.float64)
+    check(gd - st["dsdpi"], ds - st["dsdpi"], 1e-12)

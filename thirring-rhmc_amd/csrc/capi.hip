@@ -113,6 +113,16 @@ static int host_op(double *phi, const double *r, const double *u, const int dims
   return THR_OK;
 }
 
+static int derivs_dispatch(const Dims &d, int kernel, const double2 *R, const double2 *X2,
+                           double anum, int iflag, float am, int imass, double *f, hipStream_t s) {
+  if (kernel == 1) {
+    void *pb = workspace(W_D, (size_t)SLICE_PAIR_BYTES * (2 * d.ls + 4));
+    THR_CHECK(pb, THR_ERR_HIP, "workspace allocation failed");
+    return launch_derivs_wilson(d, R, X2, anum, iflag, am, imass, f, pb, s);
+  }
+  return launch_derivs(d, R, X2, anum, iflag, f, s);
+}
+
 }  // namespace thr
 
 using namespace thr;
@@ -137,6 +147,29 @@ int thr_finalize(void) {
 }
 
 const char *thr_last_error(void) { return last_error(); }
+
+int thr_context_create(void **ctx) {
+  THR_CHECK(ctx, THR_ERR_ARG, "NULL argument");
+  *ctx = (void *)context_create();
+  return THR_OK;
+}
+
+int thr_context_destroy(void *ctx) {
+  THR_CHECK(ctx, THR_ERR_ARG, "NULL context");
+  context_destroy((Context *)ctx);
+  return THR_OK;
+}
+
+int thr_context_bind(void *ctx) {
+  context_bind((Context *)ctx);
+  return THR_OK;
+}
+
+int thr_context_bytes(unsigned long long *bytes) {
+  THR_CHECK(bytes, THR_ERR_ARG, "NULL argument");
+  *bytes = (unsigned long long)workspace_bytes();
+  return THR_OK;
+}
 
 int thr_coef(double *u, const float *theta, const int dims[4], int ibound) {
   TRY(check_dims(dims));
@@ -259,11 +292,9 @@ int thr_multishift(const double *u, const int dims[4], float am, int imass, floa
 
 int thr_derivs(const double *r, const double *x2, double anum, int iflag, const int dims[4],
                float am, int imass, float am3, int kernel, double *dsdpi) {
-  (void)am;
-  (void)imass;
   (void)am3;
   TRY(check_dims(dims));
-  THR_CHECK(kernel == 0, THR_ERR_ARG, "device derivs: Shamir kernel only");
+  TRY(check_op(kernel, imass));
   THR_CHECK(r && x2 && dsdpi, THR_ERR_ARG, "NULL argument");
   const Dims d = mkdims(dims);
   hipStream_t s = default_stream();
@@ -275,7 +306,7 @@ int thr_derivs(const double *r, const double *x2, double anum, int iflag, const 
   TRY(stage_in(d, r, W_H0, a, s));
   TRY(stage_in(d, x2, W_H1, b, s));
   THR_HIP(hipMemcpyAsync(f, dsdpi, sizeof(double) * d.v4() * 3, hipMemcpyHostToDevice, s));
-  TRY(launch_derivs(d, a, b, anum, iflag, f, s));
+  TRY(derivs_dispatch(d, kernel, a, b, anum, iflag, am, imass, f, s));
   THR_HIP(hipMemcpyAsync(dsdpi, f, sizeof(double) * d.v4() * 3, hipMemcpyDeviceToHost, s));
   THR_HIP(hipStreamSynchronize(s));
   return THR_OK;
@@ -295,8 +326,7 @@ int thr_qmrherm(const double *phi, double *x, float res, int *itercg, float am, 
   THR_CHECK(iflag >= 0 && iflag <= 3, THR_ERR_ARG, "iflag must be 0..3");
   THR_CHECK(iflag != 1 || phi0, THR_ERR_ARG, "iflag 1 needs the Phi0 register");
   THR_CHECK(iflag < 2 || dsdpi, THR_ERR_ARG, "iflag 2/3 need dSdpi");
-  THR_CHECK(iflag != 3 || (phi0 && kernel == 0), THR_ERR_ARG, "iflag 3 needs Phi0 (Shamir)");
-  THR_CHECK(iflag != 2 || kernel == 0, THR_ERR_ARG, "device derivs: Shamir kernel only");
+  THR_CHECK(iflag != 3 || phi0, THR_ERR_ARG, "iflag 3 needs the Phi0 register");
   const Dims d = mkdims(dims);
   hipStream_t s = default_stream();
   double2 *u64;
@@ -365,13 +395,13 @@ int thr_qmrherm(const double *phi, double *x, float res, int *itercg, float am, 
     const double2 *xk = x1 + n * k;
     TRY(apply_op(op, 0, xk, b, s));  // x2 = M x1(k)  (143)
     if (iflag == 2) {
-      TRY(launch_derivs(d, xk, b, anum[k + 1], 0, f, s));  // (150-155)
+      TRY(derivs_dispatch(d, kernel, xk, b, anum[k + 1], 0, am, imass, f, s));  // (150-155)
     } else {
       const double coeff = -anum[k + 1];
       TRY(stage_in(d, phi0 + 2 * nh * k, W_H0, c, s));       // R = Phi0(k) (159)
-      TRY(launch_derivs(d, c, b, coeff, 0, f, s));            // (163)
+      TRY(derivs_dispatch(d, kernel, c, b, coeff, 0, am, imass, f, s));  // (163)
       TRY(apply_op(op, 0, c, b, s));                          // x2 = M R (166)
-      TRY(launch_derivs(d, b, xk, coeff, 1, f, s));           // (181)
+      TRY(derivs_dispatch(d, kernel, b, xk, coeff, 1, am, imass, f, s));  // (181)
     }
   }
   THR_HIP(hipMemcpyAsync(dsdpi, f, sizeof(double) * d.v4() * 3, hipMemcpyDeviceToHost, s));

@@ -70,4 +70,11 @@ int launch_coef(const Dims &d, const float *theta, int ibound, double2 *links, h
 int launch_derivs(const Dims &d, const double2 *R, const double2 *X2, double anum, int iflag,
                   double *dsdpi, hipStream_t s);
 
+// Wilson-kernel force (derivs_module.F90:41-97, 198-410): shared part plus the
+// s-coupling slice terms; pair_buf is device scratch for 2*Ls pair records
+int launch_derivs_wilson(const Dims &d, const double2 *R, const double2 *X2, double anum,
+                         int iflag, float am, int imass, double *dsdpi, void *pair_buf,
+                         hipStream_t s);
+constexpr int SLICE_PAIR_BYTES = 48;
+
 }  // namespace thr

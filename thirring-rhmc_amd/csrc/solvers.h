@@ -17,7 +17,13 @@ struct OpDesc {
   const float2 *u32;  // FP32 copy (may be null when only FP64 is used)
 };
 
-// growable device workspace, keyed by slot
+// growable device workspace, keyed by slot, owned by the calling thread's
+// context (or the default context)
+struct Context;
+Context *context_create();
+void context_destroy(Context *c);
+void context_bind(Context *c);
+size_t workspace_bytes();
 void *workspace(int slot, size_t bytes);
 double *partial_buffer(int nrhs_blocks);
 hipStream_t default_stream();

@@ -19,21 +19,47 @@
 namespace thr {
 
 // ---------------------------------------------------------------------------
-// workspace
+// workspace: one set of device buffers and one stream per context; a thread
+// works in the context it bound (thr_context_bind), else in the default one,
+// so independent solves on different threads never share scratch memory
 // ---------------------------------------------------------------------------
 namespace {
 struct Buf {
   void *p = nullptr;
   size_t n = 0;
 };
-std::map<int, Buf> g_ws;
-std::mutex g_ws_mu;
-hipStream_t g_stream = nullptr;
 }  // namespace
 
+struct Context {
+  std::map<int, Buf> ws;
+  std::mutex mu;
+  hipStream_t stream = nullptr;
+};
+
+namespace {
+Context g_default;
+thread_local Context *t_ctx = nullptr;
+Context &cur() { return t_ctx ? *t_ctx : g_default; }
+}  // namespace
+
+Context *context_create() { return new Context(); }
+
+void context_destroy(Context *c) {
+  if (!c) return;
+  (void)hipDeviceSynchronize();
+  for (auto &kv : c->ws)
+    if (kv.second.p) (void)hipFree(kv.second.p);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (t_ctx == c) t_ctx = nullptr;
+  delete c;
+}
+
+void context_bind(Context *c) { t_ctx = c; }
+
 void *workspace(int slot, size_t bytes) {
-  std::lock_guard<std::mutex> lk(g_ws_mu);
-  Buf &b = g_ws[slot];
+  Context &c = cur();
+  std::lock_guard<std::mutex> lk(c.mu);
+  Buf &b = c.ws[slot];
   if (b.n < bytes) {
     if (b.p) (void)hipFree(b.p);
     b.p = nullptr;
@@ -48,20 +74,30 @@ void *workspace(int slot, size_t bytes) {
   return b.p;
 }
 
+size_t workspace_bytes() {
+  Context &c = cur();
+  std::lock_guard<std::mutex> lk(c.mu);
+  size_t t = 0;
+  for (auto &kv : c.ws) t += kv.second.n;
+  return t;
+}
+
 double *partial_buffer(int nblocks) {
   return (double *)workspace(0, sizeof(double) * (size_t)nblocks);
 }
 
 hipStream_t default_stream() {
-  if (!g_stream) (void)hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking);
-  return g_stream;
+  Context &c = cur();
+  if (!c.stream) (void)hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking);
+  return c.stream;
 }
 
 void release_workspace() {
-  std::lock_guard<std::mutex> lk(g_ws_mu);
-  for (auto &kv : g_ws)
+  Context &c = cur();
+  std::lock_guard<std::mutex> lk(c.mu);
+  for (auto &kv : c.ws)
     if (kv.second.p) (void)hipFree(kv.second.p);
-  g_ws.clear();
+  c.ws.clear();
 }
 
 // ---------------------------------------------------------------------------

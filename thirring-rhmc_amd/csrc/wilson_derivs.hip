@@ -13,6 +13,8 @@
 // Force, derivs_module.F90:99-196: for every link (x, mu) the sum over s and
 // spin of the Wilson and Dirac parts of d/dtheta Re<R, M^(+) X2>, with the
 // float32 coefficient tzi_real = 2*real(anum).
+#include <vector>
+
 #include "kernels.h"
 
 namespace thr {
@@ -124,7 +126,101 @@ __global__ __launch_bounds__(BLK) void k_derivs(const Dims d, const double2 *__r
   ds[i] += acc;
 }
 
+// Wilson-kernel extra force terms (derivs_module.F90:41-97, 198-410): the
+// s-couplings of the Wilson-kernel operator, each a derivSlice() between a
+// slice of R and a slice of X2 with spin masks and a complex multiplier.
+struct SlicePair {
+  int sl, sr;       // 0-based s of the R slice and the X2 slice
+  int l0, l1;       // R spins kept [l0, l1)
+  int r0, r1;       // X2 spins kept [r0, r1)
+  double cr, ci;    // cmult
+};
+
+__global__ __launch_bounds__(BLK) void k_derivs_wilson_extra(const Dims d, const double2 *__restrict__ R,
+                                                             const double2 *__restrict__ X2,
+                                                             float tzi, int iflag,
+                                                             const SlicePair *pairs, int npairs,
+                                                             double *ds) {
+  const long v4 = d.v4(), v5 = d.v5();
+  const long i = (long)blockIdx.x * BLK + threadIdx.x;
+  if (i >= v4 * 3) return;
+  const int mu = (int)(i / v4);
+  const long s4 = i % v4;
+  const int x = (int)(s4 % d.lx), y = (int)((s4 / d.lx) % d.ly), t = (int)(s4 / ((long)d.lx * d.ly));
+  int xf = x, yf = y, tf = t;
+  if (mu == 0) xf = (x + 1 == d.lx) ? 0 : x + 1;
+  if (mu == 1) yf = (y + 1 == d.ly) ? 0 : y + 1;
+  if (mu == 2) tf = (t + 1 == d.lt) ? 0 : t + 1;
+  const long f4 = xf + (long)d.lx * (yf + (long)d.ly * tf);
+  const double w1 = (double)(tzi * 0.5f);
+  const double w2 = (double)(tzi * (iflag == 1 ? -1.0f : 1.0f));
+  double acc = ds[i];
+  for (int k = 0; k < npairs; k++) {
+    const SlicePair P = pairs[k];
+    const cx<double> cm{P.cr, P.ci};
+    const long l0 = s4 * d.ls + P.sl, l1 = f4 * d.ls + P.sl;
+    const long r0 = s4 * d.ls + P.sr, r1 = f4 * d.ls + P.sr;
+    for (int a = 0; a < 4; a++) {
+      const bool la = a >= P.l0 && a < P.l1;
+      const cx<double> zero{0.0, 0.0};
+      const cx<double> L0 = la ? ldc(R, a * v5 + l0) : zero, L1 = la ? ldc(R, a * v5 + l1) : zero;
+      const bool ra = a >= P.r0 && a < P.r1;
+      const cx<double> Ra0 = ra ? ldc(X2, a * v5 + r0) : zero, Ra1 = ra ? ldc(X2, a * v5 + r1) : zero;
+      const double v = cmul(cm, cjmul(L0, Ra1)).i - cmul(cm, cjmul(L1, Ra0)).i;
+      acc += w1 * v;
+      const int b = kGin[mu][a];
+      const bool rb = b >= P.r0 && b < P.r1;
+      const cx<double> Rb0 = rb ? ldc(X2, b * v5 + r0) : zero, Rb1 = rb ? ldc(X2, b * v5 + r1) : zero;
+      const cx<double> g{0.5 * kGre[mu][a], 0.5 * kGim[mu][a]};
+      acc += w2 * cmul(cmul(cm, g), cjmul(L0, Rb1) + cjmul(L1, Rb0)).i;
+    }
+  }
+  ds[i] = acc;
+}
+
 }  // namespace
+
+int launch_derivs_wilson(const Dims &d, const double2 *R, const double2 *X2, double anum,
+                         int iflag, float am, int imass, double *dsdpi, void *pair_buf,
+                         hipStream_t s) {
+  THR_CHECK(imass == 1 || imass == 3, THR_ERR_ARG, "Wilson kernel supports imass 1 and 3");
+  int rc = launch_derivs(d, R, X2, anum, iflag, dsdpi, s);
+  if (rc) return rc;
+  const int L = d.ls;
+  std::vector<SlicePair> pr;
+  const double m = (double)am;
+  // derivsWilsonMass (derivs_module.F90:198-249): cmplx(-am,0) | cmplx(0,-am), then
+  // cmplx(-am,0) | cmplx(0,am)
+  const double c1r = imass == 1 ? -m : 0.0, c1i = imass == 1 ? 0.0 : -m;
+  const double c2r = imass == 1 ? -m : 0.0, c2i = imass == 1 ? 0.0 : m;
+  if (iflag == 1) {
+    pr.push_back({0, L - 1, 0, 4, 0, 2, c1r, c1i});   // DP: L all, R spins 1:2
+    pr.push_back({L - 1, 0, 0, 4, 2, 4, c2r, c2i});   // DM: L all, R spins 3:4
+  } else if (iflag == 0) {
+    pr.push_back({0, L - 1, 2, 4, 0, 4, c1r, c1i});   // MD: L spins 3:4, R all
+    pr.push_back({L - 1, 0, 0, 2, 0, 4, c2r, c2i});   // PD: L spins 1:2, R all
+  }
+  // s-couplings (derivs_module.F90:73-81)
+  for (int il = 0; il < L - 1; il++) {
+    if (iflag == 1) {
+      pr.push_back({il, il + 1, 0, 4, 2, 4, 1.0, 0.0});  // DM
+      pr.push_back({il + 1, il, 0, 4, 0, 2, 1.0, 0.0});  // DP
+    } else if (iflag == 0) {
+      pr.push_back({il, il + 1, 0, 2, 0, 4, 1.0, 0.0});  // PD
+      pr.push_back({il + 1, il, 2, 4, 0, 4, 1.0, 0.0});  // MD
+    }
+  }
+  if (pr.empty()) return THR_OK;
+  THR_HIP(hipMemcpyAsync(pair_buf, pr.data(), sizeof(SlicePair) * pr.size(), hipMemcpyHostToDevice, s));
+  const float tzi = 2.0f * (float)anum;
+  const unsigned nb = (unsigned)((d.v4() * 3 + BLK - 1) / BLK);
+  hipLaunchKernelGGL(k_derivs_wilson_extra, dim3(nb), dim3(BLK), 0, s, d, R, X2, tzi, iflag,
+                     (const SlicePair *)pair_buf, (int)pr.size(), dsdpi);
+  THR_HIP(hipGetLastError());
+  // pr must stay alive until the copy is done
+  THR_HIP(hipStreamSynchronize(s));
+  return THR_OK;
+}
 
 int launch_dslash_wilson(const Dims &d, int nrhs, int dagger, int imass, float am, float am3,
                          const double2 *in, double2 *out, const double2 *links, double2 *tmp,

@@ -30,6 +30,10 @@ _SIGS = {
     "thr_init": ([I32], I32),
     "thr_finalize": ([], I32),
     "thr_last_error": ([], ctypes.c_char_p),
+    "thr_context_create": ([ctypes.POINTER(P)], I32),
+    "thr_context_destroy": ([P], I32),
+    "thr_context_bind": ([P], I32),
+    "thr_context_bytes": ([ctypes.POINTER(ctypes.c_ulonglong)], I32),
     "thr_coef": ([P, P, P, I32], I32),
     "thr_dslash": ([P, P, P, P, F32, I32, F32, I32], I32),
     "thr_dslashd": ([P, P, P, P, F32, I32, F32, I32], I32),
@@ -106,6 +110,33 @@ def _p(a):
 
 def _dims(dims):
     return (ctypes.c_int * 4)(*[int(d) for d in dims])
+
+
+class Context:
+    """Private scratch buffers + stream for one thread of work."""
+
+    def __init__(self):
+        init()
+        h = ctypes.c_void_p()
+        _check(lib().thr_context_create(ctypes.byref(h)), "thr_context_create")
+        self.handle = h.value
+
+    def bind(self):
+        _check(lib().thr_context_bind(ctypes.c_void_p(self.handle)), "thr_context_bind")
+
+    @staticmethod
+    def unbind():
+        _check(lib().thr_context_bind(None), "thr_context_bind")
+
+    def scratch_bytes(self):
+        v = ctypes.c_ulonglong()
+        _check(lib().thr_context_bytes(ctypes.byref(v)), "thr_context_bytes")
+        return int(v.value)
+
+    def destroy(self):
+        if self.handle:
+            _check(lib().thr_context_destroy(ctypes.c_void_p(self.handle)), "thr_context_destroy")
+            self.handle = None
 
 
 # ---- host-array drop-in API (reference layouts) ---------------------------
