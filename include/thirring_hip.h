@@ -113,6 +113,25 @@ int thr_qmrherm(const double *phi, double *x, float res, int *itercg, float am, 
                 int use_sp, const double *u, const int dims[4], int max_qmr_iters, int kernel,
                 double *x1, double *phi0, double *dsdpi, int *cg_returns);
 
+/* force_module.F90:9-113  force(Phi, res1, am, imass) for Nf = 1 (rational
+ * branch): four qmrherm calls with the 12-pole guidance coefficients
+ * (anum4g/aden4g, bnum2g/bden2g, anum2g/aden2g, remezg), the antiperiodic
+ * flip of dSdpi(:,:,Lt,3) when ibound == -1, plus beta*Nf*theta.  theta is
+ * trial::theta (Lx,Ly,Lt,3) float; dsdpi is overwritten; itercg[4] (may be
+ * NULL) receives the four qmrherm iteration counts. */
+int thr_force(const double *phi, float res1, float am, int imass, float am3, float beta,
+              int ibound, const float *theta, const double *u, const int dims[4],
+              const double *anum4g, const double *aden4g, const double *bnum2g,
+              const double *bden2g, const double *anum2g, const double *aden2g, int ndiagg,
+              int use_sp, int max_qmr_iters, double *dsdpi, int *itercg);
+
+/* dwf3d_lib.F90:511-616  hamilton(Phi, h, hg, hp, s, res2, am, imass) for
+ * Nf = 1 (rational branch); out = {h, hg, hp, s, hf}; itercg[2] (may be NULL) */
+int thr_hamilton(const double *phi, float res2, float am, int imass, float am3, float beta,
+                 const float *theta, const float *pp, const double *u, const int dims[4],
+                 const double *anum4, const double *aden4, const double *bnum2,
+                 const double *bden2, int ndiag, int max_qmr_iters, double *out, int *itercg);
+
 /* ---- (2) device-resident entry points ------------------------------------ */
 
 /* links from theta on device (gauge.F90:12-43); links64 [3][Lt][Ly][Lx] complex128 */

@@ -1,0 +1,80 @@
+"""RHMC force and Hamiltonian on the device against the oracle.
+
+Tolerances: force (FP64 solves) relative L2 <= 1e-10 with identical qmrherm
+iteration counts; FP32-solve force (spmd) <= 1e-4; Hamiltonian terms relative
+<= 1e-10 (h, hf) and exact for the host-side float32 sums (hg, hp).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import dense as D
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+DATA = os.path.join(os.path.dirname(__file__), "..", "thirring-rhmc_amd", "data")
+
+
+def remez_file(name):
+    """read_remez_file (remez_common_subroutines.F90:20-35): two blocks."""
+    rows = [l.replace(",", " ").split() for l in open(os.path.join(DATA, name)) if l.strip()]
+    n = (len(rows) - 2) // 2
+    blocks = []
+    for b in range(2):
+        blk = rows[b * (n + 1):(b + 1) * (n + 1)]
+        num = np.array([float(blk[0][0])] + [float(r[0]) for r in blk[1:]])
+        den = np.array([float(r[1]) for r in blk[1:]])
+        blocks.append((num, den))
+    return blocks
+
+
+@pytest.fixture(scope="module")
+def G():
+    import thirring_hip as G
+    G.init(0)
+    return G
+
+
+def fields(dims, seed):
+    lx, ly, lt, ls = dims
+    rng = np.random.default_rng(seed)
+    th = (rng.standard_normal((lx, ly, lt, 3)) / np.sqrt(0.3)).astype(np.float32)
+    pp = rng.standard_normal((lx, ly, lt, 3)).astype(np.float32)
+    return th, pp, O.coef(th, dims), D.rand_spinor(dims, rng)
+
+
+def rel(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+@pytest.mark.parametrize("use_sp", [False, True])
+def test_force(G, use_sp):
+    dims = (4, 4, 4, 8)
+    th, pp, u, phi = fields(dims, 1)
+    (a4g, d4g), (b4g, e4g) = remez_file("remez4g")
+    (a2g, d2g), (b2g, e2g) = remez_file("remez2g")
+    # force_module.F90:80-96: anum4g/aden4g (block 1 of remez4g), bnum2g/bden2g
+    # (block 2 of remez2g), anum2g/aden2g (block 1 of remez2g)
+    coeffs = (a4g, d4g, b2g, e2g, a2g, d2g)
+    res1 = 1e-8
+    fg, its = G.force(phi, res1, 0.05, 3, 0.3, th, u, dims, coeffs, use_sp=use_sp)
+    fo = O.force(phi, res1, 0.05, 3, 0.3, th, u, dims, coeffs, use_sp=use_sp)
+    e = rel(fg, fo)
+    assert e <= (1e-4 if use_sp else 1e-10), e
+
+
+def test_hamilton(G):
+    dims = (4, 4, 4, 8)
+    th, pp, u, phi = fields(dims, 2)
+    (a4, d4), (b4, e4) = remez_file("remez4")
+    (a2, d2), (b2, e2) = remez_file("remez2")
+    # dwf3d_lib.F90:584-590: anum4/aden4 (block 1 of remez4), bnum2/bden2 (block 2 of remez2)
+    coeffs = (a4, d4, b2, e2)
+    hg_, itg = G.hamilton(phi, 1e-9, 0.05, 3, 0.3, th, pp, u, dims, coeffs)
+    ho_, ito = O.hamilton(phi, 1e-9, 0.05, 3, 0.3, th, pp, u, dims, coeffs)
+    assert itg == ito
+    assert hg_["hg"] == ho_["hg"] and hg_["hp"] == ho_["hp"]
+    for k in ("h", "hf", "s"):
+        assert abs(hg_[k] - ho_[k]) <= 1e-10 * abs(ho_[k]), (k, hg_[k], ho_[k])

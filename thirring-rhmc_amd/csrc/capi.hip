@@ -11,6 +11,7 @@
 
 #include "../../include/thirring_hip.h"
 #include "msfast.h"
+#include "rhmc.h"
 #include "solvers.h"
 
 namespace thr {
@@ -111,16 +112,6 @@ static int host_op(double *phi, const double *r, const double *u, const int dims
   TRY(stage_out(d, b, W_H1, phi, s));
   THR_HIP(hipStreamSynchronize(s));
   return THR_OK;
-}
-
-static int derivs_dispatch(const Dims &d, int kernel, const double2 *R, const double2 *X2,
-                           double anum, int iflag, float am, int imass, double *f, hipStream_t s) {
-  if (kernel == 1) {
-    void *pb = workspace(W_D, (size_t)SLICE_PAIR_BYTES * (2 * d.ls + 4));
-    THR_CHECK(pb, THR_ERR_HIP, "workspace allocation failed");
-    return launch_derivs_wilson(d, R, X2, anum, iflag, am, imass, f, pb, s);
-  }
-  return launch_derivs(d, R, X2, anum, iflag, f, s);
 }
 
 }  // namespace thr
@@ -319,94 +310,121 @@ int thr_qmrherm(const double *phi, double *x, float res, int *itercg, float am, 
   TRY(check_dims(dims));
   TRY(check_op(kernel, imass));
   THR_CHECK(phi && x && itercg && anum && aden && u && x1h, THR_ERR_ARG, "NULL argument");
+  THR_CHECK((iflag != 1 && iflag != 3) || phi0, THR_ERR_ARG, "iflag 1/3 need the Phi0 register");
+  THR_CHECK(iflag < 2 || dsdpi, THR_ERR_ARG, "iflag 2/3 need dSdpi");
   THR_CHECK(ndiagq >= 1 && ndiagq <= 25, THR_ERR_ARG,
             "qmrherm requires ndiagq <= ndiag = 25 (qmrherm_module.F90:66-73)");
-  THR_CHECK(dims[3] % 4 == 0, THR_ERR_ARG,
-            "QMRHERM: kthird_l must be divisible by 4 (qmrherm_module.F90:75-81)");
-  THR_CHECK(iflag >= 0 && iflag <= 3, THR_ERR_ARG, "iflag must be 0..3");
-  THR_CHECK(iflag != 1 || phi0, THR_ERR_ARG, "iflag 1 needs the Phi0 register");
-  THR_CHECK(iflag < 2 || dsdpi, THR_ERR_ARG, "iflag 2/3 need dSdpi");
-  THR_CHECK(iflag != 3 || phi0, THR_ERR_ARG, "iflag 3 needs the Phi0 register");
   const Dims d = mkdims(dims);
   hipStream_t s = default_stream();
   double2 *u64;
   TRY(stage_links(d, u, &u64, s));
   const long n = d.nspinor();
+  const long nh = haloed5(d);
   double2 *a = (double2 *)workspace(W_A, sizeof(double2) * n);
   double2 *b = (double2 *)workspace(W_B, sizeof(double2) * n);
-  double2 *c = (double2 *)workspace(W_C, sizeof(double2) * n);
   double2 *x1 = (double2 *)workspace(W_X1, sizeof(double2) * n * ndiagq);
-  THR_CHECK(a && b && c && x1, THR_ERR_HIP, "workspace allocation failed");
+  double2 *p0 = (double2 *)workspace(W_C, sizeof(double2) * n * ndiagq);
+  double *f = (double *)workspace(W_T, sizeof(double) * d.v4() * 3);
+  THR_CHECK(a && b && x1 && p0 && f, THR_ERR_HIP, "workspace allocation failed");
   TRY(stage_in(d, phi, W_H0, a, s));
+  if (iflag == 3)
+    for (int k = 0; k < ndiagq; k++) TRY(stage_in(d, phi0 + 2 * nh * k, W_H2, p0 + n * k, s));
+  if (iflag >= 2)
+    THR_HIP(hipMemcpyAsync(f, dsdpi, sizeof(double) * d.v4() * 3, hipMemcpyHostToDevice, s));
   const OpDesc op = mkop(d, kernel, imass, am, am3, u64);
-  // single precision is not available with the Wilson kernel (qmrherm_module.F90:54-63)
-  const int sp = use_sp && kernel == 0;
-  if (sp)
-    TRY(multishift_reference_sp(op, ndiagq, aden, anum + 1, a, x1, res, max_qmr_iters, itercg,
-                                cg_returns, s));
-  else
-    TRY(multishift_reference(op, ndiagq, aden, anum + 1, a, x1, res, max_qmr_iters, itercg,
-                             cg_returns, s));
+  TRY(qmrherm_device(op, a, b, res, itercg, anum, aden, ndiagq, iflag, use_sp, max_qmr_iters, x1,
+                     p0, f, cg_returns, s));
   THR_HIP(hipMemcpyAsync(x1h, x1, sizeof(double2) * n * ndiagq, hipMemcpyDeviceToHost, s));
-  THR_HIP(hipStreamSynchronize(s));
-  // x = anum(0)*Phi over the whole haloed array (qmrherm_module.F90:83)
-  const long nh = haloed5(d);
-  for (long k = 0; k < 2 * nh; k++) x[k] = anum[0] * phi[k];
   if (iflag < 2) {
-    // x += sum anum(i) x1(i) on the interior, then the periodic halo (96-108)
-    std::vector<double> xi(2 * n);
-    // interior of x in halo-free order
-    const long S = d.ls + 2, X = d.lx + 2, Y = d.ly + 2;
-    auto hidx = [&](int sidx, int xx, int yy, int tt, int aa) {
-      return (sidx + S * (xx + X * (yy + Y * (tt + (long)(d.lt + 2) * aa))));
-    };
-    for (int aa = 0; aa < 4; aa++)
-      for (int tt = 0; tt < d.lt; tt++)
-        for (int yy = 0; yy < d.ly; yy++)
-          for (int xx = 0; xx < d.lx; xx++)
-            for (int ss = 0; ss < d.ls; ss++) {
-              const long q = ss + d.ls * (xx + (long)d.lx * (yy + (long)d.ly * (tt + (long)d.lt * aa)));
-              const long o = hidx(ss + 1, xx + 1, yy + 1, tt + 1, aa);
-              double re = x[2 * o], im = x[2 * o + 1];
-              for (int k = 0; k < ndiagq; k++) {
-                re = re + anum[k + 1] * x1h[2 * (q + n * k)];
-                im = im + anum[k + 1] * x1h[2 * (q + n * k) + 1];
-              }
-              xi[2 * q] = re;
-              xi[2 * q + 1] = im;
-            }
-    THR_HIP(hipMemcpyAsync(b, xi.data(), sizeof(double2) * n, hipMemcpyHostToDevice, s));
-    TRY(stage_out(d, b, W_H1, x, s));
-    if (iflag == 1) {
+    TRY(stage_out(d, b, W_H1, x, s));  // interior plus periodic halo (96-108)
+    if (iflag == 1)
       for (int k = 0; k < ndiagq; k++) {
         double2 *hb = (double2 *)workspace(W_H2, sizeof(double2) * nh);
         THR_CHECK(hb, THR_ERR_HIP, "workspace allocation failed");
-        TRY(launch_unpack_spinor(d, x1 + n * k, nullptr, P64, hb, 1, s));
+        TRY(launch_unpack_spinor(d, p0 + n * k, nullptr, P64, hb, 1, s));
         THR_HIP(hipMemcpyAsync(phi0 + 2 * nh * k, hb, sizeof(double2) * nh, hipMemcpyDeviceToHost, s));
+        THR_HIP(hipStreamSynchronize(s));
       }
-    }
+  } else {
+    // x = anum(0)*Phi over the whole haloed array, halo included (83)
     THR_HIP(hipStreamSynchronize(s));
-    return THR_OK;
+    for (long k = 0; k < 2 * nh; k++) x[k] = anum[0] * phi[k];
+    THR_HIP(hipMemcpyAsync(dsdpi, f, sizeof(double) * d.v4() * 3, hipMemcpyDeviceToHost, s));
   }
+  THR_HIP(hipStreamSynchronize(s));
+  return THR_OK;
+}
+
+int thr_force(const double *phi, float res1, float am, int imass, float am3, float beta,
+              int ibound, const float *theta, const double *u, const int dims[4],
+              const double *anum4g, const double *aden4g, const double *bnum2g,
+              const double *bden2g, const double *anum2g, const double *aden2g, int ndiagg,
+              int use_sp, int max_qmr_iters, double *dsdpi, int *itercg) {
+  TRY(check_dims(dims));
+  TRY(check_op(0, imass));
+  THR_CHECK(phi && theta && u && anum4g && aden4g && bnum2g && bden2g && anum2g && aden2g && dsdpi,
+            THR_ERR_ARG, "NULL argument");
+  const Dims d = mkdims(dims);
+  hipStream_t s = default_stream();
+  double2 *u64;
+  TRY(stage_links(d, u, &u64, s));
+  double2 *a = (double2 *)workspace(W_A, sizeof(double2) * d.nspinor());
+  float *th = (float *)workspace(W_F, sizeof(float) * d.v4() * 3);
   double *f = (double *)workspace(W_T, sizeof(double) * d.v4() * 3);
-  THR_CHECK(f, THR_ERR_HIP, "workspace allocation failed");
-  THR_HIP(hipMemcpyAsync(f, dsdpi, sizeof(double) * d.v4() * 3, hipMemcpyHostToDevice, s));
-  for (int k = 0; k < ndiagq; k++) {
-    const double2 *xk = x1 + n * k;
-    TRY(apply_op(op, 0, xk, b, s));  // x2 = M x1(k)  (143)
-    if (iflag == 2) {
-      TRY(derivs_dispatch(d, kernel, xk, b, anum[k + 1], 0, am, imass, f, s));  // (150-155)
-    } else {
-      const double coeff = -anum[k + 1];
-      TRY(stage_in(d, phi0 + 2 * nh * k, W_H0, c, s));       // R = Phi0(k) (159)
-      TRY(derivs_dispatch(d, kernel, c, b, coeff, 0, am, imass, f, s));  // (163)
-      TRY(apply_op(op, 0, c, b, s));                          // x2 = M R (166)
-      TRY(derivs_dispatch(d, kernel, b, xk, coeff, 1, am, imass, f, s));  // (181)
-    }
-  }
+  THR_CHECK(a && th && f, THR_ERR_HIP, "workspace allocation failed");
+  TRY(stage_in(d, phi, W_H0, a, s));
+  THR_HIP(hipMemcpyAsync(th, theta, sizeof(float) * d.v4() * 3, hipMemcpyHostToDevice, s));
+  ForceArgs A{};
+  A.op = mkop(d, 0, imass, am, am3, u64);
+  A.phi = a;
+  A.theta = th;
+  A.res1 = res1;
+  A.beta = beta;
+  A.ibound = ibound;
+  A.ndiagg = ndiagg;
+  A.use_sp = use_sp;
+  A.max_qmr_iters = max_qmr_iters;
+  A.anum4g = anum4g;
+  A.aden4g = aden4g;
+  A.bnum2g = bnum2g;
+  A.bden2g = bden2g;
+  A.anum2g = anum2g;
+  A.aden2g = aden2g;
+  TRY(force_device(A, f, itercg, s));
   THR_HIP(hipMemcpyAsync(dsdpi, f, sizeof(double) * d.v4() * 3, hipMemcpyDeviceToHost, s));
   THR_HIP(hipStreamSynchronize(s));
   return THR_OK;
+}
+
+int thr_hamilton(const double *phi, float res2, float am, int imass, float am3, float beta,
+                 const float *theta, const float *pp, const double *u, const int dims[4],
+                 const double *anum4, const double *aden4, const double *bnum2,
+                 const double *bden2, int ndiag, int max_qmr_iters, double *out, int *itercg) {
+  TRY(check_dims(dims));
+  TRY(check_op(0, imass));
+  THR_CHECK(phi && theta && pp && u && anum4 && aden4 && bnum2 && bden2 && out, THR_ERR_ARG,
+            "NULL argument");
+  const Dims d = mkdims(dims);
+  hipStream_t s = default_stream();
+  double2 *u64;
+  TRY(stage_links(d, u, &u64, s));
+  double2 *a = (double2 *)workspace(W_A, sizeof(double2) * d.nspinor());
+  THR_CHECK(a, THR_ERR_HIP, "workspace allocation failed");
+  TRY(stage_in(d, phi, W_H0, a, s));
+  HamArgs A{};
+  A.op = mkop(d, 0, imass, am, am3, u64);
+  A.phi = a;
+  A.theta = theta;
+  A.pp = pp;
+  A.res2 = res2;
+  A.beta = beta;
+  A.ndiag = ndiag;
+  A.max_qmr_iters = max_qmr_iters;
+  A.anum4 = anum4;
+  A.aden4 = aden4;
+  A.bnum2 = bnum2;
+  A.bden2 = bden2;
+  return hamilton_device(A, out, itercg, s);
 }
 
 // ---- device-resident ------------------------------------------------------

@@ -43,6 +43,9 @@ _SIGS = {
     "thr_derivs": ([P, P, F64, I32, P, F32, I32, F32, I32, P], I32),
     "thr_qmrherm": ([P, P, F32, IP, F32, I32, F32, P, P, I32, I32, I32, P, P, I32, I32, P, P, P,
                      P], I32),
+    "thr_force": ([P, F32, F32, I32, F32, F32, I32, P, P, P, P, P, P, P, P, P, I32, I32, I32, P,
+                   P], I32),
+    "thr_hamilton": ([P, F32, F32, I32, F32, F32, P, P, P, P, P, P, P, P, I32, I32, P, P], I32),
     "thr_coef_dev": ([P, P, P, I32, P], I32),
     "thr_links_to_f32_dev": ([P, P, P, P], I32),
     "thr_dslash_dev": ([P, P, P, P, P, P, I32, P, I32, F32, I32, I32, P], I32),
@@ -233,6 +236,37 @@ def qmrherm(phi, res, am, imass, anum, aden, iflag, u, dims, x1=None, phi0=None,
                              _dims(dims), int(max_qmr_iters), KERNEL[kernel], _p(x1), _p(phi0),
                              _p(dsdpi), _p(cgrs)), "qmrherm")
     return x, it.value, x1, phi0, dsdpi, cgrs
+
+
+def force(phi, res1, am, imass, beta, theta, u, dims, coeffs_g, ibound=-1, am3=1.0, use_sp=False,
+          max_qmr_iters=50000):
+    """force_module.F90:9-113; coeffs_g = (anum4g, aden4g, bnum2g, bden2g, anum2g, aden2g)."""
+    init()
+    lx, ly, lt, ls = dims
+    a4, d4, b2, e2, a2, d2 = [np.ascontiguousarray(c, dtype=np.float64) for c in coeffs_g]
+    dsdpi = np.zeros((lx, ly, lt, 3), dtype=np.float64, order="F")
+    th = np.asfortranarray(theta, dtype=np.float32)
+    its = np.zeros(4, dtype=np.int32)
+    _check(lib().thr_force(_p(phi), res1, am, int(imass), am3, beta, int(ibound), _p(th), _p(u),
+                           _dims(dims), _p(a4), _p(d4), _p(b2), _p(e2), _p(a2), _p(d2), len(d4),
+                           int(bool(use_sp)), int(max_qmr_iters), _p(dsdpi), _p(its)), "force")
+    return dsdpi, its
+
+
+def hamilton(phi, res2, am, imass, beta, theta, pp, u, dims, coeffs, am3=1.0,
+             max_qmr_iters=50000):
+    """dwf3d_lib.F90:511-616; coeffs = (anum4, aden4, bnum2, bden2)."""
+    init()
+    a4, d4, b2, e2 = [np.ascontiguousarray(c, dtype=np.float64) for c in coeffs]
+    th = np.asfortranarray(theta, dtype=np.float32)
+    ppf = np.asfortranarray(pp, dtype=np.float32)
+    out = np.zeros(5)
+    its = np.zeros(2, dtype=np.int32)
+    _check(lib().thr_hamilton(_p(phi), res2, am, int(imass), am3, beta, _p(th), _p(ppf), _p(u),
+                              _dims(dims), _p(a4), _p(d4), _p(b2), _p(e2), len(d4),
+                              int(max_qmr_iters), _p(out), _p(its)), "hamilton")
+    keys = ("h", "hg", "hp", "s", "hf")
+    return dict(zip(keys, out.tolist())), (int(its[0]), int(its[1]))
 
 
 # ---- device-resident API --------------------------------------------------
