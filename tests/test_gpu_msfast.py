@@ -118,3 +118,30 @@ def test_plain_fp64_path(G):
     oh, xh = out.to_host(), x.to_host()[0]
     d = float(np.linalg.norm(oh - xh) / np.linalg.norm(oh))
     assert d <= 1e-8, d
+
+
+def test_graph_replay_equals_eager(G):
+    """Solver blocks replayed from a captured hipGraph give bit-identical
+    solutions to eager launches, across several replays and a precision switch."""
+    import os
+    from hiprt import DeviceArray
+    dims = (4, 4, 4, 8)
+    sig = _remez4()[::3]
+    th, l64, l32, bh, b = _setup(dims, 4, 2)
+    n = len(sig)
+    nsp = bh.shape[1]
+    outs = []
+    for flag in ("0", "1", "1"):
+        os.environ["THR_GRAPHS"] = flag
+        try:
+            x = DeviceArray((2, n, nsp, 2), np.float64)
+            it, cgr, rel = G.multishift_fast_dev(l64, l32, dims, [0.02, 0.05], 3, sig, b, x,
+                                                 tol=5e-11, maxcg=20000, nsrc=2, th32=1e-4)
+            assert rel.max() <= 1e-10
+            assert it > 64  # several blocks, i.e. several replays of the captured graph
+            outs.append((it, x.to_host()))
+        finally:
+            os.environ.pop("THR_GRAPHS", None)
+    assert outs[0][0] == outs[1][0] == outs[2][0]
+    assert np.array_equal(outs[0][1], outs[1][1])
+    assert np.array_equal(outs[1][1], outs[2][1])

@@ -26,8 +26,8 @@ template <int PREC> struct Acc {
   using T = typename Store<PREC>::T;
   const T *__restrict__ p;
   const float *__restrict__ nrm;
-  long v5;
-  __device__ inline void load4(long site, cx<C> v[4]) const {
+  unsigned v5;
+  __device__ inline void load4(unsigned site, cx<C> v[4]) const {
     if constexpr (PREC == P16) {
       const float sc = nrm[site];
 #pragma unroll
@@ -37,7 +37,7 @@ template <int PREC> struct Acc {
       for (int a = 0; a < 4; a++) v[a] = ldc(p, a * v5 + site);
     }
   }
-  __device__ inline cx<C> load1(long site, int a) const {
+  __device__ inline cx<C> load1(unsigned site, unsigned a) const {
     if constexpr (PREC == P16) {
       return ldh(p, a * v5 + site, nrm[site]);
     } else {
@@ -51,8 +51,8 @@ template <int PREC> struct Out {
   using T = typename Store<PREC>::T;
   T *__restrict__ p;
   float *__restrict__ nrm;
-  long v5;
-  __device__ inline void store4(long site, const cx<C> v[4]) const {
+  unsigned v5;
+  __device__ inline void store4(unsigned site, const cx<C> v[4]) const {
     if constexpr (PREC == P16) {
       float m = 0.f;
 #pragma unroll
@@ -105,16 +105,18 @@ __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L) {
   using T = typename Store<PREC>::T;
   const int rhs = blockIdx.y;
   if (L.done && L.done[(long)rhs * L.done_stride]) return;
-  const int lx = L.d.lx, ly = L.d.ly, lt = L.d.lt, ls = L.d.ls;
-  const long v4 = (long)lx * ly * lt, v5 = v4 * ls;
-  const long i = (long)blockIdx.x * BLK + threadIdx.x;
+  // 32-bit index arithmetic: 4*V5 < 2^31 is checked on the host
+  const unsigned lx = L.d.lx, ly = L.d.ly, lt = L.d.lt, ls = L.d.ls;
+  const unsigned v4 = lx * ly * lt, v5 = v4 * ls;
+  const unsigned i = blockIdx.x * BLK + threadIdx.x;
   double red = 0.0;
   if (i < v5) {
-    const int s = (int)(i % ls);
-    const long site4 = i / ls;
-    const int x = (int)(site4 % lx);
-    const int y = (int)((site4 / lx) % ly);
-    const int t = (int)(site4 / ((long)lx * ly));
+    const unsigned s = i % ls;
+    const unsigned site4 = i / ls;
+    const unsigned x = site4 % lx;
+    const unsigned yt = site4 / lx;
+    const unsigned y = yt % ly;
+    const unsigned t = yt / ly;
     const long roff = (long)rhs * 4 * v5;
     Acc<PREC> in{(const T *)L.in + roff, L.in_norm ? L.in_norm + (long)rhs * v5 : nullptr, v5};
 
@@ -128,7 +130,7 @@ __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L) {
 
 #pragma unroll
     for (int mu = 0; mu < 3; mu++) {
-      int xf = x, yf = y, tf = t, xb = x, yb = y, tb = t;
+      unsigned xf = x, yf = y, tf = t, xb = x, yb = y, tb = t;
       if (mu == 0) {
         xf = (x + 1 == lx) ? 0 : x + 1;
         xb = (x == 0) ? lx - 1 : x - 1;
@@ -139,8 +141,8 @@ __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L) {
         tf = (t + 1 == lt) ? 0 : t + 1;
         tb = (t == 0) ? lt - 1 : t - 1;
       }
-      const long sf4 = xf + (long)lx * (yf + (long)ly * tf);
-      const long sb4 = xb + (long)lx * (yb + (long)ly * tb);
+      const unsigned sf4 = xf + lx * (yf + ly * tf);
+      const unsigned sb4 = xb + lx * (yb + ly * tb);
       const cx<C> uf = link<C>(L.links, mu * v4 + site4);
       const cx<C> ub = conj(link<C>(L.links, mu * v4 + sb4));
       cx<C> rf[4], rb[4];
@@ -173,7 +175,7 @@ __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L) {
     }
     // wall mass (dirac.F90:119-176 for M, 356-402 for M^dagger)
     const C m = (C)L.am[rhs];
-    const long col = site4 * ls;  // s = 0 of this (x,y,t)
+    const unsigned col = site4 * ls;  // s = 0 of this (x,y,t)
     if (imass == 1 || imass == 3) {
       // top: s = Ls couples to s = 1 on the up0 spins; bottom: s = 1 couples to s = Ls
       if (s == ls - 1) {
@@ -307,8 +309,8 @@ __global__ __launch_bounds__(BLK) void k_pack(const Dims d, const double2 *__res
     const double2 z = h[hidx(d, s + 1, x + 1, y + 1, t + 1, a)];
     v[a] = {(C)z.x, (C)z.y};
   }
-  Out<PREC> o{dst, nrm, v5};
-  o.store4(i, v);
+  Out<PREC> o{dst, nrm, (unsigned)v5};
+  o.store4((unsigned)i, v);
 }
 
 template <int PREC>
@@ -335,9 +337,9 @@ __global__ __launch_bounds__(BLK) void k_unpack(const Dims d, const typename Sto
   const int ti = t == 0 ? d.lt - 1 : (t == T - 1 ? 0 : t - 1);
   const long v5 = d.v5();
   const long site = si + (long)d.ls * (xi + (long)d.lx * (yi + (long)d.ly * ti));
-  Acc<PREC> in{src, nrm, v5};
+  Acc<PREC> in{src, nrm, (unsigned)v5};
   cx<typename Store<PREC>::C> v[4];
-  in.load4(site, v);
+  in.load4((unsigned)site, v);
 #pragma unroll
   for (int a = 0; a < 4; a++) h[i + nh * a] = make_double2((double)v[a].r, (double)v[a].i);
 }
@@ -362,16 +364,16 @@ __global__ __launch_bounds__(BLK) void k_convert(long nsite, long v5, const void
   const long i = (long)blockIdx.x * BLK + threadIdx.x;
   if (i >= nsite) return;
   const long rhs = i / v5, site = i % v5;
-  Acc<SP> in{(const typename Store<SP>::T *)src + rhs * 4 * v5, snrm ? snrm + rhs * v5 : nullptr, v5};
+  Acc<SP> in{(const typename Store<SP>::T *)src + rhs * 4 * v5, snrm ? snrm + rhs * v5 : nullptr, (unsigned)v5};
   using CS = typename Store<SP>::C;
   using CD = typename Store<DP>::C;
   cx<CS> v[4];
-  in.load4(site, v);
+  in.load4((unsigned)site, v);
   cx<CD> w[4];
 #pragma unroll
   for (int a = 0; a < 4; a++) w[a] = {(CD)v[a].r, (CD)v[a].i};
-  Out<DP> o{(typename Store<DP>::T *)dst + rhs * 4 * v5, dnrm ? dnrm + rhs * v5 : nullptr, v5};
-  o.store4(site, w);
+  Out<DP> o{(typename Store<DP>::T *)dst + rhs * 4 * v5, dnrm ? dnrm + rhs * v5 : nullptr, (unsigned)v5};
+  o.store4((unsigned)site, w);
 }
 
 __global__ __launch_bounds__(BLK) void k_coef(const Dims d, const float *theta, int ibound,

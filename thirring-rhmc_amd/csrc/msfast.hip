@@ -21,6 +21,7 @@
 //
 // Per-shift convergence: |r_i| = |zeta_i| |r| <= tol |b|.
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -448,30 +449,37 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
   long t = 0;  // completed iterations
   int rc = THR_OK;
   int blocks = 0;
-  for (;;) {
-    for (int j = 0; j < MSB; j++, t++) {
-      void *rin = (t == 0) ? r0[prec] : ring[prec][(t - 1) % MSB];
-      float *rinn = (prec == P16) ? ((t == 0) ? r0n : ringn[(t - 1) % MSB]) : nullptr;
-      void *rout = ring[prec][t % MSB];
-      float *routn = (prec == P16) ? ringn[t % MSB] : nullptr;
+  // one block = MSB iterations + flush + reset; every block after the first
+  // issues identical launches, so it can be captured once per precision as a
+  // hipGraph and replayed (THR_GRAPHS=0 disables)
+  const char *ge = std::getenv("THR_GRAPHS");
+  const bool use_graph = !(ge && ge[0] == '0');
+  hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
+  auto enqueue_block = [&](long t0, int pr) -> int {
+    for (int j = 0; j < MSB; j++) {
+      const long tt = t0 + j;
+      void *rin = (tt == 0) ? r0[pr] : ring[pr][(tt - 1) % MSB];
+      float *rinn = (pr == P16) ? ((tt == 0) ? r0n : ringn[(tt - 1) % MSB]) : nullptr;
+      void *rout = ring[pr][tt % MSB];
+      float *routn = (pr == P16) ? ringn[tt % MSB] : nullptr;
       // h = M p ; alpha = |h|^2
       DslashLaunch a = L;
-      a.prec = prec;
-      a.links = prec == P64 ? (const void *)A.u64 : (const void *)A.u32;
+      a.prec = pr;
+      a.links = pr == P64 ? (const void *)A.u64 : (const void *)A.u32;
       a.dagger = 0;
-      a.in = pbuf[prec];
-      a.in_norm = prec == P16 ? pn : nullptr;
-      a.out = hbuf[prec];
-      a.out_norm = prec == P16 ? hn : nullptr;
+      a.in = pbuf[pr];
+      a.in_norm = pr == P16 ? pn : nullptr;
+      a.out = hbuf[pr];
+      a.out_norm = pr == P16 ? hn : nullptr;
       a.mode = EPI_NORM;
       TRY_RC(launch_dslash(a, s));
       hipLaunchKernelGGL(k_msf_fin1, dim3(ns_), dim3(BLK), 0, s, st, aden, partial, nb);
       // r_new = r + omega M^+ h ; lambda = |r_new|^2
       DslashLaunch b = L;
-      b.prec = prec;
+      b.prec = pr;
       b.links = a.links;
       b.dagger = 1;
-      b.in = hbuf[prec];
+      b.in = hbuf[pr];
       b.in_norm = a.out_norm;
       b.mode = EPI_CG;
       b.r = rin;
@@ -483,10 +491,10 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
       TRY_RC(launch_dslash(b, s));
       hipLaunchKernelGGL(k_msf_fin2, dim3(ns_), dim3(BLK), 0, s, st, partial, nb);
       dim3 gx(nblk(v5), ns_);
-      if (prec == P64)
+      if (pr == P64)
         hipLaunchKernelGGL(k_msf_xpay<P64>, gx, dim3(BLK), 0, s, v5, (const double2 *)rout, nullptr,
                            (double2 *)pbuf[P64], nullptr, st);
-      else if (prec == P32)
+      else if (pr == P32)
         hipLaunchKernelGGL(k_msf_xpay<P32>, gx, dim3(BLK), 0, s, v5, (const float2 *)rout, nullptr,
                            (float2 *)pbuf[P32], nullptr, st);
       else
@@ -496,12 +504,12 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
     }
     // flush the block into x_i, p_i
     dim3 gf(nblk(nsp), ns_);
-    void *const *rp = (void *const *)ringptr + prec * MSB;
+    void *const *rp = (void *const *)ringptr + pr * MSB;
     float *const *rnp = (float *const *)((void **)ringptr + 3 * MSB);
-    if (prec == P64)
+    if (pr == P64)
       hipLaunchKernelGGL(k_msf_flush<P64>, gf, dim3(BLK), 0, s, v5, n, (double2 *const *)rp,
                          nullptr, A.b, A.x, Pv, st, 0);
-    else if (prec == P32)
+    else if (pr == P32)
       hipLaunchKernelGGL(k_msf_flush<P32>, gf, dim3(BLK), 0, s, v5, n, (float2 *const *)rp,
                          nullptr, A.b, A.x, Pv, st, 0);
     else
@@ -509,9 +517,53 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
                          A.b, A.x, Pv, st, 0);
     hipLaunchKernelGGL(k_msf_reset, dim3(ns_), dim3(1), 0, s, st);
     THR_HIP(hipGetLastError());
+    return THR_OK;
+  };
+  auto free_graphs = [&]() {
+    for (int k = 0; k < 3; k++)
+      if (gexec[k]) (void)hipGraphExecDestroy(gexec[k]);
+  };
+  for (;;) {
+    if (!use_graph || t == 0) {
+      rc = enqueue_block(t, prec);
+      if (rc) {
+        free_graphs();
+        return rc;
+      }
+    } else {
+      if (!gexec[prec]) {
+        hipGraph_t g = nullptr;
+        if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+          free_graphs();
+          set_error("hipStreamBeginCapture failed");
+          return THR_ERR_HIP;
+        }
+        rc = enqueue_block(t, prec);
+        hipError_t e = hipStreamEndCapture(s, &g);
+        if (rc || e != hipSuccess ||
+            hipGraphInstantiate(&gexec[prec], g, nullptr, nullptr, 0) != hipSuccess) {
+          if (g) (void)hipGraphDestroy(g);
+          free_graphs();
+          set_error("hipGraph capture of the solver block failed");
+          return rc ? rc : THR_ERR_HIP;
+        }
+        (void)hipGraphDestroy(g);
+      }
+      if (hipGraphLaunch(gexec[prec], s) != hipSuccess) {
+        free_graphs();
+        set_error("hipGraphLaunch failed");
+        return THR_ERR_HIP;
+      }
+    }
+    t += MSB;
     blocks++;
-    THR_HIP(hipMemcpyAsync(hs.data(), st, sizeof(MsfState) * ns_, hipMemcpyDeviceToHost, s));
-    THR_HIP(hipStreamSynchronize(s));
+    if (hipMemcpyAsync(hs.data(), st, sizeof(MsfState) * ns_, hipMemcpyDeviceToHost, s) !=
+            hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+      free_graphs();
+      set_error("multishift: state read-back failed");
+      return THR_ERR_HIP;
+    }
     bool all = true;
     double worst = 0.0;
     for (int k = 0; k < ns_; k++) {
@@ -520,6 +572,7 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
     }
     if (all) break;
     if (t > (long)A.maxcg + 2 * MSB) {
+      free_graphs();
       set_error("multishift: stop flag not raised");
       return THR_ERR_STATE;
     }
@@ -530,14 +583,20 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
     if (want != prec) {
       void *rl = ring[prec][(t - 1) % MSB];
       float *rln = prec == P16 ? ringn[(t - 1) % MSB] : nullptr;
-      TRY_RC(conv(d, ns_, prec, rl, rln, want, ring[want][(t - 1) % MSB],
-                  want == P16 ? ringn[(t - 1) % MSB] : nullptr, s));
-      TRY_RC(conv(d, ns_, prec, pbuf[prec], prec == P16 ? pn : nullptr, want, pbuf[want],
-                  want == P16 ? pn : nullptr, s));
+      int rcc = conv(d, ns_, prec, rl, rln, want, ring[want][(t - 1) % MSB],
+                     want == P16 ? ringn[(t - 1) % MSB] : nullptr, s);
+      if (!rcc)
+        rcc = conv(d, ns_, prec, pbuf[prec], prec == P16 ? pn : nullptr, want, pbuf[want],
+                   want == P16 ? pn : nullptr, s);
+      if (rcc) {
+        free_graphs();
+        return rcc;
+      }
       prec = want;
       if (res) res->switches[prec] = (int)t;
     }
   }
+  free_graphs();
   if (res) {
     res->iterations = (int)t;
     res->blocks = blocks;
