@@ -1,0 +1,89 @@
+"""Host logic of the serving loop (no GPU): admission by HBM footprint, slot
+limits, first-fit with anti-starvation, latency bookkeeping.  The solve is
+replaced by a stub that sleeps in proportion to the request size."""
+import threading
+import time
+
+import pytest
+
+
+@pytest.fixture
+def serving(monkeypatch):
+    import serving as S
+    import thirring_hip
+
+    class _Ctx:
+        def bind(self):
+            pass
+
+        def destroy(self):
+            pass
+
+    class _Stream:
+        pass
+
+    monkeypatch.setattr(thirring_hip, "Context", _Ctx)
+    monkeypatch.setattr(S.hr, "Stream", _Stream)
+    live = {"n": 0, "max": 0, "bytes": 0, "max_bytes": 0}
+    lock = threading.Lock()
+
+    def fake_solve(req, stream=None):
+        with lock:
+            live["n"] += 1
+            live["bytes"] += req.footprint()
+            live["max"] = max(live["max"], live["n"])
+            live["max_bytes"] = max(live["max_bytes"], live["bytes"])
+        time.sleep(0.002 * req.nsrc)
+        with lock:
+            live["n"] -= 1
+            live["bytes"] -= req.footprint()
+        return {"iterations": 1, "true_residual": [1e-11] * req.nsrc,
+                "x_norm": [1.0] * req.nsrc}
+
+    monkeypatch.setattr(S, "solve_request", fake_solve)
+    S._live = live
+    return S
+
+
+def test_slots_and_budget_respected(serving):
+    S = serving
+    reqs = [S.Request(k, 8, 8, 8, 0.3, 0.05, 1 + k % 4, k) for k in range(20)]
+    budget = 2.5 * max(r.footprint() for r in reqs)
+    eng = S.ServingEngine(slots=3, hbm_budget=budget)
+    for r in reqs:
+        eng.submit(r)
+    done = eng.drain(timeout=30)
+    assert len(done) == 20
+    assert S._live["max"] <= 3
+    assert S._live["max_bytes"] <= budget
+    for r in done:
+        assert r.finish >= r.start >= r.arrival
+
+
+def test_large_request_not_starved(serving):
+    S = serving
+    big = S.Request(0, 24, 48, 48, 0.3, 0.05, 12, 1)
+    small = [S.Request(k, 8, 8, 8, 0.3, 0.05, 1, k) for k in range(1, 30)]
+    budget = big.footprint() * 1.01
+    eng = S.ServingEngine(slots=4, hbm_budget=budget, max_skip=3)
+    eng.submit(small[0])
+    eng.submit(big)
+    for r in small[1:]:
+        eng.submit(r)
+    done = eng.drain(timeout=60)
+    assert len(done) == 30
+    order = [r.rid for r in sorted(done, key=lambda r: r.start)]
+    # the big request may be passed over at most max_skip times plus the ones already running
+    assert order.index(0) <= 3 + 4
+
+
+def test_run_load_metrics(serving):
+    S = serving
+
+    def small(rid, rng, arrival=0.0, tol=1e-10):
+        return S.Request(rid, 8, 8, 8, 0.3, 0.05, 1, rid, tol, arrival)
+
+    m = S.run_load(rate=200.0, duration=0.3, slots=4, request_fn=small)
+    assert m["requests"] >= 10
+    assert m["p99_latency_s"] >= m["p50_latency_s"] > 0
+    assert m["solves_per_s"] > 0
