@@ -111,7 +111,8 @@ def time_dslash(G, hr, dims, links, prec, steps, warmup, nbuf, stream):
     return float(np.mean(per)), float(t0.elapsed_ms(t1) / steps)
 
 
-def bench_multishift(G, hr, dims, l64, l32, am, nsrc, stream, tol=5e-11, maxcg=100000):
+def bench_multishift(G, hr, dims, l64, l32, am, nsrc, stream, tol=5e-11, maxcg=30000,
+                     mixed_th32=1e-5, mixed_th16=0.0):
     """Plain FP64 path (one source at a time) vs the batched mixed path."""
     lx, ly, lt, ls = dims
     nsp = 4 * lx * ly * lt * ls
@@ -135,23 +136,30 @@ def bench_multishift(G, hr, dims, l64, l32, am, nsrc, stream, tol=5e-11, maxcg=1
     out = {"plain_fp64": {"solves_per_s": round(1.0 / t_plain, 4), "seconds": round(t_plain, 3),
                           "iterations": int(cg_plain),
                           "max_true_residual": float(rel_plain.max())}}
-    for label, th32, th16 in (("blocked_fp64", 0.0, 0.0), ("blocked_mixed", 1e-4, 0.0)):
+    # both batched legs verify every (source, shift) in FP64 inside the timed
+    # region and correct the ones above tol (defect correction)
+    legs = (("blocked_fp64", 0.0, 0.0, 2), ("blocked_mixed", mixed_th32, mixed_th16, 4))
+    for label, th32, th16, corr in legs:
         x = hr.DeviceArray((nsrc, n, nsp, 2), np.float64)
-        progress(f"multishift {label} x{nsrc} sources")
+        progress(f"multishift {label} x{nsrc} sources (th32={th32}, th16={th16})")
         hr.synchronize()
+        st = {}
         t = time.perf_counter()
         it, cgr, _ = G.multishift_fast_dev(l64, l32, dims, am, 3, sig, b, x, tol=tol, maxcg=maxcg,
-                                           nsrc=nsrc, th32=th32, th16=th16,
-                                           true_residuals=False, stream=stream)
+                                           nsrc=nsrc, th32=th32, th16=th16, corrections=corr,
+                                           true_residuals=False, stats=st, stream=stream)
         hr.synchronize()
         dt = time.perf_counter() - t
         rel = G.true_residuals_dev(l64, dims, am, 3, sig, b, x, nsrc=nsrc, stream=stream)
         out[label] = {"solves_per_s": round(nsrc / dt, 4), "seconds": round(dt, 3),
                       "sources": nsrc, "iterations": int(it),
+                      "correction_iterations": int(st.get("corr_iterations", 0)),
+                      "th32": th32, "th16": th16,
                       "max_true_residual": float(rel.max())}
         del x
-    out["speedup_mixed_vs_plain_fp64"] = round(out["blocked_mixed"]["solves_per_s"] /
-                                               out["plain_fp64"]["solves_per_s"], 3)
+    for label in ("blocked_fp64", "blocked_mixed"):
+        out[f"speedup_{label}_vs_plain_fp64"] = round(out[label]["solves_per_s"] /
+                                                      out["plain_fp64"]["solves_per_s"], 3)
     return out
 
 
@@ -188,9 +196,15 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solver", action="store_true")
-    ap.add_argument("--solver-lattice", default="16,16,16,48")
-    ap.add_argument("--solver-am", type=float, default=0.01)
+    ap.add_argument("--solver-lattice", default="32,32,32,64")
+    ap.add_argument("--solver-am", type=float, default=0.005)
     ap.add_argument("--solver-sources", type=int, default=4)
+    ap.add_argument("--mixed-th32", type=float, default=1e-5)
+    ap.add_argument("--mixed-th16", type=float, default=0.0)
+    ap.add_argument("--tchunk", type=int, default=0,
+                    help="pin the Dslash t-chunk (with --shape-flags) instead of autotuning")
+    ap.add_argument("--shape-flags", type=int, default=7,
+                    help="with --tchunk: 1 XCD remap | 2 buffer loads | 4 lane-shift s hops")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run N device copies of 1 GiB (scales the HBM counters)")
     args = ap.parse_args()
@@ -218,6 +232,15 @@ def main():
     th, l64, l32 = make_fields(G, hr.DeviceArray, dims, 0.3, 1234)
 
     t_wall0 = time.time()
+    shapes = {}
+    for prec, lk in (("f64", l64), ("f32", l32), ("f16", l32)):
+        if args.tchunk > 0:
+            G.dslash_set_config(dims, prec, args.tchunk, args.shape_flags)
+        else:
+            progress(f"autotune dslash {prec}")
+            G.dslash_autotune(lk, dims, prec, stream=stream)
+        tc, fl = G.dslash_get_config(dims, prec)
+        shapes[prec] = {"tchunk": tc, "flags": fl}
     nbuf64 = max(2, int(np.ceil(4 * 256e6 / (2 * 16 * 4 * v5))))
     progress(f"dslash f64 {dims} x{args.steps}")
     ms64, region64 = time_dslash(G, hr, dims, l64, "f64", args.steps, args.warmup, nbuf64, stream)
@@ -233,7 +256,8 @@ def main():
     if not args.no_solver:
         sd = tuple(int(v) for v in args.solver_lattice.split(","))
         _, s64, s32 = make_fields(G, hr.DeviceArray, sd, 0.3, 4321)
-        solver = bench_multishift(G, hr, sd, s64, s32, args.solver_am, args.solver_sources, stream)
+        solver = bench_multishift(G, hr, sd, s64, s32, args.solver_am, args.solver_sources, stream,
+                                  mixed_th32=args.mixed_th32, mixed_th16=args.mixed_th16)
         solver["lattice"] = list(sd)
         solver["am"] = args.solver_am
     wall = time.time() - t_wall0
@@ -264,6 +288,7 @@ def main():
         "config": {"workload": "dslash_shamir 32x32x32 Ls=64 imass=3 am=0.005 am3=1 (M and "
                                "M^dagger alternating), rotating buffers beyond the Infinity Cache",
                    "lattice": list(dims), "rotating_buffer_sets": nbuf64,
+                   "launch_shape": shapes,
                    "region_ms_per_step": round(region64, 5)},
         "roofline": {"bound": "hbm", "achieved": round(gbs64, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(gbs64 / HBM_PEAK_GBS, 4), "traffic": traffic},

@@ -147,6 +147,25 @@ int thr_dslash_dev(void *out, float *out_norm, const void *in, const float *in_n
                    const void *links, const int dims[4], int nrhs, const float *am, int imass,
                    float am3, int dagger, int prec, void *stream);
 
+/* Launch shape of the Dslash kernel for (dims, prec): each thread marches
+ * through `tchunk` t-slices (t hops from registers); `flags` is a bit set:
+ *   THR_SHAPE_XCD_REMAP  give every XCD a contiguous range of blocks
+ *   THR_SHAPE_BUFLOAD    read fields through buffer resources (32-bit offsets)
+ *   THR_SHAPE_LANESHIFT  s hops as DPP lane shifts (used when Ls divides 64)
+ * No counterpart in the reference (a tuning knob of this library).
+ * thr_dslash_autotune times the candidates (tchunk 1..16 x flags) on scratch
+ * fields with the given links (complex128 for F64, complex64 otherwise) and
+ * pins the fastest for the process; thr_dslash_set_config pins a given shape;
+ * both apply to every later launch (solvers included) with these dims and
+ * precision. */
+#define THR_SHAPE_XCD_REMAP 1
+#define THR_SHAPE_BUFLOAD 2
+#define THR_SHAPE_LANESHIFT 4
+int thr_dslash_autotune(const void *links, const int dims[4], int prec, int *tchunk, int *flags,
+                        void *stream);
+int thr_dslash_set_config(const int dims[4], int prec, int tchunk, int flags);
+int thr_dslash_get_config(const int dims[4], int prec, int *tchunk, int *flags);
+
 /* reference-faithful FP64 CG on device fields (phi, x: 4*V5 complex128) */
 int thr_congrad_dev(const void *phi, void *x, const void *links64, const int dims[4], float res,
                     float am, int imass, float am3, int niterc, int kernel, int *itercg,
@@ -170,15 +189,21 @@ int thr_multishift_plain_dev(const void *links64, const int dims[4], float am, i
  * b[nsrc][4 V5] (complex128) with masses am[nsrc], the same nshift shifts for
  * all; solutions x[nsrc][nshift][4 V5] (complex128).  Each shift stops when
  * |b - (M^+M + aden_i) x_i| (iterated) <= tol |b|; Krylov storage drops to FP32
- * below |r|/|b| < th32 and to FP16 below th16 (<= 0 disables).  iterations:
- * iterations run; cg_return[nsrc]: per-source iterations; relres (optional,
- * nsrc*nshift): true FP64 residuals |b - (M^+M + sigma_i) x_i| / |b|.
+ * below |r|/|b| < th32 and to FP16 below th16 (<= 0 disables).
+ * max_corrections (0..16) > 0 adds FP64 defect correction: the true residual of
+ * every (source, shift) is computed in FP64 and each one above tol |b| is solved
+ * again for its residual (FP32 Krylov storage) and the correction added, up to
+ * max_corrections rounds, so low-precision storage still meets tol in FP64.
+ * iterations: iterations run; cg_return[nsrc]: per-source iterations;
+ * corr_iterations (optional): iterations spent in corrections; relres
+ * (optional, nsrc*nshift): final true FP64 residuals |b - (M^+M + sigma_i) x_i| / |b|.
  * Same recurrences as multishift_module.F90:103-282 (see thr_multishift). */
 int thr_multishift_fast_dev(const void *links64, const void *links32, const int dims[4],
                             int nsrc, const float *am, int imass, float am3, int nshift,
                             const double *aden, const void *b, void *x, double tol, int maxcg,
-                            double th32, double th16, int *iterations, int *cg_return,
-                            double *relres, void *stream);
+                            double th32, double th16, int max_corrections, int *iterations,
+                            int *cg_return, int *corr_iterations, double *relres,
+                            void *stream);
 
 /* True FP64 residuals relres[src*nshift + i] = |b - (M^+M + aden_i) x_i| / |b|
  * for solutions laid out as thr_multishift_fast_dev's x (check of any solver) */

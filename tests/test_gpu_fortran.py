@@ -77,8 +77,16 @@ def test_fortran_qmrherm_and_derivs(tmp_path):
     i = np.arange(1, 26)
     anum = np.concatenate([[0.5], np.cos(i * TAU / 25)])
     aden = np.cos(0.5 * i * TAU / 25)
+    # Phi0 as the Fortran program built it (its 1.7*exp(1.0) may be folded in
+    # higher precision than the reference's default real)
+    pin = load(tmp_path / "qmrherm_phi0_in.bin", (24, 12, 12, 12, 4, 25))
+    assert rel(pin, st["phi0"][1:-1, 1:-1, 1:-1, 1:-1]) <= 1e-6
+    phi0 = np.zeros_like(st["phi0"])
+    for l in range(25):
+        phi0[1:-1, 1:-1, 1:-1, 1:-1, :, l] = pin[..., l]
+        phi0[..., l] = refstate.halo5(np.asfortranarray(phi0[..., l]))
     for iflag in range(4):
-        p0 = st["phi0"].copy(order="F")
+        p0 = phi0.copy(order="F")
         ds = np.asfortranarray(st["dsdpi"].copy())
         x, it, x1, p0, ds, _ = O.qmrherm(st["phi"], 0.1, 0.05, 3, anum, aden, iflag, st["u"], DIMS,
                                          phi0=p0, dsdpi=ds, max_qmr_iters=2)

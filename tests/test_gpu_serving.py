@@ -39,8 +39,9 @@ def test_hbm_admission_serialises_big_requests():
     import serving
     import thirring_hip as G
     G.init(0)
-    reqs = _reqs()[:3]
-    budget = max(r.footprint() for r in reqs) * 1.01
+    # equal footprints: any two together exceed the budget
+    reqs = [serving.Request(k, 8, 8, 8, 0.3 + 0.05 * k, 0.05, 2, 20 + k) for k in range(3)]
+    budget = reqs[0].footprint() * 1.5
     eng = serving.ServingEngine(slots=3, hbm_budget=budget)
     for r in reqs:
         eng.submit(r)

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <algorithm>
 #include <vector>
 
 #include "../../include/thirring_hip.h"
@@ -469,6 +470,46 @@ int thr_dslash_dev(void *out, float *out_norm, const void *in, const float *in_n
   return launch_dslash(L, pick(stream));
 }
 
+static int shape_flags(const DslashConfig &c) {
+  return (c.remap ? THR_SHAPE_XCD_REMAP : 0) | (c.bufload ? THR_SHAPE_BUFLOAD : 0) |
+         (c.laneshift ? THR_SHAPE_LANESHIFT : 0);
+}
+
+int thr_dslash_autotune(const void *links, const int dims[4], int prec, int *tchunk, int *flags,
+                        void *stream) {
+  TRY(check_dims(dims));
+  THR_CHECK(links, THR_ERR_ARG, "NULL argument");
+  THR_CHECK(prec >= 0 && prec <= 2, THR_ERR_ARG, "prec must be THR_F64, THR_F32 or THR_F16");
+  DslashConfig c;
+  TRY(dslash_autotune(mkdims(dims), prec, links, &c, pick(stream)));
+  if (tchunk) *tchunk = c.tchunk;
+  if (flags) *flags = shape_flags(c);
+  return THR_OK;
+}
+
+int thr_dslash_set_config(const int dims[4], int prec, int tchunk, int flags) {
+  TRY(check_dims(dims));
+  THR_CHECK(prec >= 0 && prec <= 2, THR_ERR_ARG, "prec must be THR_F64, THR_F32 or THR_F16");
+  THR_CHECK(tchunk >= 1 && tchunk <= dims[2], THR_ERR_ARG, "tchunk must be in 1..Lt");
+  THR_CHECK((flags & ~7) == 0, THR_ERR_ARG, "unknown shape flags");
+  DslashConfig c;
+  c.tchunk = tchunk;
+  c.remap = (flags & THR_SHAPE_XCD_REMAP) ? 1 : 0;
+  c.bufload = (flags & THR_SHAPE_BUFLOAD) ? 1 : 0;
+  c.laneshift = (flags & THR_SHAPE_LANESHIFT) ? 1 : 0;
+  dslash_set_config(mkdims(dims), prec, c);
+  return THR_OK;
+}
+
+int thr_dslash_get_config(const int dims[4], int prec, int *tchunk, int *flags) {
+  TRY(check_dims(dims));
+  THR_CHECK(prec >= 0 && prec <= 2, THR_ERR_ARG, "prec must be THR_F64, THR_F32 or THR_F16");
+  const DslashConfig c = dslash_config(mkdims(dims), prec);
+  if (tchunk) *tchunk = c.tchunk;
+  if (flags) *flags = shape_flags(c);
+  return THR_OK;
+}
+
 int thr_congrad_dev(const void *phi, void *x, const void *links64, const int dims[4], float res,
                     float am, int imass, float am3, int niterc, int kernel, int *itercg,
                     int *iterations, void *stream) {
@@ -514,11 +555,14 @@ int thr_multishift_plain_dev(const void *links64, const int dims[4], float am, i
 int thr_multishift_fast_dev(const void *links64, const void *links32, const int dims[4],
                             int nsrc, const float *am, int imass, float am3, int nshift,
                             const double *aden, const void *b, void *x, double tol, int maxcg,
-                            double th32, double th16, int *iterations, int *cg_return,
-                            double *relres, void *stream) {
+                            double th32, double th16, int max_corrections, int *iterations,
+                            int *cg_return, int *corr_iterations, double *relres,
+                            void *stream) {
   TRY(check_dims(dims));
   TRY(check_op(0, imass));
   THR_CHECK(links64 && links32 && am && aden && b && x, THR_ERR_ARG, "NULL argument");
+  THR_CHECK(max_corrections >= 0 && max_corrections <= 16, THR_ERR_ARG,
+            "max_corrections out of range (0..16)");
   MsfArgs A{};
   A.d = mkdims(dims);
   A.nsrc = nsrc;
@@ -541,7 +585,14 @@ int thr_multishift_fast_dev(const void *links64, const void *links32, const int 
   if (iterations) *iterations = R.iterations;
   if (cg_return)
     for (int k = 0; k < nsrc; k++) cg_return[k] = R.cg_return[k];
-  if (relres) TRY(msfast_true_residuals(A, relres, s));
+  if (corr_iterations) *corr_iterations = 0;
+  if (max_corrections > 0) {
+    std::vector<double> rr((size_t)nsrc * nshift);
+    TRY(msfast_correct(A, max_corrections, rr.data(), corr_iterations, s));
+    if (relres) std::copy(rr.begin(), rr.end(), relres);
+  } else if (relres) {
+    TRY(msfast_true_residuals(A, relres, s));
+  }
   THR_HIP(hipStreamSynchronize(s));
   return THR_OK;
 }

@@ -15,6 +15,13 @@
 // dimension neighbours s+-1 are the adjacent lanes' centre loads and hit in
 // L1; the spin projection and gamma algebra are register arithmetic (no
 // MFMA: the operator is a sparse stencil, not a dense contraction).
+#include <algorithm>
+#include <cstdlib>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <type_traits>
+
 #include "kernels.h"
 
 namespace thr {
@@ -68,6 +75,92 @@ template <int PREC> struct Out {
   }
 };
 
+// Buffer-resource loads: base and size live in SGPRs, each lane supplies a
+// 32-bit byte offset and the spin plane is a scalar offset, so the four spin
+// loads of a site share one VGPR address (global loads need a 64-bit VGPR
+// pair each).  Out-of-range offsets return 0 instead of faulting.
+constexpr int BUF_DWORD3 = 0x00020000;  // gfx9 raw buffer: 32-bit data format
+
+template <int PREC> struct BAcc {
+  using C = typename Store<PREC>::C;
+  __amdgpu_buffer_rsrc_t rs, rn;
+  unsigned plane;  // bytes per spin plane
+  __device__ inline BAcc(const void *p, const float *nrm, unsigned v5) {
+    plane = v5 * (unsigned)Store<PREC>::bytes;
+    rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)(4u * plane),
+                                           BUF_DWORD3);
+    rn = __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(nrm), (short)0,
+                                           nrm ? (int)(v5 * 4u) : 0, BUF_DWORD3);
+  }
+  __device__ inline cx<C> ld(unsigned site, unsigned a, float sc) const {
+    const unsigned off = site * (unsigned)Store<PREC>::bytes;
+    if constexpr (PREC == P64) {
+      const double2 v =
+          __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(rs, off, a * plane, 0));
+      return {v.x, v.y};
+    } else if constexpr (PREC == P32) {
+      const float2 v =
+          __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rs, off, a * plane, 0));
+      return {v.x, v.y};
+    } else {
+      const __half2 h =
+          __builtin_bit_cast(__half2, __builtin_amdgcn_raw_buffer_load_b32(rs, off, a * plane, 0));
+      const float2 f = __half22float2(h);
+      return {f.x * sc, f.y * sc};
+    }
+  }
+  __device__ inline float scale(unsigned site) const {
+    if constexpr (PREC == P16)
+      return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rn, site * 4u, 0, 0));
+    else
+      return 1.0f;
+  }
+  __device__ inline void load4(unsigned site, cx<C> v[4]) const {
+    const float sc = scale(site);
+#pragma unroll
+    for (int a = 0; a < 4; a++) v[a] = ld(site, a, sc);
+  }
+  __device__ inline cx<C> load1(unsigned site, unsigned a) const { return ld(site, a, scale(site)); }
+};
+
+template <typename C> struct BLinks {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ inline BLinks(const void *p, unsigned n) {
+    r = __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0,
+                                          (int)(n * 2u * (unsigned)sizeof(C)), BUF_DWORD3);
+  }
+  __device__ inline cx<C> operator()(unsigned i) const {
+    if constexpr (sizeof(C) == 8) {
+      const double2 v =
+          __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, i * 16u, 0, 0));
+      return {v.x, v.y};
+    } else {
+      const float2 v =
+          __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, i * 8u, 0, 0));
+      return {v.x, v.y};
+    }
+  }
+};
+
+template <typename C> struct GLinks {
+  const void *p;
+  __device__ inline GLinks(const void *q, unsigned) : p(q) {}
+  __device__ inline cx<C> operator()(unsigned i) const {
+    if constexpr (sizeof(C) == 8) {
+      const double2 v = ((const double2 *)p)[i];
+      return {v.x, v.y};
+    } else {
+      const float2 v = ((const float2 *)p)[i];
+      return {v.x, v.y};
+    }
+  }
+};
+
+template <int PREC> struct GAcc : Acc<PREC> {
+  __device__ inline GAcc(const void *p, const float *nrm, unsigned v5)
+      : Acc<PREC>{(const typename Store<PREC>::T *)p, nrm, v5} {}
+};
+
 template <typename C> __device__ inline cx<C> link(const void *links, long i) {
   if constexpr (sizeof(C) == 8) {
     const double2 v = ((const double2 *)links)[i];
@@ -99,167 +192,289 @@ template <typename C> __device__ inline void add_gamma(int mu, cx<C> o[4], const
   }
 }
 
-template <int PREC, bool DAG>
-__global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L) {
+// value of v held by lane+1 (NEXT) or lane-1 (PREV) of the wave: DPP wave
+// shifts, register-to-register; lanes at the wave edge read 0 (never used:
+// they sit at s = Ls-1 / s = 0, where the s hop is absent)
+template <bool NEXT> __device__ inline float lane_shift(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), NEXT ? 0x130 : 0x138,
+                                                    0xF, 0xF, false));
+}
+template <bool NEXT> __device__ inline double lane_shift(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), NEXT ? 0x130 : 0x138,
+                                             0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), NEXT ? 0x130 : 0x138, 0xF, 0xF,
+                                             false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+template <bool NEXT, typename C> __device__ inline cx<C> lane_shift(cx<C> v) {
+  return {lane_shift<NEXT>(v.r), lane_shift<NEXT>(v.i)};
+}
+
+// 4-D hop in direction mu: o -= 1/2 (A + B), o += 1/2 gamma_mu (A - B)
+// with A = U_mu(x) R(x+mu), B = U_mu(x-mu)^* R(x-mu) (roles of A, B swapped
+// in the gamma term for M^dagger)
+template <typename C, bool DAG>
+__device__ inline void hop(int mu, cx<C> o[4], cx<C> uf, const cx<C> rf[4], cx<C> ub,
+                           const cx<C> rb[4]) {
+  cx<C> W[4], D[4];
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+    const cx<C> A = cmul(uf, rf[a]);
+    const cx<C> B = cmul(ub, rb[a]);
+    W[a] = A + B;
+    D[a] = DAG ? (B - A) : (A - B);
+  }
+#pragma unroll
+  for (int a = 0; a < 4; a++) o[a] = o[a] - C(0.5) * W[a];
+  add_gamma<C>(mu, o, D);
+}
+
+// One thread owns the s-x-y site (s fastest) of one t-chunk and marches
+// through it: R(t+1) loaded at step t is the centre at t+1 and the backward
+// t neighbour at t+2, so the t hops come from registers.  x and y hops are
+// L1/L2 hits (neighbouring lanes and waves of the same and nearby blocks).
+// With SHUF (Ls divides 64) the s hops are DPP lane shifts of the centre.
+// XCD remap: hardware block b runs on XCD b % 8; remapping gives every XCD a
+// contiguous range of t-chunks, so t and y neighbours are fetched from the
+// same L2.  Partial sums go to partial[rhs * pstride + blockIdx.x]; block 0
+// zero-fills [gridDim.x, pstride) so consumers can always sum pstride entries.
+template <int PREC, bool DAG, bool SHUF, int MODE, bool BUF>
+__global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, unsigned tchunk,
+                                                unsigned nb3, int remap, unsigned pstride) {
   using C = typename Store<PREC>::C;
   using T = typename Store<PREC>::T;
   const int rhs = blockIdx.y;
   if (L.done && L.done[(long)rhs * L.done_stride]) return;
   // 32-bit index arithmetic: 4*V5 < 2^31 is checked on the host
   const unsigned lx = L.d.lx, ly = L.d.ly, lt = L.d.lt, ls = L.d.ls;
-  const unsigned v4 = lx * ly * lt, v5 = v4 * ls;
-  const unsigned i = blockIdx.x * BLK + threadIdx.x;
+  const unsigned v2 = lx * ly, v3 = v2 * ls, v4 = v2 * lt, v5 = v4 * ls;
+  unsigned b = blockIdx.x;
+  if (remap) {
+    // blocks b, b+8, b+16, ... (one XCD) -> consecutive logical blocks
+    const unsigned nbl = gridDim.x, per = nbl >> 3, full = per << 3;
+    if (b < full) b = (b & 7) * per + (b >> 3);
+  }
+  const unsigned chunk = b / nb3;
+  const unsigned j = (b - chunk * nb3) * BLK + threadIdx.x;
+  const unsigned t0 = chunk * tchunk;
   double red = 0.0;
-  if (i < v5) {
-    const unsigned s = i % ls;
-    const unsigned site4 = i / ls;
-    const unsigned x = site4 % lx;
-    const unsigned yt = site4 / lx;
-    const unsigned y = yt % ly;
-    const unsigned t = yt / ly;
+  if (j < v3 && t0 < lt) {
+    const unsigned t1 = min(t0 + tchunk, lt);
+    const unsigned s = j % ls;
+    const unsigned xy = j / ls;
+    const unsigned x = xy % lx, y = xy / lx;
+    const unsigned xf = (x + 1 == lx) ? 0 : x + 1, xb = (x == 0) ? lx - 1 : x - 1;
+    const unsigned yf = (y + 1 == ly) ? 0 : y + 1, yb = (y == 0) ? ly - 1 : y - 1;
+    // s-x-y offsets of the x and y neighbours, and the 2-D link offsets
+    const unsigned jxf = s + ls * (xf + lx * y), jxb = s + ls * (xb + lx * y);
+    const unsigned jyf = s + ls * (x + lx * yf), jyb = s + ls * (x + lx * yb);
+    const unsigned lxb = xb + lx * y, lyb = x + lx * yb;
     const long roff = (long)rhs * 4 * v5;
-    Acc<PREC> in{(const T *)L.in + roff, L.in_norm ? L.in_norm + (long)rhs * v5 : nullptr, v5};
-
-    cx<C> o[4], c[4];
-    in.load4(i, c);
-    // D = (3.0 - am3) + 1.0 in default real (dirac.F90:45); the 4-D Wilson slice
-    // operator DWilson(.., -am3) has diag = 3.0 + (-am3) (dirac.F90:191)
+    using In = std::conditional_t<BUF, BAcc<PREC>, GAcc<PREC>>;
+    using Lk = std::conditional_t<BUF, BLinks<C>, GLinks<C>>;
+    const In in((const T *)L.in + roff, L.in_norm ? L.in_norm + (long)rhs * v5 : nullptr, v5);
+    const Lk lk(L.links, 3 * v4);
     const C diag = L.wilson4d ? (C)(3.0f + (-L.am3)) : (C)((3.0f - L.am3) + 1.0f);
-#pragma unroll
-    for (int a = 0; a < 4; a++) o[a] = diag * c[a];
-
-#pragma unroll
-    for (int mu = 0; mu < 3; mu++) {
-      unsigned xf = x, yf = y, tf = t, xb = x, yb = y, tb = t;
-      if (mu == 0) {
-        xf = (x + 1 == lx) ? 0 : x + 1;
-        xb = (x == 0) ? lx - 1 : x - 1;
-      } else if (mu == 1) {
-        yf = (y + 1 == ly) ? 0 : y + 1;
-        yb = (y == 0) ? ly - 1 : y - 1;
-      } else {
-        tf = (t + 1 == lt) ? 0 : t + 1;
-        tb = (t == 0) ? lt - 1 : t - 1;
-      }
-      const unsigned sf4 = xf + lx * (yf + ly * tf);
-      const unsigned sb4 = xb + lx * (yb + ly * tb);
-      const cx<C> uf = link<C>(L.links, mu * v4 + site4);
-      const cx<C> ub = conj(link<C>(L.links, mu * v4 + sb4));
-      cx<C> rf[4], rb[4];
-      in.load4(sf4 * ls + s, rf);
-      in.load4(sb4 * ls + s, rb);
-      cx<C> W[4], D[4];
-#pragma unroll
-      for (int a = 0; a < 4; a++) {
-        const cx<C> A = cmul(uf, rf[a]);
-        const cx<C> B = cmul(ub, rb[a]);
-        W[a] = A + B;
-        D[a] = DAG ? (B - A) : (A - B);
-      }
-#pragma unroll
-      for (int a = 0; a < 4; a++) o[a] = o[a] - C(0.5) * W[a];
-      add_gamma<C>(mu, o, D);
-    }
-
-    // hops along s (dirac.F90:80-87 for M, 317-324 for M^dagger): spins that
-    // hop from s+1 are {3,4} for M and {1,2} for M^dagger; the others from s-1.
     const int up0 = DAG ? 0 : 2, dn0 = DAG ? 2 : 0;
     const int imass = L.wilson4d ? 0 : L.imass;
-    if (!L.wilson4d && s + 1 < ls) {
-      o[up0] = o[up0] - in.load1(i + 1, up0);
-      o[up0 + 1] = o[up0 + 1] - in.load1(i + 1, up0 + 1);
-    }
-    if (!L.wilson4d && s > 0) {
-      o[dn0] = o[dn0] - in.load1(i - 1, dn0);
-      o[dn0 + 1] = o[dn0 + 1] - in.load1(i - 1, dn0 + 1);
-    }
-    // wall mass (dirac.F90:119-176 for M, 356-402 for M^dagger)
     const C m = (C)L.am[rhs];
-    const unsigned col = site4 * ls;  // s = 0 of this (x,y,t)
-    if (imass == 1 || imass == 3) {
-      // top: s = Ls couples to s = 1 on the up0 spins; bottom: s = 1 couples to s = Ls
-      if (s == ls - 1) {
-#pragma unroll
-        for (int k = 0; k < 2; k++) {
-          const cx<C> r0 = in.load1(col, up0 + k);
-          o[up0 + k] = o[up0 + k] + (imass == 1 ? m * r0 : m * muli(r0));
-        }
-      }
-      if (s == 0) {
-#pragma unroll
-        for (int k = 0; k < 2; k++) {
-          const cx<C> rL = in.load1(col + ls - 1, dn0 + k);
-          o[dn0 + k] = o[dn0 + k] + (imass == 1 ? m * rL : (-m) * muli(rL));
-        }
-      }
-    } else if (imass == 5) {
-      // same-s chirality flip at both walls: +i m (M), -i m (M^dagger)
-      const C sg = DAG ? C(-1) : C(1);
-      if (s == ls - 1) {
-#pragma unroll
-        for (int k = 0; k < 2; k++)
-          o[up0 + k] = o[up0 + k] + (sg * m) * muli(c[dn0 + k]);
-      }
-      if (s == 0) {
-#pragma unroll
-        for (int k = 0; k < 2; k++)
-          o[dn0 + k] = o[dn0 + k] + (sg * m) * muli(c[up0 + k]);
-      }
-    }
+    const bool shops = !L.wilson4d;
 
-    // epilogue
-    if (L.mode == EPI_CG) {
-      const C a = (C)L.a_ptr[(long)rhs * L.sc_stride];
-      T *r = (T *)L.r + roff;
-      T *ro = L.r_out ? (T *)L.r_out + roff : r;
-      Acc<PREC> rr{r, L.r_norm ? L.r_norm + (long)rhs * v5 : nullptr, v5};
-      cx<C> rv[4];
-      rr.load4(i, rv);
+    cx<C> prv[4], cur[4], nxt[4];
+    const unsigned tb0 = (t0 == 0) ? lt - 1 : t0 - 1;
+    in.load4(j + v3 * tb0, prv);
+    cx<C> ubt = conj(lk(2 * v4 + xy + v2 * tb0));
+    in.load4(j + v3 * t0, cur);
+    for (unsigned t = t0; t < t1; t++) {
+      const unsigned tf = (t + 1 == lt) ? 0 : t + 1;
+      in.load4(j + v3 * tf, nxt);
+      const cx<C> uft = lk(2 * v4 + xy + v2 * t);
+      const unsigned i = j + v3 * t;
+      const unsigned l4 = v2 * t;
+      cx<C> o[4];
 #pragma unroll
-      for (int k = 0; k < 4; k++) {
-        rv[k] = rv[k] - a * o[k];
-        red += (double)norm2(rv[k]);
+      for (int a = 0; a < 4; a++) o[a] = diag * cur[a];
+      {
+        cx<C> rf[4], rb[4];
+        in.load4(jxf + v3 * t, rf);
+        in.load4(jxb + v3 * t, rb);
+        hop<C, DAG>(0, o, lk(xy + l4), rf, conj(lk(lxb + l4)), rb);
       }
-      Out<PREC> rout{ro, L.r_out_norm ? L.r_out_norm + (long)rhs * v5 : nullptr, v5};
-      rout.store4(i, rv);
-      if (L.b_ptr) {
-        const C b = (C)L.b_ptr[(long)rhs * L.sc_stride];
-        Acc<PREC> pp{(const T *)L.p + roff, nullptr, v5};
-        Acc<PREC> xx{(const T *)L.x + roff, nullptr, v5};
-        cx<C> pv[4], xv[4];
-        pp.load4(i, pv);
-        xx.load4(i, xv);
-#pragma unroll
-        for (int k = 0; k < 4; k++) xv[k] = xv[k] + b * pv[k];
-        Out<PREC> xo{(T *)L.x + roff, nullptr, v5};
-        xo.store4(i, xv);
+      {
+        cx<C> rf[4], rb[4];
+        in.load4(jyf + v3 * t, rf);
+        in.load4(jyb + v3 * t, rb);
+        hop<C, DAG>(1, o, lk(v4 + xy + l4), rf,
+                    conj(lk(v4 + lyb + l4)), rb);
       }
-    } else {
-      Out<PREC> out{(T *)L.out + roff, L.out_norm ? L.out_norm + (long)rhs * v5 : nullptr, v5};
-      out.store4(i, o);
-      if (L.mode == EPI_NORM) {
-#pragma unroll
-        for (int k = 0; k < 4; k++) red += (double)norm2(o[k]);
-      } else if (L.mode == EPI_DOTW) {
-        Acc<PREC> w{(const T *)L.w + roff, nullptr, v5};
-        cx<C> wv[4];
-        w.load4(i, wv);
-#pragma unroll
-        for (int k = 0; k < 4; k++) red += (double)(wv[k].r * o[k].r + wv[k].i * o[k].i);
+      hop<C, DAG>(2, o, uft, nxt, ubt, prv);
+
+      // hops along s (dirac.F90:80-87 for M, 317-324 for M^dagger): spins that
+      // hop from s+1 are {3,4} for M and {1,2} for M^dagger; the others from s-1.
+      if constexpr (SHUF) {
+        const cx<C> u0 = lane_shift<true>(cur[up0]), u1 = lane_shift<true>(cur[up0 + 1]);
+        const cx<C> d0 = lane_shift<false>(cur[dn0]), d1 = lane_shift<false>(cur[dn0 + 1]);
+        if (shops && s + 1 < ls) {
+          o[up0] = o[up0] - u0;
+          o[up0 + 1] = o[up0 + 1] - u1;
+        }
+        if (shops && s > 0) {
+          o[dn0] = o[dn0] - d0;
+          o[dn0 + 1] = o[dn0 + 1] - d1;
+        }
+      } else {
+        if (shops && s + 1 < ls) {
+          o[up0] = o[up0] - in.load1(i + 1, up0);
+          o[up0 + 1] = o[up0 + 1] - in.load1(i + 1, up0 + 1);
+        }
+        if (shops && s > 0) {
+          o[dn0] = o[dn0] - in.load1(i - 1, dn0);
+          o[dn0 + 1] = o[dn0 + 1] - in.load1(i - 1, dn0 + 1);
+        }
       }
+      // wall mass (dirac.F90:119-176 for M, 356-402 for M^dagger)
+      const unsigned col = i - s;  // s = 0 of this (x,y,t)
+      if (imass == 1 || imass == 3) {
+        // top: s = Ls couples to s = 1 on the up0 spins; bottom: s = 1 couples to s = Ls
+        if (s == ls - 1) {
+#pragma unroll
+          for (int k = 0; k < 2; k++) {
+            const cx<C> r0 = in.load1(col, up0 + k);
+            o[up0 + k] = o[up0 + k] + (imass == 1 ? m * r0 : m * muli(r0));
+          }
+        }
+        if (s == 0) {
+#pragma unroll
+          for (int k = 0; k < 2; k++) {
+            const cx<C> rL = in.load1(col + ls - 1, dn0 + k);
+            o[dn0 + k] = o[dn0 + k] + (imass == 1 ? m * rL : (-m) * muli(rL));
+          }
+        }
+      } else if (imass == 5) {
+        // same-s chirality flip at both walls: +i m (M), -i m (M^dagger)
+        const C sg = DAG ? C(-1) : C(1);
+        if (s == ls - 1) {
+#pragma unroll
+          for (int k = 0; k < 2; k++)
+            o[up0 + k] = o[up0 + k] + (sg * m) * muli(cur[dn0 + k]);
+        }
+        if (s == 0) {
+#pragma unroll
+          for (int k = 0; k < 2; k++)
+            o[dn0 + k] = o[dn0 + k] + (sg * m) * muli(cur[up0 + k]);
+        }
+      }
+
+      // epilogue
+      if constexpr (MODE == EPI_CG) {
+        const C a = (C)L.a_ptr[(long)rhs * L.sc_stride];
+        T *r = (T *)L.r + roff;
+        T *ro = L.r_out ? (T *)L.r_out + roff : r;
+        Acc<PREC> rr{r, L.r_norm ? L.r_norm + (long)rhs * v5 : nullptr, v5};
+        cx<C> rv[4];
+        rr.load4(i, rv);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          rv[k] = rv[k] - a * o[k];
+          red += (double)norm2(rv[k]);
+        }
+        Out<PREC> rout{ro, L.r_out_norm ? L.r_out_norm + (long)rhs * v5 : nullptr, v5};
+        rout.store4(i, rv);
+        if (L.b_ptr) {
+          const C bb = (C)L.b_ptr[(long)rhs * L.sc_stride];
+          Acc<PREC> pp{(const T *)L.p + roff, nullptr, v5};
+          Acc<PREC> xx{(const T *)L.x + roff, nullptr, v5};
+          cx<C> pv[4], xv[4];
+          pp.load4(i, pv);
+          xx.load4(i, xv);
+#pragma unroll
+          for (int k = 0; k < 4; k++) xv[k] = xv[k] + bb * pv[k];
+          Out<PREC> xo{(T *)L.x + roff, nullptr, v5};
+          xo.store4(i, xv);
+        }
+      } else {
+        Out<PREC> out{(T *)L.out + roff, L.out_norm ? L.out_norm + (long)rhs * v5 : nullptr, v5};
+        out.store4(i, o);
+        if constexpr (MODE == EPI_NORM) {
+#pragma unroll
+          for (int k = 0; k < 4; k++) red += (double)norm2(o[k]);
+        } else if constexpr (MODE == EPI_DOTW) {
+          Acc<PREC> w{(const T *)L.w + roff, nullptr, v5};
+          cx<C> wv[4];
+          w.load4(i, wv);
+#pragma unroll
+          for (int k = 0; k < 4; k++) red += (double)(wv[k].r * o[k].r + wv[k].i * o[k].i);
+        }
+      }
+#pragma unroll
+      for (int a = 0; a < 4; a++) {
+        prv[a] = cur[a];
+        cur[a] = nxt[a];
+      }
+      ubt = conj(uft);
     }
   }
-  if (L.mode != EPI_STORE) {
+  if constexpr (MODE != EPI_STORE) {
     const double tot = block_sum(red);
-    if (threadIdx.x == 0) L.partial[(long)rhs * gridDim.x + blockIdx.x] = tot;
+    double *P = L.partial + (long)rhs * pstride;
+    if (threadIdx.x == 0) P[blockIdx.x] = tot;
+    if (blockIdx.x == 0)
+      for (unsigned k = gridDim.x + threadIdx.x; k < pstride; k += BLK) P[k] = 0.0;
   }
 }
 
 }  // namespace
 
-long dslash_blocks(const Dims &d) { return (d.v5() + BLK - 1) / BLK; }
+// the largest Dslash grid (t-chunk 1): ceil(Lx Ly Ls / BLK) blocks per t-slice;
+// also covers V5 sites when used as a one-site-per-thread grid
+long dslash_blocks(const Dims &d) {
+  return ((long)d.lx * d.ly * d.ls + BLK - 1) / BLK * d.lt;
+}
 
-int launch_dslash(const DslashLaunch &L, hipStream_t s) {
+// ---------------------------------------------------------------------------
+// launch configuration per (dims, precision): t-chunk length, XCD remap,
+// buffer or global loads, lane-shift s hops.  The default is fixed (results
+// identical run to run); dslash_autotune times the candidates on scratch
+// fields and pins the fastest for the process.  THR_DSLASH_TCHUNK / _REMAP /
+// _BUFLOAD / _LANESHIFT override the default.
+// ---------------------------------------------------------------------------
+namespace {
+struct TuneKey {
+  int lx, ly, lt, ls, prec;
+  bool operator<(const TuneKey &o) const {
+    return std::tie(lx, ly, lt, ls, prec) < std::tie(o.lx, o.ly, o.lt, o.ls, o.prec);
+  }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, DslashConfig> g_tuned;
+
+DslashConfig default_config(const Dims &d) {
+  DslashConfig c;
+  c.tchunk = d.lt >= 4 ? 4 : d.lt;
+  c.remap = 1;
+  c.bufload = 1;
+  if (const char *e = std::getenv("THR_DSLASH_TCHUNK")) c.tchunk = std::max(1, std::atoi(e));
+  if (const char *e = std::getenv("THR_DSLASH_REMAP")) c.remap = std::atoi(e) ? 1 : 0;
+  if (const char *e = std::getenv("THR_DSLASH_BUFLOAD")) c.bufload = std::atoi(e) ? 1 : 0;
+  if (const char *e = std::getenv("THR_DSLASH_LANESHIFT")) c.laneshift = std::atoi(e) ? 1 : 0;
+  return c;
+}
+}  // namespace
+
+DslashConfig dslash_config(const Dims &d, int prec) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tuned.find(TuneKey{d.lx, d.ly, d.lt, d.ls, prec});
+  if (it != g_tuned.end()) return it->second;
+  return default_config(d);
+}
+
+void dslash_set_config(const Dims &d, int prec, DslashConfig c) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tuned[TuneKey{d.lx, d.ly, d.lt, d.ls, prec}] = c;
+}
+
+int launch_dslash_cfg(const DslashLaunch &L, DslashConfig c, hipStream_t s) {
   THR_CHECK(L.nrhs >= 1 && L.nrhs <= MAXRHS, THR_ERR_ARG, "nrhs out of range");
   THR_CHECK(L.d.lx > 0 && L.d.ly > 0 && L.d.lt > 0 && L.d.ls > 0, THR_ERR_ARG, "bad dims");
   THR_CHECK(L.imass == 1 || L.imass == 3 || L.imass == 5, THR_ERR_ARG, "imass must be 1, 3 or 5");
@@ -268,18 +483,129 @@ int launch_dslash(const DslashLaunch &L, hipStream_t s) {
                                                    (L.r_out ? L.r_out_norm != nullptr : true)),
             THR_ERR_ARG, "fp16 CG epilogue needs r scales and no x update");
   THR_CHECK(L.prec != P16 || L.mode != EPI_DOTW, THR_ERR_ARG, "fp16 dot epilogue unsupported");
-  dim3 grid((unsigned)dslash_blocks(L.d), (unsigned)L.nrhs);
+  THR_CHECK(4 * L.d.v5() < (1L << 31), THR_ERR_SIZE, "lattice too large for 32-bit indexing");
+  const unsigned tc = (unsigned)std::max(1, std::min(c.tchunk, L.d.lt));
+  const unsigned nchunk = (unsigned)((L.d.lt + tc - 1) / tc);
+  const unsigned v3 = (unsigned)L.d.lx * L.d.ly * L.d.ls;
+  const unsigned nb3 = (v3 + BLK - 1) / BLK;
+  const unsigned pstride = (unsigned)dslash_blocks(L.d);
+  THR_CHECK((long)nb3 * nchunk <= pstride, THR_ERR_ARG, "dslash grid exceeds partial stride");
+  dim3 grid(nb3 * nchunk, (unsigned)L.nrhs);
   dim3 block(BLK);
-#define THR_LAUNCH(P, D) hipLaunchKernelGGL((k_dslash<P, D>), grid, block, 0, s, L)
-  if (L.prec == P64) {
-    if (L.dagger) THR_LAUNCH(P64, true); else THR_LAUNCH(P64, false);
-  } else if (L.prec == P32) {
-    if (L.dagger) THR_LAUNCH(P32, true); else THR_LAUNCH(P32, false);
-  } else {
-    if (L.dagger) THR_LAUNCH(P16, true); else THR_LAUNCH(P16, false);
+  const bool shuf = c.laneshift && (64 % L.d.ls) == 0;
+  const int remap = c.remap;
+  const bool buf = c.bufload != 0;
+  THR_CHECK(!buf || 4 * L.d.v5() * Store<P64>::bytes < (1L << 32), THR_ERR_SIZE,
+            "spinor too large for 32-bit buffer offsets");
+#define THR_LAUNCH(P, D, S, M)                                                              \
+  do {                                                                                      \
+    if (buf)                                                                                \
+      hipLaunchKernelGGL((k_dslash<P, D, S, M, true>), grid, block, 0, s, L, tc, nb3, remap,  \
+                         pstride);                                                          \
+    else                                                                                    \
+      hipLaunchKernelGGL((k_dslash<P, D, S, M, false>), grid, block, 0, s, L, tc, nb3, remap, \
+                         pstride);                                                          \
+  } while (0)
+#define THR_LAUNCH1(P, D, S)                        \
+  switch (L.mode) {                                 \
+    case EPI_STORE: THR_LAUNCH(P, D, S, EPI_STORE); break; \
+    case EPI_NORM: THR_LAUNCH(P, D, S, EPI_NORM); break;   \
+    case EPI_DOTW: THR_LAUNCH(P, D, S, EPI_DOTW); break;   \
+    default: THR_LAUNCH(P, D, S, EPI_CG); break;           \
   }
+#define THR_LAUNCH2(P, D) \
+  if (shuf) { THR_LAUNCH1(P, D, true); } else { THR_LAUNCH1(P, D, false); }
+  if (L.prec == P64) {
+    if (L.dagger) { THR_LAUNCH2(P64, true); } else { THR_LAUNCH2(P64, false); }
+  } else if (L.prec == P32) {
+    if (L.dagger) { THR_LAUNCH2(P32, true); } else { THR_LAUNCH2(P32, false); }
+  } else {
+    if (L.dagger) { THR_LAUNCH2(P16, true); } else { THR_LAUNCH2(P16, false); }
+  }
+#undef THR_LAUNCH2
+#undef THR_LAUNCH1
 #undef THR_LAUNCH
   THR_HIP(hipGetLastError());
+  return THR_OK;
+}
+
+int launch_dslash(const DslashLaunch &L, hipStream_t s) {
+  return launch_dslash_cfg(L, dslash_config(L.d, L.prec), s);
+}
+
+int dslash_autotune(const Dims &d, int prec, const void *links, DslashConfig *best,
+                    hipStream_t s) {
+  THR_CHECK(links, THR_ERR_ARG, "links required");
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  THR_HIP(hipStreamIsCapturing(s, &cs));
+  THR_CHECK(cs == hipStreamCaptureStatusNone, THR_ERR_STATE, "cannot autotune while capturing");
+  const size_t es = prec == P64 ? 16 : prec == P32 ? 8 : 4;
+  const size_t nsp = (size_t)4 * d.v5();
+  void *in = nullptr, *out = nullptr;
+  float *inn = nullptr, *outn = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = THR_OK;
+  DslashConfig pick = default_config(d);
+  float tbest = 1e30f;
+  auto cleanup = [&]() {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(in);
+    (void)hipFree(out);
+    (void)hipFree(inn);
+    (void)hipFree(outn);
+  };
+  if (hipMalloc(&in, es * nsp) != hipSuccess || hipMalloc(&out, es * nsp) != hipSuccess ||
+      hipMalloc((void **)&inn, sizeof(float) * d.v5()) != hipSuccess ||
+      hipMalloc((void **)&outn, sizeof(float) * d.v5()) != hipSuccess ||
+      hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    cleanup();
+    set_error("dslash_autotune: allocation failed");
+    return THR_ERR_HIP;
+  }
+  (void)hipMemsetAsync(in, 0, es * nsp, s);
+  (void)hipMemsetAsync(inn, 0, sizeof(float) * d.v5(), s);
+  DslashLaunch L{};
+  L.d = d;
+  L.nrhs = 1;
+  L.prec = prec;
+  L.imass = 3;
+  L.am3 = 1.0f;
+  L.am[0] = 0.01f;
+  L.in = in;
+  L.in_norm = inn;
+  L.out = out;
+  L.out_norm = outn;
+  L.links = links;
+  L.mode = EPI_STORE;
+  L.sc_stride = 1;
+  L.done_stride = 1;
+  const int tcs[] = {1, 2, 4, 8, 16};
+  for (int tc : tcs) {
+    if (tc > d.lt) continue;
+    for (int v = 0; v < 8 && !rc; v++) {
+      if ((v & 4) && (64 % d.ls) != 0) continue;  // lane shifts need Ls | 64
+      DslashConfig c{tc, v & 1, (v >> 1) & 1, (v >> 2) & 1};
+      for (int k = 0; k < 2 && !rc; k++) rc = launch_dslash_cfg(L, c, s);  // warm
+      if (rc) break;
+      (void)hipEventRecord(e0, s);
+      for (int k = 0; k < 6 && !rc; k++) {
+        L.dagger = k & 1;
+        rc = launch_dslash_cfg(L, c, s);
+      }
+      (void)hipEventRecord(e1, s);
+      if (hipEventSynchronize(e1) != hipSuccess) rc = THR_ERR_HIP;
+      float ms = 0.f;
+      if (!rc && hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms < tbest) {
+        tbest = ms;
+        pick = c;
+      }
+    }
+  }
+  cleanup();
+  if (rc) return rc;
+  dslash_set_config(d, prec, pick);
+  if (best) *best = pick;
   return THR_OK;
 }
 

@@ -44,8 +44,24 @@ struct DslashLaunch {
   int wilson4d;        // 1: 4-D Wilson slice operator only (diag 3 - am3, no s hops, no mass)
 };
 
+// launch shape of the Dslash kernel: t-chunk marched by each thread and
+// whether blocks are remapped so each XCD owns a contiguous lattice range
+struct DslashConfig {
+  int tchunk = 4;
+  int remap = 1;
+  int bufload = 1;    // buffer-resource loads (32-bit offsets) instead of global loads
+  int laneshift = 1;  // s hops as DPP lane shifts when Ls divides 64
+};
+
 int launch_dslash(const DslashLaunch &L, hipStream_t s);
+int launch_dslash_cfg(const DslashLaunch &L, DslashConfig c, hipStream_t s);
+// partial-sum entries per right-hand side written by every Dslash epilogue
 long dslash_blocks(const Dims &d);
+DslashConfig dslash_config(const Dims &d, int prec);
+void dslash_set_config(const Dims &d, int prec, DslashConfig c);
+// times the launch shapes on scratch fields and pins the fastest for (d, prec)
+int dslash_autotune(const Dims &d, int prec, const void *links, DslashConfig *best,
+                    hipStream_t s);
 
 // Wilson-kernel operator on device (FP64): slice-composed, imass 1 or 3
 int launch_dslash_wilson(const Dims &d, int nrhs, int dagger, int imass, float am, float am3,

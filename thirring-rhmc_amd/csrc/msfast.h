@@ -31,6 +31,8 @@ struct MsfArgs {
   int maxcg;
   double th32, th16;   // switch Krylov storage to FP32 / FP16 below these |r|/|b| (<= 0: never)
   int start_prec;      // 0 FP64 (default), 1 FP32, 2 FP16
+  const double *aden_src = nullptr;  // host [nsrc][nshift]: per-source shifts (overrides aden)
+  const double *tol_src = nullptr;   // host [nsrc]: per-source tolerances (overrides tol)
 };
 
 struct MsfResult {
@@ -43,5 +45,12 @@ struct MsfResult {
 int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s);
 // relres[src * nshift + i] = |b - (M^+M + sigma_i) x_i| / |b| in FP64
 int msfast_true_residuals(const MsfArgs &A, double *relres, hipStream_t s);
+// FP64 defect correction after msfast_solve: every (source, shift) whose true
+// residual exceeds tol is solved again for that residual (FP32 Krylov storage,
+// one shift, batched over pairs) and the correction added to x_i; at most
+// max_rounds rounds.  relres (host, nsrc*nshift) receives the final true
+// residuals, corr_iters the iterations spent in corrections.
+int msfast_correct(const MsfArgs &A, int max_rounds, double *relres, int *corr_iters,
+                   hipStream_t s);
 
 }  // namespace thr

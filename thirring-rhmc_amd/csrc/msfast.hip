@@ -20,6 +20,7 @@
 //  * Up to 32 sources with per-source masses run in one launch each step.
 //
 // Per-shift convergence: |r_i| = |zeta_i| |r| <= tol |b|.
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -58,10 +59,11 @@ template <int P> struct VAcc {
 };
 
 __global__ __launch_bounds__(BLK) void k_msf_init(MsfState *st, int n, const double *partial,
-                                                  int nb, double tol, int maxcg) {
+                                                  int nb, const double *tolv, int maxcg) {
   MsfState *S = st + blockIdx.x;
   const double b2 = fin_sum(partial + (long)blockIdx.x * nb, nb);
   if (threadIdx.x == 0) {
+    const double tol = tolv[blockIdx.x];
     S->delta = b2;
     S->bnorm2 = b2;
     S->lambda = b2;
@@ -92,26 +94,35 @@ __global__ __launch_bounds__(BLK) void k_msf_init(MsfState *st, int n, const dou
   }
 }
 
-// alpha = |h|^2, omega, zeta''' and omega_i (multishift_module.F90:118-135)
-__global__ __launch_bounds__(BLK) void k_msf_fin1(MsfState *st, const double *aden,
+// alpha = |h|^2, omega, zeta''' and omega_i (multishift_module.F90:118-135);
+// aden holds MAXSHIFT shifts per source.  Thread i updates shift i; every
+// shift's arithmetic is the serial recurrence's, so results do not depend on
+// the thread count.
+__global__ __launch_bounds__(BLK) void k_msf_fin1(MsfState *st, const double *adens,
                                                   const double *partial, int nb) {
   MsfState *S = st + blockIdx.x;
   if (S->stop) return;
+  const double *aden = adens + (long)blockIdx.x * MAXSHIFT;
+  __shared__ double s_al;
   const double al = fin_sum(partial + (long)blockIdx.x * nb, nb);
+  if (threadIdx.x == 0) s_al = al;
+  __syncthreads();
+  const double os = S->omega, gg = S->gammag;
+  const double om = -S->delta / s_al;
+  const int i = threadIdx.x;
+  if (i >= S->mini - 1 && i < S->maxi && S->active[i]) {
+    S->zeta_iii[i] = (S->zeta_i[i] * S->zeta_ii[i] * os) /
+                     (om * gg * (S->zeta_i[i] - S->zeta_ii[i]) +
+                      S->zeta_i[i] * os * (1.0 - aden[i] * om));
+    S->omegas[i] = om * S->zeta_iii[i] / S->zeta_ii[i];
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
     S->cg_return += 1;
-    S->alpha = al;
-    S->omega_save = S->omega;
-    S->omega = -S->delta / al;
-    S->minus_omega = -S->omega;
-    const double om = S->omega, os = S->omega_save, gg = S->gammag;
-    for (int i = S->mini - 1; i < S->maxi; i++) {
-      if (!S->active[i]) continue;
-      S->zeta_iii[i] = (S->zeta_i[i] * S->zeta_ii[i] * os) /
-                       (om * gg * (S->zeta_i[i] - S->zeta_ii[i]) +
-                        S->zeta_i[i] * os * (1.0 - aden[i] * om));
-      S->omegas[i] = om * S->zeta_iii[i] / S->zeta_ii[i];
-    }
+    S->alpha = s_al;
+    S->omega_save = os;
+    S->omega = om;
+    S->minus_omega = -om;
   }
 }
 
@@ -119,46 +130,58 @@ __global__ __launch_bounds__(BLK) void k_msf_fin1(MsfState *st, const double *ad
 __global__ __launch_bounds__(BLK) void k_msf_fin2(MsfState *st, const double *partial, int nb) {
   MsfState *S = st + blockIdx.x;
   if (S->stop) return;
+  __shared__ double s_lam;
+  __shared__ int s_nmax, s_nmin;
   const double lam = fin_sum(partial + (long)blockIdx.x * nb, nb);
   if (threadIdx.x == 0) {
-    S->lambda = lam;
-    const double gam = lam / S->delta;
+    s_lam = lam;
+    s_nmax = 0;
+    s_nmin = S->n + 1;
+  }
+  __syncthreads();
+  const double L = s_lam;
+  const double gam = L / S->delta;
+  const double omega = S->omega, tol2 = S->tol2;
+  const int j = S->jcount, cgr = S->cg_return;
+  const int i = threadIdx.x;
+  const bool mine = i >= S->mini - 1 && i < S->maxi && S->active[i];
+  __syncthreads();  // every thread has read the scalars before thread 0 writes them
+  if (mine) {
+    const double zt = S->zeta_iii[i], om = S->omegas[i];
+    const double gi = gam * zt * om / (S->zeta_ii[i] * omega);
+    S->gammas[i] = gi;
+    // x_i -= omega_i p_i  with p_i = ca P + sum cb R
+    S->cc[i] -= om * S->ca[i];
+    for (int q = 0; q < j; q++) S->cd[i][q] -= om * S->cb[i][q];
+    // p_i = gamma_i p_i + zeta_i''' R_j
+    S->ca[i] *= gi;
+    for (int q = 0; q < j; q++) S->cb[i][q] *= gi;
+    S->cb[i][j] = zt;
+    S->cd[i][j] = 0.0;
+    S->touched[i] = 1;
+    S->cg_returns[i] = cgr;
+    // shifted residual of the new iterate: |zeta'''| |r_new|
+    if (zt * zt * L <= tol2) {
+      S->active[i] = 0;
+    } else {
+      atomicMax(&s_nmax, i + 1);
+      atomicMin(&s_nmin, i + 1);
+    }
+  }
+  __syncthreads();
+  const int nmax = s_nmax, nmin = s_nmin;
+  if (i >= nmin - 1 && i < nmax && S->active[i]) {
+    S->zeta_i[i] = S->zeta_ii[i];
+    S->zeta_ii[i] = S->zeta_iii[i];
+  }
+  if (threadIdx.x == 0) {
+    S->lambda = L;
     S->gammag = gam;
-    const int j = S->jcount;
-    int nmax = 0, nmin = S->n + 1;
-    for (int i = S->mini - 1; i < S->maxi; i++) {
-      if (!S->active[i]) continue;
-      const double zt = S->zeta_iii[i], om = S->omegas[i];
-      const double gi = gam * zt * om / (S->zeta_ii[i] * S->omega);
-      S->gammas[i] = gi;
-      // x_i -= omega_i p_i  with p_i = ca P + sum cb R
-      S->cc[i] -= om * S->ca[i];
-      for (int q = 0; q < j; q++) S->cd[i][q] -= om * S->cb[i][q];
-      // p_i = gamma_i p_i + zeta_i''' R_j
-      S->ca[i] *= gi;
-      for (int q = 0; q < j; q++) S->cb[i][q] *= gi;
-      S->cb[i][j] = zt;
-      S->cd[i][j] = 0.0;
-      S->touched[i] = 1;
-      S->cg_returns[i] = S->cg_return;
-      // shifted residual of the new iterate: |zeta'''| |r_new|
-      if (zt * zt * lam <= S->tol2) {
-        S->active[i] = 0;
-      } else {
-        nmax = max(nmax, i + 1);
-        nmin = min(nmin, i + 1);
-      }
-    }
-    for (int i = nmin - 1; i < nmax; i++) {
-      if (!S->active[i]) continue;
-      S->zeta_i[i] = S->zeta_ii[i];
-      S->zeta_ii[i] = S->zeta_iii[i];
-    }
     S->mini = nmin;
     S->maxi = nmax;
-    S->delta = lam;
+    S->delta = L;
     S->jcount = j + 1;
-    if (nmax == 0 || S->cg_return >= S->maxcg) S->stop = 1;
+    if (nmax == 0 || cgr >= S->maxcg) S->stop = 1;
   }
 }
 
@@ -260,16 +283,19 @@ __global__ __launch_bounds__(BLK) void k_msf_flush(long v5, int n, TT<P> *const 
 
 __global__ void k_msf_reset(MsfState *st) {
   MsfState *S = st + blockIdx.x;
-  if (threadIdx.x != 0) return;
   if (S->jcount == 0) return;
-  for (int i = 0; i < S->n; i++) {
+  const int i = threadIdx.x;
+  if (i < S->n) {
     S->ca[i] = 1.0;
     S->cc[i] = 0.0;
     S->touched[i] = 0;
     for (int q = 0; q < MSB; q++) S->cb[i][q] = S->cd[i][q] = 0.0;
   }
-  S->jcount = 0;
-  S->first_block = 0;
+  __syncthreads();
+  if (i == 0) {
+    S->jcount = 0;
+    S->first_block = 0;
+  }
 }
 
 // partial |b - (M^+M + sigma) x|^2 per source: t holds M^+M x
@@ -333,6 +359,22 @@ __global__ __launch_bounds__(BLK) void k_conv(long v5, int nsrc, const TT<SP> *a
 #pragma unroll
     for (int k = 0; k < 4; k++) st(b, off + k * v5 + i, vd[k]);
   }
+}
+
+// out_j = b_j - t_j - sigma_j x_j for a batch of vectors of length ns
+__global__ __launch_bounds__(BLK) void k_resvec(long ns, const double2 *b, const double2 *t,
+                                                const double2 *x, const double *sig,
+                                                double2 *out) {
+  const long e = (long)blockIdx.x * BLK + threadIdx.x;
+  if (e >= ns) return;
+  const long o = (long)blockIdx.y * ns + e;
+  st(out, o, ldc(b, o) - ldc(t, o) - sig[blockIdx.y] * ldc(x, o));
+}
+
+__global__ __launch_bounds__(BLK) void k_addvec(long ns, double2 *y, const double2 *c) {
+  const long e = (long)blockIdx.x * BLK + threadIdx.x;
+  if (e >= ns) return;
+  st(y, e, ldc(y, e) + ldc(c, e));
 }
 
 __global__ __launch_bounds__(BLK) void k_norm_src(long v5, const double2 *x, double *partial) {
@@ -404,7 +446,9 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
   }
   double2 *Pv = (double2 *)workspace(363, sizeof(double2) * nsp * ns_ * n);
   MsfState *st = (MsfState *)workspace(364, sizeof(MsfState) * ns_);
-  double *aden = (double *)workspace(365, sizeof(double) * MAXSHIFT);
+  // per-source shifts [ns_][MAXSHIFT] followed by per-source tolerances [ns_]
+  double *aden = (double *)workspace(365, sizeof(double) * (MAXSHIFT + 1) * ns_);
+  double *tolv = aden + (long)MAXSHIFT * ns_;
   double *partial = (double *)workspace(366, sizeof(double) * nb * ns_);
   void **ringptr = (void **)workspace(367, sizeof(void *) * 3 * MSB + sizeof(float *) * MSB);
   double *sums = (double *)workspace(368, sizeof(double) * ns_);
@@ -415,14 +459,18 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
       for (int q = 0; q < MSB; q++) hp[pr * MSB + q] = ring[pr][q];
     for (int q = 0; q < MSB; q++) hp[3 * MSB + q] = A.th16 > 0 ? (void *)ringn[q] : nullptr;
     THR_HIP(hipMemcpyAsync(ringptr, hp.data(), sizeof(void *) * hp.size(), hipMemcpyHostToDevice, s));
-    double ad[MAXSHIFT] = {0};
-    for (int i = 0; i < n; i++) ad[i] = A.aden[i];
-    THR_HIP(hipMemcpyAsync(aden, ad, sizeof(ad), hipMemcpyHostToDevice, s));
+    std::vector<double> ad((MAXSHIFT + 1) * ns_, 0.0);
+    for (int k = 0; k < ns_; k++) {
+      for (int i = 0; i < n; i++)
+        ad[k * MAXSHIFT + i] = A.aden_src ? A.aden_src[k * n + i] : A.aden[i];
+      ad[MAXSHIFT * ns_ + k] = A.tol_src ? A.tol_src[k] : A.tol;
+    }
+    THR_HIP(hipMemcpyAsync(aden, ad.data(), sizeof(double) * ad.size(), hipMemcpyHostToDevice, s));
     THR_HIP(hipStreamSynchronize(s));
   }
   // |b|^2 per source, state init
   hipLaunchKernelGGL(k_norm_src, dim3(nb, ns_), dim3(BLK), 0, s, v5, A.b, partial);
-  hipLaunchKernelGGL(k_msf_init, dim3(ns_), dim3(BLK), 0, s, st, n, partial, nb, A.tol, A.maxcg);
+  hipLaunchKernelGGL(k_msf_init, dim3(ns_), dim3(BLK), 0, s, st, n, partial, nb, tolv, A.maxcg);
   THR_HIP(hipGetLastError());
 
   THR_HIP(hipMemsetAsync(A.x, 0, sizeof(double2) * nsp * ns_ * n, s));
@@ -515,7 +563,7 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
     else
       hipLaunchKernelGGL(k_msf_flush<P16>, gf, dim3(BLK), 0, s, v5, n, (__half2 *const *)rp, rnp,
                          A.b, A.x, Pv, st, 0);
-    hipLaunchKernelGGL(k_msf_reset, dim3(ns_), dim3(1), 0, s, st);
+    hipLaunchKernelGGL(k_msf_reset, dim3(ns_), dim3(MAXSHIFT), 0, s, st);
     THR_HIP(hipGetLastError());
     return THR_OK;
   };
@@ -656,6 +704,95 @@ int msfast_true_residuals(const MsfArgs &A, double *relres, hipStream_t s) {
     THR_HIP(hipStreamSynchronize(s));
     for (int k = 0; k < ns_; k++) relres[k * n + i] = std::sqrt(hsum[k] / bn[k]);
   }
+  return THR_OK;
+}
+
+int msfast_correct(const MsfArgs &A, int max_rounds, double *relres, int *corr_iters,
+                   hipStream_t s) {
+  const Dims &d = A.d;
+  const int n = A.nshift, ns_ = A.nsrc;
+  const long v5 = d.v5(), nsp = 4 * v5;
+  std::vector<double> rr((size_t)ns_ * n);
+  int total = 0;
+  for (int round = 0;; round++) {
+    TRY_RC(msfast_true_residuals(A, rr.data(), s));
+    std::vector<int> fail;  // k * n + i
+    for (int q = 0; q < ns_ * n; q++)
+      if (!(rr[q] <= A.tol)) fail.push_back(q);
+    if (fail.empty() || round >= max_rounds) break;
+    // buffers for one group of up to ns_ (source, shift) pairs
+    double2 *xs = (double2 *)workspace(372, sizeof(double2) * nsp * ns_);
+    double2 *t1 = (double2 *)workspace(370, sizeof(double2) * nsp * ns_);
+    double2 *t2 = (double2 *)workspace(371, sizeof(double2) * nsp * ns_);
+    double2 *bs = (double2 *)workspace(374, sizeof(double2) * nsp * ns_);
+    double2 *rhs = (double2 *)workspace(375, sizeof(double2) * nsp * ns_);
+    double2 *xc = (double2 *)workspace(376, sizeof(double2) * nsp * ns_);
+    double *sig = (double *)workspace(377, sizeof(double) * ns_);
+    THR_CHECK(xs && t1 && t2 && bs && rhs && xc && sig, THR_ERR_HIP, "workspace allocation failed");
+    for (size_t g0 = 0; g0 < fail.size(); g0 += ns_) {
+      const int g = (int)std::min<size_t>(ns_, fail.size() - g0);
+      std::vector<double> sg(g), tl(g);
+      std::vector<float> am(g);
+      for (int j = 0; j < g; j++) {
+        const int q = fail[g0 + j], k = q / n, i = q % n;
+        sg[j] = A.aden[i];
+        am[j] = A.am[k];
+        // solve (M^+M + sigma_i) c = r_ki to |r_ki - (M^+M + sigma_i) c| <= tol |b_k| / 2
+        tl[j] = 0.5 * A.tol / rr[q];
+        THR_HIP(hipMemcpyAsync(xs + j * nsp, A.x + (long)q * nsp, sizeof(double2) * nsp,
+                               hipMemcpyDeviceToDevice, s));
+        THR_HIP(hipMemcpyAsync(bs + j * nsp, A.b + (long)k * nsp, sizeof(double2) * nsp,
+                               hipMemcpyDeviceToDevice, s));
+      }
+      THR_HIP(hipMemcpyAsync(sig, sg.data(), sizeof(double) * g, hipMemcpyHostToDevice, s));
+      DslashLaunch L{};
+      L.d = d;
+      L.nrhs = g;
+      L.prec = P64;
+      L.imass = A.imass;
+      L.am3 = A.am3;
+      for (int j = 0; j < g; j++) L.am[j] = am[j];
+      L.links = A.u64;
+      L.mode = EPI_STORE;
+      L.sc_stride = 1;
+      L.done_stride = 1;
+      DslashLaunch a = L;
+      a.in = xs;
+      a.out = t1;
+      TRY_RC(launch_dslash(a, s));
+      DslashLaunch b = L;
+      b.dagger = 1;
+      b.in = t1;
+      b.out = t2;
+      TRY_RC(launch_dslash(b, s));
+      hipLaunchKernelGGL(k_resvec, dim3(nblk(nsp), g), dim3(BLK), 0, s, nsp, bs, t2, xs, sig, rhs);
+      THR_HIP(hipGetLastError());
+      MsfArgs B = A;
+      B.nsrc = g;
+      B.nshift = 1;
+      B.aden = sg.data();
+      B.aden_src = sg.data();
+      B.tol_src = tl.data();
+      B.am = am.data();
+      B.b = rhs;
+      B.x = xc;
+      B.th32 = 1.0;
+      B.th16 = 0.0;
+      B.start_prec = P32;
+      MsfResult R{};
+      TRY_RC(msfast_solve(B, &R, s));
+      total += R.iterations;
+      for (int j = 0; j < g; j++) {
+        hipLaunchKernelGGL(k_addvec, dim3(nblk(nsp)), dim3(BLK), 0, s, nsp,
+                           A.x + (long)fail[g0 + j] * nsp, xc + j * nsp);
+      }
+      THR_HIP(hipGetLastError());
+    }
+  }
+  if (relres)
+    for (int q = 0; q < ns_ * n; q++) relres[q] = rr[q];
+  if (corr_iters) *corr_iters = total;
+  THR_HIP(hipStreamSynchronize(s));
   return THR_OK;
 }
 

@@ -88,7 +88,7 @@ double *partial_buffer(int nblocks) {
 
 hipStream_t default_stream() {
   Context &c = cur();
-  if (!c.stream) (void)hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking);
+  if (!c.stream) (void)hipStreamCreate(&c.stream);
   return c.stream;
 }
 

@@ -47,6 +47,9 @@ _SIGS = {
                    P], I32),
     "thr_hamilton": ([P, F32, F32, I32, F32, F32, P, P, P, P, P, P, P, P, I32, I32, P, P], I32),
     "thr_coef_dev": ([P, P, P, I32, P], I32),
+    "thr_dslash_autotune": ([P, P, I32, IP, IP, P], I32),
+    "thr_dslash_set_config": ([P, I32, I32, I32], I32),
+    "thr_dslash_get_config": ([P, I32, IP, IP], I32),
     "thr_links_to_f32_dev": ([P, P, P, P], I32),
     "thr_dslash_dev": ([P, P, P, P, P, P, I32, P, I32, F32, I32, I32, P], I32),
     "thr_congrad_dev": ([P, P, P, P, F32, F32, I32, F32, I32, I32, IP, IP, P], I32),
@@ -54,8 +57,8 @@ _SIGS = {
                            I32),
     "thr_true_residuals_dev": ([P, P, I32, P, I32, F32, I32, P, P, P, P, P], I32),
     "thr_multishift_plain_dev": ([P, P, F32, I32, F32, I32, P, P, P, F32, I32, IP, P, P], I32),
-    "thr_multishift_fast_dev": ([P, P, P, I32, P, I32, F32, I32, P, P, P, F64, I32, F64, F64, IP,
-                                 P, P, P], I32),
+    "thr_multishift_fast_dev": ([P, P, P, I32, P, I32, F32, I32, P, P, P, F64, I32, F64, F64, I32,
+                                 IP, P, IP, P, P], I32),
 }
 
 _lib = None
@@ -300,6 +303,33 @@ def dslash_dev(out, inp, links, dims, am, imass=3, am3=1.0, dagger=False, prec="
                                 int(bool(dagger)), PREC[prec], _stream(stream)), "dslash_dev")
 
 
+SHAPE_XCD_REMAP, SHAPE_BUFLOAD, SHAPE_LANESHIFT = 1, 2, 4
+
+
+def dslash_autotune(links, dims, prec="f64", stream=None):
+    """Times the Dslash launch shapes and pins the fastest; returns (tchunk, flags)."""
+    init()
+    tc, fl = ctypes.c_int(), ctypes.c_int()
+    _check(lib().thr_dslash_autotune(_p(links), _dims(dims), PREC[prec], ctypes.byref(tc),
+                                     ctypes.byref(fl), _stream(stream)), "dslash_autotune")
+    return tc.value, fl.value
+
+
+def dslash_set_config(dims, prec, tchunk, flags):
+    init()
+    _check(lib().thr_dslash_set_config(_dims(dims), PREC[prec], int(tchunk), int(flags)),
+           "dslash_set_config")
+
+
+def dslash_get_config(dims, prec="f64"):
+    """(tchunk, flags) used for launches with these dims and precision."""
+    init()
+    tc, fl = ctypes.c_int(), ctypes.c_int()
+    _check(lib().thr_dslash_get_config(_dims(dims), PREC[prec], ctypes.byref(tc),
+                                       ctypes.byref(fl)), "dslash_get_config")
+    return tc.value, fl.value
+
+
 def congrad_dev(phi, x, links64, dims, res, am, imass, am3=1.0, niterc=None, kernel="shamir",
                 stream=None):
     init()
@@ -344,20 +374,30 @@ def multishift_plain_dev(links64, dims, am, imass, aden, inp, out, tol, maxcg, a
 
 
 def multishift_fast_dev(links64, links32, dims, am, imass, aden, b, x, tol, maxcg, nsrc=1,
-                        th32=0.0, th16=0.0, am3=1.0, true_residuals=True, stream=None):
-    """Batched blocked-update multishift; returns (iterations, cg_return[nsrc], relres)."""
+                        th32=0.0, th16=0.0, am3=1.0, true_residuals=True, corrections=None,
+                        stats=None, stream=None):
+    """Batched blocked-update multishift; returns (iterations, cg_return[nsrc], relres).
+
+    corrections: rounds of FP64 defect correction (default: 4 when th32 or th16
+    lowers the Krylov precision, else 0); stats (dict, optional) receives
+    "corr_iterations"."""
     init()
     n = len(aden)
+    if corrections is None:
+        corrections = 4 if (th32 > 0 or th16 > 0) else 0
     aden = np.ascontiguousarray(aden, dtype=np.float64)
     ams = np.ascontiguousarray(np.broadcast_to(np.asarray(am, dtype=np.float32), (nsrc,)))
-    it = ctypes.c_int()
+    it, ci = ctypes.c_int(), ctypes.c_int()
     cgr = np.zeros(nsrc, dtype=np.int32)
-    rel = np.zeros(nsrc * n, dtype=np.float64) if true_residuals else None
+    rel = np.zeros(nsrc * n, dtype=np.float64) if (true_residuals or corrections) else None
     _check(lib().thr_multishift_fast_dev(_p(links64), _p(links32), _dims(dims), int(nsrc),
                                          _p(ams), int(imass), am3, n, _p(aden), _p(b), _p(x),
                                          float(tol), int(maxcg), float(th32), float(th16),
-                                         ctypes.byref(it), _p(cgr), _p(rel), _stream(stream)),
+                                         int(corrections), ctypes.byref(it), _p(cgr),
+                                         ctypes.byref(ci), _p(rel), _stream(stream)),
            "multishift_fast_dev")
+    if stats is not None:
+        stats["corr_iterations"] = ci.value
     return it.value, cgr, (rel.reshape(nsrc, n) if rel is not None else None)
 
 
