@@ -111,7 +111,7 @@ def time_dslash(G, hr, dims, links, prec, steps, warmup, nbuf, stream):
     return float(np.mean(per)), float(t0.elapsed_ms(t1) / steps)
 
 
-def bench_multishift(G, hr, dims, l64, l32, am, nsrc, stream, tol=5e-11, maxcg=30000,
+def bench_multishift(G, hr, dims, l64, l32, am, nsrc, stream, tol=5e-11, maxcg=50000,
                      mixed_th32=1e-5, mixed_th16=0.0):
     """Plain FP64 path (one source at a time) vs the batched mixed path."""
     lx, ly, lt, ls = dims

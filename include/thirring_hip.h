@@ -23,7 +23,11 @@
  *
  *  (2) device-resident entry points (suffix _dev) on halo-free device
  *      buffers in the layout [rhs][spin][t][y][x][s] (s fastest), links
- *      [mu][t][y][x]; `stream` is a hipStream_t (NULL: the library's own).
+ *      [mu][t][y][x]; `stream` is a hipStream_t.  With a stream, entry
+ *      points that only enqueue work (coef, links, dslash) return at once;
+ *      with NULL they run on the calling context's own non-blocking stream
+ *      and return when the work is complete.  Solvers always return with
+ *      their results complete.
  *
  * Scalars the reference declares as default real (am, am3, res, theta) are
  * float here; coefficient arrays are double.

@@ -62,6 +62,10 @@ program test_qmrherm_drop_in
   ! tests/test_derivs_shamir.F90: dSdpi = dSdpi_orig; derivs_shamir(R, X2, tau, 0, am, imass)
   call starting_state(Phi, u, R=R, X=X, dSdpi_ref=dSdpi_ref)
   dSdpi = dSdpi_ref
+  ! the exact inputs, for the checker (constant folding may differ from the
+  ! reference compiler's in the last bit of default-real literals)
+  call dump(trim(outdir)//'/derivs_R.bin', reshape(R, [size(R)]), size(R))
+  call dump(trim(outdir)//'/derivs_X.bin', reshape(X, [size(X)]), size(X))
   call derivs_shamir(R, X, tau, 0, am, imass)
   call dump_real(trim(outdir)//'/derivs_shamir.bin', reshape(dSdpi, [kvol*3]), kvol*3)
   print *, 'test_qmrherm_drop_in done'

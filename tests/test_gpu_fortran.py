@@ -97,7 +97,11 @@ def test_fortran_qmrherm_and_derivs(tmp_path):
         check(gp, p0[1:-1, 1:-1, 1:-1, 1:-1], 1e-10)
         gd = load(tmp_path / f"qmrherm_dsdpi_{iflag}.bin", (12, 12, 12, 3), np.float64)
         check(gd, ds, 1e-10)
+    # derivs on the Fortran program's own R and X
+    R = load(tmp_path / "derivs_R.bin", (26, 14, 14, 14, 4))
+    X = load(tmp_path / "derivs_X.bin", (26, 14, 14, 14, 4))
+    assert rel(R, st["r"]) <= 1e-6 and rel(X, st["x"]) <= 1e-6
     ds = np.asfortranarray(st["dsdpi"].copy())
-    O.derivs(st["r"], st["x"], TAU, 0, DIMS, ds)
+    O.derivs(R, X, TAU, 0, DIMS, ds)
     gd = load(tmp_path / "derivs_shamir.bin", (12, 12, 12, 3), np.float64)
     check(gd - st["dsdpi"], ds - st["dsdpi"], 1e-12)

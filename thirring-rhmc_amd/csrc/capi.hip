@@ -30,6 +30,14 @@ static hipStream_t pick(void *stream) {
   return stream ? (hipStream_t)stream : default_stream();
 }
 
+// entry points that only enqueue work are synchronous when called without a
+// stream (their results are then ready for any copy on any stream)
+static int done_if_default(void *stream, int rc) {
+  if (rc || stream) return rc;
+  THR_HIP(hipStreamSynchronize(default_stream()));
+  return THR_OK;
+}
+
 static int check_dims(const int dims[4]) {
   THR_CHECK(dims, THR_ERR_ARG, "dims is NULL");
   for (int k = 0; k < 4; k++) THR_CHECK(dims[k] > 0, THR_ERR_ARG, "dims must be positive");
@@ -433,14 +441,16 @@ int thr_hamilton(const double *phi, float res2, float am, int imass, float am3, 
 int thr_coef_dev(void *links64, const float *theta, const int dims[4], int ibound, void *stream) {
   TRY(check_dims(dims));
   THR_CHECK(links64 && theta, THR_ERR_ARG, "NULL argument");
-  return launch_coef(mkdims(dims), theta, ibound, (double2 *)links64, pick(stream));
+  return done_if_default(stream,
+                         launch_coef(mkdims(dims), theta, ibound, (double2 *)links64, pick(stream)));
 }
 
 int thr_links_to_f32_dev(void *links32, const void *links64, const int dims[4], void *stream) {
   TRY(check_dims(dims));
   THR_CHECK(links32 && links64, THR_ERR_ARG, "NULL argument");
   const Dims d = mkdims(dims);
-  return convert_d2f(d.v4() * 3, (const double2 *)links64, (float2 *)links32, pick(stream));
+  return done_if_default(
+      stream, convert_d2f(d.v4() * 3, (const double2 *)links64, (float2 *)links32, pick(stream)));
 }
 
 int thr_dslash_dev(void *out, float *out_norm, const void *in, const float *in_norm,
@@ -467,7 +477,7 @@ int thr_dslash_dev(void *out, float *out_norm, const void *in, const float *in_n
   L.mode = EPI_STORE;
   L.sc_stride = 1;
   L.done_stride = 1;
-  return launch_dslash(L, pick(stream));
+  return done_if_default(stream, launch_dslash(L, pick(stream)));
 }
 
 static int shape_flags(const DslashConfig &c) {

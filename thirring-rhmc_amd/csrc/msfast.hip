@@ -571,36 +571,48 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
     for (int k = 0; k < 3; k++)
       if (gexec[k]) (void)hipGraphExecDestroy(gexec[k]);
   };
+  // graph capture is an optimisation: if another thread's work invalidates a
+  // capture (or instantiation fails), the block runs eagerly instead and the
+  // solve continues without graphs (nothing of a failed capture executed)
+  bool graphs_ok = use_graph;
   for (;;) {
-    if (!use_graph || t == 0) {
+    bool done_block = false;
+    if (graphs_ok && t > 0) {
+      if (!gexec[prec]) {
+        hipGraph_t g = nullptr;
+        if (hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed) == hipSuccess) {
+          rc = enqueue_block(t, prec);
+          const hipError_t e = hipStreamEndCapture(s, &g);
+          if (rc && e == hipSuccess) {  // argument error, not a capture problem
+            if (g) (void)hipGraphDestroy(g);
+            free_graphs();
+            return rc;
+          }
+          if (rc || e != hipSuccess ||
+              hipGraphInstantiate(&gexec[prec], g, nullptr, nullptr, 0) != hipSuccess)
+            gexec[prec] = nullptr;
+          if (g) (void)hipGraphDestroy(g);
+        }
+        if (!gexec[prec]) {
+          (void)hipGetLastError();  // clear the failed capture's error state
+          graphs_ok = false;
+          rc = THR_OK;
+        }
+      }
+      if (gexec[prec]) {
+        if (hipGraphLaunch(gexec[prec], s) != hipSuccess) {
+          free_graphs();
+          set_error("hipGraphLaunch failed");
+          return THR_ERR_HIP;
+        }
+        done_block = true;
+      }
+    }
+    if (!done_block) {
       rc = enqueue_block(t, prec);
       if (rc) {
         free_graphs();
         return rc;
-      }
-    } else {
-      if (!gexec[prec]) {
-        hipGraph_t g = nullptr;
-        if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-          free_graphs();
-          set_error("hipStreamBeginCapture failed");
-          return THR_ERR_HIP;
-        }
-        rc = enqueue_block(t, prec);
-        hipError_t e = hipStreamEndCapture(s, &g);
-        if (rc || e != hipSuccess ||
-            hipGraphInstantiate(&gexec[prec], g, nullptr, nullptr, 0) != hipSuccess) {
-          if (g) (void)hipGraphDestroy(g);
-          free_graphs();
-          set_error("hipGraph capture of the solver block failed");
-          return rc ? rc : THR_ERR_HIP;
-        }
-        (void)hipGraphDestroy(g);
-      }
-      if (hipGraphLaunch(gexec[prec], s) != hipSuccess) {
-        free_graphs();
-        set_error("hipGraphLaunch failed");
-        return THR_ERR_HIP;
       }
     }
     t += MSB;

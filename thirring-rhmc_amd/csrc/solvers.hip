@@ -88,7 +88,9 @@ double *partial_buffer(int nblocks) {
 
 hipStream_t default_stream() {
   Context &c = cur();
-  if (!c.stream) (void)hipStreamCreate(&c.stream);
+  // non-blocking: no implicit ordering with the legacy null stream, which
+  // would break graph capture while another thread copies on it
+  if (!c.stream) (void)hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking);
   return c.stream;
 }
 

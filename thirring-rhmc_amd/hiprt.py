@@ -33,7 +33,7 @@ def hip():
             "hipMemcpy": ([V, V, S, ctypes.c_int], ctypes.c_int),
             "hipMemset": ([V, ctypes.c_int, S], ctypes.c_int),
             "hipDeviceSynchronize": ([], ctypes.c_int),
-            "hipStreamCreate": ([ctypes.POINTER(V)], ctypes.c_int),
+            "hipStreamCreateWithFlags": ([ctypes.POINTER(V), ctypes.c_uint], ctypes.c_int),
             "hipStreamSynchronize": ([V], ctypes.c_int),
             "hipEventCreate": ([ctypes.POINTER(V)], ctypes.c_int),
             "hipEventRecord": ([V, V], ctypes.c_int),
@@ -86,12 +86,16 @@ class DeviceArray:
         return a
 
     def copy_from(self, other):
+        """Device-to-device copy, complete on return."""
         assert other.nbytes == self.nbytes
         check(hip().hipMemcpy(ctypes.c_void_p(self.ptr), ctypes.c_void_p(other.ptr), self.nbytes,
                               D2D), "hipMemcpy D2D")
+        check(hip().hipStreamSynchronize(None), "hipStreamSynchronize")
 
     def zero(self):
+        """Fill with zero bytes, complete on return."""
         check(hip().hipMemset(ctypes.c_void_p(self.ptr), 0, self.nbytes), "hipMemset")
+        check(hip().hipStreamSynchronize(None), "hipStreamSynchronize")
 
     def free(self):
         if self.ptr:
@@ -110,9 +114,13 @@ def synchronize():
 
 
 class Stream:
+    """A non-blocking stream: no implicit ordering with the legacy null stream,
+    so synchronous copies issued by other threads never wait on (or break the
+    graph capture of) work queued here."""
+
     def __init__(self):
         p = ctypes.c_void_p()
-        check(hip().hipStreamCreate(ctypes.byref(p)), "hipStreamCreate")
+        check(hip().hipStreamCreateWithFlags(ctypes.byref(p), 1), "hipStreamCreateWithFlags")
         self.ptr = p.value
 
     def synchronize(self):
