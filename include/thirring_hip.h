@@ -156,6 +156,9 @@ int thr_dslash_dev(void *out, float *out_norm, const void *in, const float *in_n
  *   THR_SHAPE_XCD_REMAP  give every XCD a contiguous range of blocks
  *   THR_SHAPE_BUFLOAD    read fields through buffer resources (32-bit offsets)
  *   THR_SHAPE_LANESHIFT  s hops as DPP lane shifts (used when Ls divides 64)
+ *   THR_SHAPE_XCD_BOXES  every XCD owns one box of a split of (x, y, t) into 8
+ *                        (used when such a split divides the lattice; the
+ *                        remap bit is then ignored)
  * No counterpart in the reference (a tuning knob of this library).
  * thr_dslash_autotune times the candidates (tchunk 1..16 x flags) on scratch
  * fields with the given links (complex128 for F64, complex64 otherwise) and
@@ -165,6 +168,7 @@ int thr_dslash_dev(void *out, float *out_norm, const void *in, const float *in_n
 #define THR_SHAPE_XCD_REMAP 1
 #define THR_SHAPE_BUFLOAD 2
 #define THR_SHAPE_LANESHIFT 4
+#define THR_SHAPE_XCD_BOXES 8
 int thr_dslash_autotune(const void *links, const int dims[4], int prec, int *tchunk, int *flags,
                         void *stream);
 int thr_dslash_set_config(const int dims[4], int prec, int tchunk, int flags);

@@ -235,13 +235,18 @@ __device__ inline void hop(int mu, cx<C> o[4], cx<C> uf, const cx<C> rf[4], cx<C
 // t neighbour at t+2, so the t hops come from registers.  x and y hops are
 // L1/L2 hits (neighbouring lanes and waves of the same and nearby blocks).
 // With SHUF (Ls divides 64) the s hops are DPP lane shifts of the centre.
-// XCD remap: hardware block b runs on XCD b % 8; remapping gives every XCD a
-// contiguous range of t-chunks, so t and y neighbours are fetched from the
-// same L2.  Partial sums go to partial[rhs * pstride + blockIdx.x]; block 0
-// zero-fills [gridDim.x, pstride) so consumers can always sum pstride entries.
+// Hardware block b runs on XCD b % 8, and each XCD has its own L2, so the
+// tiling decides which neighbours are L2 hits:
+//   linear  blocks walk the s-x-y slab of a t-chunk in order;
+//   remap   XCD k gets the k-th contiguous eighth of the linear order;
+//   boxes   XCD k owns one box of a px x py x pt split of (x, y, t) (the split
+//           with the smallest halo) and its blocks tile that box's s-x-y
+//           cross-section, marching its t-range in chunks: only box faces
+//           are fetched across XCDs.
+// Partial sums go to partial[rhs * pstride + blockIdx.x]; block 0 zero-fills
+// [gridDim.x, pstride) so consumers can always sum pstride entries.
 template <int PREC, bool DAG, bool SHUF, int MODE, bool BUF>
-__global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, unsigned tchunk,
-                                                unsigned nb3, int remap, unsigned pstride) {
+__global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, const Tiling TL) {
   using C = typename Store<PREC>::C;
   using T = typename Store<PREC>::T;
   const int rhs = blockIdx.y;
@@ -249,21 +254,41 @@ __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, unsigned t
   // 32-bit index arithmetic: 4*V5 < 2^31 is checked on the host
   const unsigned lx = L.d.lx, ly = L.d.ly, lt = L.d.lt, ls = L.d.ls;
   const unsigned v2 = lx * ly, v3 = v2 * ls, v4 = v2 * lt, v5 = v4 * ls;
-  unsigned b = blockIdx.x;
-  if (remap) {
-    // blocks b, b+8, b+16, ... (one XCD) -> consecutive logical blocks
-    const unsigned nbl = gridDim.x, per = nbl >> 3, full = per << 3;
-    if (b < full) b = (b & 7) * per + (b >> 3);
-  }
-  const unsigned chunk = b / nb3;
-  const unsigned j = (b - chunk * nb3) * BLK + threadIdx.x;
-  const unsigned t0 = chunk * tchunk;
-  double red = 0.0;
-  if (j < v3 && t0 < lt) {
-    const unsigned t1 = min(t0 + tchunk, lt);
-    const unsigned s = j % ls;
+  unsigned j, s, x, y, t0, t1;
+  bool valid;
+  if (TL.mode == TILE_BOXES) {
+    const unsigned k = blockIdx.x & 7, l = blockIdx.x >> 3;
+    const unsigned rx = k % TL.px, ry = (k / TL.px) % TL.py, rt = k / (TL.px * TL.py);
+    const unsigned chunk = l / TL.nb3;
+    const unsigned jr = (l - chunk * TL.nb3) * BLK + threadIdx.x;
+    t0 = rt * TL.bt + chunk * TL.tchunk;
+    t1 = min(t0 + TL.tchunk, rt * TL.bt + TL.bt);
+    valid = jr < ls * TL.bx * TL.by && chunk * TL.tchunk < TL.bt;
+    s = jr % ls;
+    const unsigned q = jr / ls;
+    x = rx * TL.bx + q % TL.bx;
+    y = ry * TL.by + q / TL.bx;
+    j = s + ls * (x + lx * y);
+  } else {
+    unsigned b = blockIdx.x;
+    if (TL.mode == TILE_REMAP) {
+      // blocks b, b+8, b+16, ... (one XCD) -> consecutive logical blocks
+      const unsigned nbl = gridDim.x, per = nbl >> 3, full = per << 3;
+      if (b < full) b = (b & 7) * per + (b >> 3);
+    }
+    const unsigned chunk = b / TL.nb3;
+    j = (b - chunk * TL.nb3) * BLK + threadIdx.x;
+    t0 = chunk * TL.tchunk;
+    t1 = min(t0 + TL.tchunk, lt);
+    valid = j < v3 && t0 < lt;
+    s = j % ls;
     const unsigned xy = j / ls;
-    const unsigned x = xy % lx, y = xy / lx;
+    x = xy % lx;
+    y = xy / lx;
+  }
+  double red = 0.0;
+  if (valid) {
+    const unsigned xy = x + lx * y;  // 2-D site (link offset within a t-slice)
     const unsigned xf = (x + 1 == lx) ? 0 : x + 1, xb = (x == 0) ? lx - 1 : x - 1;
     const unsigned yf = (y + 1 == ly) ? 0 : y + 1, yb = (y == 0) ? ly - 1 : y - 1;
     // s-x-y offsets of the x and y neighbours, and the 2-D link offsets
@@ -417,27 +442,52 @@ __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, unsigned t
   }
   if constexpr (MODE != EPI_STORE) {
     const double tot = block_sum(red);
-    double *P = L.partial + (long)rhs * pstride;
+    double *P = L.partial + (long)rhs * TL.pstride;
     if (threadIdx.x == 0) P[blockIdx.x] = tot;
     if (blockIdx.x == 0)
-      for (unsigned k = gridDim.x + threadIdx.x; k < pstride; k += BLK) P[k] = 0.0;
+      for (unsigned k = gridDim.x + threadIdx.x; k < TL.pstride; k += BLK) P[k] = 0.0;
   }
 }
 
 }  // namespace
 
-// the largest Dslash grid (t-chunk 1): ceil(Lx Ly Ls / BLK) blocks per t-slice;
-// also covers V5 sites when used as a one-site-per-thread grid
+// bound on every Dslash grid: t-chunk 1 with ceil(Lx Ly Ls / BLK) blocks per
+// t-slice, plus up to 8 partial blocks per slice from the XCD boxes; also
+// covers V5 sites when used as a one-site-per-thread grid
 long dslash_blocks(const Dims &d) {
-  return ((long)d.lx * d.ly * d.ls + BLK - 1) / BLK * d.lt;
+  return (((long)d.lx * d.ly * d.ls + BLK - 1) / BLK + 8) * d.lt;
+}
+
+// px x py x pt = 8 split of (x, y, t) with the smallest halo fraction
+// 2/(L/p) summed over split directions; false if no split divides evenly
+static bool box_split(const Dims &d, unsigned &px, unsigned &py, unsigned &pt) {
+  double best = 1e30;
+  bool found = false;
+  const unsigned ps[4] = {1, 2, 4, 8};
+  for (unsigned a : ps)
+    for (unsigned b : ps) {
+      if (a * b > 8 || 8 % (a * b)) continue;
+      const unsigned c = 8 / (a * b);
+      if (d.lx % a || d.ly % b || d.lt % c) continue;
+      const double cost = (a > 1 ? 2.0 * a / d.lx : 0.0) + (b > 1 ? 2.0 * b / d.ly : 0.0) +
+                          (c > 1 ? 2.0 * c / d.lt : 0.0);
+      if (cost < best - 1e-12) {
+        best = cost;
+        px = a;
+        py = b;
+        pt = c;
+        found = true;
+      }
+    }
+  return found;
 }
 
 // ---------------------------------------------------------------------------
-// launch configuration per (dims, precision): t-chunk length, XCD remap,
-// buffer or global loads, lane-shift s hops.  The default is fixed (results
+// launch configuration per (dims, precision): t-chunk length, XCD remap or
+// XCD boxes, buffer or global loads, lane-shift s hops.  The default is fixed (results
 // identical run to run); dslash_autotune times the candidates on scratch
 // fields and pins the fastest for the process.  THR_DSLASH_TCHUNK / _REMAP /
-// _BUFLOAD / _LANESHIFT override the default.
+// _BUFLOAD / _LANESHIFT / _BOXES override the default.
 // ---------------------------------------------------------------------------
 namespace {
 struct TuneKey {
@@ -458,6 +508,7 @@ DslashConfig default_config(const Dims &d) {
   if (const char *e = std::getenv("THR_DSLASH_REMAP")) c.remap = std::atoi(e) ? 1 : 0;
   if (const char *e = std::getenv("THR_DSLASH_BUFLOAD")) c.bufload = std::atoi(e) ? 1 : 0;
   if (const char *e = std::getenv("THR_DSLASH_LANESHIFT")) c.laneshift = std::atoi(e) ? 1 : 0;
+  if (const char *e = std::getenv("THR_DSLASH_BOXES")) c.box = std::atoi(e) ? 1 : 0;
   return c;
 }
 }  // namespace
@@ -484,27 +535,39 @@ int launch_dslash_cfg(const DslashLaunch &L, DslashConfig c, hipStream_t s) {
             THR_ERR_ARG, "fp16 CG epilogue needs r scales and no x update");
   THR_CHECK(L.prec != P16 || L.mode != EPI_DOTW, THR_ERR_ARG, "fp16 dot epilogue unsupported");
   THR_CHECK(4 * L.d.v5() < (1L << 31), THR_ERR_SIZE, "lattice too large for 32-bit indexing");
-  const unsigned tc = (unsigned)std::max(1, std::min(c.tchunk, L.d.lt));
-  const unsigned nchunk = (unsigned)((L.d.lt + tc - 1) / tc);
-  const unsigned v3 = (unsigned)L.d.lx * L.d.ly * L.d.ls;
-  const unsigned nb3 = (v3 + BLK - 1) / BLK;
-  const unsigned pstride = (unsigned)dslash_blocks(L.d);
-  THR_CHECK((long)nb3 * nchunk <= pstride, THR_ERR_ARG, "dslash grid exceeds partial stride");
-  dim3 grid(nb3 * nchunk, (unsigned)L.nrhs);
+  Tiling TL{};
+  TL.pstride = (unsigned)dslash_blocks(L.d);
+  unsigned nblocks = 0;
+  unsigned px = 1, py = 1, pt = 8;
+  if (c.box && box_split(L.d, px, py, pt)) {
+    TL.mode = TILE_BOXES;
+    TL.px = px;
+    TL.py = py;
+    TL.bx = L.d.lx / px;
+    TL.by = L.d.ly / py;
+    TL.bt = L.d.lt / pt;
+    TL.tchunk = (unsigned)std::max(1, std::min<int>(c.tchunk, (int)TL.bt));
+    TL.nb3 = ((unsigned)L.d.ls * TL.bx * TL.by + BLK - 1) / BLK;
+    nblocks = 8 * TL.nb3 * ((TL.bt + TL.tchunk - 1) / TL.tchunk);
+  } else {
+    TL.mode = c.remap ? TILE_REMAP : TILE_LINEAR;
+    TL.tchunk = (unsigned)std::max(1, std::min(c.tchunk, L.d.lt));
+    TL.nb3 = ((unsigned)L.d.lx * L.d.ly * L.d.ls + BLK - 1) / BLK;
+    nblocks = TL.nb3 * ((L.d.lt + TL.tchunk - 1) / TL.tchunk);
+  }
+  THR_CHECK(nblocks <= TL.pstride, THR_ERR_ARG, "dslash grid exceeds partial stride");
+  dim3 grid(nblocks, (unsigned)L.nrhs);
   dim3 block(BLK);
   const bool shuf = c.laneshift && (64 % L.d.ls) == 0;
-  const int remap = c.remap;
   const bool buf = c.bufload != 0;
   THR_CHECK(!buf || 4 * L.d.v5() * Store<P64>::bytes < (1L << 32), THR_ERR_SIZE,
             "spinor too large for 32-bit buffer offsets");
 #define THR_LAUNCH(P, D, S, M)                                                              \
   do {                                                                                      \
     if (buf)                                                                                \
-      hipLaunchKernelGGL((k_dslash<P, D, S, M, true>), grid, block, 0, s, L, tc, nb3, remap,  \
-                         pstride);                                                          \
+      hipLaunchKernelGGL((k_dslash<P, D, S, M, true>), grid, block, 0, s, L, TL);           \
     else                                                                                    \
-      hipLaunchKernelGGL((k_dslash<P, D, S, M, false>), grid, block, 0, s, L, tc, nb3, remap, \
-                         pstride);                                                          \
+      hipLaunchKernelGGL((k_dslash<P, D, S, M, false>), grid, block, 0, s, L, TL);          \
   } while (0)
 #define THR_LAUNCH1(P, D, S)                        \
   switch (L.mode) {                                 \
@@ -583,9 +646,10 @@ int dslash_autotune(const Dims &d, int prec, const void *links, DslashConfig *be
   const int tcs[] = {1, 2, 4, 8, 16};
   for (int tc : tcs) {
     if (tc > d.lt) continue;
-    for (int v = 0; v < 8 && !rc; v++) {
+    for (int v = 0; v < 16 && !rc; v++) {
       if ((v & 4) && (64 % d.ls) != 0) continue;  // lane shifts need Ls | 64
-      DslashConfig c{tc, v & 1, (v >> 1) & 1, (v >> 2) & 1};
+      if ((v & 8) && (v & 1)) continue;           // boxes replace the remap
+      DslashConfig c{tc, v & 1, (v >> 1) & 1, (v >> 2) & 1, (v >> 3) & 1};
       for (int k = 0; k < 2 && !rc; k++) rc = launch_dslash_cfg(L, c, s);  // warm
       if (rc) break;
       (void)hipEventRecord(e0, s);

@@ -51,6 +51,16 @@ struct DslashConfig {
   int remap = 1;
   int bufload = 1;    // buffer-resource loads (32-bit offsets) instead of global loads
   int laneshift = 1;  // s hops as DPP lane shifts when Ls divides 64
+  int box = 0;        // XCD boxes: each XCD owns one box of an (x, y, t) split
+};
+
+// block -> lattice mapping of one Dslash launch (see dslash.hip)
+enum TileMode { TILE_LINEAR = 0, TILE_REMAP = 1, TILE_BOXES = 2 };
+struct Tiling {
+  int mode;
+  unsigned tchunk, nb3, pstride;
+  unsigned px, py;      // boxes: split of x and y (t split = 8 / (px py))
+  unsigned bx, by, bt;  // boxes: box extents
 };
 
 int launch_dslash(const DslashLaunch &L, hipStream_t s);
