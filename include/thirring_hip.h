@@ -62,7 +62,9 @@ const char *thr_last_error(void);
  * thread that binds a context (NULL: the default one) runs every later call
  * in it, so concurrent solves on different threads share no scratch memory.
  * No counterpart in the reference (one MPI rank per process there); used by
- * the serving loop to overlap independent requests on one GPU. */
+ * the serving loop to overlap independent requests on one GPU.  Solvers in a
+ * created context launch eagerly (no hipGraph capture, which allocations or
+ * copies on other threads would invalidate); the default context captures. */
 int thr_context_create(void **ctx);
 int thr_context_destroy(void *ctx);
 int thr_context_bind(void *ctx);
@@ -159,6 +161,7 @@ int thr_dslash_dev(void *out, float *out_norm, const void *in, const float *in_n
  *   THR_SHAPE_XCD_BOXES  every XCD owns one box of a split of (x, y, t) into 8
  *                        (used when such a split divides the lattice; the
  *                        remap bit is then ignored)
+ *   THR_SHAPE_PREFETCH   load the centre field one t-step further ahead
  * No counterpart in the reference (a tuning knob of this library).
  * thr_dslash_autotune times the candidates (tchunk 1..16 x flags) on scratch
  * fields with the given links (complex128 for F64, complex64 otherwise) and
@@ -169,6 +172,7 @@ int thr_dslash_dev(void *out, float *out_norm, const void *in, const float *in_n
 #define THR_SHAPE_BUFLOAD 2
 #define THR_SHAPE_LANESHIFT 4
 #define THR_SHAPE_XCD_BOXES 8
+#define THR_SHAPE_PREFETCH 16
 int thr_dslash_autotune(const void *links, const int dims[4], int prec, int *tchunk, int *flags,
                         void *stream);
 int thr_dslash_set_config(const int dims[4], int prec, int tchunk, int flags);

@@ -235,6 +235,7 @@ __device__ inline void hop(int mu, cx<C> o[4], cx<C> uf, const cx<C> rf[4], cx<C
 // t neighbour at t+2, so the t hops come from registers.  x and y hops are
 // L1/L2 hits (neighbouring lanes and waves of the same and nearby blocks).
 // With SHUF (Ls divides 64) the s hops are DPP lane shifts of the centre.
+// With PF the centre stream is software-pipelined one step deeper.
 // Hardware block b runs on XCD b % 8, and each XCD has its own L2, so the
 // tiling decides which neighbours are L2 hits:
 //   linear  blocks walk the s-x-y slab of a t-chunk in order;
@@ -245,7 +246,7 @@ __device__ inline void hop(int mu, cx<C> o[4], cx<C> uf, const cx<C> rf[4], cx<C
 //           are fetched across XCDs.
 // Partial sums go to partial[rhs * pstride + blockIdx.x]; block 0 zero-fills
 // [gridDim.x, pstride) so consumers can always sum pstride entries.
-template <int PREC, bool DAG, bool SHUF, int MODE, bool BUF>
+template <int PREC, bool DAG, bool SHUF, int MODE, bool BUF, bool PF>
 __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, const Tiling TL) {
   using C = typename Store<PREC>::C;
   using T = typename Store<PREC>::T;
@@ -306,32 +307,50 @@ __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, const Tili
     const C m = (C)L.am[rhs];
     const bool shops = !L.wilson4d;
 
-    cx<C> prv[4], cur[4], nxt[4];
+    cx<C> prv[4], cur[4], nxt[4], nn[4];
     const unsigned tb0 = (t0 == 0) ? lt - 1 : t0 - 1;
     in.load4(j + v3 * tb0, prv);
     cx<C> ubt = conj(lk(2 * v4 + xy + v2 * tb0));
     in.load4(j + v3 * t0, cur);
+    if constexpr (PF) in.load4(j + v3 * ((t0 + 1 == lt) ? 0 : t0 + 1), nxt);
     for (unsigned t = t0; t < t1; t++) {
       const unsigned tf = (t + 1 == lt) ? 0 : t + 1;
-      in.load4(j + v3 * tf, nxt);
+      if constexpr (!PF) in.load4(j + v3 * tf, nxt);
       const cx<C> uft = lk(2 * v4 + xy + v2 * t);
       const unsigned i = j + v3 * t;
       const unsigned l4 = v2 * t;
       cx<C> o[4];
+      if constexpr (PF) {
+        cx<C> rxf[4], rxb[4], ryf[4], ryb[4];
+        in.load4(jxf + v3 * t, rxf);
+        in.load4(jxb + v3 * t, rxb);
+        in.load4(jyf + v3 * t, ryf);
+        in.load4(jyb + v3 * t, ryb);
+        // R(t+2) for the step after next, issued after this step's neighbour
+        // loads so that waiting for those does not wait for it: its HBM
+        // latency overlaps this step's arithmetic and stores
+        __builtin_amdgcn_sched_barrier(0);
+        if (t + 1 < t1) in.load4(j + v3 * ((tf + 1 == lt) ? 0 : tf + 1), nn);
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-      for (int a = 0; a < 4; a++) o[a] = diag * cur[a];
-      {
-        cx<C> rf[4], rb[4];
-        in.load4(jxf + v3 * t, rf);
-        in.load4(jxb + v3 * t, rb);
-        hop<C, DAG>(0, o, lk(xy + l4), rf, conj(lk(lxb + l4)), rb);
-      }
-      {
-        cx<C> rf[4], rb[4];
-        in.load4(jyf + v3 * t, rf);
-        in.load4(jyb + v3 * t, rb);
-        hop<C, DAG>(1, o, lk(v4 + xy + l4), rf,
-                    conj(lk(v4 + lyb + l4)), rb);
+        for (int a = 0; a < 4; a++) o[a] = diag * cur[a];
+        hop<C, DAG>(0, o, lk(xy + l4), rxf, conj(lk(lxb + l4)), rxb);
+        hop<C, DAG>(1, o, lk(v4 + xy + l4), ryf, conj(lk(v4 + lyb + l4)), ryb);
+      } else {
+#pragma unroll
+        for (int a = 0; a < 4; a++) o[a] = diag * cur[a];
+        {
+          cx<C> rf[4], rb[4];
+          in.load4(jxf + v3 * t, rf);
+          in.load4(jxb + v3 * t, rb);
+          hop<C, DAG>(0, o, lk(xy + l4), rf, conj(lk(lxb + l4)), rb);
+        }
+        {
+          cx<C> rf[4], rb[4];
+          in.load4(jyf + v3 * t, rf);
+          in.load4(jyb + v3 * t, rb);
+          hop<C, DAG>(1, o, lk(v4 + xy + l4), rf, conj(lk(v4 + lyb + l4)), rb);
+        }
       }
       hop<C, DAG>(2, o, uft, nxt, ubt, prv);
 
@@ -436,6 +455,7 @@ __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, const Tili
       for (int a = 0; a < 4; a++) {
         prv[a] = cur[a];
         cur[a] = nxt[a];
+        if constexpr (PF) nxt[a] = nn[a];
       }
       ubt = conj(uft);
     }
@@ -487,7 +507,7 @@ static bool box_split(const Dims &d, unsigned &px, unsigned &py, unsigned &pt) {
 // XCD boxes, buffer or global loads, lane-shift s hops.  The default is fixed (results
 // identical run to run); dslash_autotune times the candidates on scratch
 // fields and pins the fastest for the process.  THR_DSLASH_TCHUNK / _REMAP /
-// _BUFLOAD / _LANESHIFT / _BOXES override the default.
+// _BUFLOAD / _LANESHIFT / _BOXES / _PREFETCH override the default.
 // ---------------------------------------------------------------------------
 namespace {
 struct TuneKey {
@@ -509,6 +529,7 @@ DslashConfig default_config(const Dims &d) {
   if (const char *e = std::getenv("THR_DSLASH_BUFLOAD")) c.bufload = std::atoi(e) ? 1 : 0;
   if (const char *e = std::getenv("THR_DSLASH_LANESHIFT")) c.laneshift = std::atoi(e) ? 1 : 0;
   if (const char *e = std::getenv("THR_DSLASH_BOXES")) c.box = std::atoi(e) ? 1 : 0;
+  if (const char *e = std::getenv("THR_DSLASH_PREFETCH")) c.prefetch = std::atoi(e) ? 1 : 0;
   return c;
 }
 }  // namespace
@@ -560,14 +581,19 @@ int launch_dslash_cfg(const DslashLaunch &L, DslashConfig c, hipStream_t s) {
   dim3 block(BLK);
   const bool shuf = c.laneshift && (64 % L.d.ls) == 0;
   const bool buf = c.bufload != 0;
+  const bool pf = c.prefetch != 0;
   THR_CHECK(!buf || 4 * L.d.v5() * Store<P64>::bytes < (1L << 32), THR_ERR_SIZE,
             "spinor too large for 32-bit buffer offsets");
-#define THR_LAUNCH(P, D, S, M)                                                              \
-  do {                                                                                      \
-    if (buf)                                                                                \
-      hipLaunchKernelGGL((k_dslash<P, D, S, M, true>), grid, block, 0, s, L, TL);           \
-    else                                                                                    \
-      hipLaunchKernelGGL((k_dslash<P, D, S, M, false>), grid, block, 0, s, L, TL);          \
+#define THR_LAUNCH(P, D, S, M)                                                        \
+  do {                                                                                \
+    if (buf && pf)                                                                    \
+      hipLaunchKernelGGL((k_dslash<P, D, S, M, true, true>), grid, block, 0, s, L, TL);   \
+    else if (buf)                                                                     \
+      hipLaunchKernelGGL((k_dslash<P, D, S, M, true, false>), grid, block, 0, s, L, TL);  \
+    else if (pf)                                                                      \
+      hipLaunchKernelGGL((k_dslash<P, D, S, M, false, true>), grid, block, 0, s, L, TL);  \
+    else                                                                              \
+      hipLaunchKernelGGL((k_dslash<P, D, S, M, false, false>), grid, block, 0, s, L, TL); \
   } while (0)
 #define THR_LAUNCH1(P, D, S)                        \
   switch (L.mode) {                                 \
@@ -646,10 +672,10 @@ int dslash_autotune(const Dims &d, int prec, const void *links, DslashConfig *be
   const int tcs[] = {1, 2, 4, 8, 16};
   for (int tc : tcs) {
     if (tc > d.lt) continue;
-    for (int v = 0; v < 16 && !rc; v++) {
+    for (int v = 0; v < 32 && !rc; v++) {
       if ((v & 4) && (64 % d.ls) != 0) continue;  // lane shifts need Ls | 64
       if ((v & 8) && (v & 1)) continue;           // boxes replace the remap
-      DslashConfig c{tc, v & 1, (v >> 1) & 1, (v >> 2) & 1, (v >> 3) & 1};
+      DslashConfig c{tc, v & 1, (v >> 1) & 1, (v >> 2) & 1, (v >> 3) & 1, (v >> 4) & 1};
       for (int k = 0; k < 2 && !rc; k++) rc = launch_dslash_cfg(L, c, s);  // warm
       if (rc) break;
       (void)hipEventRecord(e0, s);

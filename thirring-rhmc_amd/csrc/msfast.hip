@@ -501,7 +501,7 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
   // issues identical launches, so it can be captured once per precision as a
   // hipGraph and replayed (THR_GRAPHS=0 disables)
   const char *ge = std::getenv("THR_GRAPHS");
-  const bool use_graph = !(ge && ge[0] == '0');
+  const bool use_graph = !(ge && ge[0] == '0') && context_graphs();
   hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
   auto enqueue_block = [&](long t0, int pr) -> int {
     for (int j = 0; j < MSB; j++) {
@@ -583,7 +583,7 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
         if (hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed) == hipSuccess) {
           rc = enqueue_block(t, prec);
           const hipError_t e = hipStreamEndCapture(s, &g);
-          if (rc && e == hipSuccess) {  // argument error, not a capture problem
+          if (rc && rc != THR_ERR_HIP) {  // argument error, not a capture problem
             if (g) (void)hipGraphDestroy(g);
             free_graphs();
             return rc;

@@ -27,6 +27,8 @@ size_t workspace_bytes();
 void *workspace(int slot, size_t bytes);
 double *partial_buffer(int nrhs_blocks);
 hipStream_t default_stream();
+// whether solvers may capture their blocks as hipGraphs in the bound context
+bool context_graphs();
 
 // reference-faithful CG on M^+M (measure_module.F90:9-140); x in/out (FP64 device)
 int cg_reference(const OpDesc &op, const double2 *phi, double2 *x, float res, int niterc,

@@ -34,6 +34,10 @@ struct Context {
   std::map<int, Buf> ws;
   std::mutex mu;
   hipStream_t stream = nullptr;
+  // hipGraph capture of solver blocks; off in created contexts, which run
+  // next to other threads' allocations and copies (a stream capture is
+  // invalidated by device-synchronising calls from any thread)
+  bool graphs = true;
 };
 
 namespace {
@@ -42,7 +46,13 @@ thread_local Context *t_ctx = nullptr;
 Context &cur() { return t_ctx ? *t_ctx : g_default; }
 }  // namespace
 
-Context *context_create() { return new Context(); }
+Context *context_create() {
+  Context *c = new Context();
+  c->graphs = false;
+  return c;
+}
+
+bool context_graphs() { return cur().graphs; }
 
 void context_destroy(Context *c) {
   if (!c) return;
