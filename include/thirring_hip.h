@@ -199,13 +199,15 @@ int thr_multishift_plain_dev(const void *links64, const int dims[4], float am, i
 
 /* Batched mixed-precision multishift with blocked shift updates: nsrc sources
  * b[nsrc][4 V5] (complex128) with masses am[nsrc], the same nshift shifts for
- * all; solutions x[nsrc][nshift][4 V5] (complex128).  Each shift stops when
- * |b - (M^+M + aden_i) x_i| (iterated) <= tol |b|; Krylov storage drops to FP32
- * below |r|/|b| < th32 and to FP16 below th16 (<= 0 disables).
+ * all; solutions x[nsrc][nshift][4 V5] (complex128).  tol is the FP64 true
+ * residual asked for: each shift iterates until |b - (M^+M + aden_i) x_i|
+ * (iterated) <= 0.75 tol |b|; Krylov storage drops to FP32 below
+ * |r|/|b| < th32 and to FP16 below th16 (<= 0 disables).
  * max_corrections (0..16) > 0 adds FP64 defect correction: the true residual of
  * every (source, shift) is computed in FP64 and each one above tol |b| is solved
- * again for its residual (FP32 Krylov storage) and the correction added, up to
- * max_corrections rounds, so low-precision storage still meets tol in FP64.
+ * again for its residual (FP32 Krylov storage, to tol |b| / 2) and the
+ * correction added, up to max_corrections rounds, so low-precision storage
+ * still meets tol in FP64.
  * iterations: iterations run; cg_return[nsrc]: per-source iterations;
  * corr_iterations (optional): iterations spent in corrections; relres
  * (optional, nsrc*nshift): final true FP64 residuals |b - (M^+M + sigma_i) x_i| / |b|.

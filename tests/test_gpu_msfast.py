@@ -109,7 +109,9 @@ def test_plain_fp64_path(G):
     n = len(sig)
     nsp = bh.shape[1]
     out = DeviceArray((n, nsp, 2), np.float64)
-    cg, cgs = G.multishift_plain_dev(l64, dims, 0.05, 3, sig, b, out, tol=5e-11, maxcg=20000)
+    # the fast path iterates to 3/4 of the requested true residual
+    cg, cgs = G.multishift_plain_dev(l64, dims, 0.05, 3, sig, b, out, tol=0.75 * 5e-11,
+                                     maxcg=20000)
     x = DeviceArray((1, n, nsp, 2), np.float64)
     it, cgr, rel = G.multishift_fast_dev(l64, l32, dims, 0.05, 3, sig, b, x, tol=5e-11,
                                          maxcg=20000)

@@ -588,7 +588,10 @@ int thr_multishift_fast_dev(const void *links64, const void *links32, const int 
   A.u32 = (const float2 *)links32;
   A.b = (const double2 *)b;
   A.x = (double2 *)x;
-  A.tol = tol;
+  // iterate each shift to 3/4 of the requested true residual, which leaves an
+  // FP64 solve below tol without corrections (its true residual exceeds the
+  // iterated one by rounding only)
+  A.tol = 0.75 * tol;
   A.maxcg = maxcg;
   A.th32 = th32;
   A.th16 = th16;
@@ -601,7 +604,7 @@ int thr_multishift_fast_dev(const void *links64, const void *links32, const int 
   if (corr_iterations) *corr_iterations = 0;
   if (max_corrections > 0) {
     std::vector<double> rr((size_t)nsrc * nshift);
-    TRY(msfast_correct(A, max_corrections, rr.data(), corr_iterations, s));
+    TRY(msfast_correct(A, max_corrections, tol, rr.data(), corr_iterations, s));
     if (relres) std::copy(rr.begin(), rr.end(), relres);
   } else if (relres) {
     TRY(msfast_true_residuals(A, relres, s));

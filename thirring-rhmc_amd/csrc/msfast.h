@@ -46,11 +46,12 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s);
 // relres[src * nshift + i] = |b - (M^+M + sigma_i) x_i| / |b| in FP64
 int msfast_true_residuals(const MsfArgs &A, double *relres, hipStream_t s);
 // FP64 defect correction after msfast_solve: every (source, shift) whose true
-// residual exceeds tol is solved again for that residual (FP32 Krylov storage,
-// one shift, batched over pairs) and the correction added to x_i; at most
-// max_rounds rounds.  relres (host, nsrc*nshift) receives the final true
-// residuals, corr_iters the iterations spent in corrections.
-int msfast_correct(const MsfArgs &A, int max_rounds, double *relres, int *corr_iters,
-                   hipStream_t s);
+// residual exceeds tol_true is solved again for that residual (FP32 Krylov
+// storage, one shift, batched over pairs) to tol_true / 2 and the correction
+// added to x_i; at most max_rounds rounds.  relres (host, nsrc*nshift)
+// receives the final true residuals, corr_iters the iterations spent in
+// corrections.
+int msfast_correct(const MsfArgs &A, int max_rounds, double tol_true, double *relres,
+                   int *corr_iters, hipStream_t s);
 
 }  // namespace thr

@@ -719,8 +719,8 @@ int msfast_true_residuals(const MsfArgs &A, double *relres, hipStream_t s) {
   return THR_OK;
 }
 
-int msfast_correct(const MsfArgs &A, int max_rounds, double *relres, int *corr_iters,
-                   hipStream_t s) {
+int msfast_correct(const MsfArgs &A, int max_rounds, double tol_true, double *relres,
+                   int *corr_iters, hipStream_t s) {
   const Dims &d = A.d;
   const int n = A.nshift, ns_ = A.nsrc;
   const long v5 = d.v5(), nsp = 4 * v5;
@@ -730,7 +730,7 @@ int msfast_correct(const MsfArgs &A, int max_rounds, double *relres, int *corr_i
     TRY_RC(msfast_true_residuals(A, rr.data(), s));
     std::vector<int> fail;  // k * n + i
     for (int q = 0; q < ns_ * n; q++)
-      if (!(rr[q] <= A.tol)) fail.push_back(q);
+      if (!(rr[q] <= tol_true)) fail.push_back(q);
     if (fail.empty() || round >= max_rounds) break;
     // buffers for one group of up to ns_ (source, shift) pairs
     double2 *xs = (double2 *)workspace(372, sizeof(double2) * nsp * ns_);
@@ -750,7 +750,7 @@ int msfast_correct(const MsfArgs &A, int max_rounds, double *relres, int *corr_i
         sg[j] = A.aden[i];
         am[j] = A.am[k];
         // solve (M^+M + sigma_i) c = r_ki to |r_ki - (M^+M + sigma_i) c| <= tol |b_k| / 2
-        tl[j] = 0.5 * A.tol / rr[q];
+        tl[j] = 0.5 * tol_true / rr[q];
         THR_HIP(hipMemcpyAsync(xs + j * nsp, A.x + (long)q * nsp, sizeof(double2) * nsp,
                                hipMemcpyDeviceToDevice, s));
         THR_HIP(hipMemcpyAsync(bs + j * nsp, A.b + (long)k * nsp, sizeof(double2) * nsp,

@@ -88,7 +88,7 @@ def solve_request(req, stream=None, maxcg=200000):
     bd = hr.DeviceArray.from_host(b)
     x = hr.DeviceArray((req.nsrc, 1, nsp, 2), np.float64)
     sig = np.zeros(1)
-    it, cgr, rel = G.multishift_fast_dev(l64, l32, req.dims, req.am, 3, sig, bd, x, tol=req.tol * 0.5,
+    it, cgr, rel = G.multishift_fast_dev(l64, l32, req.dims, req.am, 3, sig, bd, x, tol=req.tol,
                                          maxcg=maxcg, nsrc=req.nsrc, th32=1e-4,
                                          true_residuals=True, stream=stream)
     xh = x.to_host()
