@@ -439,6 +439,19 @@ __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, const Tili
         }
       } else {
         Out<PREC> out{(T *)L.out + roff, L.out_norm ? L.out_norm + (long)rhs * v5 : nullptr, v5};
+        if constexpr (MODE == EPI_HREC) {
+          // out = M in + b out: this thread reads its own site before writing it
+          const C bb = (C)L.b_ptr[(long)rhs * L.sc_stride];
+          Acc<PREC> ho{(const T *)L.out + roff, L.out_norm ? L.out_norm + (long)rhs * v5 : nullptr,
+                       v5};
+          cx<C> hv[4];
+          ho.load4(i, hv);
+#pragma unroll
+          for (int k = 0; k < 4; k++) {
+            o[k] = o[k] + bb * hv[k];
+            red += (double)norm2(o[k]);
+          }
+        }
         out.store4(i, o);
         if constexpr (MODE == EPI_NORM) {
 #pragma unroll
@@ -555,6 +568,8 @@ int launch_dslash_cfg(const DslashLaunch &L, DslashConfig c, hipStream_t s) {
                                                    (L.r_out ? L.r_out_norm != nullptr : true)),
             THR_ERR_ARG, "fp16 CG epilogue needs r scales and no x update");
   THR_CHECK(L.prec != P16 || L.mode != EPI_DOTW, THR_ERR_ARG, "fp16 dot epilogue unsupported");
+  THR_CHECK(L.mode != EPI_HREC || (L.b_ptr && (L.prec != P16 || L.out_norm)), THR_ERR_ARG,
+            "recurrence epilogue needs its scalar (and fp16 scales)");
   THR_CHECK(4 * L.d.v5() < (1L << 31), THR_ERR_SIZE, "lattice too large for 32-bit indexing");
   Tiling TL{};
   TL.pstride = (unsigned)dslash_blocks(L.d);
@@ -600,6 +615,7 @@ int launch_dslash_cfg(const DslashLaunch &L, DslashConfig c, hipStream_t s) {
     case EPI_STORE: THR_LAUNCH(P, D, S, EPI_STORE); break; \
     case EPI_NORM: THR_LAUNCH(P, D, S, EPI_NORM); break;   \
     case EPI_DOTW: THR_LAUNCH(P, D, S, EPI_DOTW); break;   \
+    case EPI_HREC: THR_LAUNCH(P, D, S, EPI_HREC); break;   \
     default: THR_LAUNCH(P, D, S, EPI_CG); break;           \
   }
 #define THR_LAUNCH2(P, D) \

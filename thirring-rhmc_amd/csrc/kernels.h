@@ -12,6 +12,7 @@ enum EpiMode {
   EPI_NORM = 1,      // out = M r ; partial += |out|^2
   EPI_DOTW = 2,      // out = M r ; partial += Re <w, out>
   EPI_CG = 3,        // (CG dagger step) r -= a*out ; x += b*p ; partial += |r|^2 ; out not stored
+  EPI_HREC = 4,      // out = M r + b*out (in place) ; partial += |out|^2  (h = M p by recurrence)
 };
 
 struct DslashLaunch {
@@ -36,7 +37,8 @@ struct DslashLaunch {
   const float *r_norm; // EPI_CG with FP16 storage: per-site scales of r / r_out
   float *r_out_norm;
   const double *a_ptr; // EPI_CG: r -= a_ptr[rhs*sc_stride] * out (device scalar)
-  const double *b_ptr; // EPI_CG: x += b_ptr[rhs*sc_stride] * p  (null: no x update)
+  const double *b_ptr; // EPI_CG: x += b_ptr[rhs*sc_stride] * p  (null: no x update);
+                       // EPI_HREC: out = M in + b_ptr[rhs*sc_stride] * out
   int sc_stride;       // stride (in doubles) between the per-rhs scalars
   double *partial;     // [nrhs][nblocks]
   const int *done;     // optional early-exit flag, done[rhs*done_stride] (device)

@@ -6,6 +6,10 @@
 //  * alpha = Re<p, M^+M p> = |M p|^2 is taken from the M pass, so omega is
 //    known before M^+ runs and the M^+ pass updates r in its epilogue
 //    (s = M^+ h is never stored).
+//  * The search direction p is never stored: h = M p obeys
+//    h_new = M r_new + gamma h_old, so the M pass reads r and the old h and
+//    writes h in place (no p update pass: 2 fewer field streams per
+//    iteration).
 //  * Shift vectors are not touched every iteration.  Every p_i and x_i is a
 //    linear combination of its value at the start of a block and the block's
 //    residuals R_0..R_{B-1}; the per-iteration recurrences
@@ -182,42 +186,6 @@ __global__ __launch_bounds__(BLK) void k_msf_fin2(MsfState *st, const double *pa
     S->delta = L;
     S->jcount = j + 1;
     if (nmax == 0 || cgr >= S->maxcg) S->stop = 1;
-  }
-}
-
-// p = r_new + gamma p  (per source)
-template <int P>
-__global__ __launch_bounds__(BLK) void k_msf_xpay(long v5, const TT<P> *r, const float *rn,
-                                                  TT<P> *p, float *pn, const MsfState *sts) {
-  const int src = blockIdx.y;
-  const MsfState *S = sts + src;
-  if (S->stop) return;
-  const long i = (long)blockIdx.x * BLK + threadIdx.x;
-  if (i >= v5) return;
-  const long off = (long)src * 4 * v5;
-  using C = CC<P>;
-  const C g = (C)S->gammag;
-  cx<C> v[4];
-  if constexpr (P == P16) {
-    const float sr = rn[(long)src * v5 + i], sp = pn[(long)src * v5 + i];
-    float m = 0.f;
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      const float2 a = __half22float2(r[off + k * v5 + i]);
-      const float2 b = __half22float2(p[off + k * v5 + i]);
-      v[k] = {a.x * sr + g * b.x * sp, a.y * sr + g * b.y * sp};
-      m = fmaxf(m, fmaxf(fabsf(v[k].r), fabsf(v[k].i)));
-    }
-    const float inv = m > 0.f ? 1.f / m : 0.f;
-    pn[(long)src * v5 + i] = m;
-#pragma unroll
-    for (int k = 0; k < 4; k++) p[off + k * v5 + i] = __floats2half2_rn(v[k].r * inv, v[k].i * inv);
-  } else {
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      const long e = off + k * v5 + i;
-      st(p, e, ldc(r, e) + g * ldc(p, e));
-    }
   }
 }
 
@@ -426,21 +394,19 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
   size_t bytes_p[3] = {16, 8, 4};
   void *ring[3][MSB];
   float *ringn[MSB];
-  void *pbuf[3], *hbuf[3], *r0[3];
-  float *pn = nullptr, *hn = nullptr, *r0n = nullptr;
+  void *hbuf[3], *r0[3];
+  float *hn = nullptr, *r0n = nullptr;
   for (int pr = 0; pr < 3; pr++) {
     const bool need = (pr == P64) || (pr == P32 && A.th32 > 0) || (pr == P16 && A.th16 > 0);
     for (int q = 0; q < MSB; q++)
       ring[pr][q] = need ? workspace(200 + pr * 40 + q, bytes_p[pr] * nsb) : nullptr;
-    pbuf[pr] = need ? workspace(200 + pr * 40 + 20, bytes_p[pr] * nsb) : nullptr;
     hbuf[pr] = need ? workspace(200 + pr * 40 + 21, bytes_p[pr] * nsb) : nullptr;
     r0[pr] = need ? workspace(200 + pr * 40 + 22, bytes_p[pr] * nsb) : nullptr;
     if (need)
-      THR_CHECK(pbuf[pr] && hbuf[pr] && r0[pr], THR_ERR_HIP, "workspace allocation failed");
+      THR_CHECK(hbuf[pr] && r0[pr], THR_ERR_HIP, "workspace allocation failed");
   }
   if (A.th16 > 0) {
     for (int q = 0; q < MSB; q++) ringn[q] = (float *)workspace(340 + q, sizeof(float) * v5 * ns_);
-    pn = (float *)workspace(360, sizeof(float) * v5 * ns_);
     hn = (float *)workspace(361, sizeof(float) * v5 * ns_);
     r0n = (float *)workspace(362, sizeof(float) * v5 * ns_);
   }
@@ -480,7 +446,6 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
             "start precision needs its buffers (set th32/th16)");
   // r0 and p in the starting precision
   TRY_RC(conv(d, ns_, P64, A.b, nullptr, prec, r0[prec], prec == P16 ? r0n : nullptr, s));
-  TRY_RC(conv(d, ns_, P64, A.b, nullptr, prec, pbuf[prec], prec == P16 ? pn : nullptr, s));
 
   DslashLaunch L{};
   L.d = d;
@@ -510,16 +475,23 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
       float *rinn = (pr == P16) ? ((tt == 0) ? r0n : ringn[(tt - 1) % MSB]) : nullptr;
       void *rout = ring[pr][tt % MSB];
       float *routn = (pr == P16) ? ringn[tt % MSB] : nullptr;
-      // h = M p ; alpha = |h|^2
+      // h = M p ; alpha = |h|^2.  p = r + gamma p is never formed: h obeys
+      // h_new = M r_new + gamma h_old (M is linear), so the pass reads r and
+      // the old h and writes h in place
       DslashLaunch a = L;
       a.prec = pr;
       a.links = pr == P64 ? (const void *)A.u64 : (const void *)A.u32;
       a.dagger = 0;
-      a.in = pbuf[pr];
-      a.in_norm = pr == P16 ? pn : nullptr;
+      a.in = rin;
+      a.in_norm = rinn;
       a.out = hbuf[pr];
       a.out_norm = pr == P16 ? hn : nullptr;
-      a.mode = EPI_NORM;
+      if (tt == 0) {
+        a.mode = EPI_NORM;  // p_0 = r_0 = b
+      } else {
+        a.mode = EPI_HREC;
+        a.b_ptr = &st->gammag;
+      }
       TRY_RC(launch_dslash(a, s));
       hipLaunchKernelGGL(k_msf_fin1, dim3(ns_), dim3(BLK), 0, s, st, aden, partial, nb);
       // r_new = r + omega M^+ h ; lambda = |r_new|^2
@@ -538,16 +510,6 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
       b.b_ptr = nullptr;
       TRY_RC(launch_dslash(b, s));
       hipLaunchKernelGGL(k_msf_fin2, dim3(ns_), dim3(BLK), 0, s, st, partial, nb);
-      dim3 gx(nblk(v5), ns_);
-      if (pr == P64)
-        hipLaunchKernelGGL(k_msf_xpay<P64>, gx, dim3(BLK), 0, s, v5, (const double2 *)rout, nullptr,
-                           (double2 *)pbuf[P64], nullptr, st);
-      else if (pr == P32)
-        hipLaunchKernelGGL(k_msf_xpay<P32>, gx, dim3(BLK), 0, s, v5, (const float2 *)rout, nullptr,
-                           (float2 *)pbuf[P32], nullptr, st);
-      else
-        hipLaunchKernelGGL(k_msf_xpay<P16>, gx, dim3(BLK), 0, s, v5, (const __half2 *)rout, routn,
-                           (__half2 *)pbuf[P16], pn, st);
       THR_HIP(hipGetLastError());
     }
     // flush the block into x_i, p_i
@@ -646,8 +608,8 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
       int rcc = conv(d, ns_, prec, rl, rln, want, ring[want][(t - 1) % MSB],
                      want == P16 ? ringn[(t - 1) % MSB] : nullptr, s);
       if (!rcc)
-        rcc = conv(d, ns_, prec, pbuf[prec], prec == P16 ? pn : nullptr, want, pbuf[want],
-                   want == P16 ? pn : nullptr, s);
+        rcc = conv(d, ns_, prec, hbuf[prec], prec == P16 ? hn : nullptr, want, hbuf[want],
+                   want == P16 ? hn : nullptr, s);
       if (rcc) {
         free_graphs();
         return rcc;
