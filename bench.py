@@ -19,6 +19,8 @@ Also reported in the same JSON line:
     x^(-1/4) approximation (remez4 block 2): the plain FP64 GPU path (the
     reference's recurrences) against the batched, blocked-update,
     mixed-precision path, with the true residuals of both;
+  * serving: p50/p99 latency and solves/s of a Poisson load of measurement
+    solves with continuous batching (thirring-rhmc_amd/serving.py);
   * the CPU baseline: the FP64 C oracle's Dslash on the host cores.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
@@ -165,6 +167,15 @@ def bench_multishift(G, hr, dims, l64, l32, am, nsrc, stream, tol=1e-10, maxcg=5
     return out
 
 
+def bench_serving(rate, seconds, slots, max_requests):
+    """Poisson load of measurement solves (L in {8,12,16,24}, T in {L,2L}, Ls in
+    {8,16,32,48}, beta in [0.2,0.5], am in {0.005..0.05}, 1-12 sources, FP64
+    true residual 1e-10), continuous batching over `slots` contexts."""
+    import serving as S
+    progress(f"serving: Poisson rate {rate}/s for {seconds} s, {slots} slots")
+    return S.run_load(rate=rate, duration=seconds, seed=3, slots=slots, max_requests=max_requests)
+
+
 def cpu_baseline(dims, seconds=12.0):
     """Oracle (own FP64 C restatement, OpenMP) Dslash on the host cores."""
     import dense as D
@@ -198,6 +209,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solver", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-rate", type=float, default=2.0)
+    ap.add_argument("--serve-seconds", type=float, default=15.0)
+    ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--solver-lattice", default="32,32,32,64")
     ap.add_argument("--solver-am", type=float, default=0.005)
     ap.add_argument("--solver-sources", type=int, default=4)
@@ -263,6 +278,9 @@ def main():
                                   mixed_th32=args.mixed_th32, mixed_th16=args.mixed_th16)
         solver["lattice"] = list(sd)
         solver["am"] = args.solver_am
+    serving = None
+    if not args.no_serving:
+        serving = bench_serving(args.serve_rate, args.serve_seconds, args.serve_slots, 60)
     wall = time.time() - t_wall0
 
     traffic = None
@@ -298,6 +316,7 @@ def main():
         "cpu_baseline": cpu,
         "dslash_other_precisions": extra,
         "multishift": solver,
+        "serving": serving,
         "bench_wall_s": round(wall, 2),
     }
     print(json.dumps(line))
