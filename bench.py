@@ -221,8 +221,8 @@ def main():
     ap.add_argument("--tchunk", type=int, default=0,
                     help="pin the Dslash t-chunk (with --shape-flags) instead of autotuning")
     ap.add_argument("--shape-flags", type=int, default=7,
-                    help="with --tchunk: 1 XCD remap | 2 buffer loads | 4 lane-shift s hops | "
-                         "8 XCD boxes | 16 prefetch")
+                    help="with --tchunk: 1 XCD remap | 2 uniform links | 4 lane-shift s hops | "
+                         "8 XCD boxes")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run N device copies of 1 GiB (scales the HBM counters)")
     args = ap.parse_args()

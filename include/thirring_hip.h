@@ -155,13 +155,13 @@ int thr_dslash_dev(void *out, float *out_norm, const void *in, const float *in_n
 
 /* Launch shape of the Dslash kernel for (dims, prec): each thread marches
  * through `tchunk` t-slices (t hops from registers); `flags` is a bit set:
- *   THR_SHAPE_XCD_REMAP  give every XCD a contiguous range of blocks
- *   THR_SHAPE_BUFLOAD    read fields through buffer resources (32-bit offsets)
- *   THR_SHAPE_LANESHIFT  s hops as DPP lane shifts (used when Ls divides 64)
- *   THR_SHAPE_XCD_BOXES  every XCD owns one box of a split of (x, y, t) into 8
- *                        (used when such a split divides the lattice; the
- *                        remap bit is then ignored)
- *   THR_SHAPE_PREFETCH   load the centre field one t-step further ahead
+ *   THR_SHAPE_XCD_REMAP      give every XCD a contiguous range of blocks
+ *   THR_SHAPE_UNIFORM_LINKS  Ls = 64 (a wave is one s-line): links through
+ *                            scalar loads, wall couplings as lane reads
+ *   THR_SHAPE_LANESHIFT      s hops as DPP lane shifts (used when Ls divides 64)
+ *   THR_SHAPE_XCD_BOXES      every XCD owns one box of a split of (x, y, t)
+ *                            into 8 (used when such a split divides the
+ *                            lattice; the remap bit is then ignored)
  * No counterpart in the reference (a tuning knob of this library).
  * thr_dslash_autotune times the candidates (tchunk 1..16 x flags) on scratch
  * fields with the given links (complex128 for F64, complex64 otherwise) and
@@ -169,10 +169,9 @@ int thr_dslash_dev(void *out, float *out_norm, const void *in, const float *in_n
  * both apply to every later launch (solvers included) with these dims and
  * precision. */
 #define THR_SHAPE_XCD_REMAP 1
-#define THR_SHAPE_BUFLOAD 2
+#define THR_SHAPE_UNIFORM_LINKS 2
 #define THR_SHAPE_LANESHIFT 4
 #define THR_SHAPE_XCD_BOXES 8
-#define THR_SHAPE_PREFETCH 16
 int thr_dslash_autotune(const void *links, const int dims[4], int prec, int *tchunk, int *flags,
                         void *stream);
 int thr_dslash_set_config(const int dims[4], int prec, int tchunk, int flags);

@@ -142,24 +142,40 @@ template <typename C> struct BLinks {
   }
 };
 
-template <typename C> struct GLinks {
+// Links through the scalar cache: when a wave is exactly one s-line (Ls = 64)
+// every lane of the wave needs the same links, so the index is made
+// wave-uniform and the load is a scalar load (no vector memory instruction);
+// the links are read-only for the whole kernel.
+typedef const __attribute__((address_space(4))) double ConstF64;
+typedef const __attribute__((address_space(4))) float ConstF32;
+template <typename C> struct SLinks {
   const void *p;
-  __device__ inline GLinks(const void *q, unsigned) : p(q) {}
+  __device__ inline SLinks(const void *q, unsigned) : p(q) {}
   __device__ inline cx<C> operator()(unsigned i) const {
+    const unsigned u = __builtin_amdgcn_readfirstlane(i);
     if constexpr (sizeof(C) == 8) {
-      const double2 v = ((const double2 *)p)[i];
-      return {v.x, v.y};
+      ConstF64 *q = (ConstF64 *)p;
+      return {q[2 * u], q[2 * u + 1]};
     } else {
-      const float2 v = ((const float2 *)p)[i];
-      return {v.x, v.y};
+      ConstF32 *q = (ConstF32 *)p;
+      return {q[2 * u], q[2 * u + 1]};
     }
   }
 };
 
-template <int PREC> struct GAcc : Acc<PREC> {
-  __device__ inline GAcc(const void *p, const float *nrm, unsigned v5)
-      : Acc<PREC>{(const typename Store<PREC>::T *)p, nrm, v5} {}
-};
+// value of lane k of the wave (a wave-uniform broadcast)
+__device__ inline float lane_value(float v, int k) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
+}
+__device__ inline double lane_value(double v, int k) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), k);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+template <typename C> __device__ inline cx<C> lane_value(cx<C> v, int k) {
+  return {lane_value(v.r, k), lane_value(v.i, k)};
+}
 
 template <typename C> __device__ inline cx<C> link(const void *links, long i) {
   if constexpr (sizeof(C) == 8) {
@@ -235,7 +251,8 @@ __device__ inline void hop(int mu, cx<C> o[4], cx<C> uf, const cx<C> rf[4], cx<C
 // t neighbour at t+2, so the t hops come from registers.  x and y hops are
 // L1/L2 hits (neighbouring lanes and waves of the same and nearby blocks).
 // With SHUF (Ls divides 64) the s hops are DPP lane shifts of the centre.
-// With PF the centre stream is software-pipelined one step deeper.
+// With UL (Ls = 64: a wave is one s-line) links are scalar loads and the
+// wall couplings s = 0 <-> Ls-1 are lane reads of the centre.
 // Hardware block b runs on XCD b % 8, and each XCD has its own L2, so the
 // tiling decides which neighbours are L2 hits:
 //   linear  blocks walk the s-x-y slab of a t-chunk in order;
@@ -246,7 +263,7 @@ __device__ inline void hop(int mu, cx<C> o[4], cx<C> uf, const cx<C> rf[4], cx<C
 //           are fetched across XCDs.
 // Partial sums go to partial[rhs * pstride + blockIdx.x]; block 0 zero-fills
 // [gridDim.x, pstride) so consumers can always sum pstride entries.
-template <int PREC, bool DAG, bool SHUF, int MODE, bool BUF, bool PF>
+template <int PREC, bool DAG, bool SHUF, int MODE, bool UL>
 __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, const Tiling TL) {
   using C = typename Store<PREC>::C;
   using T = typename Store<PREC>::T;
@@ -297,8 +314,8 @@ __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, const Tili
     const unsigned jyf = s + ls * (x + lx * yf), jyb = s + ls * (x + lx * yb);
     const unsigned lxb = xb + lx * y, lyb = x + lx * yb;
     const long roff = (long)rhs * 4 * v5;
-    using In = std::conditional_t<BUF, BAcc<PREC>, GAcc<PREC>>;
-    using Lk = std::conditional_t<BUF, BLinks<C>, GLinks<C>>;
+    using In = BAcc<PREC>;
+    using Lk = std::conditional_t<UL, SLinks<C>, BLinks<C>>;
     const In in((const T *)L.in + roff, L.in_norm ? L.in_norm + (long)rhs * v5 : nullptr, v5);
     const Lk lk(L.links, 3 * v4);
     const C diag = L.wilson4d ? (C)(3.0f + (-L.am3)) : (C)((3.0f - L.am3) + 1.0f);
@@ -307,50 +324,31 @@ __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, const Tili
     const C m = (C)L.am[rhs];
     const bool shops = !L.wilson4d;
 
-    cx<C> prv[4], cur[4], nxt[4], nn[4];
+    cx<C> prv[4], cur[4], nxt[4];
     const unsigned tb0 = (t0 == 0) ? lt - 1 : t0 - 1;
     in.load4(j + v3 * tb0, prv);
     cx<C> ubt = conj(lk(2 * v4 + xy + v2 * tb0));
     in.load4(j + v3 * t0, cur);
-    if constexpr (PF) in.load4(j + v3 * ((t0 + 1 == lt) ? 0 : t0 + 1), nxt);
     for (unsigned t = t0; t < t1; t++) {
       const unsigned tf = (t + 1 == lt) ? 0 : t + 1;
-      if constexpr (!PF) in.load4(j + v3 * tf, nxt);
+      in.load4(j + v3 * tf, nxt);
       const cx<C> uft = lk(2 * v4 + xy + v2 * t);
       const unsigned i = j + v3 * t;
       const unsigned l4 = v2 * t;
       cx<C> o[4];
-      if constexpr (PF) {
-        cx<C> rxf[4], rxb[4], ryf[4], ryb[4];
-        in.load4(jxf + v3 * t, rxf);
-        in.load4(jxb + v3 * t, rxb);
-        in.load4(jyf + v3 * t, ryf);
-        in.load4(jyb + v3 * t, ryb);
-        // R(t+2) for the step after next, issued after this step's neighbour
-        // loads so that waiting for those does not wait for it: its HBM
-        // latency overlaps this step's arithmetic and stores
-        __builtin_amdgcn_sched_barrier(0);
-        if (t + 1 < t1) in.load4(j + v3 * ((tf + 1 == lt) ? 0 : tf + 1), nn);
-        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int a = 0; a < 4; a++) o[a] = diag * cur[a];
-        hop<C, DAG>(0, o, lk(xy + l4), rxf, conj(lk(lxb + l4)), rxb);
-        hop<C, DAG>(1, o, lk(v4 + xy + l4), ryf, conj(lk(v4 + lyb + l4)), ryb);
-      } else {
-#pragma unroll
-        for (int a = 0; a < 4; a++) o[a] = diag * cur[a];
-        {
-          cx<C> rf[4], rb[4];
-          in.load4(jxf + v3 * t, rf);
-          in.load4(jxb + v3 * t, rb);
-          hop<C, DAG>(0, o, lk(xy + l4), rf, conj(lk(lxb + l4)), rb);
-        }
-        {
-          cx<C> rf[4], rb[4];
-          in.load4(jyf + v3 * t, rf);
-          in.load4(jyb + v3 * t, rb);
-          hop<C, DAG>(1, o, lk(v4 + xy + l4), rf, conj(lk(v4 + lyb + l4)), rb);
-        }
+      for (int a = 0; a < 4; a++) o[a] = diag * cur[a];
+      {
+        cx<C> rf[4], rb[4];
+        in.load4(jxf + v3 * t, rf);
+        in.load4(jxb + v3 * t, rb);
+        hop<C, DAG>(0, o, lk(xy + l4), rf, conj(lk(lxb + l4)), rb);
+      }
+      {
+        cx<C> rf[4], rb[4];
+        in.load4(jyf + v3 * t, rf);
+        in.load4(jyb + v3 * t, rb);
+        hop<C, DAG>(1, o, lk(v4 + xy + l4), rf, conj(lk(v4 + lyb + l4)), rb);
       }
       hop<C, DAG>(2, o, uft, nxt, ubt, prv);
 
@@ -381,17 +379,26 @@ __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, const Tili
       const unsigned col = i - s;  // s = 0 of this (x,y,t)
       if (imass == 1 || imass == 3) {
         // top: s = Ls couples to s = 1 on the up0 spins; bottom: s = 1 couples to s = Ls
+        cx<C> w0[2], wL[2];
+        if constexpr (UL) {
+          // the wave is the s-line: s = 0 is lane 0, s = Ls-1 is lane 63
+#pragma unroll
+          for (int k = 0; k < 2; k++) {
+            w0[k] = lane_value(cur[up0 + k], 0);
+            wL[k] = lane_value(cur[dn0 + k], 63);
+          }
+        }
         if (s == ls - 1) {
 #pragma unroll
           for (int k = 0; k < 2; k++) {
-            const cx<C> r0 = in.load1(col, up0 + k);
+            const cx<C> r0 = UL ? w0[k] : in.load1(col, up0 + k);
             o[up0 + k] = o[up0 + k] + (imass == 1 ? m * r0 : m * muli(r0));
           }
         }
         if (s == 0) {
 #pragma unroll
           for (int k = 0; k < 2; k++) {
-            const cx<C> rL = in.load1(col + ls - 1, dn0 + k);
+            const cx<C> rL = UL ? wL[k] : in.load1(col + ls - 1, dn0 + k);
             o[dn0 + k] = o[dn0 + k] + (imass == 1 ? m * rL : (-m) * muli(rL));
           }
         }
@@ -468,7 +475,6 @@ __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, const Tili
       for (int a = 0; a < 4; a++) {
         prv[a] = cur[a];
         cur[a] = nxt[a];
-        if constexpr (PF) nxt[a] = nn[a];
       }
       ubt = conj(uft);
     }
@@ -517,10 +523,10 @@ static bool box_split(const Dims &d, unsigned &px, unsigned &py, unsigned &pt) {
 
 // ---------------------------------------------------------------------------
 // launch configuration per (dims, precision): t-chunk length, XCD remap or
-// XCD boxes, buffer or global loads, lane-shift s hops.  The default is fixed (results
+// XCD boxes, uniform (scalar) links, lane-shift s hops.  The default is fixed (results
 // identical run to run); dslash_autotune times the candidates on scratch
 // fields and pins the fastest for the process.  THR_DSLASH_TCHUNK / _REMAP /
-// _BUFLOAD / _LANESHIFT / _BOXES / _PREFETCH override the default.
+// _UNIFORM / _LANESHIFT / _BOXES override the default.
 // ---------------------------------------------------------------------------
 namespace {
 struct TuneKey {
@@ -536,13 +542,11 @@ DslashConfig default_config(const Dims &d) {
   DslashConfig c;
   c.tchunk = d.lt >= 4 ? 4 : d.lt;
   c.remap = 1;
-  c.bufload = 1;
   if (const char *e = std::getenv("THR_DSLASH_TCHUNK")) c.tchunk = std::max(1, std::atoi(e));
   if (const char *e = std::getenv("THR_DSLASH_REMAP")) c.remap = std::atoi(e) ? 1 : 0;
-  if (const char *e = std::getenv("THR_DSLASH_BUFLOAD")) c.bufload = std::atoi(e) ? 1 : 0;
+  if (const char *e = std::getenv("THR_DSLASH_UNIFORM")) c.uniform_links = std::atoi(e) ? 1 : 0;
   if (const char *e = std::getenv("THR_DSLASH_LANESHIFT")) c.laneshift = std::atoi(e) ? 1 : 0;
   if (const char *e = std::getenv("THR_DSLASH_BOXES")) c.box = std::atoi(e) ? 1 : 0;
-  if (const char *e = std::getenv("THR_DSLASH_PREFETCH")) c.prefetch = std::atoi(e) ? 1 : 0;
   return c;
 }
 }  // namespace
@@ -595,20 +599,17 @@ int launch_dslash_cfg(const DslashLaunch &L, DslashConfig c, hipStream_t s) {
   dim3 grid(nblocks, (unsigned)L.nrhs);
   dim3 block(BLK);
   const bool shuf = c.laneshift && (64 % L.d.ls) == 0;
-  const bool buf = c.bufload != 0;
-  const bool pf = c.prefetch != 0;
-  THR_CHECK(!buf || 4 * L.d.v5() * Store<P64>::bytes < (1L << 32), THR_ERR_SIZE,
+  // a wave is exactly one s-line when Ls = 64 (blocks are whole waves and
+  // every tiling starts a wave at s = 0)
+  const bool ul = c.uniform_links && shuf && L.d.ls == 64;
+  THR_CHECK(4 * L.d.v5() * Store<P64>::bytes < (1L << 32), THR_ERR_SIZE,
             "spinor too large for 32-bit buffer offsets");
-#define THR_LAUNCH(P, D, S, M)                                                        \
-  do {                                                                                \
-    if (buf && pf)                                                                    \
-      hipLaunchKernelGGL((k_dslash<P, D, S, M, true, true>), grid, block, 0, s, L, TL);   \
-    else if (buf)                                                                     \
-      hipLaunchKernelGGL((k_dslash<P, D, S, M, true, false>), grid, block, 0, s, L, TL);  \
-    else if (pf)                                                                      \
-      hipLaunchKernelGGL((k_dslash<P, D, S, M, false, true>), grid, block, 0, s, L, TL);  \
-    else                                                                              \
-      hipLaunchKernelGGL((k_dslash<P, D, S, M, false, false>), grid, block, 0, s, L, TL); \
+#define THR_LAUNCH(P, D, S, M)                                                     \
+  do {                                                                             \
+    if (ul)                                                                        \
+      hipLaunchKernelGGL((k_dslash<P, D, true, M, true>), grid, block, 0, s, L, TL); \
+    else                                                                           \
+      hipLaunchKernelGGL((k_dslash<P, D, S, M, false>), grid, block, 0, s, L, TL);   \
   } while (0)
 #define THR_LAUNCH1(P, D, S)                        \
   switch (L.mode) {                                 \
@@ -688,10 +689,16 @@ int dslash_autotune(const Dims &d, int prec, const void *links, DslashConfig *be
   const int tcs[] = {1, 2, 4, 8, 16};
   for (int tc : tcs) {
     if (tc > d.lt) continue;
-    for (int v = 0; v < 32 && !rc; v++) {
-      if ((v & 4) && (64 % d.ls) != 0) continue;  // lane shifts need Ls | 64
-      if ((v & 8) && (v & 1)) continue;           // boxes replace the remap
-      DslashConfig c{tc, v & 1, (v >> 1) & 1, (v >> 2) & 1, (v >> 3) & 1, (v >> 4) & 1};
+    for (int v = 0; v < 16 && !rc; v++) {
+      if ((v & 4) && (64 % d.ls) != 0) continue;   // lane shifts need Ls | 64
+      if ((v & 2) && (!(v & 4) || d.ls != 64)) continue;  // uniform links need Ls = 64
+      if ((v & 8) && (v & 1)) continue;            // boxes replace the remap
+      DslashConfig c;
+      c.tchunk = tc;
+      c.remap = v & 1;
+      c.uniform_links = (v >> 1) & 1;
+      c.laneshift = (v >> 2) & 1;
+      c.box = (v >> 3) & 1;
       for (int k = 0; k < 2 && !rc; k++) rc = launch_dslash_cfg(L, c, s);  // warm
       if (rc) break;
       (void)hipEventRecord(e0, s);

@@ -51,10 +51,9 @@ struct DslashLaunch {
 struct DslashConfig {
   int tchunk = 4;
   int remap = 1;
-  int bufload = 1;    // buffer-resource loads (32-bit offsets) instead of global loads
-  int laneshift = 1;  // s hops as DPP lane shifts when Ls divides 64
-  int box = 0;        // XCD boxes: each XCD owns one box of an (x, y, t) split
-  int prefetch = 0;   // load R(t+2) one step ahead (software pipelining)
+  int uniform_links = 1;  // Ls = 64: scalar link loads, lane-read wall couplings
+  int laneshift = 1;      // s hops as DPP lane shifts when Ls divides 64
+  int box = 0;            // XCD boxes: each XCD owns one box of an (x, y, t) split
 };
 
 // block -> lattice mapping of one Dslash launch (see dslash.hip)
