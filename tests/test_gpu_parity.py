@@ -240,12 +240,14 @@ def test_device_precisions(G):
     got = (outh.to_host().astype(np.float32) * on.to_host()[None, :, None]).reshape(-1, 2)
     check(got.astype(np.float64), rc, 4e-3)
 
-@pytest.mark.parametrize("dims", [(4, 6, 8, 8), (6, 4, 6, 12), (4, 4, 3, 64), (4, 2, 8, 16)])
+@pytest.mark.parametrize("dims", [(4, 6, 8, 8), (6, 4, 6, 12), (4, 4, 3, 64), (4, 2, 8, 16),
+                                  (8, 4, 4, 64)])
 def test_dslash_launch_shapes(G, dims):
     """Every launch shape (t-chunk marched per thread, XCD remap or XCD boxes,
-    scalar links and lane-read walls when Ls = 64; s hops by lane shifts when
-    Ls divides 64, loads otherwise) gives the same operator to the last bits
-    (fused multiply-add placement may differ), equal to the oracle's to 1e-13."""
+    scalar links, lane-read walls and LDS x exchange when Ls = 64; s hops by
+    lane shifts when Ls divides 64, loads otherwise) gives the same operator
+    to the last bits (fused multiply-add placement may differ), equal to the
+    oracle's to 1e-13."""
     from hiprt import DeviceArray
     lx, ly, lt, ls = dims
     v5 = lx * ly * lt * ls
@@ -269,7 +271,7 @@ def test_dslash_launch_shapes(G, dims):
                     x = DeviceArray.from_host(xh.astype(dt))
                     first = None
                     for tc in sorted({1, 2, 3, 4, lt}):
-                        for flags in range(16):
+                        for flags in range(32):
                             G.dslash_set_config(dims, prec, min(tc, lt), flags)
                             out = DeviceArray((4 * v5, 2), dt)
                             G.dslash_dev(out, x, links, dims, 0.05, imass=imass, dagger=dagger,

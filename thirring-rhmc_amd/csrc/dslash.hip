@@ -21,6 +21,7 @@
 #include <mutex>
 #include <tuple>
 #include <type_traits>
+#include <vector>
 
 #include "kernels.h"
 
@@ -252,7 +253,8 @@ __device__ inline void hop(int mu, cx<C> o[4], cx<C> uf, const cx<C> rf[4], cx<C
 // L1/L2 hits (neighbouring lanes and waves of the same and nearby blocks).
 // With SHUF (Ls divides 64) the s hops are DPP lane shifts of the centre.
 // With UL (Ls = 64: a wave is one s-line) links are scalar loads and the
-// wall couplings s = 0 <-> Ls-1 are lane reads of the centre.
+// wall couplings s = 0 <-> Ls-1 are lane reads of the centre; with XL the
+// inner x neighbours come from the other waves of the block through LDS.
 // Hardware block b runs on XCD b % 8, and each XCD has its own L2, so the
 // tiling decides which neighbours are L2 hits:
 //   linear  blocks walk the s-x-y slab of a t-chunk in order;
@@ -263,7 +265,7 @@ __device__ inline void hop(int mu, cx<C> o[4], cx<C> uf, const cx<C> rf[4], cx<C
 //           are fetched across XCDs.
 // Partial sums go to partial[rhs * pstride + blockIdx.x]; block 0 zero-fills
 // [gridDim.x, pstride) so consumers can always sum pstride entries.
-template <int PREC, bool DAG, bool SHUF, int MODE, bool UL>
+template <int PREC, bool DAG, bool SHUF, int MODE, bool UL, bool XL>
 __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, const Tiling TL) {
   using C = typename Store<PREC>::C;
   using T = typename Store<PREC>::T;
@@ -338,7 +340,34 @@ __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, const Tili
       cx<C> o[4];
 #pragma unroll
       for (int a = 0; a < 4; a++) o[a] = diag * cur[a];
-      {
+      if constexpr (XL) {
+        // the block's 4 waves are 4 consecutive x at one (y, t): the x
+        // neighbours of the inner waves are the centre registers of the
+        // next/previous wave, exchanged through LDS (double-buffered by step
+        // parity, one barrier per step); the outer waves load theirs
+        __shared__ C xbuf[2][4][4][64][2];
+        const unsigned w = threadIdx.x >> 6, ln = threadIdx.x & 63, pb = (t - t0) & 1;
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+          xbuf[pb][w][a][ln][0] = cur[a].r;
+          xbuf[pb][w][a][ln][1] = cur[a].i;
+        }
+        __syncthreads();
+        cx<C> rf[4], rb[4];
+        if (w < 3) {
+#pragma unroll
+          for (int a = 0; a < 4; a++) rf[a] = {xbuf[pb][w + 1][a][ln][0], xbuf[pb][w + 1][a][ln][1]};
+        } else {
+          in.load4(jxf + v3 * t, rf);
+        }
+        if (w > 0) {
+#pragma unroll
+          for (int a = 0; a < 4; a++) rb[a] = {xbuf[pb][w - 1][a][ln][0], xbuf[pb][w - 1][a][ln][1]};
+        } else {
+          in.load4(jxb + v3 * t, rb);
+        }
+        hop<C, DAG>(0, o, lk(xy + l4), rf, conj(lk(lxb + l4)), rb);
+      } else {
         cx<C> rf[4], rb[4];
         in.load4(jxf + v3 * t, rf);
         in.load4(jxb + v3 * t, rb);
@@ -526,7 +555,7 @@ static bool box_split(const Dims &d, unsigned &px, unsigned &py, unsigned &pt) {
 // XCD boxes, uniform (scalar) links, lane-shift s hops.  The default is fixed (results
 // identical run to run); dslash_autotune times the candidates on scratch
 // fields and pins the fastest for the process.  THR_DSLASH_TCHUNK / _REMAP /
-// _UNIFORM / _LANESHIFT / _BOXES override the default.
+// _UNIFORM / _LANESHIFT / _BOXES / _LDSX override the default.
 // ---------------------------------------------------------------------------
 namespace {
 struct TuneKey {
@@ -540,13 +569,13 @@ std::map<TuneKey, DslashConfig> g_tuned;
 
 DslashConfig default_config(const Dims &d) {
   DslashConfig c;
-  c.tchunk = d.lt >= 4 ? 4 : d.lt;
-  c.remap = 1;
+  c.tchunk = std::min(8, d.lt);
   if (const char *e = std::getenv("THR_DSLASH_TCHUNK")) c.tchunk = std::max(1, std::atoi(e));
   if (const char *e = std::getenv("THR_DSLASH_REMAP")) c.remap = std::atoi(e) ? 1 : 0;
   if (const char *e = std::getenv("THR_DSLASH_UNIFORM")) c.uniform_links = std::atoi(e) ? 1 : 0;
   if (const char *e = std::getenv("THR_DSLASH_LANESHIFT")) c.laneshift = std::atoi(e) ? 1 : 0;
   if (const char *e = std::getenv("THR_DSLASH_BOXES")) c.box = std::atoi(e) ? 1 : 0;
+  if (const char *e = std::getenv("THR_DSLASH_LDSX")) c.lds_x = std::atoi(e) ? 1 : 0;
   return c;
 }
 }  // namespace
@@ -602,14 +631,20 @@ int launch_dslash_cfg(const DslashLaunch &L, DslashConfig c, hipStream_t s) {
   // a wave is exactly one s-line when Ls = 64 (blocks are whole waves and
   // every tiling starts a wave at s = 0)
   const bool ul = c.uniform_links && shuf && L.d.ls == 64;
+  // LDS x exchange: every block must be 4 full waves at 4 consecutive x of
+  // one (y, t) (all waves then take the same barriers)
+  const bool xl = ul && c.lds_x && L.d.lx % 4 == 0 &&
+                  (TL.mode != TILE_BOXES || TL.bx % 4 == 0);
   THR_CHECK(4 * L.d.v5() * Store<P64>::bytes < (1L << 32), THR_ERR_SIZE,
             "spinor too large for 32-bit buffer offsets");
-#define THR_LAUNCH(P, D, S, M)                                                     \
-  do {                                                                             \
-    if (ul)                                                                        \
-      hipLaunchKernelGGL((k_dslash<P, D, true, M, true>), grid, block, 0, s, L, TL); \
-    else                                                                           \
-      hipLaunchKernelGGL((k_dslash<P, D, S, M, false>), grid, block, 0, s, L, TL);   \
+#define THR_LAUNCH(P, D, S, M)                                                            \
+  do {                                                                                    \
+    if (xl)                                                                               \
+      hipLaunchKernelGGL((k_dslash<P, D, true, M, true, true>), grid, block, 0, s, L, TL);  \
+    else if (ul)                                                                          \
+      hipLaunchKernelGGL((k_dslash<P, D, true, M, true, false>), grid, block, 0, s, L, TL); \
+    else                                                                                  \
+      hipLaunchKernelGGL((k_dslash<P, D, S, M, false, false>), grid, block, 0, s, L, TL);   \
   } while (0)
 #define THR_LAUNCH1(P, D, S)                        \
   switch (L.mode) {                                 \
@@ -647,8 +682,12 @@ int dslash_autotune(const Dims &d, int prec, const void *links, DslashConfig *be
   THR_CHECK(cs == hipStreamCaptureStatusNone, THR_ERR_STATE, "cannot autotune while capturing");
   const size_t es = prec == P64 ? 16 : prec == P32 ? 8 : 4;
   const size_t nsp = (size_t)4 * d.v5();
-  void *in = nullptr, *out = nullptr;
-  float *inn = nullptr, *outn = nullptr;
+  // rotate over enough input/output pairs to exceed the 256 MB Infinity
+  // Cache, so every candidate is timed on cold fields as in a solver
+  const size_t pair = 2 * (es * nsp + sizeof(float) * d.v5());
+  const int nbuf = (int)std::min<size_t>(8, std::max<size_t>(2, (768u << 20) / pair + 1));
+  std::vector<void *> in(nbuf, nullptr), out(nbuf, nullptr);
+  std::vector<float *> inn(nbuf, nullptr), outn(nbuf, nullptr);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = THR_OK;
   DslashConfig pick = default_config(d);
@@ -656,21 +695,27 @@ int dslash_autotune(const Dims &d, int prec, const void *links, DslashConfig *be
   auto cleanup = [&]() {
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
-    (void)hipFree(in);
-    (void)hipFree(out);
-    (void)hipFree(inn);
-    (void)hipFree(outn);
+    for (int k = 0; k < nbuf; k++) {
+      (void)hipFree(in[k]);
+      (void)hipFree(out[k]);
+      (void)hipFree(inn[k]);
+      (void)hipFree(outn[k]);
+    }
   };
-  if (hipMalloc(&in, es * nsp) != hipSuccess || hipMalloc(&out, es * nsp) != hipSuccess ||
-      hipMalloc((void **)&inn, sizeof(float) * d.v5()) != hipSuccess ||
-      hipMalloc((void **)&outn, sizeof(float) * d.v5()) != hipSuccess ||
-      hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+  bool ok = hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
+  for (int k = 0; k < nbuf && ok; k++)
+    ok = hipMalloc(&in[k], es * nsp) == hipSuccess && hipMalloc(&out[k], es * nsp) == hipSuccess &&
+         hipMalloc((void **)&inn[k], sizeof(float) * d.v5()) == hipSuccess &&
+         hipMalloc((void **)&outn[k], sizeof(float) * d.v5()) == hipSuccess;
+  if (!ok) {
     cleanup();
     set_error("dslash_autotune: allocation failed");
     return THR_ERR_HIP;
   }
-  (void)hipMemsetAsync(in, 0, es * nsp, s);
-  (void)hipMemsetAsync(inn, 0, sizeof(float) * d.v5(), s);
+  for (int k = 0; k < nbuf; k++) {
+    (void)hipMemsetAsync(in[k], 0, es * nsp, s);
+    (void)hipMemsetAsync(inn[k], 0, sizeof(float) * d.v5(), s);
+  }
   DslashLaunch L{};
   L.d = d;
   L.nrhs = 1;
@@ -678,20 +723,27 @@ int dslash_autotune(const Dims &d, int prec, const void *links, DslashConfig *be
   L.imass = 3;
   L.am3 = 1.0f;
   L.am[0] = 0.01f;
-  L.in = in;
-  L.in_norm = inn;
-  L.out = out;
-  L.out_norm = outn;
   L.links = links;
   L.mode = EPI_STORE;
   L.sc_stride = 1;
   L.done_stride = 1;
+  int turn = 0;
+  auto launch = [&](const DslashConfig &c) {
+    const int k = turn++ % nbuf;
+    L.in = in[k];
+    L.in_norm = inn[k];
+    L.out = out[k];
+    L.out_norm = outn[k];
+    L.dagger = turn & 1;
+    return launch_dslash_cfg(L, c, s);
+  };
   const int tcs[] = {1, 2, 4, 8, 16};
   for (int tc : tcs) {
     if (tc > d.lt) continue;
-    for (int v = 0; v < 16 && !rc; v++) {
+    for (int v = 0; v < 32 && !rc; v++) {
       if ((v & 4) && (64 % d.ls) != 0) continue;   // lane shifts need Ls | 64
       if ((v & 2) && (!(v & 4) || d.ls != 64)) continue;  // uniform links need Ls = 64
+      if ((v & 16) && !(v & 2)) continue;          // the LDS x exchange needs them
       if ((v & 8) && (v & 1)) continue;            // boxes replace the remap
       DslashConfig c;
       c.tchunk = tc;
@@ -699,13 +751,11 @@ int dslash_autotune(const Dims &d, int prec, const void *links, DslashConfig *be
       c.uniform_links = (v >> 1) & 1;
       c.laneshift = (v >> 2) & 1;
       c.box = (v >> 3) & 1;
-      for (int k = 0; k < 2 && !rc; k++) rc = launch_dslash_cfg(L, c, s);  // warm
+      c.lds_x = (v >> 4) & 1;
+      for (int k = 0; k < 2 && !rc; k++) rc = launch(c);  // warm
       if (rc) break;
       (void)hipEventRecord(e0, s);
-      for (int k = 0; k < 6 && !rc; k++) {
-        L.dagger = k & 1;
-        rc = launch_dslash_cfg(L, c, s);
-      }
+      for (int k = 0; k < 2 * nbuf && !rc; k++) rc = launch(c);
       (void)hipEventRecord(e1, s);
       if (hipEventSynchronize(e1) != hipSuccess) rc = THR_ERR_HIP;
       float ms = 0.f;

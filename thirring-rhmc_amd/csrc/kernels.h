@@ -48,12 +48,15 @@ struct DslashLaunch {
 
 // launch shape of the Dslash kernel: t-chunk marched by each thread and
 // whether blocks are remapped so each XCD owns a contiguous lattice range
+// (the defaults are the fastest shape measured at 32^3 x 64; options whose
+// preconditions a lattice does not meet fall back by themselves)
 struct DslashConfig {
-  int tchunk = 4;
+  int tchunk = 8;
   int remap = 1;
   int uniform_links = 1;  // Ls = 64: scalar link loads, lane-read wall couplings
   int laneshift = 1;      // s hops as DPP lane shifts when Ls divides 64
-  int box = 0;            // XCD boxes: each XCD owns one box of an (x, y, t) split
+  int box = 1;            // XCD boxes: each XCD owns one box of an (x, y, t) split
+  int lds_x = 1;          // with uniform links: inner x neighbours through LDS
 };
 
 // block -> lattice mapping of one Dslash launch (see dslash.hip)
