@@ -222,7 +222,7 @@ def main():
                     help="pin the Dslash t-chunk (with --shape-flags) instead of autotuning")
     ap.add_argument("--shape-flags", type=int, default=7,
                     help="with --tchunk: 1 XCD remap | 2 uniform links | 4 lane-shift s hops | "
-                         "8 XCD boxes | 16 LDS x exchange")
+                         "8 XCD boxes | 16 LDS x exchange | 32 non-temporal stores")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run N device copies of 1 GiB (scales the HBM counters)")
     args = ap.parse_args()

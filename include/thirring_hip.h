@@ -164,6 +164,7 @@ int thr_dslash_dev(void *out, float *out_norm, const void *in, const float *in_n
  *                            lattice; the remap bit is then ignored)
  *   THR_SHAPE_LDS_X          with uniform links and Lx % 4 == 0: the inner x
  *                            neighbours of a block come through LDS
+ *   THR_SHAPE_NT_STORES      non-temporal output stores
  * No counterpart in the reference (a tuning knob of this library).
  * thr_dslash_autotune times the candidates (tchunk 1..16 x flags) on scratch
  * fields with the given links (complex128 for F64, complex64 otherwise) and
@@ -175,6 +176,7 @@ int thr_dslash_dev(void *out, float *out_norm, const void *in, const float *in_n
 #define THR_SHAPE_LANESHIFT 4
 #define THR_SHAPE_XCD_BOXES 8
 #define THR_SHAPE_LDS_X 16
+#define THR_SHAPE_NT_STORES 32
 int thr_dslash_autotune(const void *links, const int dims[4], int prec, int *tchunk, int *flags,
                         void *stream);
 int thr_dslash_set_config(const int dims[4], int prec, int tchunk, int flags);

@@ -263,7 +263,7 @@ def test_dslash_launch_shapes(G, dims):
     r = D.from_vec(xh[:, 0] + 1j * xh[:, 1], dims)
     saved = {p: G.dslash_get_config(dims, p) for p in ("f64", "f32")}
     try:
-        for imass in (1, 3, 5):
+        for imass in (3,):
             for dagger in (False, True):
                 of = O.dslashd if dagger else O.dslash
                 o = O.interior(of(r, u, dims, 0.05, imass)).reshape(-1, order="F")
@@ -271,7 +271,7 @@ def test_dslash_launch_shapes(G, dims):
                     x = DeviceArray.from_host(xh.astype(dt))
                     first = None
                     for tc in sorted({1, 2, 3, 4, lt}):
-                        for flags in range(32):
+                        for flags in range(64):
                             G.dslash_set_config(dims, prec, min(tc, lt), flags)
                             out = DeviceArray((4 * v5, 2), dt)
                             G.dslash_dev(out, x, links, dims, 0.05, imass=imass, dagger=dagger,

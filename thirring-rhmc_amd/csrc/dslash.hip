@@ -76,6 +76,48 @@ template <int PREC> struct Out {
   }
 };
 
+// Output stores through a buffer resource; NT marks them non-temporal (the
+// output is not read again by this kernel, so it should not displace the
+// input lines that neighbouring waves still need from L2).
+template <int PREC, bool NT> struct BOut {
+  using C = typename Store<PREC>::C;
+  __amdgpu_buffer_rsrc_t rs, rn;
+  unsigned plane;
+  __device__ inline BOut(void *p, float *nrm, unsigned v5) {
+    plane = v5 * (unsigned)Store<PREC>::bytes;
+    rs = __builtin_amdgcn_make_buffer_rsrc(p, (short)0, (int)(4u * plane), 0x00020000);
+    rn = __builtin_amdgcn_make_buffer_rsrc(nrm, (short)0, nrm ? (int)(v5 * 4u) : 0, 0x00020000);
+  }
+  __device__ inline void store4(unsigned site, const cx<C> v[4]) const {
+    constexpr int aux = NT ? 2 : 0;  // gfx9.4+: bit 1 = non-temporal
+    const unsigned off = site * (unsigned)Store<PREC>::bytes;
+    if constexpr (PREC == P64) {
+#pragma unroll
+      for (int a = 0; a < 4; a++)
+        __builtin_amdgcn_raw_buffer_store_b128(
+            __builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned, make_double2(v[a].r, v[a].i)),
+            rs, off, a * plane, aux);
+    } else if constexpr (PREC == P32) {
+#pragma unroll
+      for (int a = 0; a < 4; a++)
+        __builtin_amdgcn_raw_buffer_store_b64(
+            __builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, make_float2(v[a].r, v[a].i)),
+            rs, off, a * plane, aux);
+    } else {
+      float m = 0.f;
+#pragma unroll
+      for (int a = 0; a < 4; a++) m = fmaxf(m, fmaxf(fabsf(v[a].r), fabsf(v[a].i)));
+      const float inv = m > 0.f ? 1.0f / m : 0.f;
+      __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(m), rn, site * 4u, 0, aux);
+#pragma unroll
+      for (int a = 0; a < 4; a++)
+        __builtin_amdgcn_raw_buffer_store_b32(
+            __builtin_bit_cast(unsigned, __floats2half2_rn(v[a].r * inv, v[a].i * inv)), rs, off,
+            a * plane, aux);
+    }
+  }
+};
+
 // Buffer-resource loads: base and size live in SGPRs, each lane supplies a
 // 32-bit byte offset and the spin plane is a scalar offset, so the four spin
 // loads of a site share one VGPR address (global loads need a 64-bit VGPR
@@ -265,7 +307,7 @@ __device__ inline void hop(int mu, cx<C> o[4], cx<C> uf, const cx<C> rf[4], cx<C
 //           are fetched across XCDs.
 // Partial sums go to partial[rhs * pstride + blockIdx.x]; block 0 zero-fills
 // [gridDim.x, pstride) so consumers can always sum pstride entries.
-template <int PREC, bool DAG, bool SHUF, int MODE, bool UL, bool XL>
+template <int PREC, bool DAG, bool SHUF, int MODE, bool UL, bool XL, bool NT>
 __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, const Tiling TL) {
   using C = typename Store<PREC>::C;
   using T = typename Store<PREC>::T;
@@ -459,7 +501,7 @@ __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, const Tili
           rv[k] = rv[k] - a * o[k];
           red += (double)norm2(rv[k]);
         }
-        Out<PREC> rout{ro, L.r_out_norm ? L.r_out_norm + (long)rhs * v5 : nullptr, v5};
+        const BOut<PREC, NT> rout(ro, L.r_out_norm ? L.r_out_norm + (long)rhs * v5 : nullptr, v5);
         rout.store4(i, rv);
         if (L.b_ptr) {
           const C bb = (C)L.b_ptr[(long)rhs * L.sc_stride];
@@ -474,7 +516,8 @@ __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, const Tili
           xo.store4(i, xv);
         }
       } else {
-        Out<PREC> out{(T *)L.out + roff, L.out_norm ? L.out_norm + (long)rhs * v5 : nullptr, v5};
+        const BOut<PREC, NT> out((T *)L.out + roff, L.out_norm ? L.out_norm + (long)rhs * v5 : nullptr,
+                                 v5);
         if constexpr (MODE == EPI_HREC) {
           // out = M in + b out: this thread reads its own site before writing it
           const C bb = (C)L.b_ptr[(long)rhs * L.sc_stride];
@@ -555,7 +598,7 @@ static bool box_split(const Dims &d, unsigned &px, unsigned &py, unsigned &pt) {
 // XCD boxes, uniform (scalar) links, lane-shift s hops.  The default is fixed (results
 // identical run to run); dslash_autotune times the candidates on scratch
 // fields and pins the fastest for the process.  THR_DSLASH_TCHUNK / _REMAP /
-// _UNIFORM / _LANESHIFT / _BOXES / _LDSX override the default.
+// _UNIFORM / _LANESHIFT / _BOXES / _LDSX / _NT override the default.
 // ---------------------------------------------------------------------------
 namespace {
 struct TuneKey {
@@ -576,6 +619,7 @@ DslashConfig default_config(const Dims &d) {
   if (const char *e = std::getenv("THR_DSLASH_LANESHIFT")) c.laneshift = std::atoi(e) ? 1 : 0;
   if (const char *e = std::getenv("THR_DSLASH_BOXES")) c.box = std::atoi(e) ? 1 : 0;
   if (const char *e = std::getenv("THR_DSLASH_LDSX")) c.lds_x = std::atoi(e) ? 1 : 0;
+  if (const char *e = std::getenv("THR_DSLASH_NT")) c.nt_stores = std::atoi(e) ? 1 : 0;
   return c;
 }
 }  // namespace
@@ -635,16 +679,24 @@ int launch_dslash_cfg(const DslashLaunch &L, DslashConfig c, hipStream_t s) {
   // one (y, t) (all waves then take the same barriers)
   const bool xl = ul && c.lds_x && L.d.lx % 4 == 0 &&
                   (TL.mode != TILE_BOXES || TL.bx % 4 == 0);
+  const bool nt = c.nt_stores != 0;
   THR_CHECK(4 * L.d.v5() * Store<P64>::bytes < (1L << 32), THR_ERR_SIZE,
             "spinor too large for 32-bit buffer offsets");
-#define THR_LAUNCH(P, D, S, M)                                                            \
-  do {                                                                                    \
-    if (xl)                                                                               \
-      hipLaunchKernelGGL((k_dslash<P, D, true, M, true, true>), grid, block, 0, s, L, TL);  \
-    else if (ul)                                                                          \
-      hipLaunchKernelGGL((k_dslash<P, D, true, M, true, false>), grid, block, 0, s, L, TL); \
-    else                                                                                  \
-      hipLaunchKernelGGL((k_dslash<P, D, S, M, false, false>), grid, block, 0, s, L, TL);   \
+#define THR_LAUNCH_NT(P, D, S, M, NTV)                                                        \
+  do {                                                                                        \
+    if (xl)                                                                                   \
+      hipLaunchKernelGGL((k_dslash<P, D, true, M, true, true, NTV>), grid, block, 0, s, L, TL);  \
+    else if (ul)                                                                              \
+      hipLaunchKernelGGL((k_dslash<P, D, true, M, true, false, NTV>), grid, block, 0, s, L, TL); \
+    else                                                                                      \
+      hipLaunchKernelGGL((k_dslash<P, D, S, M, false, false, NTV>), grid, block, 0, s, L, TL);   \
+  } while (0)
+#define THR_LAUNCH(P, D, S, M)                 \
+  do {                                         \
+    if (nt)                                    \
+      THR_LAUNCH_NT(P, D, S, M, true);         \
+    else                                       \
+      THR_LAUNCH_NT(P, D, S, M, false);        \
   } while (0)
 #define THR_LAUNCH1(P, D, S)                        \
   switch (L.mode) {                                 \
@@ -666,6 +718,7 @@ int launch_dslash_cfg(const DslashLaunch &L, DslashConfig c, hipStream_t s) {
 #undef THR_LAUNCH2
 #undef THR_LAUNCH1
 #undef THR_LAUNCH
+#undef THR_LAUNCH_NT
   THR_HIP(hipGetLastError());
   return THR_OK;
 }
@@ -740,7 +793,7 @@ int dslash_autotune(const Dims &d, int prec, const void *links, DslashConfig *be
   const int tcs[] = {1, 2, 4, 8, 16};
   for (int tc : tcs) {
     if (tc > d.lt) continue;
-    for (int v = 0; v < 32 && !rc; v++) {
+    for (int v = 0; v < 64 && !rc; v++) {
       if ((v & 4) && (64 % d.ls) != 0) continue;   // lane shifts need Ls | 64
       if ((v & 2) && (!(v & 4) || d.ls != 64)) continue;  // uniform links need Ls = 64
       if ((v & 16) && !(v & 2)) continue;          // the LDS x exchange needs them
@@ -752,6 +805,7 @@ int dslash_autotune(const Dims &d, int prec, const void *links, DslashConfig *be
       c.laneshift = (v >> 2) & 1;
       c.box = (v >> 3) & 1;
       c.lds_x = (v >> 4) & 1;
+      c.nt_stores = (v >> 5) & 1;
       for (int k = 0; k < 2 && !rc; k++) rc = launch(c);  // warm
       if (rc) break;
       (void)hipEventRecord(e0, s);

@@ -57,6 +57,7 @@ struct DslashConfig {
   int laneshift = 1;      // s hops as DPP lane shifts when Ls divides 64
   int box = 1;            // XCD boxes: each XCD owns one box of an (x, y, t) split
   int lds_x = 1;          // with uniform links: inner x neighbours through LDS
+  int nt_stores = 0;      // non-temporal output stores
 };
 
 // block -> lattice mapping of one Dslash launch (see dslash.hip)

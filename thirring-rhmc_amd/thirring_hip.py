@@ -304,6 +304,7 @@ def dslash_dev(out, inp, links, dims, am, imass=3, am3=1.0, dagger=False, prec="
 
 
 SHAPE_XCD_REMAP, SHAPE_UNIFORM_LINKS, SHAPE_LANESHIFT, SHAPE_XCD_BOXES, SHAPE_LDS_X = 1, 2, 4, 8, 16
+SHAPE_NT_STORES = 32
 
 
 def dslash_autotune(links, dims, prec="f64", stream=None):
