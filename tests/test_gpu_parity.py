@@ -144,7 +144,7 @@ def test_multishift(G, sp):
         # a shift whose stopping test sits at the threshold may stop one or two
         # iterations apart (GPU and CPU sum the dot products in different
         # orders); its solution then differs at the solver tolerance
-        assert cg == co and np.abs(cgs - cos).max() <= 2, (cgs, cos)
+        assert abs(cg - co) <= 2 and np.abs(cgs - cos).max() <= 2, (cgs, cos)
         check(og, oo, 50 * res)
 def test_multishift_maxcg_cap(G):
     dims = (4, 4, 4, 4)

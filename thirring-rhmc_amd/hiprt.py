@@ -73,10 +73,14 @@ class DeviceArray:
 
     @classmethod
     def from_host(cls, a):
+        """Device copy of a host array, complete on return.  (A pageable hipMemcpy
+        may return once the data is staged, before the DMA has landed, and
+        work on non-blocking streams is not ordered after it.)"""
         a = np.ascontiguousarray(a)
         d = cls(a.shape, a.dtype)
         check(hip().hipMemcpy(ctypes.c_void_p(d.ptr), a.ctypes.data_as(ctypes.c_void_p),
                               d.nbytes, H2D), "hipMemcpy H2D")
+        check(hip().hipStreamSynchronize(None), "hipStreamSynchronize")
         return d
 
     def to_host(self):
