@@ -76,16 +76,23 @@ template <int PREC> struct Out {
   }
 };
 
-// Output stores through a buffer resource; NT marks them non-temporal (the
+// Output stores through buffer resources; NT marks them non-temporal (the
 // output is not read again by this kernel, so it should not displace the
 // input lines that neighbouring waves still need from L2).
+// One resource per spin plane, stored with soffset 0: a store of more than
+// 64 bits reads its data VGPRs after issue, and the compiler's wait-state
+// check for a following VALU write to those VGPRs skips stores whose soffset
+// is an SGPR (the data can then be overwritten before it is read: measured as
+// corrupted imaginary parts in FP64 output when other kernels share the CU).
 template <int PREC, bool NT> struct BOut {
   using C = typename Store<PREC>::C;
-  __amdgpu_buffer_rsrc_t rs, rn;
-  unsigned plane;
+  __amdgpu_buffer_rsrc_t rs[4], rn;
   __device__ inline BOut(void *p, float *nrm, unsigned v5) {
-    plane = v5 * (unsigned)Store<PREC>::bytes;
-    rs = __builtin_amdgcn_make_buffer_rsrc(p, (short)0, (int)(4u * plane), 0x00020000);
+    const unsigned plane = v5 * (unsigned)Store<PREC>::bytes;
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+      rs[a] = __builtin_amdgcn_make_buffer_rsrc((char *)p + (size_t)a * plane, (short)0, (int)plane,
+                                                0x00020000);
     rn = __builtin_amdgcn_make_buffer_rsrc(nrm, (short)0, nrm ? (int)(v5 * 4u) : 0, 0x00020000);
   }
   __device__ inline void store4(unsigned site, const cx<C> v[4]) const {
@@ -96,13 +103,13 @@ template <int PREC, bool NT> struct BOut {
       for (int a = 0; a < 4; a++)
         __builtin_amdgcn_raw_buffer_store_b128(
             __builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned, make_double2(v[a].r, v[a].i)),
-            rs, off, a * plane, aux);
+            rs[a], off, 0, aux);
     } else if constexpr (PREC == P32) {
 #pragma unroll
       for (int a = 0; a < 4; a++)
         __builtin_amdgcn_raw_buffer_store_b64(
             __builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, make_float2(v[a].r, v[a].i)),
-            rs, off, a * plane, aux);
+            rs[a], off, 0, aux);
     } else {
       float m = 0.f;
 #pragma unroll
@@ -112,8 +119,8 @@ template <int PREC, bool NT> struct BOut {
 #pragma unroll
       for (int a = 0; a < 4; a++)
         __builtin_amdgcn_raw_buffer_store_b32(
-            __builtin_bit_cast(unsigned, __floats2half2_rn(v[a].r * inv, v[a].i * inv)), rs, off,
-            a * plane, aux);
+            __builtin_bit_cast(unsigned, __floats2half2_rn(v[a].r * inv, v[a].i * inv)), rs[a], off,
+            0, aux);
     }
   }
 };
@@ -367,6 +374,14 @@ __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, const Tili
     const int imass = L.wilson4d ? 0 : L.imass;
     const C m = (C)L.am[rhs];
     const bool shops = !L.wilson4d;
+    // epilogue scalars (device-resident solver state)
+    C ea = C(0), eb = C(0);
+    if constexpr (MODE == EPI_CG) {
+      ea = (C)L.a_ptr[(long)rhs * L.sc_stride];
+      if (L.b_ptr) eb = (C)L.b_ptr[(long)rhs * L.sc_stride];
+    } else if constexpr (MODE == EPI_HREC) {
+      eb = (C)L.b_ptr[(long)rhs * L.sc_stride];
+    }
 
     cx<C> prv[4], cur[4], nxt[4];
     const unsigned tb0 = (t0 == 0) ? lt - 1 : t0 - 1;
@@ -490,7 +505,7 @@ __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, const Tili
 
       // epilogue
       if constexpr (MODE == EPI_CG) {
-        const C a = (C)L.a_ptr[(long)rhs * L.sc_stride];
+        const C a = ea;
         T *r = (T *)L.r + roff;
         T *ro = L.r_out ? (T *)L.r_out + roff : r;
         Acc<PREC> rr{r, L.r_norm ? L.r_norm + (long)rhs * v5 : nullptr, v5};
@@ -504,7 +519,7 @@ __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, const Tili
         const BOut<PREC, NT> rout(ro, L.r_out_norm ? L.r_out_norm + (long)rhs * v5 : nullptr, v5);
         rout.store4(i, rv);
         if (L.b_ptr) {
-          const C bb = (C)L.b_ptr[(long)rhs * L.sc_stride];
+          const C bb = eb;
           Acc<PREC> pp{(const T *)L.p + roff, nullptr, v5};
           Acc<PREC> xx{(const T *)L.x + roff, nullptr, v5};
           cx<C> pv[4], xv[4];
@@ -520,7 +535,7 @@ __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, const Tili
                                  v5);
         if constexpr (MODE == EPI_HREC) {
           // out = M in + b out: this thread reads its own site before writing it
-          const C bb = (C)L.b_ptr[(long)rhs * L.sc_stride];
+          const C bb = eb;
           Acc<PREC> ho{(const T *)L.out + roff, L.out_norm ? L.out_norm + (long)rhs * v5 : nullptr,
                        v5};
           cx<C> hv[4];

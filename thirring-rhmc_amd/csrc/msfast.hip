@@ -420,18 +420,22 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
   double *sums = (double *)workspace(368, sizeof(double) * ns_);
   THR_CHECK(Pv && st && aden && partial && ringptr && sums, THR_ERR_HIP, "workspace allocation failed");
   {
-    std::vector<void *> hp(3 * MSB + MSB);
+    const int np = 4 * MSB;
+    const size_t nad = (size_t)(MAXSHIFT + 1) * ns_;
+    void **hp = (void **)host_workspace(0, sizeof(void *) * np);
+    double *ad = (double *)host_workspace(1, sizeof(double) * nad);
+    THR_CHECK(hp && ad, THR_ERR_HIP, "host workspace allocation failed");
     for (int pr = 0; pr < 3; pr++)
       for (int q = 0; q < MSB; q++) hp[pr * MSB + q] = ring[pr][q];
     for (int q = 0; q < MSB; q++) hp[3 * MSB + q] = A.th16 > 0 ? (void *)ringn[q] : nullptr;
-    THR_HIP(hipMemcpyAsync(ringptr, hp.data(), sizeof(void *) * hp.size(), hipMemcpyHostToDevice, s));
-    std::vector<double> ad((MAXSHIFT + 1) * ns_, 0.0);
+    THR_HIP(hipMemcpyAsync(ringptr, hp, sizeof(void *) * np, hipMemcpyHostToDevice, s));
+    for (size_t q = 0; q < nad; q++) ad[q] = 0.0;
     for (int k = 0; k < ns_; k++) {
       for (int i = 0; i < n; i++)
         ad[k * MAXSHIFT + i] = A.aden_src ? A.aden_src[k * n + i] : A.aden[i];
       ad[MAXSHIFT * ns_ + k] = A.tol_src ? A.tol_src[k] : A.tol;
     }
-    THR_HIP(hipMemcpyAsync(aden, ad.data(), sizeof(double) * ad.size(), hipMemcpyHostToDevice, s));
+    THR_HIP(hipMemcpyAsync(aden, ad, sizeof(double) * nad, hipMemcpyHostToDevice, s));
     THR_HIP(hipStreamSynchronize(s));
   }
   // |b|^2 per source, state init
@@ -458,7 +462,8 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
   L.done_stride = (int)(sizeof(MsfState) / sizeof(int));
   L.partial = partial;
 
-  std::vector<MsfState> hs(ns_);
+  MsfState *hs = (MsfState *)host_workspace(2, sizeof(MsfState) * ns_);
+  THR_CHECK(hs, THR_ERR_HIP, "host workspace allocation failed");
   long t = 0;  // completed iterations
   int rc = THR_OK;
   int blocks = 0;
@@ -579,7 +584,7 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
     }
     t += MSB;
     blocks++;
-    if (hipMemcpyAsync(hs.data(), st, sizeof(MsfState) * ns_, hipMemcpyDeviceToHost, s) !=
+    if (hipMemcpyAsync(hs, st, sizeof(MsfState) * ns_, hipMemcpyDeviceToHost, s) !=
             hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess) {
       free_graphs();
@@ -641,10 +646,12 @@ int msfast_true_residuals(const MsfArgs &A, double *relres, hipStream_t s) {
   double *sums = (double *)workspace(368, sizeof(double) * ns_);
   double2 *xs = (double2 *)workspace(372, sizeof(double2) * nsp * ns_);
   THR_CHECK(t1 && t2 && partial && sums && xs, THR_ERR_HIP, "workspace allocation failed");
-  std::vector<double> bn(ns_), hsum(ns_);
+  double *bn = (double *)host_workspace(3, sizeof(double) * ns_);
+  double *hsum = (double *)host_workspace(4, sizeof(double) * ns_);
+  THR_CHECK(bn && hsum, THR_ERR_HIP, "host workspace allocation failed");
   hipLaunchKernelGGL(k_norm_src, dim3(nb, ns_), dim3(BLK), 0, s, v5, A.b, partial);
   hipLaunchKernelGGL(k_sum_rows, dim3(ns_), dim3(BLK), 0, s, partial, nb, sums);
-  THR_HIP(hipMemcpyAsync(bn.data(), sums, sizeof(double) * ns_, hipMemcpyDeviceToHost, s));
+  THR_HIP(hipMemcpyAsync(bn, sums, sizeof(double) * ns_, hipMemcpyDeviceToHost, s));
   THR_HIP(hipStreamSynchronize(s));
   DslashLaunch L{};
   L.d = d;
@@ -674,7 +681,7 @@ int msfast_true_residuals(const MsfArgs &A, double *relres, hipStream_t s) {
     hipLaunchKernelGGL(k_true_res, dim3(nb, ns_), dim3(BLK), 0, s, v5, A.b, t2, xs, A.aden[i],
                        nsp, partial);
     hipLaunchKernelGGL(k_sum_rows, dim3(ns_), dim3(BLK), 0, s, partial, nb, sums);
-    THR_HIP(hipMemcpyAsync(hsum.data(), sums, sizeof(double) * ns_, hipMemcpyDeviceToHost, s));
+    THR_HIP(hipMemcpyAsync(hsum, sums, sizeof(double) * ns_, hipMemcpyDeviceToHost, s));
     THR_HIP(hipStreamSynchronize(s));
     for (int k = 0; k < ns_; k++) relres[k * n + i] = std::sqrt(hsum[k] / bn[k]);
   }
@@ -718,7 +725,10 @@ int msfast_correct(const MsfArgs &A, int max_rounds, double tol_true, double *re
         THR_HIP(hipMemcpyAsync(bs + j * nsp, A.b + (long)k * nsp, sizeof(double2) * nsp,
                                hipMemcpyDeviceToDevice, s));
       }
-      THR_HIP(hipMemcpyAsync(sig, sg.data(), sizeof(double) * g, hipMemcpyHostToDevice, s));
+      double *sgp = (double *)host_workspace(5, sizeof(double) * g);
+      THR_CHECK(sgp, THR_ERR_HIP, "host workspace allocation failed");
+      std::copy(sg.begin(), sg.end(), sgp);
+      THR_HIP(hipMemcpyAsync(sig, sgp, sizeof(double) * g, hipMemcpyHostToDevice, s));
       DslashLaunch L{};
       L.d = d;
       L.nrhs = g;

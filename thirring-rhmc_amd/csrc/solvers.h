@@ -25,6 +25,10 @@ void context_destroy(Context *c);
 void context_bind(Context *c);
 size_t workspace_bytes();
 void *workspace(int slot, size_t bytes);
+// pinned host buffer of the bound context (state read-backs and uploads go
+// through these: pageable async copies are staged through runtime buffers
+// shared by all threads)
+void *host_workspace(int slot, size_t bytes);
 double *partial_buffer(int nrhs_blocks);
 hipStream_t default_stream();
 // whether solvers may capture their blocks as hipGraphs in the bound context

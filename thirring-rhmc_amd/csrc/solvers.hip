@@ -32,6 +32,7 @@ struct Buf {
 
 struct Context {
   std::map<int, Buf> ws;
+  std::map<int, Buf> hws;  // pinned host buffers for small state transfers
   std::mutex mu;
   hipStream_t stream = nullptr;
   // hipGraph capture of solver blocks; off in created contexts, which run
@@ -59,6 +60,8 @@ void context_destroy(Context *c) {
   (void)hipDeviceSynchronize();
   for (auto &kv : c->ws)
     if (kv.second.p) (void)hipFree(kv.second.p);
+  for (auto &kv : c->hws)
+    if (kv.second.p) (void)hipHostFree(kv.second.p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   if (t_ctx == c) t_ctx = nullptr;
   delete c;
@@ -76,6 +79,24 @@ void *workspace(int slot, size_t bytes) {
     b.n = 0;
     if (hipMalloc(&b.p, bytes) != hipSuccess) {
       set_error("hipMalloc of workspace failed");
+      b.p = nullptr;
+      return nullptr;
+    }
+    b.n = bytes;
+  }
+  return b.p;
+}
+
+void *host_workspace(int slot, size_t bytes) {
+  Context &c = cur();
+  std::lock_guard<std::mutex> lk(c.mu);
+  Buf &b = c.hws[slot];
+  if (b.n < bytes) {
+    if (b.p) (void)hipHostFree(b.p);
+    b.p = nullptr;
+    b.n = 0;
+    if (hipHostMalloc(&b.p, bytes, hipHostMallocDefault) != hipSuccess) {
+      set_error("hipHostMalloc of host workspace failed");
       b.p = nullptr;
       return nullptr;
     }
