@@ -176,6 +176,24 @@ def bench_serving(rate, seconds, slots, max_requests):
     return S.run_load(rate=rate, duration=seconds, seed=3, slots=slots, max_requests=max_requests)
 
 
+def copy_rate(hr, nbytes=1 << 30, reps=10):
+    """Device-to-device copy rate (bytes read + written per second) on this GPU:
+    the practical HBM ceiling the Dslash is compared with."""
+    a = hr.DeviceArray((nbytes // 8,), np.float64)
+    a.zero()
+    b = hr.DeviceArray((nbytes // 8,), np.float64)
+    b.copy_from(a)
+    e0, e1 = hr.Event(), hr.Event()
+    e0.record(None)
+    for _ in range(reps):
+        b.copy_from(a)
+    e1.record(None)
+    ms = e0.elapsed_ms(e1) / reps
+    a.free()
+    b.free()
+    return 2 * nbytes / (ms * 1e-3) / 1e9
+
+
 def cpu_baseline(dims, seconds=12.0):
     """Oracle (own FP64 C restatement, OpenMP) Dslash on the host cores."""
     import dense as D
@@ -270,6 +288,8 @@ def main():
         ms, _ = time_dslash(G, hr, dims, l32, prec, args.steps, args.warmup, nb, stream)
         extra[prec] = {"ms": round(ms, 5),
                        "GB_s": round(dslash_bytes(v4, v5, prec) / (ms * 1e-3) / 1e9, 1)}
+    progress("device copy rate")
+    copy_gbs = copy_rate(hr)
     solver = None
     if not args.no_solver:
         sd = tuple(int(v) for v in args.solver_lattice.split(","))
@@ -313,6 +333,8 @@ def main():
                    "region_ms_per_step": round(region64, 5)},
         "roofline": {"bound": "hbm", "achieved": round(gbs64, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(gbs64 / HBM_PEAK_GBS, 4), "traffic": traffic},
+        "hbm_copy_GB_s": round(copy_gbs, 1),
+        "frac_of_copy": round(gbs64 / copy_gbs, 4),
         "cpu_baseline": cpu,
         "dslash_other_precisions": extra,
         "multishift": solver,
