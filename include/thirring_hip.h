@@ -146,6 +146,13 @@ int thr_coef_dev(void *links64, const float *theta, const int dims[4], int iboun
 /* convert FP64 device links to the FP32 copy used by the FP32/FP16 operators */
 int thr_links_to_f32_dev(void *links32, const void *links64, const int dims[4], void *stream);
 
+/* streaming device-to-device copy of `bytes` (a multiple of 16; 16-byte
+ * aligned pointers), non-temporal.  No counterpart in the reference: it is
+ * the plain read+write stream that bench.py times as the practical HBM
+ * ceiling next to the Dslash, and a field copy that avoids the runtime's
+ * blit path. */
+int thr_device_copy(void *dst, const void *src, unsigned long long bytes, void *stream);
+
 /* out = M in (dagger = 0) or M^dagger in (dagger = 1) for nrhs right-hand
  * sides with per-rhs masses am[nrhs]; prec THR_F64/F32/F16 (links must be
  * complex128 for F64, complex64 otherwise; norms only for F16) */

@@ -294,3 +294,19 @@ def test_dslash_launch_shapes(G, dims):
     finally:
         for p, (tc, fl) in saved.items():
             G.dslash_set_config(dims, p, tc, fl)
+
+
+def test_device_copy(G):
+    """The streaming copy moves every byte (sizes around the grid stride) and
+    rejects sizes that are not a multiple of 16."""
+    from hiprt import DeviceArray
+    rng = np.random.default_rng(12)
+    for nbytes in (16, 4096 + 16, (1 << 20) * 3 + 48, 256 * 4 * 256 * 16 * 3 + 16):
+        h = rng.integers(0, 255, size=nbytes, dtype=np.uint8)
+        a = DeviceArray.from_host(h)
+        b = DeviceArray((nbytes,), np.uint8)
+        b.zero()
+        G.device_copy(b, a, nbytes)
+        assert np.array_equal(b.to_host(), h), nbytes
+    with pytest.raises(G.ThrError):
+        G.device_copy(b, a, 24)

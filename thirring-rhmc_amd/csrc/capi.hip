@@ -453,6 +453,11 @@ int thr_links_to_f32_dev(void *links32, const void *links64, const int dims[4], 
       stream, convert_d2f(d.v4() * 3, (const double2 *)links64, (float2 *)links32, pick(stream)));
 }
 
+int thr_device_copy(void *dst, const void *src, unsigned long long bytes, void *stream) {
+  THR_CHECK(dst && src, THR_ERR_ARG, "NULL argument");
+  return done_if_default(stream, stream_copy(src, dst, (size_t)bytes, pick(stream)));
+}
+
 int thr_dslash_dev(void *out, float *out_norm, const void *in, const float *in_norm,
                    const void *links, const int dims[4], int nrhs, const float *am, int imass,
                    float am3, int dagger, int prec, void *stream) {

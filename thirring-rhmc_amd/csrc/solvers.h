@@ -54,6 +54,8 @@ int multishift_reference_sp(const OpDesc &op, int n, const double *aden, const d
 
 // element-wise FP64 -> FP32 complex conversion
 int convert_d2f(long n, const double2 *a, float2 *b, hipStream_t s);
+// streaming device-to-device copy (bytes % 16 == 0, 16-byte aligned pointers)
+int stream_copy(const void *src, void *dst, size_t bytes, hipStream_t s);
 
 // apply M or M^+ (FP64) into out
 int apply_op(const OpDesc &op, int dagger, const double2 *in, double2 *out, hipStream_t s);

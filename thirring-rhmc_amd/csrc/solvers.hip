@@ -8,7 +8,9 @@
 // iterations.  Kernels of an iteration that starts after the stop flag was
 // raised exit immediately, so the iteration counts and final vectors are
 // exactly those of the reference's loop structure.
+#include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -426,7 +428,33 @@ __global__ __launch_bounds__(BLK) void k_f2d(long n, const float2 *a, double2 *b
 
 inline unsigned nblk(long n) { return (unsigned)((n + BLK - 1) / BLK); }
 
+// streaming copy, 16 bytes per lane, non-temporal loads and stores, a
+// grid-stride loop over a few blocks per CU: the fastest plain read+write
+// stream measured on gfx950 (the HBM ceiling the Dslash is compared with)
+typedef __attribute__((ext_vector_type(4))) unsigned U4;
+__global__ __launch_bounds__(BLK) void k_stream_copy(const U4 *__restrict__ a, U4 *__restrict__ b,
+                                                     long n) {
+  const long stride = (long)gridDim.x * BLK;
+  for (long i = (long)blockIdx.x * BLK + threadIdx.x; i < n; i += stride)
+    __builtin_nontemporal_store(__builtin_nontemporal_load(a + i), b + i);
+}
+
 }  // namespace
+
+int stream_copy(const void *src, void *dst, size_t bytes, hipStream_t s) {
+  THR_CHECK(bytes % 16 == 0, THR_ERR_ARG, "byte count must be a multiple of 16");
+  THR_CHECK(((uintptr_t)src | (uintptr_t)dst) % 16 == 0, THR_ERR_ARG, "pointers must be 16-byte aligned");
+  if (bytes == 0) return THR_OK;
+  int dev = 0, cus = 256;
+  THR_HIP(hipGetDevice(&dev));
+  THR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const long n = (long)(bytes / 16);
+  const long want = (long)cus * 4;
+  const unsigned grid = (unsigned)std::max(1L, std::min(want, (long)nblk(n)));
+  hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(BLK), 0, s, (const U4 *)src, (U4 *)dst, n);
+  THR_HIP(hipGetLastError());
+  return THR_OK;
+}
 
 int convert_d2f(long n, const double2 *a, float2 *b, hipStream_t s) {
   hipLaunchKernelGGL(k_d2f, dim3(nblk(n)), dim3(BLK), 0, s, n, a, b);

@@ -51,6 +51,7 @@ _SIGS = {
     "thr_dslash_set_config": ([P, I32, I32, I32], I32),
     "thr_dslash_get_config": ([P, I32, IP, IP], I32),
     "thr_links_to_f32_dev": ([P, P, P, P], I32),
+    "thr_device_copy": ([P, P, ctypes.c_ulonglong, P], I32),
     "thr_dslash_dev": ([P, P, P, P, P, P, I32, P, I32, F32, I32, I32, P], I32),
     "thr_congrad_dev": ([P, P, P, P, F32, F32, I32, F32, I32, I32, IP, IP, P], I32),
     "thr_multishift_dev": ([P, P, F32, I32, F32, I32, P, P, P, P, F32, I32, I32, I32, IP, P, P],
@@ -292,6 +293,12 @@ def links_to_f32_dev(links32, links64, dims, stream=None):
     init()
     _check(lib().thr_links_to_f32_dev(_p(links32), _p(links64), _dims(dims), _stream(stream)),
            "links_to_f32_dev")
+
+
+def device_copy(dst, src, nbytes, stream=None):
+    """Streaming device-to-device copy (nbytes a multiple of 16)."""
+    init()
+    _check(lib().thr_device_copy(_p(dst), _p(src), int(nbytes), _stream(stream)), "device_copy")
 
 
 def dslash_dev(out, inp, links, dims, am, imass=3, am3=1.0, dagger=False, prec="f64",
