@@ -651,6 +651,37 @@ void dslash_set_config(const Dims &d, int prec, DslashConfig c) {
   g_tuned[TuneKey{d.lx, d.ly, d.lt, d.ls, prec}] = c;
 }
 
+// block -> lattice mapping of a launch with configuration c; *nblocks
+// receives the grid's x extent (blocks per right-hand side)
+static Tiling make_tiling(const Dims &d, const DslashConfig &c, unsigned *nblocks) {
+  Tiling TL{};
+  TL.pstride = (unsigned)dslash_blocks(d);
+  unsigned px = 1, py = 1, pt = 8;
+  if (c.box && box_split(d, px, py, pt)) {
+    TL.mode = TILE_BOXES;
+    TL.px = px;
+    TL.py = py;
+    TL.bx = d.lx / px;
+    TL.by = d.ly / py;
+    TL.bt = d.lt / pt;
+    TL.tchunk = (unsigned)std::max(1, std::min<int>(c.tchunk, (int)TL.bt));
+    TL.nb3 = ((unsigned)d.ls * TL.bx * TL.by + BLK - 1) / BLK;
+    *nblocks = 8 * TL.nb3 * ((TL.bt + TL.tchunk - 1) / TL.tchunk);
+  } else {
+    TL.mode = c.remap ? TILE_REMAP : TILE_LINEAR;
+    TL.tchunk = (unsigned)std::max(1, std::min(c.tchunk, d.lt));
+    TL.nb3 = ((unsigned)d.lx * d.ly * d.ls + BLK - 1) / BLK;
+    *nblocks = TL.nb3 * ((d.lt + TL.tchunk - 1) / TL.tchunk);
+  }
+  return TL;
+}
+
+long dslash_grid_blocks(const Dims &d, const DslashConfig &c) {
+  unsigned nb = 0;
+  (void)make_tiling(d, c, &nb);
+  return nb;
+}
+
 int launch_dslash_cfg(const DslashLaunch &L, DslashConfig c, hipStream_t s) {
   THR_CHECK(L.nrhs >= 1 && L.nrhs <= MAXRHS, THR_ERR_ARG, "nrhs out of range");
   THR_CHECK(L.d.lx > 0 && L.d.ly > 0 && L.d.lt > 0 && L.d.ls > 0, THR_ERR_ARG, "bad dims");
@@ -663,26 +694,8 @@ int launch_dslash_cfg(const DslashLaunch &L, DslashConfig c, hipStream_t s) {
   THR_CHECK(L.mode != EPI_HREC || (L.b_ptr && (L.prec != P16 || L.out_norm)), THR_ERR_ARG,
             "recurrence epilogue needs its scalar (and fp16 scales)");
   THR_CHECK(4 * L.d.v5() < (1L << 31), THR_ERR_SIZE, "lattice too large for 32-bit indexing");
-  Tiling TL{};
-  TL.pstride = (unsigned)dslash_blocks(L.d);
   unsigned nblocks = 0;
-  unsigned px = 1, py = 1, pt = 8;
-  if (c.box && box_split(L.d, px, py, pt)) {
-    TL.mode = TILE_BOXES;
-    TL.px = px;
-    TL.py = py;
-    TL.bx = L.d.lx / px;
-    TL.by = L.d.ly / py;
-    TL.bt = L.d.lt / pt;
-    TL.tchunk = (unsigned)std::max(1, std::min<int>(c.tchunk, (int)TL.bt));
-    TL.nb3 = ((unsigned)L.d.ls * TL.bx * TL.by + BLK - 1) / BLK;
-    nblocks = 8 * TL.nb3 * ((TL.bt + TL.tchunk - 1) / TL.tchunk);
-  } else {
-    TL.mode = c.remap ? TILE_REMAP : TILE_LINEAR;
-    TL.tchunk = (unsigned)std::max(1, std::min(c.tchunk, L.d.lt));
-    TL.nb3 = ((unsigned)L.d.lx * L.d.ly * L.d.ls + BLK - 1) / BLK;
-    nblocks = TL.nb3 * ((L.d.lt + TL.tchunk - 1) / TL.tchunk);
-  }
+  const Tiling TL = make_tiling(L.d, c, &nblocks);
   THR_CHECK(nblocks <= TL.pstride, THR_ERR_ARG, "dslash grid exceeds partial stride");
   dim3 grid(nblocks, (unsigned)L.nrhs);
   dim3 block(BLK);

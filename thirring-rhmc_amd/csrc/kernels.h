@@ -72,7 +72,11 @@ struct Tiling {
 int launch_dslash(const DslashLaunch &L, hipStream_t s);
 int launch_dslash_cfg(const DslashLaunch &L, DslashConfig c, hipStream_t s);
 // partial-sum entries per right-hand side written by every Dslash epilogue
+// (the row stride of the partial buffer; entries past a launch's grid are 0)
 long dslash_blocks(const Dims &d);
+// blocks per right-hand side of a launch with configuration c: the partial
+// entries that launch fills with its sums
+long dslash_grid_blocks(const Dims &d, const DslashConfig &c);
 DslashConfig dslash_config(const Dims &d, int prec);
 void dslash_set_config(const Dims &d, int prec, DslashConfig c);
 // times the launch shapes on scratch fields and pins the fastest for (d, prec)

@@ -4,7 +4,7 @@
 
 namespace thr {
 
-constexpr int MSB = 16;  // iterations per shift-update block
+constexpr int MSB = 32;  // iterations per shift-update block
 
 struct MsfState {
   double delta, omega, omega_save, minus_omega, gammag, alpha, lambda, bnorm2, tol2;

@@ -103,12 +103,12 @@ __global__ __launch_bounds__(BLK) void k_msf_init(MsfState *st, int n, const dou
 // shift's arithmetic is the serial recurrence's, so results do not depend on
 // the thread count.
 __global__ __launch_bounds__(BLK) void k_msf_fin1(MsfState *st, const double *adens,
-                                                  const double *partial, int nb) {
+                                                  const double *partial, int stride, int nb) {
   MsfState *S = st + blockIdx.x;
   if (S->stop) return;
   const double *aden = adens + (long)blockIdx.x * MAXSHIFT;
   __shared__ double s_al;
-  const double al = fin_sum(partial + (long)blockIdx.x * nb, nb);
+  const double al = fin_sum(partial + (long)blockIdx.x * stride, nb);
   if (threadIdx.x == 0) s_al = al;
   __syncthreads();
   const double os = S->omega, gg = S->gammag;
@@ -131,12 +131,13 @@ __global__ __launch_bounds__(BLK) void k_msf_fin1(MsfState *st, const double *ad
 }
 
 // lambda, gamma, gamma_i; block coefficients; convergence (136-176, 230-275)
-__global__ __launch_bounds__(BLK) void k_msf_fin2(MsfState *st, const double *partial, int nb) {
+__global__ __launch_bounds__(BLK) void k_msf_fin2(MsfState *st, const double *partial, int stride,
+                                                  int nb) {
   MsfState *S = st + blockIdx.x;
   if (S->stop) return;
   __shared__ double s_lam;
   __shared__ int s_nmax, s_nmin;
-  const double lam = fin_sum(partial + (long)blockIdx.x * nb, nb);
+  const double lam = fin_sum(partial + (long)blockIdx.x * stride, nb);
   if (threadIdx.x == 0) {
     s_lam = lam;
     s_nmax = 0;
@@ -154,12 +155,24 @@ __global__ __launch_bounds__(BLK) void k_msf_fin2(MsfState *st, const double *pa
     const double zt = S->zeta_iii[i], om = S->omegas[i];
     const double gi = gam * zt * om / (S->zeta_ii[i] * omega);
     S->gammas[i] = gi;
-    // x_i -= omega_i p_i  with p_i = ca P + sum cb R
+    // x_i -= omega_i p_i  with p_i = ca P + sum cb R ; then
+    // p_i = gamma_i p_i + zeta_i''' R_j.  The rows are loaded whole: static
+    // indices keep them in registers and make the loads independent
+    double cbr[MSB], cdr[MSB];
+#pragma unroll
+    for (int q = 0; q < MSB; q++) {
+      cbr[q] = S->cb[i][q];
+      cdr[q] = S->cd[i][q];
+    }
     S->cc[i] -= om * S->ca[i];
-    for (int q = 0; q < j; q++) S->cd[i][q] -= om * S->cb[i][q];
-    // p_i = gamma_i p_i + zeta_i''' R_j
     S->ca[i] *= gi;
-    for (int q = 0; q < j; q++) S->cb[i][q] *= gi;
+#pragma unroll
+    for (int q = 0; q < MSB; q++) {
+      if (q < j) {
+        S->cd[i][q] = cdr[q] - om * cbr[q];
+        S->cb[i][q] = cbr[q] * gi;
+      }
+    }
     S->cb[i][j] = zt;
     S->cd[i][j] = 0.0;
     S->touched[i] = 1;
@@ -390,7 +403,12 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
   THR_CHECK(A.u64 && A.u32 && A.b && A.x, THR_ERR_ARG, "NULL argument");
   const long v5 = d.v5(), nsp = 4 * v5, nsb = nsp * ns_;
   const int nb = (int)dslash_blocks(d);
-  // buffers (workspace slots 200+)
+  // workspace slots of the Krylov buffers per precision pr: ring slot q,
+  // h, r0, and the FP16 scales
+  auto w_ring = [](int pr, int q) { return 1000 + pr * MSB + q; };
+  auto w_h = [](int pr) { return 1100 + pr; };
+  auto w_r0 = [](int pr) { return 1110 + pr; };
+  auto w_ringn = [](int q) { return 1200 + q; };
   size_t bytes_p[3] = {16, 8, 4};
   void *ring[3][MSB];
   float *ringn[MSB];
@@ -399,16 +417,16 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
   for (int pr = 0; pr < 3; pr++) {
     const bool need = (pr == P64) || (pr == P32 && A.th32 > 0) || (pr == P16 && A.th16 > 0);
     for (int q = 0; q < MSB; q++)
-      ring[pr][q] = need ? workspace(200 + pr * 40 + q, bytes_p[pr] * nsb) : nullptr;
-    hbuf[pr] = need ? workspace(200 + pr * 40 + 21, bytes_p[pr] * nsb) : nullptr;
-    r0[pr] = need ? workspace(200 + pr * 40 + 22, bytes_p[pr] * nsb) : nullptr;
+      ring[pr][q] = need ? workspace(w_ring(pr, q), bytes_p[pr] * nsb) : nullptr;
+    hbuf[pr] = need ? workspace(w_h(pr), bytes_p[pr] * nsb) : nullptr;
+    r0[pr] = need ? workspace(w_r0(pr), bytes_p[pr] * nsb) : nullptr;
     if (need)
       THR_CHECK(hbuf[pr] && r0[pr], THR_ERR_HIP, "workspace allocation failed");
   }
   if (A.th16 > 0) {
-    for (int q = 0; q < MSB; q++) ringn[q] = (float *)workspace(340 + q, sizeof(float) * v5 * ns_);
-    hn = (float *)workspace(361, sizeof(float) * v5 * ns_);
-    r0n = (float *)workspace(362, sizeof(float) * v5 * ns_);
+    for (int q = 0; q < MSB; q++) ringn[q] = (float *)workspace(w_ringn(q), sizeof(float) * v5 * ns_);
+    hn = (float *)workspace(1300, sizeof(float) * v5 * ns_);
+    r0n = (float *)workspace(1301, sizeof(float) * v5 * ns_);
   }
   double2 *Pv = (double2 *)workspace(363, sizeof(double2) * nsp * ns_ * n);
   MsfState *st = (MsfState *)workspace(364, sizeof(MsfState) * ns_);
@@ -444,6 +462,15 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
   THR_HIP(hipGetLastError());
 
   THR_HIP(hipMemsetAsync(A.x, 0, sizeof(double2) * nsp * ns_ * n, s));
+  // launch shapes are fixed for the whole solve, so the finalize kernels sum
+  // exactly the partial entries each precision's Dslash grid writes
+  DslashConfig cfg[3];
+  int nbl[3];
+  for (int pr = 0; pr < 3; pr++) {
+    cfg[pr] = dslash_config(d, pr);
+    nbl[pr] = (int)dslash_grid_blocks(d, cfg[pr]);
+    THR_CHECK(nbl[pr] <= nb, THR_ERR_STATE, "dslash grid exceeds partial stride");
+  }
   int prec = P64;
   if (A.start_prec > 0) prec = A.start_prec;
   THR_CHECK(prec == P64 || (prec == P32 && A.th32 > 0) || (prec == P16 && A.th16 > 0), THR_ERR_ARG,
@@ -497,8 +524,8 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
         a.mode = EPI_HREC;
         a.b_ptr = &st->gammag;
       }
-      TRY_RC(launch_dslash(a, s));
-      hipLaunchKernelGGL(k_msf_fin1, dim3(ns_), dim3(BLK), 0, s, st, aden, partial, nb);
+      TRY_RC(launch_dslash_cfg(a, cfg[pr], s));
+      hipLaunchKernelGGL(k_msf_fin1, dim3(ns_), dim3(BLK), 0, s, st, aden, partial, nb, nbl[pr]);
       // r_new = r + omega M^+ h ; lambda = |r_new|^2
       DslashLaunch b = L;
       b.prec = pr;
@@ -513,8 +540,8 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
       b.r_out_norm = routn;
       b.a_ptr = &st->minus_omega;
       b.b_ptr = nullptr;
-      TRY_RC(launch_dslash(b, s));
-      hipLaunchKernelGGL(k_msf_fin2, dim3(ns_), dim3(BLK), 0, s, st, partial, nb);
+      TRY_RC(launch_dslash_cfg(b, cfg[pr], s));
+      hipLaunchKernelGGL(k_msf_fin2, dim3(ns_), dim3(BLK), 0, s, st, partial, nb, nbl[pr]);
       THR_HIP(hipGetLastError());
     }
     // flush the block into x_i, p_i

@@ -58,8 +58,8 @@ class Request:
         """Device bytes: links, sources, solutions and the solver scratch."""
         v5 = self.L * self.L * self.T * self.Ls
         spinor = 4 * v5 * 16
-        # ring of 16 FP64 residual batches + FP32 copies + p, h, r0 + shift x/p
-        per_src = spinor * (16 + 3 + 2) + spinor // 2 * (16 + 3) + 2 * spinor
+        # ring of 32 FP64 residual batches + FP32 copies + p, h, r0 + shift x/p
+        per_src = spinor * (32 + 3 + 2) + spinor // 2 * (32 + 3) + 2 * spinor
         return 3 * self.L * self.L * self.T * 24 + self.nsrc * (per_src + 2 * spinor)
 
 
