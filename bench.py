@@ -114,7 +114,8 @@ def time_dslash(G, hr, dims, links, prec, steps, warmup, nbuf, stream):
 
 
 def bench_multishift(G, hr, dims, l64, l32, am, nsrc, stream, tol=1e-10, maxcg=50000,
-                     mixed_th32=1e-4, mixed_th16=1e-7):
+                     mixed_th32=1e-4, mixed_th16=1e-7,
+                     legs_wanted=("plain_fp64", "blocked_fp64", "blocked_mixed")):
     """Plain FP64 path (one source at a time) vs the batched mixed path."""
     lx, ly, lt, ls = dims
     nsp = 4 * lx * ly * lt * ls
@@ -123,27 +124,15 @@ def bench_multishift(G, hr, dims, l64, l32, am, nsrc, stream, tol=1e-10, maxcg=5
     rng = np.random.default_rng(7)
     bh = rng.standard_normal((nsrc, nsp, 2)) / np.sqrt(2)
     b = hr.DeviceArray.from_host(bh)
-    # plain FP64: the reference's recurrences, one source
-    b0 = hr.DeviceArray.from_host(bh[:1])
-    xp = hr.DeviceArray((1, n, nsp, 2), np.float64)
-    progress(f"multishift plain FP64 {dims} am={am}")
-    hr.synchronize()
-    t = time.perf_counter()
-    # iterated stop at 3/4 of the target, as the fast path does, so that the
-    # FP64 true residual ends below the target
-    cg_plain, _ = G.multishift_plain_dev(l64, dims, am, 3, sig, b0, xp, tol=0.75 * tol, maxcg=maxcg,
-                                         stream=stream)
-    hr.synchronize()
-    t_plain = time.perf_counter() - t
-    rel_plain = G.true_residuals_dev(l64, dims, am, 3, sig, b0, xp, stream=stream)
-    del xp
-    out = {"plain_fp64": {"solves_per_s": round(1.0 / t_plain, 4), "seconds": round(t_plain, 3),
-                          "iterations": int(cg_plain),
-                          "max_true_residual": float(rel_plain.max())}}
+    out = {}
+    if "plain_fp64" in legs_wanted:
+        out.update(_plain_leg(G, hr, dims, l64, am, bh, sig, n, nsp, tol, maxcg, stream))
     # both batched legs verify every (source, shift) in FP64 inside the timed
     # region and correct the ones above tol (defect correction)
     legs = (("blocked_fp64", 0.0, 0.0, 2), ("blocked_mixed", mixed_th32, mixed_th16, 4))
     for label, th32, th16, corr in legs:
+        if label not in legs_wanted:
+            continue
         x = hr.DeviceArray((nsrc, n, nsp, 2), np.float64)
         progress(f"multishift {label} x{nsrc} sources (th32={th32}, th16={th16})")
         hr.synchronize()
@@ -162,9 +151,30 @@ def bench_multishift(G, hr, dims, l64, l32, am, nsrc, stream, tol=1e-10, maxcg=5
                       "max_true_residual": float(rel.max())}
         del x
     for label in ("blocked_fp64", "blocked_mixed"):
-        out[f"speedup_{label}_vs_plain_fp64"] = round(out[label]["solves_per_s"] /
-                                                      out["plain_fp64"]["solves_per_s"], 3)
+        if label in out and "plain_fp64" in out:
+            out[f"speedup_{label}_vs_plain_fp64"] = round(out[label]["solves_per_s"] /
+                                                          out["plain_fp64"]["solves_per_s"], 3)
     return out
+
+
+def _plain_leg(G, hr, dims, l64, am, bh, sig, n, nsp, tol, maxcg, stream):
+    # plain FP64: the reference's recurrences, one source
+    b0 = hr.DeviceArray.from_host(bh[:1])
+    xp = hr.DeviceArray((1, n, nsp, 2), np.float64)
+    progress(f"multishift plain FP64 {dims} am={am}")
+    hr.synchronize()
+    t = time.perf_counter()
+    # iterated stop at 3/4 of the target, as the fast path does, so that the
+    # FP64 true residual ends below the target
+    cg_plain, _ = G.multishift_plain_dev(l64, dims, am, 3, sig, b0, xp, tol=0.75 * tol, maxcg=maxcg,
+                                         stream=stream)
+    hr.synchronize()
+    t_plain = time.perf_counter() - t
+    rel_plain = G.true_residuals_dev(l64, dims, am, 3, sig, b0, xp, stream=stream)
+    del xp
+    return {"plain_fp64": {"solves_per_s": round(1.0 / t_plain, 4), "seconds": round(t_plain, 3),
+                           "iterations": int(cg_plain),
+                           "max_true_residual": float(rel_plain.max())}}
 
 
 def bench_serving(rate, seconds, slots, max_requests):
@@ -238,6 +248,8 @@ def main():
     ap.add_argument("--solver-sources", type=int, default=4)
     ap.add_argument("--mixed-th32", type=float, default=1e-4)
     ap.add_argument("--mixed-th16", type=float, default=1e-7)
+    ap.add_argument("--solver-legs", default="plain_fp64,blocked_fp64,blocked_mixed",
+                    help="comma-separated subset of the multishift legs to run")
     ap.add_argument("--tchunk", type=int, default=0,
                     help="pin the Dslash t-chunk (with --shape-flags) instead of autotuning")
     ap.add_argument("--shape-flags", type=int, default=7,
@@ -297,7 +309,8 @@ def main():
         sd = tuple(int(v) for v in args.solver_lattice.split(","))
         _, s64, s32 = make_fields(G, hr.DeviceArray, sd, 0.3, 4321)
         solver = bench_multishift(G, hr, sd, s64, s32, args.solver_am, args.solver_sources, stream,
-                                  mixed_th32=args.mixed_th32, mixed_th16=args.mixed_th16)
+                                  mixed_th32=args.mixed_th32, mixed_th16=args.mixed_th16,
+                                  legs_wanted=tuple(args.solver_legs.split(",")))
         solver["lattice"] = list(sd)
         solver["am"] = args.solver_am
     serving = None
