@@ -16,6 +16,10 @@ def serving(monkeypatch):
         def bind(self):
             pass
 
+        @staticmethod
+        def unbind():
+            pass
+
         def destroy(self):
             pass
 
@@ -28,6 +32,8 @@ def serving(monkeypatch):
     lock = threading.Lock()
 
     def fake_solve(req, stream=None):
+        if req.seed == 666:
+            raise RuntimeError("solver failure")
         with lock:
             live["n"] += 1
             live["bytes"] += req.footprint()
@@ -87,3 +93,29 @@ def test_run_load_metrics(serving):
     assert m["requests"] >= 10
     assert m["p99_latency_s"] >= m["p50_latency_s"] > 0
     assert m["solves_per_s"] > 0
+
+
+def test_failed_solve_frees_its_slot(serving):
+    S = serving
+    reqs = [S.Request(k, 8, 8, 8, 0.3, 0.05, 1, 666 if k == 2 else k) for k in range(8)]
+    eng = S.ServingEngine(slots=1, hbm_budget=1.5 * reqs[0].footprint())
+    for r in reqs:
+        eng.submit(r)
+    done = eng.drain(timeout=30)
+    assert len(done) == 8
+    failed = [r for r in done if "error" in r.result]
+    assert [r.rid for r in failed] == [2] and "solver failure" in failed[0].result["error"]
+
+
+def test_request_over_budget_is_rejected(serving):
+    S = serving
+    small = S.Request(0, 8, 8, 8, 0.3, 0.05, 1, 0)
+    huge = S.Request(1, 24, 48, 48, 0.3, 0.05, 12, 1)
+    eng = S.ServingEngine(slots=2, hbm_budget=2 * small.footprint())
+    eng.submit(huge)
+    eng.submit(small)
+    done = eng.drain(timeout=30)
+    assert len(done) == 2
+    byid = {r.rid: r for r in done}
+    assert "exceeds the HBM budget" in byid[1].result["error"]
+    assert "error" not in byid[0].result

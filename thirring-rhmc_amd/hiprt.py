@@ -35,6 +35,7 @@ def hip():
             "hipDeviceSynchronize": ([], ctypes.c_int),
             "hipStreamCreateWithFlags": ([ctypes.POINTER(V), ctypes.c_uint], ctypes.c_int),
             "hipStreamSynchronize": ([V], ctypes.c_int),
+            "hipStreamDestroy": ([V], ctypes.c_int),
             "hipEventCreate": ([ctypes.POINTER(V)], ctypes.c_int),
             "hipEventRecord": ([V, V], ctypes.c_int),
             "hipEventSynchronize": ([V], ctypes.c_int),
@@ -129,6 +130,11 @@ class Stream:
 
     def synchronize(self):
         check(hip().hipStreamSynchronize(ctypes.c_void_p(self.ptr)), "hipStreamSynchronize")
+
+    def destroy(self):
+        if self.ptr:
+            check(hip().hipStreamDestroy(ctypes.c_void_p(self.ptr)), "hipStreamDestroy")
+            self.ptr = None
 
 
 class Event:

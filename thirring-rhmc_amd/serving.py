@@ -120,10 +120,18 @@ class ServingEngine:
         self.scheduler.start()
 
     def submit(self, req):
+        """Queues a request; one that could never fit the HBM budget is
+        finished at once with an error instead of blocking the queue."""
         with self.lock:
             if req.arrival == 0.0:
                 req.arrival = time.perf_counter()
-            self.pending.append(req)
+            if req.footprint() > self.hbm_budget:
+                req.start = req.finish = time.perf_counter()
+                req.result = {"error": f"footprint {req.footprint():.3e} B exceeds the HBM budget "
+                                       f"{self.hbm_budget:.3e} B"}
+                self.done.append(req)
+            else:
+                self.pending.append(req)
             self.lock.notify_all()
 
     def _pick(self):
@@ -168,15 +176,21 @@ class ServingEngine:
             if req is None:
                 break
             req.start = time.perf_counter()
-            req.result = solve_request(req, stream=stream)
-            if not self.keep:
-                req.result.pop("x", None)
+            try:
+                req.result = solve_request(req, stream=stream)
+                if not self.keep:
+                    req.result.pop("x", None)
+            except Exception as e:  # a failed solve still frees its slot and memory
+                req.result = {"error": f"{type(e).__name__}: {e}"}
             req.finish = time.perf_counter()
             with self.lock:
                 self.in_use -= req.footprint()
                 self.running -= 1
                 self.done.append(req)
                 self.lock.notify_all()
+        if hasattr(stream, "destroy"):
+            stream.destroy()
+        G.Context.unbind()
         ctx.destroy()
 
     def drain(self, timeout=None):
@@ -210,10 +224,12 @@ def run_load(rate, duration, seed=0, slots=4, hbm_budget=200e9, tol=1e-10, max_r
             time.sleep(min(0.002, t_next - now))
     done = eng.drain()
     wall = time.perf_counter() - t0
-    lat = np.array([r.finish - r.arrival for r in done]) if done else np.zeros(1)
-    solves = sum(r.nsrc for r in done)
-    worst = max((max(r.result["true_residual"]) for r in done), default=0.0)
-    return {"rate_per_s": rate, "requests": len(done), "sources_solved": int(solves),
+    ok = [r for r in done if "error" not in r.result]
+    lat = np.array([r.finish - r.arrival for r in ok]) if ok else np.zeros(1)
+    solves = sum(r.nsrc for r in ok)
+    worst = max((max(r.result["true_residual"]) for r in ok), default=0.0)
+    return {"rate_per_s": rate, "requests": len(ok), "failed": len(done) - len(ok),
+            "sources_solved": int(solves),
             "solves_per_s": round(solves / wall, 3), "requests_per_s": round(len(done) / wall, 3),
             "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
             "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
