@@ -14,8 +14,15 @@
  * brute-force operator assembly, adjointness <a,Mb> = <M^+a,b>, U(1) gauge
  * covariance, free-field plane waves, exact finite differences of the action
  * for derivs (M is affine in theta), multishift iterates vs. plain CG on each
- * shifted system, dense solves for qmrherm, and the reference's own known
- * answer itercg == 27 for congrad (tests/test_congrad.F90:47).
+ * shifted system, dense solves for qmrherm, the reference's own known
+ * answer itercg == 27 for congrad (tests/test_congrad.F90:47), and force ==
+ * theta-derivative of the hamilton action (central differences, same
+ * rational approximations in both).
+ *
+ * Parity unpinned: the absolute sign of the imass = 1/3/5 wall-mass terms
+ * (adjointness fixes M against M^+, not the sign itself; a restatement of
+ * dirac.F90:119-176, 356-402), and the exact iteration counts of the FP32
+ * multishift (multishift_module.F90:289-534).
  */
 #include "oracle.h"
 

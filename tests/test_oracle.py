@@ -402,3 +402,56 @@ def test_congrad_wilson_true_residual():
     it, _ = O.congrad(phi, x, u, dims, res=1e-6, am=0.1, imass=1, kernel="wilson")
     r = D.interior_vec(phi) - D.interior_vec(O.dirac_op_shifted(x, u, dims, 0.1, 1, kernel="wilson"))
     assert np.sum(np.abs(r) ** 2) < 4 * 4 ** 4 * np.float32(1e-6) ** 2
+
+
+def _remez_data(name, block):
+    import os
+    return _remez(os.path.join(os.path.dirname(__file__), "..", "thirring-rhmc_amd", "data", name),
+                  block)
+
+
+def test_force_is_derivative_of_hamilton():
+    """force_module.F90 against dwf3d_lib.F90 hamilton: with the same rational
+    approximations in both, dSdpi - beta*theta is the theta-derivative of the
+    pseudofermion action hf = <R, r2(K) R>, R = r4(K_PV) Phi (central
+    differences; u = 1 + i theta is linear in theta, antiperiodic in t, so the
+    t-boundary links enter with the opposite sign).  Pins the composition of
+    the four qmrherm calls, the sign flip at the t boundary and the gauge term."""
+    a0_4, a_4, s_4 = _remez_data("remez4", 0)   # x^(1/4)
+    b0_2, b_2, t_2 = _remez_data("remez2", 1)   # x^(-1/2)
+    a0_2, a_2, s_2 = _remez_data("remez2", 0)   # x^(1/2)
+    x = np.logspace(-3, 1, 200)
+    r = lambda c0, c, s: c0 + np.sum(c[None, :] / (x[:, None] + s[None, :]), axis=1)
+    assert np.max(np.abs(r(a0_4, a_4, s_4) / x ** 0.25 - 1)) < 1e-6
+    assert np.max(np.abs(r(b0_2, b_2, t_2) * x ** 0.5 - 1)) < 1e-6
+    assert np.max(np.abs(r(a0_2, a_2, s_2) / x ** 0.5 - 1)) < 1e-6
+    anum4, aden4 = np.concatenate([[a0_4], a_4]), s_4
+    bnum2, bden2 = np.concatenate([[b0_2], b_2]), t_2
+    anum2, aden2 = np.concatenate([[a0_2], a_2]), s_2
+
+    rng = np.random.default_rng(60)
+    dims = (4, 4, 4, 4)
+    lx, ly, lt, ls = dims
+    am, imass, beta = 0.2, 3, 0.6
+    th = (0.3 * rng.standard_normal((lx, ly, lt, 3))).astype(np.float32)
+    phi = D.rand_spinor(dims, rng)
+    pp = np.zeros((lx, ly, lt, 3), dtype=np.float32)
+    ham = (anum4, aden4, bnum2, bden2)
+    frc = (anum4, aden4, bnum2, bden2, anum2, aden2)
+    res = np.float32(1e-11)
+
+    def hf(t32):
+        h, _ = O.hamilton(phi, res, am, imass, beta, t32, pp, O.coef(t32, dims), dims, ham)
+        return h["hf"]
+
+    f = O.force(phi, res, am, imass, beta, th, O.coef(th, dims), dims, frc)
+    gauge = (np.float32(beta) * th).astype(np.float64)  # d/dtheta of hg = beta/2 sum theta^2
+    for (xx, yy, tt, mu) in [(0, 0, 0, 0), (1, 2, 1, 1), (2, 1, 2, 2), (3, 3, lt - 1, 2),
+                             (0, 3, lt - 1, 0)]:
+        tp, tm = th.copy(), th.copy()
+        tp[xx, yy, tt, mu] += np.float32(2e-3)
+        tm[xx, yy, tt, mu] -= np.float32(2e-3)
+        step = float(tp[xx, yy, tt, mu]) - float(tm[xx, yy, tt, mu])
+        dS = (hf(tp) - hf(tm)) / step
+        got = f[xx, yy, tt, mu] - gauge[xx, yy, tt, mu]
+        assert abs(got - dS) < 1e-6 * max(1.0, abs(dS)), ((xx, yy, tt, mu), got, dS)
