@@ -19,7 +19,11 @@
  * theta-derivative of the hamilton action (central differences, same
  * rational approximations in both).
  *
- * Parity unpinned: the absolute sign of the imass = 1/3/5 wall-mass terms
+ * The reference's psibarpsi known answers (tests/test_measure_shamir.F90,
+ * test_measure_wilson.F90) pin dslashd, congrad and the imass = 3 wall-mass
+ * sign of both kernels.
+ *
+ * Parity unpinned: the absolute sign of the imass = 1 and 5 wall-mass terms
  * (adjointness fixes M against M^+, not the sign itself; a restatement of
  * dirac.F90:119-176, 356-402), and the exact iteration counts of the FP32
  * multishift (multishift_module.F90:289-534).
@@ -425,6 +429,14 @@ static void dwilson(zdp *Phi, const zdp *R, const zdp *u, const geom *gp, float 
           }
     }
 #undef I3
+}
+
+/* DWilson (dag 0) / DWilsonD (dag 1) on one haloed 4-D slice (X,Y,T,4), as
+ * measure_wilson calls it (measure_module.F90 at the P+/P- wall projections) */
+void orc_dwilson_slice(double *phi, const double *r, const double *u, const int dims[4],
+                       float mass, int dag) {
+  geom g = mkgeom(dims);
+  dwilson((zdp *)phi, (const zdp *)r, (const zdp *)u, &g, mass, dag);
 }
 
 /* copy spins [a0,a1) of slice s of R (whole x,y,t halo range) into a slice */

@@ -35,6 +35,10 @@ void orc_set_threads(int n);
 
 /* periodic halo fills (uncomms5.f90:6-19 / comms5.F90:135-192 semantics) */
 void orc_halo5(double *a, const int dims[4], int nspin);
+/* 4-D Wilson operator (mass: -am3 in the kernel) on one haloed slice
+ * [T+2][Y+2][X+2] x 4 spins, complex interleaved; dag selects DWilsonD */
+void orc_dwilson_slice(double *phi, const double *r, const double *u, const int dims[4],
+                       float mass, int dag);
 void orc_halo4(double *u, const int dims[4]);
 
 /* u = 1 + i theta, antiperiodic t-links, halo (gauge.F90:12-43) */

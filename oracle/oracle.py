@@ -46,6 +46,7 @@ def lib():
             "orc_num_threads": ([], I32),
             "orc_set_threads": ([I32], None),
             "orc_halo5": ([P, P, I32], None),
+            "orc_dwilson_slice": ([P, P, P, P, F32, I32], None),
             "orc_halo4": ([P, P], None),
             "orc_coef": ([P, P, P, I32], None),
             "orc_dslash_shamir": ([P, P, P, P, F32, I32, F32], None),
@@ -112,6 +113,14 @@ def set_threads(n):
 def halo5(a, dims):
     lib().orc_halo5(_p(a), _dims(dims), a.shape[4])
     return a
+
+
+def dwilson_slice(r, u, dims, mass, dag=False):
+    """DWilson / DWilsonD on a haloed slice (Lx+2, Ly+2, Lt+2, 4), Fortran order."""
+    r = np.asfortranarray(r, dtype=np.complex128)
+    out = np.zeros_like(r)
+    lib().orc_dwilson_slice(_p(out), _p(r), _p(u), _dims(dims), mass, int(bool(dag)))
+    return out
 
 
 def halo4(u, dims):

@@ -22,6 +22,7 @@ import pytest
 
 import dense as D
 import oracle as O
+import measure_ref
 import refstate
 
 
@@ -455,3 +456,60 @@ def test_force_is_derivative_of_hamilton():
         dS = (hf(tp) - hf(tm)) / step
         got = f[xx, yy, tt, mu] - gauge[xx, yy, tt, mu]
         assert abs(got - dS) < 1e-6 * max(1.0, abs(dS)), ((xx, yy, tt, mu), got, dS)
+
+
+def test_site_random_gauss_known_answers():
+    """tests/test_gauss0.F90:83-86 and tests/test_gaussp.F90:83-86 (SITE_RANDOM
+    build, seed 1.0, 12^3 sites): sum, max and min of one noise field."""
+    from site_random import SiteRandom
+    ps = SiteRandom(12, 12, 12, 1.0).gauss0()
+    for got, want in ((ps.sum(dtype=np.float32), 14.460), (ps.max(), 2.59536),
+                      (ps.min(), -2.441575)):
+        assert abs(got - want) <= 1e-3 * abs(want), (got, want)
+    pp = SiteRandom(12, 12, 12, 1.0).gaussp()
+    for got, want in ((pp.sum(dtype=np.float32), 20.4506), (pp.max(), 3.67040),
+                      (pp.min(), -3.452905)):
+        assert abs(got - want) <= 1e-3 * abs(want), (got, want)
+
+
+def test_measure_shamir_known_answer():
+    """tests/test_measure_shamir.F90:24-46 (12^3 x Ls 24, closed-form links,
+    SITE_RANDOM noise seeded 4139764973254, res 0.1, am 0.05, imass 3):
+    psibarpsi == 2.504295e-04 and 5 CG iterations per solve, to 0.1% (the
+    oracle gives 2.5042944e-04).  Flipping the sign of the wall-mass term flips
+    the sign of psibarpsi, so this pins that sign."""
+    from site_random import SiteRandom
+    dims = (12, 12, 12, 24)
+    st = refstate.starting_state(dims, want=("u",))
+    # `seed = 4139764973254.0` is a default-real literal: the seed is its float32 value
+    rng = SiteRandom(12, 12, 12, float(np.float32(4139764973254.0)))
+    pbp, aviter = measure_ref.measure_shamir(O.dslashd, O.congrad, st["u"], dims, np.float32(0.1),
+                                             np.float32(0.05), 3, rng)
+    assert abs(pbp - 2.504295e-04) <= 1e-3 * 2.504295e-04, pbp
+    assert abs(aviter - 5.0) <= 1e-3 * 5.0, aviter
+
+
+def test_measure_wilson_known_answer():
+    """tests/test_measure_wilson.F90:24-47 (same inputs, Wilson-kernel wall
+    sources, DWilson at the projected walls, congrad with the Shamir
+    operator the reference's test values were generated with): aviter == 10.9
+    and |psibarpsi| == 5.4226685e-02 to 0.1% (the oracle gives 5.4226685e-02).
+    The reference's value is negative: it predates the swap of the imass = 3
+    phases in measure_wilson (measure_module.F90:612-618, "The signs here are
+    swapped compared to measure_shamir, fixed by SJH 7/6/24"): the phases
+    before that swap negate every noise term, giving -5.4226685e-02.  Flipping
+    the wall-mass sign of the Wilson kernel instead gives -8.05e-02, so the
+    value pins that sign."""
+    from site_random import SiteRandom
+    dims = (12, 12, 12, 24)
+    st = refstate.starting_state(dims, want=("u",))
+    seed = float(np.float32(4139764973254.0))
+
+    def dslashd_w(r, u, d, am, imass):
+        return O.dslashd(r, u, d, am, imass, kernel="wilson")
+
+    pbp, aviter = measure_ref.measure_wilson(dslashd_w, O.congrad, O.dwilson_slice, st["u"], dims,
+                                             np.float32(0.1), np.float32(0.05), 3,
+                                             SiteRandom(12, 12, 12, seed))
+    assert abs(pbp - 5.4226685e-02) <= 1e-3 * 5.4226685e-02, pbp
+    assert abs(aviter - 10.9) <= 1e-3 * 10.9, aviter
