@@ -79,12 +79,13 @@ def time_dslash(G, hr, dims, links, prec, steps, warmup, nbuf, stream):
     ins, outs, inn, outn = [], [], [], []
     for _ in range(nbuf):
         x = (rng.standard_normal((4, v5, 2)) / np.sqrt(2)).astype(np.float32)
-        if prec == "f16":
-            sc = np.abs(x).max(axis=(0, 2)).astype(np.float32)
-            ins.append(hr.DeviceArray.from_host((x / sc[None, :, None]).astype(np.float16)))
+        if prec == "f16":  # [site][spin] with a per-site scale
+            xs = np.ascontiguousarray(x.transpose(1, 0, 2))
+            sc = np.abs(xs).max(axis=(1, 2)).astype(np.float32)
+            ins.append(hr.DeviceArray.from_host((xs / sc[:, None, None]).astype(np.float16)))
             inn.append(hr.DeviceArray.from_host(sc))
             outn.append(hr.DeviceArray((v5,), np.float32))
-            outs.append(hr.DeviceArray((4, v5, 2), np.float16))
+            outs.append(hr.DeviceArray((v5, 4, 2), np.float16))
         else:
             dt = np.float64 if prec == "f64" else np.float32
             ins.append(hr.DeviceArray.from_host(x.astype(dt)))

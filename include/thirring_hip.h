@@ -22,7 +22,9 @@
  *      multishift_module, qmrherm_module, derivs_module, gauge) bind.
  *
  *  (2) device-resident entry points (suffix _dev) on halo-free device
- *      buffers in the layout [rhs][spin][t][y][x][s] (s fastest), links
+ *      buffers in the layout [rhs][spin][t][y][x][s] (s fastest; FP16
+ *      spinors keep the 4 spins of a site together, [rhs][t][y][x][s][spin],
+ *      plus one float scale per site), links
  *      [mu][t][y][x]; `stream` is a hipStream_t.  With a stream, entry
  *      points that only enqueue work (coef, links, dslash) return at once;
  *      with NULL they run on the calling context's own non-blocking stream
@@ -49,7 +51,9 @@ extern "C" {
 /* precisions of device spinors */
 #define THR_F64 0
 #define THR_F32 1
-#define THR_F16 2 /* half2 per complex + one float scale per 5-D site */
+#define THR_F16 2 /* half2 per complex + one float scale per 5-D site; the 4 spins
+                     of a site are stored together: [rhs][site][spin] (F64 and F32
+                     store spin planes: [rhs][spin][site]) */
 
 /* ---- lifetime ------------------------------------------------------------ */
 /* select the device and create the library stream; replaces init_MPI and

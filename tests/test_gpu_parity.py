@@ -228,16 +228,18 @@ def test_device_precisions(G):
     out32 = DeviceArray((4 * v5, 2), np.float32)
     G.dslash_dev(out32, x32, l32, dims, 0.05, imass=3, prec="f32")
     check(out32.to_host().astype(np.float64), rc, 2e-6)
-    # fp16 + per-site scale: quantise the source the way the kernel stores it
-    xs = xh.astype(np.float32).reshape(4, v5, 2)
-    scale = np.abs(xs).max(axis=(0, 2)).astype(np.float32)
-    xq = (xs / scale[None, :, None]).astype(np.float16)
+    # fp16 + per-site scale, spins of a site together [site][spin]: quantise
+    # the source the way the kernel stores it
+    xs = xh.astype(np.float32).reshape(4, v5, 2).transpose(1, 0, 2)
+    scale = np.abs(xs).max(axis=(1, 2)).astype(np.float32)
+    xq = np.ascontiguousarray((xs / scale[:, None, None]).astype(np.float16))
     xhd = DeviceArray.from_host(xq)
     sd = DeviceArray.from_host(scale)
-    outh = DeviceArray((4, v5, 2), np.float16)
+    outh = DeviceArray((v5, 4, 2), np.float16)
     on = DeviceArray((v5,), np.float32)
     G.dslash_dev(outh, xhd, l32, dims, 0.05, imass=3, prec="f16", in_norm=sd, out_norm=on)
-    got = (outh.to_host().astype(np.float32) * on.to_host()[None, :, None]).reshape(-1, 2)
+    got = outh.to_host().astype(np.float32) * on.to_host()[:, None, None]
+    got = got.transpose(1, 0, 2).reshape(-1, 2)
     check(got.astype(np.float64), rc, 4e-3)
 
 @pytest.mark.parametrize("dims", [(4, 6, 8, 8), (6, 4, 6, 12), (4, 4, 3, 64), (4, 2, 8, 16),

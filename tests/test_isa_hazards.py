@@ -1,5 +1,5 @@
-"""CPU check of the built gfx950 code: no buffer store of more than 64 bits
-uses an SGPR soffset.
+"""CPU checks of the built gfx950 code.  First: no buffer store of more than
+64 bits uses an SGPR soffset.
 
 Such a store reads its data VGPRs after issue; the compiler inserts the
 wait state before a VALU write to those VGPRs only when soffset is not an
@@ -90,3 +90,24 @@ def test_no_wide_buffer_store_with_sgpr_soffset(tmp_path):
     assert total > 0, "expected the FP64 Dslash output stores in the code"
     assert not hazards, hazards[:5]
     assert not wide, wide[:5]
+
+
+def test_f16_dslash_reads_sites_whole(tmp_path):
+    """FP16 spinors keep the 4 spins of a site in 16 bytes: every FP16 Dslash
+    kernel must read the centre, the two t neighbours and the four x/y
+    neighbours with one 128-bit load each.  (A bit_cast of an ext-vector
+    element lvalue once compiled to a 32-bit load of spin 0 only.)"""
+    label = re.compile(r"^[0-9a-f]+ <(.*)>:$")
+    counts, cur = {}, None
+    for text in _disassemble(tmp_path):
+        for line in text.splitlines():
+            m = label.match(line)
+            if m:
+                cur = m.group(1)
+                counts[cur] = 0
+            elif cur is not None and "buffer_load_dwordx4" in line:
+                counts[cur] += 1
+    f16 = {k: v for k, v in counts.items() if "8k_dslashILi2E" in k}
+    assert f16, "no FP16 Dslash kernels found"
+    short = {k: v for k, v in f16.items() if v < 7}
+    assert not short, list(short.items())[:5]

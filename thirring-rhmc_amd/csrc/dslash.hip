@@ -29,6 +29,9 @@ namespace thr {
 
 namespace {
 
+// Spinor layouts: FP64 and FP32 store spin planes [spin][site]; FP16 stores
+// the 4 spins of a site together [site][spin] (16 bytes, one 128-bit access)
+// with a per-site float scale.
 template <int PREC> struct Acc {
   using C = typename Store<PREC>::C;
   using T = typename Store<PREC>::T;
@@ -38,8 +41,13 @@ template <int PREC> struct Acc {
   __device__ inline void load4(unsigned site, cx<C> v[4]) const {
     if constexpr (PREC == P16) {
       const float sc = nrm[site];
+      const uint4 q = *reinterpret_cast<const uint4 *>(p + 4 * (long)site);
+      const unsigned w[4] = {q.x, q.y, q.z, q.w};
 #pragma unroll
-      for (int a = 0; a < 4; a++) v[a] = ldh(p, a * v5 + site, sc);
+      for (int a = 0; a < 4; a++) {
+        const float2 f = __half22float2(__builtin_bit_cast(__half2, w[a]));
+        v[a] = {f.x * sc, f.y * sc};
+      }
     } else {
 #pragma unroll
       for (int a = 0; a < 4; a++) v[a] = ldc(p, a * v5 + site);
@@ -47,7 +55,7 @@ template <int PREC> struct Acc {
   }
   __device__ inline cx<C> load1(unsigned site, unsigned a) const {
     if constexpr (PREC == P16) {
-      return ldh(p, a * v5 + site, nrm[site]);
+      return ldh(p, 4 * (long)site + a, nrm[site]);
     } else {
       return ldc(p, a * v5 + site);
     }
@@ -68,7 +76,7 @@ template <int PREC> struct Out {
       const float inv = m > 0.f ? 1.0f / m : 0.f;
       nrm[site] = m;
 #pragma unroll
-      for (int a = 0; a < 4; a++) p[a * v5 + site] = __floats2half2_rn(v[a].r * inv, v[a].i * inv);
+      for (int a = 0; a < 4; a++) p[4 * (long)site + a] = __floats2half2_rn(v[a].r * inv, v[a].i * inv);
     } else {
 #pragma unroll
       for (int a = 0; a < 4; a++) st(p, a * v5 + site, v[a]);
@@ -89,10 +97,15 @@ template <int PREC, bool NT> struct BOut {
   __amdgpu_buffer_rsrc_t rs[4], rn;
   __device__ inline BOut(void *p, float *nrm, unsigned v5) {
     const unsigned plane = v5 * (unsigned)Store<PREC>::bytes;
+    if constexpr (PREC == P16) {  // [site][spin]: one resource over the field
+      rs[0] = __builtin_amdgcn_make_buffer_rsrc(p, (short)0, (int)(4u * plane), 0x00020000);
+      rs[1] = rs[2] = rs[3] = rs[0];
+    } else {
 #pragma unroll
-    for (int a = 0; a < 4; a++)
-      rs[a] = __builtin_amdgcn_make_buffer_rsrc((char *)p + (size_t)a * plane, (short)0, (int)plane,
-                                                0x00020000);
+      for (int a = 0; a < 4; a++)
+        rs[a] = __builtin_amdgcn_make_buffer_rsrc((char *)p + (size_t)a * plane, (short)0,
+                                                  (int)plane, 0x00020000);
+    }
     rn = __builtin_amdgcn_make_buffer_rsrc(nrm, (short)0, nrm ? (int)(v5 * 4u) : 0, 0x00020000);
   }
   __device__ inline void store4(unsigned site, const cx<C> v[4]) const {
@@ -116,11 +129,11 @@ template <int PREC, bool NT> struct BOut {
       for (int a = 0; a < 4; a++) m = fmaxf(m, fmaxf(fabsf(v[a].r), fabsf(v[a].i)));
       const float inv = m > 0.f ? 1.0f / m : 0.f;
       __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(m), rn, site * 4u, 0, aux);
+      __attribute__((ext_vector_type(4))) unsigned q;
 #pragma unroll
       for (int a = 0; a < 4; a++)
-        __builtin_amdgcn_raw_buffer_store_b32(
-            __builtin_bit_cast(unsigned, __floats2half2_rn(v[a].r * inv, v[a].i * inv)), rs[a], off,
-            0, aux);
+        q[a] = __builtin_bit_cast(unsigned, __floats2half2_rn(v[a].r * inv, v[a].i * inv));
+      __builtin_amdgcn_raw_buffer_store_b128(q, rs[0], site * 16u, 0, aux);
     }
   }
 };
@@ -152,9 +165,9 @@ template <int PREC> struct BAcc {
       const float2 v =
           __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rs, off, a * plane, 0));
       return {v.x, v.y};
-    } else {
+    } else {  // [site][spin]
       const __half2 h =
-          __builtin_bit_cast(__half2, __builtin_amdgcn_raw_buffer_load_b32(rs, off, a * plane, 0));
+          __builtin_bit_cast(__half2, __builtin_amdgcn_raw_buffer_load_b32(rs, site * 16u + a * 4u, 0, 0));
       const float2 f = __half22float2(h);
       return {f.x * sc, f.y * sc};
     }
@@ -167,8 +180,20 @@ template <int PREC> struct BAcc {
   }
   __device__ inline void load4(unsigned site, cx<C> v[4]) const {
     const float sc = scale(site);
+    if constexpr (PREC == P16) {  // the site's 4 spins in one 128-bit load
+      const auto q = __builtin_amdgcn_raw_buffer_load_b128(rs, site * 16u, 0, 0);
 #pragma unroll
-    for (int a = 0; a < 4; a++) v[a] = ld(site, a, sc);
+      for (int a = 0; a < 4; a++) {
+        // take the element as a value first: clang's bit_cast of an
+        // ext-vector element lvalue reads the vector's first element
+        const unsigned e = q[a];
+        const float2 f = __half22float2(__builtin_bit_cast(__half2, e));
+        v[a] = {f.x * sc, f.y * sc};
+      }
+    } else {
+#pragma unroll
+      for (int a = 0; a < 4; a++) v[a] = ld(site, a, sc);
+    }
   }
   __device__ inline cx<C> load1(unsigned site, unsigned a) const { return ld(site, a, scale(site)); }
 };

@@ -47,14 +47,17 @@ __device__ inline double fin_sum(const double *partial, int n) {
   return block_sum(v);
 }
 
+// element e = spin * V5 + site of one source's field (FP16 fields store the
+// spins of a site together: [site][spin])
 template <int P> struct VAcc {
   const TT<P> *p;
   const float *nrm;
   long v5;
   __device__ inline cx<CC<P>> ld(long e) const {
     if constexpr (P == P16) {
-      const float2 f = __half22float2(p[e]);
-      const float sc = nrm[e % v5];
+      const long site = e % v5;
+      const float2 f = __half22float2(p[4 * site + e / v5]);
+      const float sc = nrm[site];
       return {f.x * sc, f.y * sc};
     } else {
       return ldc(p, e);
@@ -316,7 +319,7 @@ __global__ __launch_bounds__(BLK) void k_conv(long v5, int nsrc, const TT<SP> *a
 #pragma unroll
   for (int k = 0; k < 4; k++) {
     if constexpr (SP == P16) {
-      const float2 f = __half22float2(a[off + k * v5 + i]);
+      const float2 f = __half22float2(a[off + 4 * i + k]);
       const float s = an[(long)src * v5 + i];
       vd[k] = {(double)(f.x * s), (double)(f.y * s)};
     } else {
@@ -332,7 +335,7 @@ __global__ __launch_bounds__(BLK) void k_conv(long v5, int nsrc, const TT<SP> *a
     const float inv = m > 0.f ? 1.f / m : 0.f;
     bn[(long)src * v5 + i] = m;
 #pragma unroll
-    for (int k = 0; k < 4; k++) b[off + k * v5 + i] = __floats2half2_rn(vf[k].r * inv, vf[k].i * inv);
+    for (int k = 0; k < 4; k++) b[off + 4 * i + k] = __floats2half2_rn(vf[k].r * inv, vf[k].i * inv);
   } else if constexpr (DP == P32) {
 #pragma unroll
     for (int k = 0; k < 4; k++) st(b, off + k * v5 + i, vf[k]);
