@@ -72,31 +72,33 @@ def make_fields(G, DA, dims, beta, seed):
     return th, l64, l32
 
 
-def time_dslash(G, hr, dims, links, prec, steps, warmup, nbuf, stream):
+def time_dslash(G, hr, dims, links, prec, steps, warmup, nbuf, stream, nrhs=1, am=0.005):
+    """Mean per-launch time (HIP events around each launch on `stream`) and the
+    whole region's time per launch; `nrhs` spinors per launch, masses `am`."""
     lx, ly, lt, ls = dims
     v5 = lx * ly * lt * ls
     rng = np.random.default_rng(1)
     ins, outs, inn, outn = [], [], [], []
     for _ in range(nbuf):
-        x = (rng.standard_normal((4, v5, 2)) / np.sqrt(2)).astype(np.float32)
+        x = (rng.standard_normal((nrhs, 4, v5, 2)) / np.sqrt(2)).astype(np.float32)
         if prec == "f16":  # [site][spin] with a per-site scale
-            xs = np.ascontiguousarray(x.transpose(1, 0, 2))
-            sc = np.abs(xs).max(axis=(1, 2)).astype(np.float32)
-            ins.append(hr.DeviceArray.from_host((xs / sc[:, None, None]).astype(np.float16)))
+            xs = np.ascontiguousarray(x.transpose(0, 2, 1, 3))
+            sc = np.abs(xs).max(axis=(2, 3)).astype(np.float32)
+            ins.append(hr.DeviceArray.from_host((xs / sc[:, :, None, None]).astype(np.float16)))
             inn.append(hr.DeviceArray.from_host(sc))
-            outn.append(hr.DeviceArray((v5,), np.float32))
-            outs.append(hr.DeviceArray((v5, 4, 2), np.float16))
+            outn.append(hr.DeviceArray((nrhs, v5), np.float32))
+            outs.append(hr.DeviceArray((nrhs, v5, 4, 2), np.float16))
         else:
             dt = np.float64 if prec == "f64" else np.float32
             ins.append(hr.DeviceArray.from_host(x.astype(dt)))
-            outs.append(hr.DeviceArray((4, v5, 2), dt))
+            outs.append(hr.DeviceArray((nrhs, 4, v5, 2), dt))
             inn.append(None)
             outn.append(None)
 
     def launch(k):
         j = k % nbuf
-        G.dslash_dev(outs[j], ins[j], links, dims, 0.005, imass=3, am3=1.0, dagger=bool(k & 1),
-                     prec=prec, in_norm=inn[j], out_norm=outn[j], stream=stream)
+        G.dslash_dev(outs[j], ins[j], links, dims, am, imass=3, am3=1.0, dagger=bool(k & 1),
+                     prec=prec, in_norm=inn[j], out_norm=outn[j], nrhs=nrhs, stream=stream)
 
     for k in range(warmup):
         launch(k)
@@ -111,7 +113,35 @@ def time_dslash(G, hr, dims, links, prec, steps, warmup, nbuf, stream):
     t1.record(stream)
     stream.synchronize()
     per = np.array([a.elapsed_ms(b) for a, b in ev])
+    for a in ins + outs + inn + outn:
+        if a is not None:
+            a.free()
     return float(np.mean(per)), float(t0.elapsed_ms(t1) / steps)
+
+
+def multi_rhs_legs(G, hr, steps, warmup, stream):
+    """FP64 Dslash over several right-hand sides per launch (links read once
+    per launch): 4 sources at 32^3 x Ls=64, and 16 sources with masses drawn
+    from {0.005, 0.01, 0.02, 0.05} at 24^3 x Ls=32."""
+    out = {}
+    rng = np.random.default_rng(7)
+    for dims, nrhs in (((32, 32, 32, 64), 4), ((24, 24, 24, 32), 16)):
+        lx, ly, lt, ls = dims
+        v4, v5 = lx * ly * lt, lx * ly * lt * ls
+        _, l64, l32 = make_fields(G, hr.DeviceArray, dims, 0.3, 99)
+        G.dslash_autotune(l64, dims, "f64", stream=stream)
+        tc, fl = G.dslash_get_config(dims, "f64")
+        am = rng.choice([0.005, 0.01, 0.02, 0.05], size=nrhs).astype(np.float32)
+        nb = max(2, int(np.ceil(4 * 256e6 / (2 * 16 * 4 * v5 * nrhs))))
+        progress(f"dslash f64 {dims} nrhs={nrhs}")
+        ms, _ = time_dslash(G, hr, dims, l64, "f64", steps, warmup, nb, stream, nrhs=nrhs, am=am)
+        gbs = dslash_bytes(v4, v5, "f64", nrhs) / (ms * 1e-3) / 1e9
+        out["x".join(map(str, dims)) + f"_nrhs{nrhs}"] = {
+            "ms": round(ms, 5), "GB_s": round(gbs, 1), "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4),
+            "launch_shape": {"tchunk": tc, "flags": fl}}
+        l64.free()
+        l32.free()
+    return out
 
 
 def bench_multishift(G, hr, dims, l64, l32, am, nsrc, stream, tol=1e-10, maxcg=50000,
@@ -241,6 +271,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solver", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-multi-rhs", action="store_true")
     ap.add_argument("--serve-rate", type=float, default=2.0)
     ap.add_argument("--serve-seconds", type=float, default=15.0)
     ap.add_argument("--serve-slots", type=int, default=4)
@@ -303,6 +334,8 @@ def main():
         ms, _ = time_dslash(G, hr, dims, l32, prec, args.steps, args.warmup, nb, stream)
         extra[prec] = {"ms": round(ms, 5),
                        "GB_s": round(dslash_bytes(v4, v5, prec) / (ms * 1e-3) / 1e9, 1)}
+    multi = None if args.no_multi_rhs else multi_rhs_legs(G, hr, max(20, args.steps // 4),
+                                                           args.warmup, stream)
     progress("device copy rate")
     copy_gbs = copy_rate(G, hr, stream)
     solver = None
@@ -353,6 +386,7 @@ def main():
         "frac_of_copy": round(gbs64 / copy_gbs, 4),
         "cpu_baseline": cpu,
         "dslash_other_precisions": extra,
+        "dslash_multi_rhs": multi,
         "multishift": solver,
         "serving": serving,
         "bench_wall_s": round(wall, 2),

@@ -142,6 +142,28 @@ int thr_hamilton(const double *phi, float res2, float am, int imass, float am3, 
                  const double *anum4, const double *aden4, const double *bnum2,
                  const double *bden2, int ndiag, int max_qmr_iters, double *out, int *itercg);
 
+/* Source of the uniform deviates that end a molecular-dynamics trajectory:
+ * returns the caller's next rano(yran, idum, 1, 1, 1) (evolution.F90:159-161). */
+typedef float (*thr_uniform_fn)(void *ctx);
+
+/* evolution.F90:113-179  evolve_theta_pp(iterl, dt, pp, phi, res, am, imass, itot)
+ * for Nf = 1 (rational branch of force):
+ *   force; pp -= real(0.5*dt*dSdpi);
+ *   for iter = 1..4*iterl: theta += dt*pp; u = coef(theta); force;
+ *     ytest = ytest_fn(ctx); if ytest < 1/iterl: pp -= real(0.5*dt*dSdpi), stop;
+ *     else pp -= real(dt*dSdpi).
+ * theta (trial::theta, (Lx,Ly,Lt,3) float) and pp ((Lx,Ly,Lt,3) float) are
+ * updated in place; u (trial::u, haloed as thr_coef writes it) is rewritten
+ * from the final theta; dsdpi receives the last force; *itot is increased by
+ * min(iter, 4*iterl) (178).  The force arguments are those of thr_force;
+ * ytest_fn is called once per MD step, in the reference's order. */
+int thr_evolve_theta_pp(int iterl, float dt, float *pp, const double *phi, float res, float am,
+                        int imass, float am3, float beta, int ibound, float *theta, double *u,
+                        const int dims[4], const double *anum4g, const double *aden4g,
+                        const double *bnum2g, const double *bden2g, const double *anum2g,
+                        const double *aden2g, int ndiagg, int use_sp, int max_qmr_iters,
+                        thr_uniform_fn ytest_fn, void *ctx, double *dsdpi, int *itot);
+
 /* ---- (2) device-resident entry points ------------------------------------ */
 
 /* links from theta on device (gauge.F90:12-43); links64 [3][Lt][Ly][Lx] complex128 */
@@ -239,6 +261,19 @@ int thr_multishift_fast_dev(const void *links64, const void *links32, const int 
 int thr_true_residuals_dev(const void *links64, const int dims[4], int nsrc, const float *am,
                            int imass, float am3, int nshift, const double *aden, const void *b,
                            const void *x, double *relres, void *stream);
+
+/* evolution.F90:113-179  thr_evolve_theta_pp on device-resident fields: phi
+ * [4][Lt][Ly][Lx][Ls] complex128 (halo-free), theta and pp [3][Lt][Ly][Lx]
+ * float, links64 [3][Lt][Ly][Lx] complex128 (rewritten by coef each step),
+ * dsdpi [3][Lt][Ly][Lx] float64 (the last force); coefficients are host
+ * arrays.  Returns when the trajectory has ended and its work is complete. */
+int thr_evolve_theta_pp_dev(int iterl, float dt, float *pp, const void *phi, float res, float am,
+                            int imass, float am3, float beta, int ibound, float *theta,
+                            void *links64, const int dims[4], const double *anum4g,
+                            const double *aden4g, const double *bnum2g, const double *bden2g,
+                            const double *anum2g, const double *aden2g, int ndiagg, int use_sp,
+                            int max_qmr_iters, thr_uniform_fn ytest_fn, void *ctx,
+                            double *dsdpi, int *itot, void *stream);
 
 #ifdef __cplusplus
 }

@@ -245,6 +245,39 @@ def force(phi, res1, am, imass, beta, theta, u, dims, coeffs_g, ibound=-1, am3=1
     return dsdpi
 
 
+def evolve_theta_pp(iterl, dt, pp, phi, res, am, imass, beta, theta, dims, coeffs_g, ytest,
+                    ibound=-1, am3=1.0, use_sp=False, max_qmr_iters=50000):
+    """evolution.F90:113-179 (Nf = 1) composed from the oracle's force and coef.
+
+    Half step of pp (126-130); then per MD step theta += dt*pp in float32
+    (152), u = coef(theta) (155), force (156); ytest() < 1/iterl ends the
+    trajectory with a closing half step (159-168), else a full step (170).
+    pp updates are real(c*dSdpi) with c = 0.5*dt or dt a float32 product.
+    Returns (theta, pp, u, dSdpi, steps) with steps = min(iter, 4*iterl) (178).
+    """
+    f32 = np.float32
+    theta = np.array(theta, dtype=np.float32, order="F")
+    pp = np.array(pp, dtype=np.float32, order="F")
+    half, full = np.float64(f32(0.5) * f32(dt)), np.float64(f32(dt))
+    u = coef(theta, dims, ibound)
+    ds = force(phi, res, am, imass, beta, theta, u, dims, coeffs_g, ibound, am3, use_sp,
+               max_qmr_iters)
+    pp = np.asfortranarray(pp - (half * ds).astype(np.float32))
+    proby = f32(1.0) / f32(iterl)
+    steps = 4 * iterl
+    for it in range(1, 4 * iterl + 1):
+        theta = np.asfortranarray(theta + f32(dt) * pp)
+        u = coef(theta, dims, ibound)
+        ds = force(phi, res, am, imass, beta, theta, u, dims, coeffs_g, ibound, am3, use_sp,
+                   max_qmr_iters)
+        if f32(ytest()) < proby:
+            pp = np.asfortranarray(pp - (half * ds).astype(np.float32))
+            steps = it
+            break
+        pp = np.asfortranarray(pp - (full * ds).astype(np.float32))
+    return theta, pp, u, ds, steps
+
+
 def hamilton(phi, res2, am, imass, beta, theta, pp, u, dims, coeffs, am3=1.0,
              max_qmr_iters=50000):
     """coeffs = (anum4, aden4, bnum2, bden2); returns ({h,hg,hp,s,hf}, (it1, it2))."""

@@ -83,3 +83,142 @@ def test_hamilton(G):
     assert hg_["hg"] == ho_["hg"] and hg_["hp"] == ho_["hp"]
     for k in ("h", "hf", "s"):
         assert abs(hg_[k] - ho_[k]) <= tol * abs(ho_[k]), (k, hg_[k], ho_[k])
+
+
+def _md_coeffs():
+    (a4g, d4g), _ = remez_file("remez4g")
+    (a2g, d2g), (b2g, e2g) = remez_file("remez2g")
+    return (a4g, d4g, b2g, e2g, a2g, d2g)
+
+
+def test_evolve_theta_pp(G):
+    """evolution.F90:113-179 on the device against the oracle's composition:
+    the same uniforms (site (1,1,1) of the reference's site generator) end both
+    trajectories at the same step; theta and pp agree to float32 rounding of
+    forces that agree to 1e-10; the links are coef(theta) of the final theta."""
+    from site_random import SiteRandom
+    dims = (4, 4, 4, 8)
+    lx, ly, lt, ls = dims
+    th, pp, u, phi = fields(dims, 3)
+    th = np.asfortranarray(th)
+    pp = np.asfortranarray(pp)
+    coeffs = _md_coeffs()
+    iterl, dt, res = 3, np.float32(0.05), np.float32(1e-9)
+    ra, rb = SiteRandom(lx, ly, lt, 11.0), SiteRandom(lx, ly, lt, 11.0)
+    tho, ppo, uo, fo, ko = O.evolve_theta_pp(iterl, dt, pp, phi, res, 0.05, 3, 0.3, th, dims,
+                                             coeffs, lambda: ra.rano_at(0, 0, 0))
+    thg, ppg, ug = th.copy(order="F"), pp.copy(order="F"), u.copy(order="F")
+    fg, kg = G.evolve_theta_pp(iterl, dt, ppg, phi, res, 0.05, 3, 0.3, thg, ug, dims, coeffs,
+                               lambda: rb.rano_at(0, 0, 0))
+    print("steps", kg, ko, flush=True)
+    assert kg == ko >= 1
+    assert np.max(np.abs(thg - tho)) <= 1e-5
+    assert np.max(np.abs(ppg - ppo)) <= 1e-5
+    assert rel(fg, fo) <= 1e-7
+    assert np.array_equal(ug, O.coef(thg, dims))
+    # both generators were drawn from the same number of times
+    assert ra.rano_at(0, 0, 0) == rb.rano_at(0, 0, 0)
+
+
+def test_evolve_theta_pp_dev_matches_host(G):
+    """The device-resident entry point runs the same kernels as the host-array
+    one: identical theta, pp and force for the same inputs."""
+    from hiprt import DeviceArray
+    dims = (4, 4, 4, 8)
+    lx, ly, lt, ls = dims
+    th, pp, u, phi = fields(dims, 4)
+    th = np.asfortranarray(th)
+    pp = np.asfortranarray(pp)
+    coeffs = _md_coeffs()
+    stops = [np.float32(0.9), np.float32(0.8), np.float32(0.1)]
+    seq = iter(stops)
+    thh, pph, uh = th.copy(order="F"), pp.copy(order="F"), u.copy(order="F")
+    fh, kh = G.evolve_theta_pp(2, np.float32(0.05), pph, phi, np.float32(1e-9), 0.05, 3, 0.3,
+                               thh, uh, dims, coeffs, lambda: next(seq))
+    assert kh == 3
+    thd = DeviceArray.from_host(th.reshape(-1, order="F"))
+    ppd = DeviceArray.from_host(pp.reshape(-1, order="F"))
+    l64 = DeviceArray((lx * ly * lt * 3, 2), np.float64)
+    G.coef_dev(l64, thd, dims)
+    v = O.interior(phi).reshape(-1, order="F")
+    phid = DeviceArray.from_host(np.stack([v.real, v.imag], axis=-1))
+    fd = DeviceArray((3 * lx * ly * lt,), np.float64)
+    seq = iter(stops)
+    kd = G.evolve_theta_pp_dev(2, np.float32(0.05), ppd, phid, np.float32(1e-9), 0.05, 3, 0.3,
+                               thd, l64, dims, coeffs, lambda: next(seq), fd)
+    assert kd == 3
+    assert np.array_equal(thd.to_host(), thh.reshape(-1, order="F"))
+    assert np.array_equal(ppd.to_host(), pph.reshape(-1, order="F"))
+    assert np.array_equal(fd.to_host(), fh.reshape(-1, order="F"))
+
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _e2e_inputs():
+    """The reference's end-to-end sample: con_4x4x4 (theta and seed) and
+    midout (dt 0.14, beta 0.4, am3 1, am 0.01, imass 3, iterl 7), Ls = 8
+    (tests/e2e_tests/README.md, build_test_dir.sh defaults)."""
+    import confio
+    th, seed, _ = confio.read_con(os.path.join(GOLD, "con_4x4x4_serial"), 4, 4, 4)
+    with open(os.path.join(GOLD, "midout_e2e")) as fh:
+        fields = fh.readline().replace("<ITER2>", "100").split()
+    mid = dict(dt=float(fields[0]), beta=float(fields[1]), am3=float(fields[2]),
+               am=float(fields[3]), imass=int(fields[4]), iterl=int(fields[5]))
+    return th, seed, mid, (4, 4, 4, 8)
+
+
+def test_trajectory_against_oracle(G):
+    """Two RHMC trajectories (dwf3d_lib.F90:263-318) from the reference's
+    sample configuration and seed, GPU library vs the oracle behind the same
+    trajectory code: same MD step counts and accept decisions, dH within
+    the acceptance solver's tolerance, theta within float32 rounding."""
+    import md
+    from md_oracle_ops import OracleOps
+    from site_random import SiteRandom
+    th0, seed, mid, dims = _e2e_inputs()
+    lx, ly, lt, ls = dims
+    runs = {}
+    for name, ops in (("gpu", G), ("oracle", OracleOps)):
+        rng = SiteRandom(lx, ly, lt, seed)
+        th, out = th0.copy(order="F"), []
+        for _ in range(2):
+            r = md.trajectory(th, rng, dims, mid["am"], mid["imass"], mid["beta"], mid["iterl"],
+                              mid["dt"], np.float32(1e-8), np.float32(1e-6), am3=mid["am3"],
+                              ops=ops)
+            th = r["theta"]
+            print(name, "H0", r["H0"]["h"], "H1", r["H1"]["h"], flush=True)
+            out.append((r["steps"], r["accepted"], r["dH"], th.copy()))
+        runs[name] = out
+    for (sg, ag, dg, tg), (so, ao, do, to) in zip(runs["gpu"], runs["oracle"]):
+        print("steps", sg, so, "accepted", ag, ao, "dH", dg, do, flush=True)
+        assert sg == so and ag == ao
+        assert abs(dg - do) <= 1e-5 + 1e-3 * abs(do)
+        assert np.max(np.abs(tg - to)) <= 1e-4
+
+
+def test_e2e_acceptance_and_exp_dH(G):
+    """The reference's end-to-end criteria (tests/e2e_tests/test_suite.py):
+    100 trajectories from its sample configuration and seed with its sample
+    midout, acceptance >= 0.8 and <exp(dH)> within its error of 1 (the error
+    as counters.F90:59-62 computes it)."""
+    import md
+    from site_random import SiteRandom
+    th, seed, mid, dims = _e2e_inputs()
+    rng = SiteRandom(4, 4, 4, seed)
+    c = md.coefficient_sets()
+    ys, acc, n = [], 0, 100
+    for k in range(n):
+        r = md.trajectory(th, rng, dims, mid["am"], mid["imass"], mid["beta"], mid["iterl"],
+                          mid["dt"], np.float32(1e-8), np.float32(1e-6), coeffs=c, am3=mid["am3"])
+        th = r["theta"]
+        ys.append(float(np.exp(np.float32(r["dH"]))))
+        acc += r["accepted"]
+        if k % 20 == 19:
+            print("trajectories", k + 1, "accepted", acc, flush=True)
+    y = np.array(ys)
+    yav = y.sum() / n
+    err = np.sqrt(((y * y).sum() / n - yav * yav) / (n - 1))
+    print("acceptance", acc / n, "<exp-dH>", yav, "+/-", err, flush=True)
+    assert acc / n >= 0.8
+    assert abs(1.0 - yav) <= err

@@ -513,3 +513,91 @@ def test_measure_wilson_known_answer():
                                              SiteRandom(12, 12, 12, seed))
     assert abs(pbp - 5.4226685e-02) <= 1e-3 * 5.4226685e-02, pbp
     assert abs(aviter - 10.9) <= 1e-3 * 10.9, aviter
+
+
+def _md_setup(seed=61):
+    """Small lattice, rational approximations shared by force and hamilton (so
+    H is the action whose gradient the force is)."""
+    a0_4, a_4, s_4 = _remez_data("remez4", 0)
+    b0_2, b_2, t_2 = _remez_data("remez2", 1)
+    a0_2, a_2, s_2 = _remez_data("remez2", 0)
+    anum4, aden4 = np.concatenate([[a0_4], a_4]), s_4
+    bnum2, bden2 = np.concatenate([[b0_2], b_2]), t_2
+    anum2, aden2 = np.concatenate([[a0_2], a_2]), s_2
+    rng = np.random.default_rng(seed)
+    dims = (4, 4, 4, 4)
+    lx, ly, lt, ls = dims
+    th = np.asfortranarray((0.3 * rng.standard_normal((lx, ly, lt, 3))).astype(np.float32))
+    pp = np.asfortranarray(rng.standard_normal((lx, ly, lt, 3)).astype(np.float32))
+    phi = D.rand_spinor(dims, rng)
+    return dict(dims=dims, th=th, pp=pp, phi=phi, am=0.2, imass=3, beta=0.6,
+                ham=(anum4, aden4, bnum2, bden2),
+                frc=(anum4, aden4, bnum2, bden2, anum2, aden2), res=np.float32(1e-11))
+
+
+def _stop_after(n, iterl):
+    """ytest sequence that ends the trajectory after MD step n (evolution.F90:159-168)."""
+    seq = iter([np.float32(1.0)] * (n - 1) + [np.float32(0.5) / np.float32(iterl)])
+    return lambda: next(seq)
+
+
+def test_leapfrog_is_reversible():
+    """evolution.F90:113-179: half step, full steps, closing half step.  With
+    the momenta negated the integrator retraces its path (a misplaced half
+    step or a force taken at the wrong theta breaks this)."""
+    m = _md_setup()
+    dims, iterl, dt, n = m["dims"], 4, np.float32(0.1), 5
+    th1, pp1, _, _, k = O.evolve_theta_pp(iterl, dt, m["pp"], m["phi"], m["res"], m["am"],
+                                          m["imass"], m["beta"], m["th"], dims, m["frc"],
+                                          _stop_after(n, iterl))
+    assert k == n
+    assert np.max(np.abs(th1 - m["th"])) > 0.05  # it moved
+    th2, pp2, _, _, k2 = O.evolve_theta_pp(iterl, dt, -pp1, m["phi"], m["res"], m["am"],
+                                           m["imass"], m["beta"], th1, dims, m["frc"],
+                                           _stop_after(n, iterl))
+    assert k2 == n
+    assert np.max(np.abs(th2 - m["th"])) < 2e-5
+    assert np.max(np.abs(pp2 + m["pp"])) < 2e-5
+
+
+def test_leapfrog_energy_error_is_second_order():
+    """dH over a fixed trajectory length falls as dt^2 for the leapfrog
+    (evolution.F90:113-179 with hamilton, dwf3d_lib.F90:511-616): halving dt
+    divides dH by ~4.  A wrong force sign or factor leaves dH of order 1."""
+    m = _md_setup()
+    dims = m["dims"]
+
+    def dH(dt, n):
+        h0, _ = O.hamilton(m["phi"], m["res"], m["am"], m["imass"], m["beta"], m["th"], m["pp"],
+                           O.coef(m["th"], dims), dims, m["ham"])
+        th, pp, u, _, k = O.evolve_theta_pp(n, np.float32(dt), m["pp"], m["phi"], m["res"],
+                                            m["am"], m["imass"], m["beta"], m["th"], dims,
+                                            m["frc"], _stop_after(n, n))
+        assert k == n
+        h1, _ = O.hamilton(m["phi"], m["res"], m["am"], m["imass"], m["beta"], th, pp, u, dims,
+                           m["ham"])
+        return h1["h"] - h0["h"]
+
+    d1, d2 = dH(0.08, 5), dH(0.04, 10)
+    assert abs(d1) < 0.5
+    assert 3.0 < d1 / d2 < 5.0, (d1, d2)
+
+
+def test_site_random_single_site_stream():
+    """rano(yran, idum, 1, 1, 1) advances only the generator of site (1,1,1)
+    (site_random.f90:33-60): drawing from it leaves every other site's next
+    value unchanged, and that site's values follow its own stream."""
+    from site_random import SiteRandom
+    a, b = SiteRandom(4, 4, 4, 3.0), SiteRandom(4, 4, 4, 3.0)
+    seq = [a.rano_at(0, 0, 0) for _ in range(5)]
+    full = [b.rano()[0, 0, 0] for _ in range(5)]
+    assert seq == full
+    na, nb = a.rano(), b.rano()
+    assert na[0, 0, 0] == nb[0, 0, 0]
+    na[0, 0, 0] = nb[0, 0, 0] = 0
+    # b advanced all sites 5 times; a only site (0,0,0): the others differ
+    assert not np.array_equal(na, nb)
+    c = SiteRandom(4, 4, 4, 3.0)
+    nc = c.rano()
+    nc[0, 0, 0] = 0
+    assert np.array_equal(na, nc)

@@ -53,7 +53,7 @@ static long haloed4(const Dims &d) { return (long)(d.lx + 2) * (d.ly + 2) * (d.l
 
 // workspace slots of the host staging path
 enum {
-  W_H0 = 100, W_H1, W_H2, W_L, W_LH, W_A, W_B, W_C, W_D, W_X1, W_F, W_T
+  W_H0 = 100, W_H1, W_H2, W_L, W_LH, W_A, W_B, W_C, W_D, W_X1, W_F, W_T, W_P
 };
 
 static int stage_links(const Dims &d, const double *u, double2 **u64, hipStream_t s) {
@@ -171,6 +171,8 @@ int thr_context_bytes(unsigned long long *bytes) {
   return THR_OK;
 }
 
+static int links_to_host(const Dims &d, const double2 *u64, double *u, hipStream_t s);
+
 int thr_coef(double *u, const float *theta, const int dims[4], int ibound) {
   TRY(check_dims(dims));
   THR_CHECK(u && theta, THR_ERR_ARG, "NULL array");
@@ -181,7 +183,12 @@ int thr_coef(double *u, const float *theta, const int dims[4], int ibound) {
   THR_CHECK(th && u64, THR_ERR_HIP, "workspace allocation failed");
   THR_HIP(hipMemcpyAsync(th, theta, sizeof(float) * d.v4() * 3, hipMemcpyHostToDevice, s));
   TRY(launch_coef(d, th, ibound, u64, s));
-  // host halo of the link array: periodic copy of the interior
+  return links_to_host(d, u64, u, s);
+}
+
+// device links [mu][t][y][x] -> the haloed host array u(0:Lx+1,0:Ly+1,0:Lt+1,3)
+// with the periodic copy of the interior in the halo
+static int links_to_host(const Dims &d, const double2 *u64, double *u, hipStream_t s) {
   std::vector<double2> h(d.v4() * 3);
   THR_HIP(hipMemcpyAsync(h.data(), u64, sizeof(double2) * h.size(), hipMemcpyDeviceToHost, s));
   THR_HIP(hipStreamSynchronize(s));
@@ -402,6 +409,95 @@ int thr_force(const double *phi, float res1, float am, int imass, float am3, flo
   TRY(force_device(A, f, itercg, s));
   THR_HIP(hipMemcpyAsync(dsdpi, f, sizeof(double) * d.v4() * 3, hipMemcpyDeviceToHost, s));
   THR_HIP(hipStreamSynchronize(s));
+  return THR_OK;
+}
+
+static EvolveArgs evolve_args(int iterl, float dt, float res, float am, int imass, float am3,
+                              float beta, int ibound, const Dims &d, const double *anum4g,
+                              const double *aden4g, const double *bnum2g, const double *bden2g,
+                              const double *anum2g, const double *aden2g, int ndiagg, int use_sp,
+                              int max_qmr_iters, thr_uniform_fn ytest, void *ytest_ctx) {
+  EvolveArgs A{};
+  A.f.op = mkop(d, 0, imass, am, am3, nullptr);
+  A.f.res1 = res;
+  A.f.beta = beta;
+  A.f.ibound = ibound;
+  A.f.ndiagg = ndiagg;
+  A.f.use_sp = use_sp;
+  A.f.max_qmr_iters = max_qmr_iters;
+  A.f.anum4g = anum4g;
+  A.f.aden4g = aden4g;
+  A.f.bnum2g = bnum2g;
+  A.f.bden2g = bden2g;
+  A.f.anum2g = anum2g;
+  A.f.aden2g = aden2g;
+  A.iterl = iterl;
+  A.dt = dt;
+  A.ytest = ytest;
+  A.ytest_ctx = ytest_ctx;
+  return A;
+}
+
+int thr_evolve_theta_pp(int iterl, float dt, float *pp, const double *phi, float res, float am,
+                        int imass, float am3, float beta, int ibound, float *theta, double *u,
+                        const int dims[4], const double *anum4g, const double *aden4g,
+                        const double *bnum2g, const double *bden2g, const double *anum2g,
+                        const double *aden2g, int ndiagg, int use_sp, int max_qmr_iters,
+                        thr_uniform_fn ytest, void *ytest_ctx, double *dsdpi, int *itot) {
+  TRY(check_dims(dims));
+  TRY(check_op(0, imass));
+  THR_CHECK(pp && phi && theta && u && anum4g && aden4g && bnum2g && bden2g && anum2g && aden2g &&
+                ytest && dsdpi,
+            THR_ERR_ARG, "NULL argument");
+  THR_CHECK(iterl >= 1, THR_ERR_ARG, "iterl must be positive");
+  const Dims d = mkdims(dims);
+  const long n4 = d.v4() * 3;
+  hipStream_t s = default_stream();
+  double2 *u64;
+  TRY(stage_links(d, u, &u64, s));
+  double2 *a = (double2 *)workspace(W_A, sizeof(double2) * d.nspinor());
+  float *th = (float *)workspace(W_F, sizeof(float) * n4);
+  double *f = (double *)workspace(W_T, sizeof(double) * n4);
+  float *pd = (float *)workspace(W_P, sizeof(float) * n4);
+  THR_CHECK(a && th && f && pd, THR_ERR_HIP, "workspace allocation failed");
+  TRY(stage_in(d, phi, W_H0, a, s));
+  THR_HIP(hipMemcpyAsync(th, theta, sizeof(float) * n4, hipMemcpyHostToDevice, s));
+  THR_HIP(hipMemcpyAsync(pd, pp, sizeof(float) * n4, hipMemcpyHostToDevice, s));
+  EvolveArgs A = evolve_args(iterl, dt, res, am, imass, am3, beta, ibound, d, anum4g, aden4g,
+                             bnum2g, bden2g, anum2g, aden2g, ndiagg, use_sp, max_qmr_iters, ytest,
+                             ytest_ctx);
+  A.f.phi = a;
+  int steps = 0;
+  TRY(evolve_device(A, th, pd, u64, f, &steps, s));
+  THR_HIP(hipMemcpyAsync(theta, th, sizeof(float) * n4, hipMemcpyDeviceToHost, s));
+  THR_HIP(hipMemcpyAsync(pp, pd, sizeof(float) * n4, hipMemcpyDeviceToHost, s));
+  THR_HIP(hipMemcpyAsync(dsdpi, f, sizeof(double) * n4, hipMemcpyDeviceToHost, s));
+  TRY(links_to_host(d, u64, u, s));
+  if (itot) *itot += steps;
+  return THR_OK;
+}
+
+int thr_evolve_theta_pp_dev(int iterl, float dt, float *pp, const void *phi, float res, float am,
+                            int imass, float am3, float beta, int ibound, float *theta,
+                            void *links64, const int dims[4], const double *anum4g,
+                            const double *aden4g, const double *bnum2g, const double *bden2g,
+                            const double *anum2g, const double *aden2g, int ndiagg, int use_sp,
+                            int max_qmr_iters, thr_uniform_fn ytest, void *ytest_ctx,
+                            double *dsdpi, int *itot, void *stream) {
+  TRY(check_dims(dims));
+  TRY(check_op(0, imass));
+  THR_CHECK(pp && phi && theta && links64 && anum4g && aden4g && bnum2g && bden2g && anum2g &&
+                aden2g && ytest && dsdpi,
+            THR_ERR_ARG, "NULL argument");
+  THR_CHECK(iterl >= 1, THR_ERR_ARG, "iterl must be positive");
+  const Dims d = mkdims(dims);
+  EvolveArgs A = evolve_args(iterl, dt, res, am, imass, am3, beta, ibound, d, anum4g, aden4g,
+                             bnum2g, bden2g, anum2g, aden2g, ndiagg, use_sp, max_qmr_iters, ytest,
+                             ytest_ctx);
+  A.f.phi = (const double2 *)phi;
+  int steps = 0;
+  TRY(evolve_device(A, theta, pp, (double2 *)links64, dsdpi, &steps, pick(stream)));
+  if (itot) *itot += steps;
   return THR_OK;
 }
 

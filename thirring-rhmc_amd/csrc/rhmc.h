@@ -37,4 +37,21 @@ struct HamArgs {
 // dwf3d_lib.F90:511-616 (rational branch, Nf = 1): out = {h, hg, hp, s, hf}
 int hamilton_device(const HamArgs &A, double out[5], int itercg[2], hipStream_t s);
 
+// evolution.F90:113-179 (Nf = 1): a half step of pp, then leapfrog steps
+// theta += dt pp, links = coef(theta), force, pp -= dt dSdpi; after step k the
+// trajectory ends with a closing half step when ytest(ytest_ctx) < 1/iterl
+// (the reference draws rano(yran, idum, 1, 1, 1) there), else it runs 4 iterl
+// full steps.  theta, pp [3 V4] float, links [3 V4] and dsdpi [3 V4] are
+// device arrays updated in place (dsdpi holds the last force); *steps
+// receives min(k, 4 iterl).  f.op.u64 and f.theta are replaced by links and theta.
+struct EvolveArgs {
+  ForceArgs f;
+  int iterl;
+  float dt;
+  float (*ytest)(void *);
+  void *ytest_ctx;
+};
+int evolve_device(const EvolveArgs &A, float *theta, float *pp, double2 *links, double *dsdpi,
+                  int *steps, hipStream_t s);
+
 }  // namespace thr

@@ -3,6 +3,7 @@
 // qmrherm_device follows qmrherm_module.F90:20-194 (iflag 0..3) with every
 // field resident in HBM; force_device follows force_module.F90:9-113 (the
 // rational branch, Nf = 1) and hamilton_device dwf3d_lib.F90:511-616.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -56,6 +57,26 @@ __global__ __launch_bounds__(BLK) void k_force_finish(const Dims d, double *ds, 
   double v = ds[i];
   if (ibound == -1 && mu == 2 && t == d.lt - 1) v = -v;
   ds[i] = v + (double)(beta * 1.0f * theta[i]);
+}
+
+// pp = pp - real(c*dSdpi) (evolution.F90:130, 171-174): c = 0.5*dt or dt is a
+// default-real product, promoted to double by the real(dp) force, and the
+// difference is formed in float32
+__global__ __launch_bounds__(BLK) void k_pp_step(long n, float *pp, const double *ds, float c) {
+  const long i = (long)blockIdx.x * BLK + threadIdx.x;
+  if (i >= n) return;
+  pp[i] = pp[i] - (float)((double)c * ds[i]);
+}
+
+// thetat = thetat + dt*pp (evolution.F90:152), float32 with a separately
+// rounded product as the Fortran statement computes it
+__global__ __launch_bounds__(BLK) void k_theta_step(long n, float *theta, const float *pp,
+                                                    float dt) {
+#pragma clang fp contract(off)
+  const long i = (long)blockIdx.x * BLK + threadIdx.x;
+  if (i >= n) return;
+  const float step = dt * pp[i];
+  theta[i] = theta[i] + step;
 }
 
 }  // namespace
@@ -200,6 +221,42 @@ int hamilton_device(const HamArgs &A, double out[5], int itercg[2], hipStream_t 
     itercg[0] = it1;
     itercg[1] = it2;
   }
+  return THR_OK;
+}
+
+int evolve_device(const EvolveArgs &A, float *theta, float *pp, double2 *links, double *dsdpi,
+                  int *steps, hipStream_t s) {
+  const Dims &d = A.f.op.d;
+  const long n = d.v4() * 3;
+  THR_CHECK(A.iterl >= 1, THR_ERR_ARG, "iterl must be positive");
+  THR_CHECK(A.ytest, THR_ERR_ARG, "a uniform source for the trajectory end is required");
+  ForceArgs F = A.f;
+  F.op.u64 = links;
+  F.op.u32 = nullptr;
+  F.theta = theta;
+  const dim3 g(nblk(n));
+  // half step forward for p (126-130)
+  TRY_RC(force_device(F, dsdpi, nullptr, s));
+  hipLaunchKernelGGL(k_pp_step, g, dim3(BLK), 0, s, n, pp, dsdpi, 0.5f * A.dt);
+  THR_HIP(hipGetLastError());
+  const float proby = 1.0f / (float)A.iterl;
+  int iter = 1;
+  for (; iter <= 4 * A.iterl; iter++) {
+    hipLaunchKernelGGL(k_theta_step, g, dim3(BLK), 0, s, n, theta, pp, A.dt);  // (152)
+    THR_HIP(hipGetLastError());
+    TRY_RC(launch_coef(d, theta, F.ibound, links, s));  // (155)
+    TRY_RC(force_device(F, dsdpi, nullptr, s));          // (156)
+    const float ytest = A.ytest(A.ytest_ctx);            // (159-161)
+    if (ytest < proby) {
+      hipLaunchKernelGGL(k_pp_step, g, dim3(BLK), 0, s, n, pp, dsdpi, 0.5f * A.dt);  // (167)
+      THR_HIP(hipGetLastError());
+      break;
+    }
+    hipLaunchKernelGGL(k_pp_step, g, dim3(BLK), 0, s, n, pp, dsdpi, A.dt);  // (170)
+    THR_HIP(hipGetLastError());
+  }
+  THR_HIP(hipStreamSynchronize(s));
+  if (steps) *steps = std::min(iter, 4 * A.iterl);  // itot increment (178)
   return THR_OK;
 }
 

@@ -38,17 +38,21 @@ class SiteRandom:
         self.y = self._rranf()
         self.rano()
 
-    def _dranf(self):
-        dl = self.ds1 * _DM1
+    def _dranf(self, idx=Ellipsis):
+        """One step of the 48-bit generator of every site (idx = Ellipsis) or
+        of the one site idx = (x, y, t)."""
+        ds1, ds2 = self.ds1[idx], self.ds2[idx]
+        dl = ds1 * _DM1
         dc = np.trunc(dl / _DX24)
         dl = dl - dc * _DX24
-        du = self.ds1 * _DM2 + self.ds2 * _DM1 + dc
-        self.ds2 = du - np.trunc(du / _DX24) * _DX24
-        self.ds1 = dl
-        return (self.ds2 * _DX24 + self.ds1) / _DX48
+        du = ds1 * _DM2 + ds2 * _DM1 + dc
+        ds2 = du - np.trunc(du / _DX24) * _DX24
+        self.ds1[idx] = dl
+        self.ds2[idx] = ds2
+        return (ds2 * _DX24 + dl) / _DX48
 
-    def _rranf(self):
-        return self._dranf().astype(np.float32)
+    def _rranf(self, idx=Ellipsis):
+        return np.asarray(self._dranf(idx)).astype(np.float32)
 
     def rano(self):
         """One shuffled single-precision uniform per site (site_random.f90:33-60)."""
@@ -58,6 +62,19 @@ class SiteRandom:
         self.y = self.v[(j,) + tuple(ix)]
         out = self.y.copy()
         self.v[(j,) + tuple(ix)] = self._rranf()
+        return out
+
+    def rano_at(self, ix, iy, it):
+        """rano(y, i, ix, iy, it) for one site, 0-based (site_random.f90:33-60):
+        advances only that site's stream.  rano_at(0, 0, 0) is the
+        reference's rano(yran, idum, 1, 1, 1), which ends MD trajectories
+        (evolution.F90:159-161) and draws the Metropolis test
+        (dwf3d_lib.F90:309)."""
+        s = (ix, iy, it)
+        j = min(max(int(np.float32(97.0) * self.y[s]), 0), 96)
+        out = np.float32(self.v[(j,) + s])
+        self.y[s] = out
+        self.v[(j,) + s] = self._rranf(s)
         return out
 
     def _gauss(self, two):

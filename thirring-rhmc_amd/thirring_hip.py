@@ -22,6 +22,7 @@ I32 = ctypes.c_int
 F64 = ctypes.c_double
 P = ctypes.c_void_p
 IP = ctypes.POINTER(ctypes.c_int)
+UFN = ctypes.CFUNCTYPE(ctypes.c_float, ctypes.c_void_p)  # thr_uniform_fn
 
 PREC = {"f64": 0, "f32": 1, "f16": 2}
 KERNEL = {"shamir": 0, "wilson": 1}
@@ -46,6 +47,10 @@ _SIGS = {
     "thr_force": ([P, F32, F32, I32, F32, F32, I32, P, P, P, P, P, P, P, P, P, I32, I32, I32, P,
                    P], I32),
     "thr_hamilton": ([P, F32, F32, I32, F32, F32, P, P, P, P, P, P, P, P, I32, I32, P, P], I32),
+    "thr_evolve_theta_pp": ([I32, F32, P, P, F32, F32, I32, F32, F32, I32, P, P, P, P, P, P, P,
+                             P, P, I32, I32, I32, UFN, P, P, IP], I32),
+    "thr_evolve_theta_pp_dev": ([I32, F32, P, P, F32, F32, I32, F32, F32, I32, P, P, P, P, P, P,
+                                 P, P, P, I32, I32, I32, UFN, P, P, IP, P], I32),
     "thr_coef_dev": ([P, P, P, I32, P], I32),
     "thr_dslash_autotune": ([P, P, I32, IP, IP, P], I32),
     "thr_dslash_set_config": ([P, I32, I32, I32], I32),
@@ -255,6 +260,72 @@ def force(phi, res1, am, imass, beta, theta, u, dims, coeffs_g, ibound=-1, am3=1
                            _dims(dims), _p(a4), _p(d4), _p(b2), _p(e2), _p(a2), _p(d2), len(d4),
                            int(bool(use_sp)), int(max_qmr_iters), _p(dsdpi), _p(its)), "force")
     return dsdpi, its
+
+
+def _uniform_source(ytest):
+    """ctypes callback over a Python callable; an exception ends the
+    trajectory at once (the callback returns 0) and is re-raised afterwards."""
+    err = []
+
+    def cb(_ctx):
+        if err:
+            return 0.0
+        try:
+            return float(ytest())
+        except BaseException as e:  # re-raised by the caller of the C entry point
+            err.append(e)
+            return 0.0
+    return UFN(cb), err
+
+
+def evolve_theta_pp(iterl, dt, pp, phi, res, am, imass, beta, theta, u, dims, coeffs_g, ytest,
+                    ibound=-1, am3=1.0, use_sp=False, max_qmr_iters=50000):
+    """evolution.F90:113-179 (Nf = 1).  theta, pp ((Lx,Ly,Lt,3) float32,
+    Fortran order) and the haloed links u are updated in place; `ytest()`
+    returns the next uniform of the caller's generator (the reference's
+    rano(yran, idum, 1, 1, 1)), called once per MD step.  Returns
+    (dSdpi of the last step, number of MD steps)."""
+    init()
+    lx, ly, lt, ls = dims
+    for name, a, dt_ in (("theta", theta, np.float32), ("pp", pp, np.float32),
+                         ("u", u, np.complex128)):
+        if not (isinstance(a, np.ndarray) and a.dtype == dt_ and a.flags.f_contiguous):
+            raise TypeError(f"{name} must be a Fortran-ordered {np.dtype(dt_).name} array "
+                            "(updated in place)")
+    a4, d4, b2, e2, a2, d2 = [np.ascontiguousarray(c, dtype=np.float64) for c in coeffs_g]
+    dsdpi = np.zeros((lx, ly, lt, 3), dtype=np.float64, order="F")
+    itot = ctypes.c_int(0)
+    fn, err = _uniform_source(ytest)
+    rc = lib().thr_evolve_theta_pp(int(iterl), dt, _p(pp), _p(phi), res, am, int(imass), am3,
+                                   beta, int(ibound), _p(theta), _p(u), _dims(dims), _p(a4),
+                                   _p(d4), _p(b2), _p(e2), _p(a2), _p(d2), len(d4),
+                                   int(bool(use_sp)), int(max_qmr_iters), fn, None, _p(dsdpi),
+                                   ctypes.byref(itot))
+    if err:
+        raise err[0]
+    _check(rc, "evolve_theta_pp")
+    return dsdpi, itot.value
+
+
+def evolve_theta_pp_dev(iterl, dt, pp, phi, res, am, imass, beta, theta, links64, dims, coeffs_g,
+                        ytest, dsdpi, ibound=-1, am3=1.0, use_sp=False, max_qmr_iters=50000,
+                        stream=None):
+    """evolve_theta_pp on device buffers (theta, pp [3 V4] float32, links64
+    [3 V4] complex128, phi [4 V5] complex128, dsdpi [3 V4] float64); returns
+    the number of MD steps."""
+    init()
+    a4, d4, b2, e2, a2, d2 = [np.ascontiguousarray(c, dtype=np.float64) for c in coeffs_g]
+    itot = ctypes.c_int(0)
+    fn, err = _uniform_source(ytest)
+    rc = lib().thr_evolve_theta_pp_dev(int(iterl), dt, _p(pp), _p(phi), res, am, int(imass), am3,
+                                       beta, int(ibound), _p(theta), _p(links64), _dims(dims),
+                                       _p(a4), _p(d4), _p(b2), _p(e2), _p(a2), _p(d2), len(d4),
+                                       int(bool(use_sp)), int(max_qmr_iters), fn, None, _p(dsdpi),
+                                       ctypes.byref(itot), _stream(stream))
+    if err:
+        raise err[0]
+    _check(rc, "evolve_theta_pp_dev")
+    return itot.value
 
 
 def hamilton(phi, res2, am, imass, beta, theta, pp, u, dims, coeffs, am3=1.0,
