@@ -21,7 +21,9 @@
  *
  * The reference's psibarpsi known answers (tests/test_measure_shamir.F90,
  * test_measure_wilson.F90) pin dslashd, congrad and the imass = 3 wall-mass
- * sign of both kernels.
+ * sign of both kernels.  The leapfrog composed over force in oracle.py
+ * (evolution.F90:113-179) is pinned by reversibility and by the dt^2 scaling
+ * of the energy error.
  *
  * Parity unpinned: the absolute sign of the imass = 1 and 5 wall-mass terms
  * (adjointness fixes M against M^+, not the sign itself; a restatement of
