@@ -128,19 +128,23 @@ int thr_qmrherm(const double *phi, double *x, float res, int *itercg, float am, 
  * (anum4g/aden4g, bnum2g/bden2g, anum2g/aden2g, remezg), the antiperiodic
  * flip of dSdpi(:,:,Lt,3) when ibound == -1, plus beta*Nf*theta.  theta is
  * trial::theta (Lx,Ly,Lt,3) float; dsdpi is overwritten; itercg[4] (may be
- * NULL) receives the four qmrherm iteration counts. */
+ * NULL) receives the four qmrherm iteration counts.  kernel 0 = Shamir, 1 =
+ * Wilson kernel (the GENERATE_WITH_* switch; single precision is not used with
+ * the Wilson kernel, qmrherm_module.F90:54-63). */
 int thr_force(const double *phi, float res1, float am, int imass, float am3, float beta,
               int ibound, const float *theta, const double *u, const int dims[4],
               const double *anum4g, const double *aden4g, const double *bnum2g,
               const double *bden2g, const double *anum2g, const double *aden2g, int ndiagg,
-              int use_sp, int max_qmr_iters, double *dsdpi, int *itercg);
+              int use_sp, int max_qmr_iters, int kernel, double *dsdpi, int *itercg);
 
 /* dwf3d_lib.F90:511-616  hamilton(Phi, h, hg, hp, s, res2, am, imass) for
- * Nf = 1 (rational branch); out = {h, hg, hp, s, hf}; itercg[2] (may be NULL) */
+ * Nf = 1 (rational branch); out = {h, hg, hp, s, hf}; itercg[2] (may be NULL);
+ * kernel as for thr_force */
 int thr_hamilton(const double *phi, float res2, float am, int imass, float am3, float beta,
                  const float *theta, const float *pp, const double *u, const int dims[4],
                  const double *anum4, const double *aden4, const double *bnum2,
-                 const double *bden2, int ndiag, int max_qmr_iters, double *out, int *itercg);
+                 const double *bden2, int ndiag, int max_qmr_iters, int kernel, double *out,
+                 int *itercg);
 
 /* Source of the uniform deviates that end a molecular-dynamics trajectory:
  * returns the caller's next rano(yran, idum, 1, 1, 1) (evolution.F90:159-161). */
@@ -162,7 +166,7 @@ int thr_evolve_theta_pp(int iterl, float dt, float *pp, const double *phi, float
                         const int dims[4], const double *anum4g, const double *aden4g,
                         const double *bnum2g, const double *bden2g, const double *anum2g,
                         const double *aden2g, int ndiagg, int use_sp, int max_qmr_iters,
-                        thr_uniform_fn ytest_fn, void *ctx, double *dsdpi, int *itot);
+                        int kernel, thr_uniform_fn ytest_fn, void *ctx, double *dsdpi, int *itot);
 
 /* ---- (2) device-resident entry points ------------------------------------ */
 
@@ -272,7 +276,7 @@ int thr_evolve_theta_pp_dev(int iterl, float dt, float *pp, const void *phi, flo
                             void *links64, const int dims[4], const double *anum4g,
                             const double *aden4g, const double *bnum2g, const double *bden2g,
                             const double *anum2g, const double *aden2g, int ndiagg, int use_sp,
-                            int max_qmr_iters, thr_uniform_fn ytest_fn, void *ctx,
+                            int max_qmr_iters, int kernel, thr_uniform_fn ytest_fn, void *ctx,
                             double *dsdpi, int *itot, void *stream);
 
 #ifdef __cplusplus

@@ -1145,7 +1145,7 @@ void orc_force(const double *phi, float res1, float am, int imass, float am3, fl
                int ibound, const float *theta, const double *u, const int dims[4],
                const double *anum4g, const double *aden4g, const double *bnum2g,
                const double *bden2g, const double *anum2g, const double *aden2g, int ndiagg,
-               int use_sp, int max_qmr_iters, double *dsdpi) {
+               int use_sp, int max_qmr_iters, int kernel, double *dsdpi) {
   geom g = mkgeom(dims);
   const long nh = (long)g.ls * g.lx * g.ly * g.lt * 4;
   const long n4 = (long)g.lx * g.ly * g.lt;
@@ -1157,15 +1157,15 @@ void orc_force(const double *phi, float res1, float am, int imass, float am3, fl
   const float One = 1.0f;
   memcpy(X2, phi, sizeof(zdp) * g.n5);
   orc_qmrherm(X2, Xr, res1, &it, One, 1, am3, anum4g, aden4g, ndiagg, 1, use_sp, u, dims,
-              max_qmr_iters, 0, x1, phi0, dsdpi, NULL);
+              max_qmr_iters, kernel, x1, phi0, dsdpi, NULL);
   memcpy(X2, Xr, sizeof(zdp) * g.n5);
   orc_qmrherm(X2, Xr, res1, &it, am, imass, am3, bnum2g, bden2g, ndiagg, 0, use_sp, u, dims,
-              max_qmr_iters, 0, x1, phi0, dsdpi, NULL);
+              max_qmr_iters, kernel, x1, phi0, dsdpi, NULL);
   memcpy(X2, Xr, sizeof(zdp) * g.n5);
   orc_qmrherm(X2, Xr, res1, &it, am, imass, am3, anum2g, aden2g, ndiagg, 2, use_sp, u, dims,
-              max_qmr_iters, 0, x1, phi0, dsdpi, NULL);
+              max_qmr_iters, kernel, x1, phi0, dsdpi, NULL);
   orc_qmrherm(X2, Xr, res1, &it, One, 1, am3, anum4g, aden4g, ndiagg, 3, use_sp, u, dims,
-              max_qmr_iters, 0, x1, phi0, dsdpi, NULL);
+              max_qmr_iters, kernel, x1, phi0, dsdpi, NULL);
   if (ibound == -1)
     for (int y = 1; y <= g.ly; y++)
       for (int x = 1; x <= g.lx; x++) dsdpi[IF4(g, x, y, g.lt, 2)] = -dsdpi[IF4(g, x, y, g.lt, 2)];
@@ -1183,7 +1183,7 @@ void orc_force(const double *phi, float res1, float am, int imass, float am3, fl
 void orc_hamilton(const double *phi, float res2, float am, int imass, float am3, float beta,
                   const float *theta, const float *pp, const double *u, const int dims[4],
                   const double *anum4, const double *aden4, const double *bnum2,
-                  const double *bden2, int ndiag, int max_qmr_iters, double out[5],
+                  const double *bden2, int ndiag, int max_qmr_iters, int kernel, double out[5],
                   int itercg_out[2]) {
   geom g = mkgeom(dims);
   const long nh = (long)g.ls * g.lx * g.ly * g.lt * 4;
@@ -1202,10 +1202,10 @@ void orc_hamilton(const double *phi, float res2, float am, int imass, float am3,
   int it1, it2;
   memcpy(R, phi, sizeof(zdp) * g.n5);
   orc_qmrherm(R, Xr, res2, &it1, 1.0f, 1, am3, anum4, aden4, ndiag, 0, 0, u, dims, max_qmr_iters,
-              0, x1, phi0, NULL, NULL);
+              kernel, x1, phi0, NULL, NULL);
   memcpy(R, Xr, sizeof(zdp) * g.n5);
   orc_qmrherm(R, Xr, res2, &it2, am, imass, am3, bnum2, bden2, ndiag, 0, 0, u, dims,
-              max_qmr_iters, 0, x1, phi0, NULL, NULL);
+              max_qmr_iters, kernel, x1, phi0, NULL, NULL);
   double hf = 0.0 + redot_int((const zdp *)R, (const zdp *)Xr, &g);
   out[0] = hg + hp + hf;
   out[1] = hg;

@@ -100,13 +100,13 @@ void orc_force(const double *phi, float res1, float am, int imass, float am3, fl
                int ibound, const float *theta, const double *u, const int dims[4],
                const double *anum4g, const double *aden4g, const double *bnum2g,
                const double *bden2g, const double *anum2g, const double *aden2g, int ndiagg,
-               int use_sp, int max_qmr_iters, double *dsdpi);
+               int use_sp, int max_qmr_iters, int kernel, double *dsdpi);
 /* Hamiltonian (dwf3d_lib.F90:511-616, rational branch, Nf = 1);
  * out = {h, hg, hp, s, hf} */
 void orc_hamilton(const double *phi, float res2, float am, int imass, float am3, float beta,
                   const float *theta, const float *pp, const double *u, const int dims[4],
                   const double *anum4, const double *aden4, const double *bnum2,
-                  const double *bden2, int ndiag, int max_qmr_iters, double out[5],
+                  const double *bden2, int ndiag, int max_qmr_iters, int kernel, double out[5],
                   int itercg_out[2]);
 
 #ifdef __cplusplus

@@ -64,9 +64,9 @@ def lib():
             "orc_qmrherm": ([P, P, F32, IP, F32, I32, F32, P, P, I32, I32, I32, P, P, I32, I32,
                              P, P, P, P], None),
             "orc_force": ([P, F32, F32, I32, F32, F32, I32, P, P, P, P, P, P, P, P, P, I32, I32,
-                           I32, P], None),
-            "orc_hamilton": ([P, F32, F32, I32, F32, F32, P, P, P, P, P, P, P, P, I32, I32, P, P],
-                             None),
+                           I32, I32, P], None),
+            "orc_hamilton": ([P, F32, F32, I32, F32, F32, P, P, P, P, P, P, P, P, I32, I32, I32, P,
+                              P], None),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -233,7 +233,7 @@ def qmrherm(phi, res, am, imass, anum, aden, iflag, u, dims, x1=None, phi0=None,
 
 
 def force(phi, res1, am, imass, beta, theta, u, dims, coeffs_g, ibound=-1, am3=1.0, use_sp=False,
-          max_qmr_iters=50000):
+          max_qmr_iters=50000, kernel="shamir"):
     """coeffs_g = (anum4g, aden4g, bnum2g, bden2g, anum2g, aden2g)."""
     lx, ly, lt, ls = dims
     a4, d4, b2, e2, a2, d2 = [np.ascontiguousarray(c, dtype=np.float64) for c in coeffs_g]
@@ -241,12 +241,12 @@ def force(phi, res1, am, imass, beta, theta, u, dims, coeffs_g, ibound=-1, am3=1
     th = np.asfortranarray(theta, dtype=np.float32)
     lib().orc_force(_p(phi), res1, am, int(imass), am3, beta, int(ibound), _p(th), _p(u),
                     _dims(dims), _p(a4), _p(d4), _p(b2), _p(e2), _p(a2), _p(d2), len(d4),
-                    int(bool(use_sp)), int(max_qmr_iters), _p(dsdpi))
+                    int(bool(use_sp)), int(max_qmr_iters), _kid(kernel), _p(dsdpi))
     return dsdpi
 
 
 def evolve_theta_pp(iterl, dt, pp, phi, res, am, imass, beta, theta, dims, coeffs_g, ytest,
-                    ibound=-1, am3=1.0, use_sp=False, max_qmr_iters=50000):
+                    ibound=-1, am3=1.0, use_sp=False, max_qmr_iters=50000, kernel="shamir"):
     """evolution.F90:113-179 (Nf = 1) composed from the oracle's force and coef.
 
     Half step of pp (126-130); then per MD step theta += dt*pp in float32
@@ -261,7 +261,7 @@ def evolve_theta_pp(iterl, dt, pp, phi, res, am, imass, beta, theta, dims, coeff
     half, full = np.float64(f32(0.5) * f32(dt)), np.float64(f32(dt))
     u = coef(theta, dims, ibound)
     ds = force(phi, res, am, imass, beta, theta, u, dims, coeffs_g, ibound, am3, use_sp,
-               max_qmr_iters)
+               max_qmr_iters, kernel)
     pp = np.asfortranarray(pp - (half * ds).astype(np.float32))
     proby = f32(1.0) / f32(iterl)
     steps = 4 * iterl
@@ -269,7 +269,7 @@ def evolve_theta_pp(iterl, dt, pp, phi, res, am, imass, beta, theta, dims, coeff
         theta = np.asfortranarray(theta + f32(dt) * pp)
         u = coef(theta, dims, ibound)
         ds = force(phi, res, am, imass, beta, theta, u, dims, coeffs_g, ibound, am3, use_sp,
-                   max_qmr_iters)
+                   max_qmr_iters, kernel)
         if f32(ytest()) < proby:
             pp = np.asfortranarray(pp - (half * ds).astype(np.float32))
             steps = it
@@ -279,7 +279,7 @@ def evolve_theta_pp(iterl, dt, pp, phi, res, am, imass, beta, theta, dims, coeff
 
 
 def hamilton(phi, res2, am, imass, beta, theta, pp, u, dims, coeffs, am3=1.0,
-             max_qmr_iters=50000):
+             max_qmr_iters=50000, kernel="shamir"):
     """coeffs = (anum4, aden4, bnum2, bden2); returns ({h,hg,hp,s,hf}, (it1, it2))."""
     a4, d4, b2, e2 = [np.ascontiguousarray(c, dtype=np.float64) for c in coeffs]
     th = np.asfortranarray(theta, dtype=np.float32)
@@ -288,6 +288,6 @@ def hamilton(phi, res2, am, imass, beta, theta, pp, u, dims, coeffs, am3=1.0,
     its = np.zeros(2, dtype=np.int32)
     lib().orc_hamilton(_p(phi), res2, am, int(imass), am3, beta, _p(th), _p(ppf), _p(u),
                        _dims(dims), _p(a4), _p(d4), _p(b2), _p(e2), len(d4), int(max_qmr_iters),
-                       _p(out), _p(its))
+                       _kid(kernel), _p(out), _p(its))
     keys = ("h", "hg", "hp", "s", "hf")
     return dict(zip(keys, out.tolist())), (int(its[0]), int(its[1]))

@@ -11,9 +11,10 @@ class OracleOps:
 
     @staticmethod
     def evolve_theta_pp(iterl, dt, pp, phi, res, am, imass, beta, theta, u, dims, coeffs_g, ytest,
-                        ibound=-1, am3=1.0, use_sp=False, max_qmr_iters=50000):
+                        ibound=-1, am3=1.0, use_sp=False, max_qmr_iters=50000, kernel="shamir"):
         th, p, uu, ds, k = O.evolve_theta_pp(iterl, dt, pp, phi, res, am, imass, beta, theta, dims,
-                                             coeffs_g, ytest, ibound, am3, use_sp, max_qmr_iters)
+                                             coeffs_g, ytest, ibound, am3, use_sp, max_qmr_iters,
+                                             kernel)
         theta[...] = th
         pp[...] = p
         u[...] = uu

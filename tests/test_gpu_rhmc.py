@@ -51,8 +51,9 @@ def rel(a, b):
     return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
 
 
-@pytest.mark.parametrize("use_sp", [False, True])
-def test_force(G, use_sp):
+@pytest.mark.parametrize("use_sp,kernel", [(False, "shamir"), (True, "shamir"),
+                                           (False, "wilson")])
+def test_force(G, use_sp, kernel):
     dims = (4, 4, 4, 8)
     th, pp, u, phi = fields(dims, 1)
     (a4g, d4g), (b4g, e4g) = remez_file("remez4g")
@@ -61,21 +62,22 @@ def test_force(G, use_sp):
     # (block 2 of remez2g), anum2g/aden2g (block 1 of remez2g)
     coeffs = (a4g, d4g, b2g, e2g, a2g, d2g)
     res1 = 1e-8
-    fg, its = G.force(phi, res1, 0.05, 3, 0.3, th, u, dims, coeffs, use_sp=use_sp)
-    fo = O.force(phi, res1, 0.05, 3, 0.3, th, u, dims, coeffs, use_sp=use_sp)
+    fg, its = G.force(phi, res1, 0.05, 3, 0.3, th, u, dims, coeffs, use_sp=use_sp, kernel=kernel)
+    fo = O.force(phi, res1, 0.05, 3, 0.3, th, u, dims, coeffs, use_sp=use_sp, kernel=kernel)
     e = rel(fg, fo)
     assert e <= (1e-4 if use_sp else 1e-10), e
 
 
-def test_hamilton(G):
+@pytest.mark.parametrize("kernel", ["shamir", "wilson"])
+def test_hamilton(G, kernel):
     dims = (4, 4, 4, 8)
     th, pp, u, phi = fields(dims, 2)
     (a4, d4), (b4, e4) = remez_file("remez4")
     (a2, d2), (b2, e2) = remez_file("remez2")
     # dwf3d_lib.F90:584-590: anum4/aden4 (block 1 of remez4), bnum2/bden2 (block 2 of remez2)
     coeffs = (a4, d4, b2, e2)
-    hg_, itg = G.hamilton(phi, 1e-9, 0.05, 3, 0.3, th, pp, u, dims, coeffs)
-    ho_, ito = O.hamilton(phi, 1e-9, 0.05, 3, 0.3, th, pp, u, dims, coeffs)
+    hg_, itg = G.hamilton(phi, 1e-9, 0.05, 3, 0.3, th, pp, u, dims, coeffs, kernel=kernel)
+    ho_, ito = O.hamilton(phi, 1e-9, 0.05, 3, 0.3, th, pp, u, dims, coeffs, kernel=kernel)
     # solver stopping tests may flip at the threshold (dot products summed in
     # a different order); the fermion action then agrees at the solver tolerance
     assert all(abs(a - b) <= 2 for a, b in zip(itg, ito)), (itg, ito)
@@ -91,7 +93,8 @@ def _md_coeffs():
     return (a4g, d4g, b2g, e2g, a2g, d2g)
 
 
-def test_evolve_theta_pp(G):
+@pytest.mark.parametrize("kernel", ["shamir", "wilson"])
+def test_evolve_theta_pp(G, kernel):
     """evolution.F90:113-179 on the device against the oracle's composition:
     the same uniforms (site (1,1,1) of the reference's site generator) end both
     trajectories at the same step; theta and pp agree to float32 rounding of
@@ -106,10 +109,10 @@ def test_evolve_theta_pp(G):
     iterl, dt, res = 3, np.float32(0.05), np.float32(1e-9)
     ra, rb = SiteRandom(lx, ly, lt, 11.0), SiteRandom(lx, ly, lt, 11.0)
     tho, ppo, uo, fo, ko = O.evolve_theta_pp(iterl, dt, pp, phi, res, 0.05, 3, 0.3, th, dims,
-                                             coeffs, lambda: ra.rano_at(0, 0, 0))
+                                             coeffs, lambda: ra.rano_at(0, 0, 0), kernel=kernel)
     thg, ppg, ug = th.copy(order="F"), pp.copy(order="F"), u.copy(order="F")
     fg, kg = G.evolve_theta_pp(iterl, dt, ppg, phi, res, 0.05, 3, 0.3, thg, ug, dims, coeffs,
-                               lambda: rb.rano_at(0, 0, 0))
+                               lambda: rb.rano_at(0, 0, 0), kernel=kernel)
     print("steps", kg, ko, flush=True)
     assert kg == ko >= 1
     assert np.max(np.abs(thg - tho)) <= 1e-5
@@ -168,7 +171,8 @@ def _e2e_inputs():
     return th, seed, mid, (4, 4, 4, 8)
 
 
-def test_trajectory_against_oracle(G):
+@pytest.mark.parametrize("kernel", ["shamir", "wilson"])
+def test_trajectory_against_oracle(G, kernel):
     """Two RHMC trajectories (dwf3d_lib.F90:263-318) from the reference's
     sample configuration and seed, GPU library vs the oracle behind the same
     trajectory code: same MD step counts and accept decisions, dH within
@@ -185,7 +189,7 @@ def test_trajectory_against_oracle(G):
         for _ in range(2):
             r = md.trajectory(th, rng, dims, mid["am"], mid["imass"], mid["beta"], mid["iterl"],
                               mid["dt"], np.float32(1e-8), np.float32(1e-6), am3=mid["am3"],
-                              ops=ops)
+                              ops=ops, kernel=kernel)
             th = r["theta"]
             print(name, "H0", r["H0"]["h"], "H1", r["H1"]["h"], flush=True)
             out.append((r["steps"], r["accepted"], r["dH"], th.copy()))

@@ -411,7 +411,8 @@ def _remez_data(name, block):
                   block)
 
 
-def test_force_is_derivative_of_hamilton():
+@pytest.mark.parametrize("kernel", ["shamir", "wilson"])
+def test_force_is_derivative_of_hamilton(kernel):
     """force_module.F90 against dwf3d_lib.F90 hamilton: with the same rational
     approximations in both, dSdpi - beta*theta is the theta-derivative of the
     pseudofermion action hf = <R, r2(K) R>, R = r4(K_PV) Phi (central
@@ -442,10 +443,11 @@ def test_force_is_derivative_of_hamilton():
     res = np.float32(1e-11)
 
     def hf(t32):
-        h, _ = O.hamilton(phi, res, am, imass, beta, t32, pp, O.coef(t32, dims), dims, ham)
+        h, _ = O.hamilton(phi, res, am, imass, beta, t32, pp, O.coef(t32, dims), dims, ham,
+                          kernel=kernel)
         return h["hf"]
 
-    f = O.force(phi, res, am, imass, beta, th, O.coef(th, dims), dims, frc)
+    f = O.force(phi, res, am, imass, beta, th, O.coef(th, dims), dims, frc, kernel=kernel)
     gauge = (np.float32(beta) * th).astype(np.float64)  # d/dtheta of hg = beta/2 sum theta^2
     for (xx, yy, tt, mu) in [(0, 0, 0, 0), (1, 2, 1, 1), (2, 1, 2, 2), (3, 3, lt - 1, 2),
                              (0, 3, lt - 1, 0)]:

@@ -375,9 +375,9 @@ int thr_force(const double *phi, float res1, float am, int imass, float am3, flo
               int ibound, const float *theta, const double *u, const int dims[4],
               const double *anum4g, const double *aden4g, const double *bnum2g,
               const double *bden2g, const double *anum2g, const double *aden2g, int ndiagg,
-              int use_sp, int max_qmr_iters, double *dsdpi, int *itercg) {
+              int use_sp, int max_qmr_iters, int kernel, double *dsdpi, int *itercg) {
   TRY(check_dims(dims));
-  TRY(check_op(0, imass));
+  TRY(check_op(kernel, imass));
   THR_CHECK(phi && theta && u && anum4g && aden4g && bnum2g && bden2g && anum2g && aden2g && dsdpi,
             THR_ERR_ARG, "NULL argument");
   const Dims d = mkdims(dims);
@@ -391,7 +391,7 @@ int thr_force(const double *phi, float res1, float am, int imass, float am3, flo
   TRY(stage_in(d, phi, W_H0, a, s));
   THR_HIP(hipMemcpyAsync(th, theta, sizeof(float) * d.v4() * 3, hipMemcpyHostToDevice, s));
   ForceArgs A{};
-  A.op = mkop(d, 0, imass, am, am3, u64);
+  A.op = mkop(d, kernel, imass, am, am3, u64);
   A.phi = a;
   A.theta = th;
   A.res1 = res1;
@@ -416,9 +416,10 @@ static EvolveArgs evolve_args(int iterl, float dt, float res, float am, int imas
                               float beta, int ibound, const Dims &d, const double *anum4g,
                               const double *aden4g, const double *bnum2g, const double *bden2g,
                               const double *anum2g, const double *aden2g, int ndiagg, int use_sp,
-                              int max_qmr_iters, thr_uniform_fn ytest, void *ytest_ctx) {
+                              int max_qmr_iters, int kernel, thr_uniform_fn ytest,
+                              void *ytest_ctx) {
   EvolveArgs A{};
-  A.f.op = mkop(d, 0, imass, am, am3, nullptr);
+  A.f.op = mkop(d, kernel, imass, am, am3, nullptr);
   A.f.res1 = res;
   A.f.beta = beta;
   A.f.ibound = ibound;
@@ -443,9 +444,10 @@ int thr_evolve_theta_pp(int iterl, float dt, float *pp, const double *phi, float
                         const int dims[4], const double *anum4g, const double *aden4g,
                         const double *bnum2g, const double *bden2g, const double *anum2g,
                         const double *aden2g, int ndiagg, int use_sp, int max_qmr_iters,
-                        thr_uniform_fn ytest, void *ytest_ctx, double *dsdpi, int *itot) {
+                        int kernel, thr_uniform_fn ytest, void *ytest_ctx, double *dsdpi,
+                        int *itot) {
   TRY(check_dims(dims));
-  TRY(check_op(0, imass));
+  TRY(check_op(kernel, imass));
   THR_CHECK(pp && phi && theta && u && anum4g && aden4g && bnum2g && bden2g && anum2g && aden2g &&
                 ytest && dsdpi,
             THR_ERR_ARG, "NULL argument");
@@ -464,8 +466,8 @@ int thr_evolve_theta_pp(int iterl, float dt, float *pp, const double *phi, float
   THR_HIP(hipMemcpyAsync(th, theta, sizeof(float) * n4, hipMemcpyHostToDevice, s));
   THR_HIP(hipMemcpyAsync(pd, pp, sizeof(float) * n4, hipMemcpyHostToDevice, s));
   EvolveArgs A = evolve_args(iterl, dt, res, am, imass, am3, beta, ibound, d, anum4g, aden4g,
-                             bnum2g, bden2g, anum2g, aden2g, ndiagg, use_sp, max_qmr_iters, ytest,
-                             ytest_ctx);
+                             bnum2g, bden2g, anum2g, aden2g, ndiagg, use_sp, max_qmr_iters, kernel,
+                             ytest, ytest_ctx);
   A.f.phi = a;
   int steps = 0;
   TRY(evolve_device(A, th, pd, u64, f, &steps, s));
@@ -482,18 +484,18 @@ int thr_evolve_theta_pp_dev(int iterl, float dt, float *pp, const void *phi, flo
                             void *links64, const int dims[4], const double *anum4g,
                             const double *aden4g, const double *bnum2g, const double *bden2g,
                             const double *anum2g, const double *aden2g, int ndiagg, int use_sp,
-                            int max_qmr_iters, thr_uniform_fn ytest, void *ytest_ctx,
+                            int max_qmr_iters, int kernel, thr_uniform_fn ytest, void *ytest_ctx,
                             double *dsdpi, int *itot, void *stream) {
   TRY(check_dims(dims));
-  TRY(check_op(0, imass));
+  TRY(check_op(kernel, imass));
   THR_CHECK(pp && phi && theta && links64 && anum4g && aden4g && bnum2g && bden2g && anum2g &&
                 aden2g && ytest && dsdpi,
             THR_ERR_ARG, "NULL argument");
   THR_CHECK(iterl >= 1, THR_ERR_ARG, "iterl must be positive");
   const Dims d = mkdims(dims);
   EvolveArgs A = evolve_args(iterl, dt, res, am, imass, am3, beta, ibound, d, anum4g, aden4g,
-                             bnum2g, bden2g, anum2g, aden2g, ndiagg, use_sp, max_qmr_iters, ytest,
-                             ytest_ctx);
+                             bnum2g, bden2g, anum2g, aden2g, ndiagg, use_sp, max_qmr_iters, kernel,
+                             ytest, ytest_ctx);
   A.f.phi = (const double2 *)phi;
   int steps = 0;
   TRY(evolve_device(A, theta, pp, (double2 *)links64, dsdpi, &steps, pick(stream)));
@@ -504,9 +506,10 @@ int thr_evolve_theta_pp_dev(int iterl, float dt, float *pp, const void *phi, flo
 int thr_hamilton(const double *phi, float res2, float am, int imass, float am3, float beta,
                  const float *theta, const float *pp, const double *u, const int dims[4],
                  const double *anum4, const double *aden4, const double *bnum2,
-                 const double *bden2, int ndiag, int max_qmr_iters, double *out, int *itercg) {
+                 const double *bden2, int ndiag, int max_qmr_iters, int kernel, double *out,
+                 int *itercg) {
   TRY(check_dims(dims));
-  TRY(check_op(0, imass));
+  TRY(check_op(kernel, imass));
   THR_CHECK(phi && theta && pp && u && anum4 && aden4 && bnum2 && bden2 && out, THR_ERR_ARG,
             "NULL argument");
   const Dims d = mkdims(dims);
@@ -517,7 +520,7 @@ int thr_hamilton(const double *phi, float res2, float am, int imass, float am3, 
   THR_CHECK(a, THR_ERR_HIP, "workspace allocation failed");
   TRY(stage_in(d, phi, W_H0, a, s));
   HamArgs A{};
-  A.op = mkop(d, 0, imass, am, am3, u64);
+  A.op = mkop(d, kernel, imass, am, am3, u64);
   A.phi = a;
   A.theta = theta;
   A.pp = pp;

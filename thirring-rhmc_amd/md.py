@@ -61,7 +61,8 @@ def initialise_pp(rng):
     return pp
 
 
-def initialise_phi_1flavour(rng, u, dims, am, imass, rescga, coeffs, am3=1.0, ops=G):
+def initialise_phi_1flavour(rng, u, dims, am, imass, rescga, coeffs, am3=1.0, ops=G,
+                            kernel="shamir"):
     """evolution.F90:7-74 (rational branch).  coeffs = (anum4, aden4, bnum4,
     bden4); returns (Phi haloed, (itercg of the two qmrherm calls))."""
     lx, ly, lt, ls = dims
@@ -72,31 +73,35 @@ def initialise_phi_1flavour(rng, u, dims, am, imass, rescga, coeffs, am3=1.0, op
             ps = rng.gauss0()
             r[ithird + 1, 1:-1, 1:-1, 1:-1, idirac] = ps[..., 0] + 1j * ps[..., 1]
     _periodic_halo(r)
-    x, it1, *_ = ops.qmrherm(r, rescga, am, imass, anum4, aden4, 0, u, dims, am3=am3)
-    phi, it2, *_ = ops.qmrherm(x, rescga, 1.0, 1, bnum4, bden4, 0, u, dims, am3=am3)
+    x, it1, *_ = ops.qmrherm(r, rescga, am, imass, anum4, aden4, 0, u, dims, am3=am3,
+                             kernel=kernel)
+    phi, it2, *_ = ops.qmrherm(x, rescga, 1.0, 1, bnum4, bden4, 0, u, dims, am3=am3,
+                               kernel=kernel)
     return phi, (it1, it2)
 
 
 def trajectory(theta, rng, dims, am, imass, beta, iterl, dt, rescga, rescgg, coeffs=None,
-               ibound=-1, am3=1.0, max_qmr_iters=50000, ops=G):
+               ibound=-1, am3=1.0, max_qmr_iters=50000, ops=G, kernel="shamir"):
     """One sweep of dwf3d_lib.F90:263-318.  theta (Lx,Ly,Lt,3) float32 is not
     modified; returns a dict with the new theta (the trial theta if accepted,
     else a copy of the old one), accepted, dH = H0 - H1, dS = S0 - S1, the MD
     step count, and H0/H1 (h, hg, hp, s, hf).  `ops` supplies coef, qmrherm,
     hamilton and evolve_theta_pp with this module's signatures (the GPU
-    library by default)."""
+    library by default); kernel "shamir" or "wilson" is the generation kernel
+    (GENERATE_WITH_SHAMIR / GENERATE_WITH_WILSON)."""
     c = coefficient_sets() if coeffs is None else coeffs
     thetat = np.array(theta, dtype=np.float32, order="F")
     u = ops.coef(thetat, dims, ibound)
-    phi, it_phi = initialise_phi_1flavour(rng, u, dims, am, imass, rescga, c["phi"], am3, ops)
+    phi, it_phi = initialise_phi_1flavour(rng, u, dims, am, imass, rescga, c["phi"], am3, ops,
+                                          kernel)
     pp = initialise_pp(rng)
     h0, _ = ops.hamilton(phi, rescga, am, imass, beta, thetat, pp, u, dims, c["hamilton"], am3,
-                         max_qmr_iters)
+                         max_qmr_iters, kernel=kernel)
     _, steps = ops.evolve_theta_pp(iterl, np.float32(dt), pp, phi, rescgg, am, imass, beta,
                                    thetat, u, dims, c["force"], lambda: rng.rano_at(0, 0, 0),
-                                   ibound, am3, max_qmr_iters=max_qmr_iters)
+                                   ibound, am3, max_qmr_iters=max_qmr_iters, kernel=kernel)
     h1, _ = ops.hamilton(phi, rescga, am, imass, beta, thetat, pp, u, dims, c["hamilton"], am3,
-                         max_qmr_iters)
+                         max_qmr_iters, kernel=kernel)
     dH = h0["h"] - h1["h"]
     dS = h0["s"] - h1["s"]
     y = np.exp(np.float32(dH))  # y = exp(real(dH)) (301)

@@ -44,13 +44,14 @@ _SIGS = {
     "thr_derivs": ([P, P, F64, I32, P, F32, I32, F32, I32, P], I32),
     "thr_qmrherm": ([P, P, F32, IP, F32, I32, F32, P, P, I32, I32, I32, P, P, I32, I32, P, P, P,
                      P], I32),
-    "thr_force": ([P, F32, F32, I32, F32, F32, I32, P, P, P, P, P, P, P, P, P, I32, I32, I32, P,
-                   P], I32),
-    "thr_hamilton": ([P, F32, F32, I32, F32, F32, P, P, P, P, P, P, P, P, I32, I32, P, P], I32),
+    "thr_force": ([P, F32, F32, I32, F32, F32, I32, P, P, P, P, P, P, P, P, P, I32, I32, I32, I32,
+                   P, P], I32),
+    "thr_hamilton": ([P, F32, F32, I32, F32, F32, P, P, P, P, P, P, P, P, I32, I32, I32, P, P],
+                     I32),
     "thr_evolve_theta_pp": ([I32, F32, P, P, F32, F32, I32, F32, F32, I32, P, P, P, P, P, P, P,
-                             P, P, I32, I32, I32, UFN, P, P, IP], I32),
+                             P, P, I32, I32, I32, I32, UFN, P, P, IP], I32),
     "thr_evolve_theta_pp_dev": ([I32, F32, P, P, F32, F32, I32, F32, F32, I32, P, P, P, P, P, P,
-                                 P, P, P, I32, I32, I32, UFN, P, P, IP, P], I32),
+                                 P, P, P, I32, I32, I32, I32, UFN, P, P, IP, P], I32),
     "thr_coef_dev": ([P, P, P, I32, P], I32),
     "thr_dslash_autotune": ([P, P, I32, IP, IP, P], I32),
     "thr_dslash_set_config": ([P, I32, I32, I32], I32),
@@ -248,7 +249,7 @@ def qmrherm(phi, res, am, imass, anum, aden, iflag, u, dims, x1=None, phi0=None,
 
 
 def force(phi, res1, am, imass, beta, theta, u, dims, coeffs_g, ibound=-1, am3=1.0, use_sp=False,
-          max_qmr_iters=50000):
+          max_qmr_iters=50000, kernel="shamir"):
     """force_module.F90:9-113; coeffs_g = (anum4g, aden4g, bnum2g, bden2g, anum2g, aden2g)."""
     init()
     lx, ly, lt, ls = dims
@@ -258,7 +259,8 @@ def force(phi, res1, am, imass, beta, theta, u, dims, coeffs_g, ibound=-1, am3=1
     its = np.zeros(4, dtype=np.int32)
     _check(lib().thr_force(_p(phi), res1, am, int(imass), am3, beta, int(ibound), _p(th), _p(u),
                            _dims(dims), _p(a4), _p(d4), _p(b2), _p(e2), _p(a2), _p(d2), len(d4),
-                           int(bool(use_sp)), int(max_qmr_iters), _p(dsdpi), _p(its)), "force")
+                           int(bool(use_sp)), int(max_qmr_iters), KERNEL[kernel], _p(dsdpi),
+                           _p(its)), "force")
     return dsdpi, its
 
 
@@ -279,7 +281,7 @@ def _uniform_source(ytest):
 
 
 def evolve_theta_pp(iterl, dt, pp, phi, res, am, imass, beta, theta, u, dims, coeffs_g, ytest,
-                    ibound=-1, am3=1.0, use_sp=False, max_qmr_iters=50000):
+                    ibound=-1, am3=1.0, use_sp=False, max_qmr_iters=50000, kernel="shamir"):
     """evolution.F90:113-179 (Nf = 1).  theta, pp ((Lx,Ly,Lt,3) float32,
     Fortran order) and the haloed links u are updated in place; `ytest()`
     returns the next uniform of the caller's generator (the reference's
@@ -299,8 +301,8 @@ def evolve_theta_pp(iterl, dt, pp, phi, res, am, imass, beta, theta, u, dims, co
     rc = lib().thr_evolve_theta_pp(int(iterl), dt, _p(pp), _p(phi), res, am, int(imass), am3,
                                    beta, int(ibound), _p(theta), _p(u), _dims(dims), _p(a4),
                                    _p(d4), _p(b2), _p(e2), _p(a2), _p(d2), len(d4),
-                                   int(bool(use_sp)), int(max_qmr_iters), fn, None, _p(dsdpi),
-                                   ctypes.byref(itot))
+                                   int(bool(use_sp)), int(max_qmr_iters), KERNEL[kernel], fn,
+                                   None, _p(dsdpi), ctypes.byref(itot))
     if err:
         raise err[0]
     _check(rc, "evolve_theta_pp")
@@ -309,7 +311,7 @@ def evolve_theta_pp(iterl, dt, pp, phi, res, am, imass, beta, theta, u, dims, co
 
 def evolve_theta_pp_dev(iterl, dt, pp, phi, res, am, imass, beta, theta, links64, dims, coeffs_g,
                         ytest, dsdpi, ibound=-1, am3=1.0, use_sp=False, max_qmr_iters=50000,
-                        stream=None):
+                        kernel="shamir", stream=None):
     """evolve_theta_pp on device buffers (theta, pp [3 V4] float32, links64
     [3 V4] complex128, phi [4 V5] complex128, dsdpi [3 V4] float64); returns
     the number of MD steps."""
@@ -320,8 +322,8 @@ def evolve_theta_pp_dev(iterl, dt, pp, phi, res, am, imass, beta, theta, links64
     rc = lib().thr_evolve_theta_pp_dev(int(iterl), dt, _p(pp), _p(phi), res, am, int(imass), am3,
                                        beta, int(ibound), _p(theta), _p(links64), _dims(dims),
                                        _p(a4), _p(d4), _p(b2), _p(e2), _p(a2), _p(d2), len(d4),
-                                       int(bool(use_sp)), int(max_qmr_iters), fn, None, _p(dsdpi),
-                                       ctypes.byref(itot), _stream(stream))
+                                       int(bool(use_sp)), int(max_qmr_iters), KERNEL[kernel], fn,
+                                       None, _p(dsdpi), ctypes.byref(itot), _stream(stream))
     if err:
         raise err[0]
     _check(rc, "evolve_theta_pp_dev")
@@ -329,7 +331,7 @@ def evolve_theta_pp_dev(iterl, dt, pp, phi, res, am, imass, beta, theta, links64
 
 
 def hamilton(phi, res2, am, imass, beta, theta, pp, u, dims, coeffs, am3=1.0,
-             max_qmr_iters=50000):
+             max_qmr_iters=50000, kernel="shamir"):
     """dwf3d_lib.F90:511-616; coeffs = (anum4, aden4, bnum2, bden2)."""
     init()
     a4, d4, b2, e2 = [np.ascontiguousarray(c, dtype=np.float64) for c in coeffs]
@@ -339,7 +341,7 @@ def hamilton(phi, res2, am, imass, beta, theta, pp, u, dims, coeffs, am3=1.0,
     its = np.zeros(2, dtype=np.int32)
     _check(lib().thr_hamilton(_p(phi), res2, am, int(imass), am3, beta, _p(th), _p(ppf), _p(u),
                               _dims(dims), _p(a4), _p(d4), _p(b2), _p(e2), len(d4),
-                              int(max_qmr_iters), _p(out), _p(its)), "hamilton")
+                              int(max_qmr_iters), KERNEL[kernel], _p(out), _p(its)), "hamilton")
     keys = ("h", "hg", "hp", "s", "hf")
     return dict(zip(keys, out.tolist())), (int(its[0]), int(its[1]))
 
