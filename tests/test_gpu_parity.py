@@ -322,14 +322,14 @@ def test_measure_known_answers(G):
     |psibarpsi| 5.4226685e-02 with 10.9 (the reference's value is negative:
     it predates the swap of the imass = 3 phases in measure_wilson).  The
     DWilson wall projection of the Wilson measurement runs on the oracle."""
+    import measure
     import measure_ref
     from site_random import SiteRandom
     dims = (12, 12, 12, 24)
     st = refstate.starting_state(dims, want=("u",))
     seed = float(np.float32(4139764973254.0))  # a default-real literal in the test
     res, am = np.float32(0.1), np.float32(0.05)
-    pbp, aviter = measure_ref.measure_shamir(G.dslashd, G.congrad, st["u"], dims, res, am, 3,
-                                             SiteRandom(12, 12, 12, seed))
+    pbp, aviter = measure.measure_shamir(st["u"], dims, res, am, 3, SiteRandom(12, 12, 12, seed))
     assert abs(pbp - 2.504295e-04) <= 1e-3 * 2.504295e-04, pbp
     assert abs(aviter - 5.0) <= 1e-3 * 5.0, aviter
 

@@ -22,6 +22,7 @@ import pytest
 
 import dense as D
 import oracle as O
+import measure
 import measure_ref
 import refstate
 
@@ -485,8 +486,8 @@ def test_measure_shamir_known_answer():
     st = refstate.starting_state(dims, want=("u",))
     # `seed = 4139764973254.0` is a default-real literal: the seed is its float32 value
     rng = SiteRandom(12, 12, 12, float(np.float32(4139764973254.0)))
-    pbp, aviter = measure_ref.measure_shamir(O.dslashd, O.congrad, st["u"], dims, np.float32(0.1),
-                                             np.float32(0.05), 3, rng)
+    pbp, aviter = measure.measure_shamir(st["u"], dims, np.float32(0.1), np.float32(0.05), 3, rng,
+                                         ops=O)
     assert abs(pbp - 2.504295e-04) <= 1e-3 * 2.504295e-04, pbp
     assert abs(aviter - 5.0) <= 1e-3 * 5.0, aviter
 

@@ -37,21 +37,6 @@ def coefficient_sets(data=None):
             "force": (anum4g, aden4g, bnum2g, bden2g, anum2g, aden2g)}
 
 
-def _periodic_halo(a):
-    """Fills the one-cell halo of a haloed spinor (Ls+2, Lx+2, Ly+2, Lt+2, 4)
-    with periodic copies of the interior, x then y then t as update_halo_5
-    (uncomms5.f90:6-19), and Ls as the third-axis update (evolution.F90:55).
-    The library's kernels wrap indices themselves; callers that read the
-    haloed array directly (the reference's own routines) need the halo."""
-    for ax in (1, 2, 3, 0):
-        n = a.shape[ax]
-        lo, hi, first, last = ([slice(None)] * a.ndim for _ in range(4))
-        lo[ax], hi[ax], first[ax], last[ax] = 0, n - 1, 1, n - 2
-        a[tuple(lo)] = a[tuple(last)]
-        a[tuple(hi)] = a[tuple(first)]
-    return a
-
-
 def initialise_pp(rng):
     """evolution.F90:76-108: one gaussp call per direction, real part kept."""
     lx, ly, lt = rng.shape
@@ -72,7 +57,7 @@ def initialise_phi_1flavour(rng, u, dims, am, imass, rescga, coeffs, am3=1.0, op
         for ithird in range(ls):
             ps = rng.gauss0()
             r[ithird + 1, 1:-1, 1:-1, 1:-1, idirac] = ps[..., 0] + 1j * ps[..., 1]
-    _periodic_halo(r)
+    G.halo5(r)
     x, it1, *_ = ops.qmrherm(r, rescga, am, imass, anum4, aden4, 0, u, dims, am3=am3,
                              kernel=kernel)
     phi, it2, *_ = ops.qmrherm(x, rescga, 1.0, 1, bnum4, bden4, 0, u, dims, am3=am3,

@@ -25,6 +25,7 @@ IP = ctypes.POINTER(ctypes.c_int)
 UFN = ctypes.CFUNCTYPE(ctypes.c_float, ctypes.c_void_p)  # thr_uniform_fn
 
 PREC = {"f64": 0, "f32": 1, "f16": 2}
+MAXRHS = 32  # right-hand sides per multi-RHS launch (nrhs 1..32 in the C ABI)
 KERNEL = {"shamir": 0, "wilson": 1}
 
 _SIGS = {
@@ -123,6 +124,21 @@ def _p(a):
 
 def _dims(dims):
     return (ctypes.c_int * 4)(*[int(d) for d in dims])
+
+
+def halo5(a):
+    """Fills, in place, the one-cell halo of a haloed spinor
+    (Ls+2, Lx+2, Ly+2, Lt+2, 4) with periodic copies of the interior: x, y, t
+    as update_halo_5 (uncomms5.f90:6-19), then Ls as the third-axis update.
+    The library's kernels wrap indices themselves; routines that read the
+    haloed array directly (the reference's own) need the halo."""
+    for ax in (1, 2, 3, 0):
+        n = a.shape[ax]
+        lo, hi, first, last = ([slice(None)] * a.ndim for _ in range(4))
+        lo[ax], hi[ax], first[ax], last[ax] = 0, n - 1, 1, n - 2
+        a[tuple(lo)] = a[tuple(last)]
+        a[tuple(hi)] = a[tuple(first)]
+    return a
 
 
 class Context:
