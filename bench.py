@@ -119,7 +119,7 @@ def time_dslash(G, hr, dims, links, prec, steps, warmup, nbuf, stream, nrhs=1, a
     return float(np.mean(per)), float(t0.elapsed_ms(t1) / steps)
 
 
-def multi_rhs_legs(G, hr, steps, warmup, stream):
+def multi_rhs_legs(G, hr, steps, warmup, stream, pinned=None):
     """FP64 Dslash over several right-hand sides per launch (links read once
     per launch): 4 sources at 32^3 x Ls=64, and 16 sources with masses drawn
     from {0.005, 0.01, 0.02, 0.05} at 24^3 x Ls=32."""
@@ -129,7 +129,10 @@ def multi_rhs_legs(G, hr, steps, warmup, stream):
         lx, ly, lt, ls = dims
         v4, v5 = lx * ly * lt, lx * ly * lt * ls
         _, l64, l32 = make_fields(G, hr.DeviceArray, dims, 0.3, 99)
-        G.dslash_autotune(l64, dims, "f64", stream=stream)
+        if pinned:
+            G.dslash_set_config(dims, "f64", *pinned)
+        else:
+            G.dslash_autotune(l64, dims, "f64", stream=stream)
         tc, fl = G.dslash_get_config(dims, "f64")
         am = rng.choice([0.005, 0.01, 0.02, 0.05], size=nrhs).astype(np.float32)
         nb = max(2, int(np.ceil(4 * 256e6 / (2 * 16 * 4 * v5 * nrhs))))
@@ -334,8 +337,9 @@ def main():
         ms, _ = time_dslash(G, hr, dims, l32, prec, args.steps, args.warmup, nb, stream)
         extra[prec] = {"ms": round(ms, 5),
                        "GB_s": round(dslash_bytes(v4, v5, prec) / (ms * 1e-3) / 1e9, 1)}
+    pinned = (args.tchunk, args.shape_flags) if args.tchunk > 0 else None
     multi = None if args.no_multi_rhs else multi_rhs_legs(G, hr, max(20, args.steps // 4),
-                                                           args.warmup, stream)
+                                                           args.warmup, stream, pinned)
     progress("device copy rate")
     copy_gbs = copy_rate(G, hr, stream)
     solver = None
