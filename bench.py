@@ -294,8 +294,9 @@ def main():
                     help="only run N device copies of 1 GiB (scales the HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this build measures "
-              "the single-GPU kernel at its real shape", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); it measures the "
+              "single-GPU kernels at their real shape (independent configurations and sources "
+              "shard one process per GPU)", file=sys.stderr)
         sys.exit(2)
 
     import hiprt as hr
