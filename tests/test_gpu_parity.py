@@ -243,11 +243,12 @@ def test_device_precisions(G):
     check(got.astype(np.float64), rc, 4e-3)
 
 @pytest.mark.parametrize("dims", [(4, 6, 8, 8), (6, 4, 6, 12), (4, 4, 3, 64), (4, 2, 8, 16),
-                                  (8, 4, 4, 64)])
+                                  (8, 4, 4, 64), (4, 4, 4, 48)])
 def test_dslash_launch_shapes(G, dims):
     """Every launch shape (t-chunk marched per thread, XCD remap or XCD boxes,
     scalar links, lane-read walls and LDS x exchange when Ls = 64; s hops by
-    lane shifts when Ls divides 64, loads otherwise) gives the same operator,
+    lane shifts, with loads at the wave edges when an s-line crosses waves
+    (Ls = 12, 48), or by loads) gives the same operator,
     equal to the oracle's to 1e-13: bit-identical across shapes, except the
     scalar-link variants (Ls = 64), whose compiled code places fused
     multiply-adds differently (last-bit differences)."""

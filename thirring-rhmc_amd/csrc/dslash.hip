@@ -284,8 +284,9 @@ template <typename C> __device__ inline void add_gamma(int mu, cx<C> o[4], const
 }
 
 // value of v held by lane+1 (NEXT) or lane-1 (PREV) of the wave: DPP wave
-// shifts, register-to-register; lanes at the wave edge read 0 (never used:
-// they sit at s = Ls-1 / s = 0, where the s hop is absent)
+// shifts, register-to-register; lanes at the wave edge read 0 (when Ls
+// divides 64 they sit at s = Ls-1 / s = 0, where the s hop is absent;
+// otherwise the kernel loads their neighbour)
 template <bool NEXT> __device__ inline float lane_shift(float v) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), NEXT ? 0x130 : 0x138,
                                                     0xF, 0xF, false));
@@ -466,8 +467,21 @@ __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, const Tili
       // hops along s (dirac.F90:80-87 for M, 317-324 for M^dagger): spins that
       // hop from s+1 are {3,4} for M and {1,2} for M^dagger; the others from s-1.
       if constexpr (SHUF) {
-        const cx<C> u0 = lane_shift<true>(cur[up0]), u1 = lane_shift<true>(cur[up0 + 1]);
-        const cx<C> d0 = lane_shift<false>(cur[dn0]), d1 = lane_shift<false>(cur[dn0 + 1]);
+        cx<C> u0 = lane_shift<true>(cur[up0]), u1 = lane_shift<true>(cur[up0 + 1]);
+        cx<C> d0 = lane_shift<false>(cur[dn0]), d1 = lane_shift<false>(cur[dn0 + 1]);
+        if (64 % ls != 0) {
+          // s-lines cross wave boundaries: the neighbour of the last lane
+          // (s+1) and of the first lane (s-1) sit in the next/previous wave
+          const unsigned ln = threadIdx.x & 63;
+          if (ln == 63 && s + 1 < ls) {
+            u0 = in.load1(i + 1, up0);
+            u1 = in.load1(i + 1, up0 + 1);
+          }
+          if (ln == 0 && s > 0) {
+            d0 = in.load1(i - 1, dn0);
+            d1 = in.load1(i - 1, dn0 + 1);
+          }
+        }
         if (shops && s + 1 < ls) {
           o[up0] = o[up0] - u0;
           o[up0 + 1] = o[up0 + 1] - u1;
@@ -653,6 +667,9 @@ std::map<TuneKey, DslashConfig> g_tuned;
 DslashConfig default_config(const Dims &d) {
   DslashConfig c;
   c.tchunk = std::min(8, d.lt);
+  // lane shifts pay when every wave holds whole s-lines; when lines cross
+  // waves the edge-lane loads cost as much as they save (measured at Ls = 48)
+  c.laneshift = (64 % d.ls) == 0 ? 1 : 0;
   if (const char *e = std::getenv("THR_DSLASH_TCHUNK")) c.tchunk = std::max(1, std::atoi(e));
   if (const char *e = std::getenv("THR_DSLASH_REMAP")) c.remap = std::atoi(e) ? 1 : 0;
   if (const char *e = std::getenv("THR_DSLASH_UNIFORM")) c.uniform_links = std::atoi(e) ? 1 : 0;
@@ -724,7 +741,7 @@ int launch_dslash_cfg(const DslashLaunch &L, DslashConfig c, hipStream_t s) {
   THR_CHECK(nblocks <= TL.pstride, THR_ERR_ARG, "dslash grid exceeds partial stride");
   dim3 grid(nblocks, (unsigned)L.nrhs);
   dim3 block(BLK);
-  const bool shuf = c.laneshift && (64 % L.d.ls) == 0;
+  const bool shuf = c.laneshift != 0;
   // a wave is exactly one s-line when Ls = 64 (blocks are whole waves and
   // every tiling starts a wave at s = 0)
   const bool ul = c.uniform_links && shuf && L.d.ls == 64;
@@ -847,7 +864,6 @@ int dslash_autotune(const Dims &d, int prec, const void *links, DslashConfig *be
   for (int tc : tcs) {
     if (tc > d.lt) continue;
     for (int v = 0; v < 64 && !rc; v++) {
-      if ((v & 4) && (64 % d.ls) != 0) continue;   // lane shifts need Ls | 64
       if ((v & 2) && (!(v & 4) || d.ls != 64)) continue;  // uniform links need Ls = 64
       if ((v & 16) && !(v & 2)) continue;          // the LDS x exchange needs them
       if ((v & 8) && (v & 1)) continue;            // boxes replace the remap

@@ -54,7 +54,8 @@ struct DslashConfig {
   int tchunk = 8;
   int remap = 1;
   int uniform_links = 1;  // Ls = 64: scalar link loads, lane-read wall couplings
-  int laneshift = 1;      // s hops as DPP lane shifts when Ls divides 64
+  int laneshift = 1;      // s hops as DPP lane shifts (wave-edge lanes load theirs when
+                          // Ls does not divide 64; default on only when it does)
   int box = 1;            // XCD boxes: each XCD owns one box of an (x, y, t) split
   int lds_x = 1;          // with uniform links: inner x neighbours through LDS
   int nt_stores = 0;      // non-temporal output stores
