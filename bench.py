@@ -14,11 +14,14 @@ Kernel durations come from HIP events recorded around every launch on the
 stream the kernels run on.
 
 Also reported in the same JSON line:
-  * FP32 / FP16 operator bandwidth;
+  * FP32 / FP16 operator bandwidth, and the FP64 operator over 4 and 16
+    right-hand sides per launch;
   * multishift solves/s to FP64 true residual 1e-10 for all 25 poles of the
     x^(-1/4) approximation (remez4 block 2): the plain FP64 GPU path (the
     reference's recurrences) against the batched, blocked-update,
     mixed-precision path, with the true residuals of both;
+  * the RHMC force at 16^3 x Ls=48: the FP64 solver against the
+    mixed-precision solver with FP64 correction (time and difference);
   * serving: p50/p99 latency and solves/s of a Poisson load of measurement
     solves with continuous batching (thirring-rhmc_amd/serving.py);
   * the CPU baseline: the FP64 C oracle's Dslash on the host cores.
@@ -211,6 +214,39 @@ def _plain_leg(G, hr, dims, l64, am, bh, sig, n, nsp, tol, maxcg, stream):
                            "max_true_residual": float(rel_plain.max())}}
 
 
+def bench_force(G, dims=(16, 16, 16, 48), am=0.01, beta=0.3, res1=1e-8, seed=77):
+    """force_module.F90 at the production lattice (16^3 x Ls=48, am 0.01,
+    beta 0.3, 12-pole guidance approximations): the reference's FP64 solver
+    against the mixed-precision solver with FP64 correction (use_sp =
+    "mixed"), both on this GPU through the host-array entry point; reports
+    seconds per force and the relative L2 difference of dSdpi."""
+    import confio
+    lx, ly, lt, ls = dims
+    rng = np.random.default_rng(seed)
+    th = np.asfortranarray((rng.standard_normal((lx, ly, lt, 3)) / np.sqrt(beta))
+                           .astype(np.float32))
+    u = G.coef(th, dims)
+    phi = np.zeros((ls + 2, lx + 2, ly + 2, lt + 2, 4), dtype=np.complex128, order="F")
+    phi[1:-1, 1:-1, 1:-1, 1:-1] = (rng.standard_normal((ls, lx, ly, lt, 4))
+                                   + 1j * rng.standard_normal((ls, lx, ly, lt, 4))) / np.sqrt(2)
+    G.halo5(phi)
+    (a4g, d4g), _ = confio.remez("remez4g")
+    (a2g, d2g), (b2g, e2g) = confio.remez("remez2g")
+    coeffs = (a4g, d4g, b2g, e2g, a2g, d2g)
+    out = {"lattice": list(dims), "am": am, "res1": res1}
+    forces = {}
+    for name, sp in (("fp64", False), ("mixed", "mixed")):
+        progress(f"force {name} {dims}")
+        t0 = time.perf_counter()
+        f, its = G.force(phi, res1, am, 3, beta, th, u, dims, coeffs, use_sp=sp)
+        out[name] = {"seconds": round(time.perf_counter() - t0, 3), "itercg": [int(i) for i in its]}
+        forces[name] = f
+    out["mixed_vs_fp64_rel_l2"] = float(np.linalg.norm(forces["mixed"] - forces["fp64"])
+                                        / np.linalg.norm(forces["fp64"]))
+    out["speedup"] = round(out["fp64"]["seconds"] / out["mixed"]["seconds"], 3)
+    return out
+
+
 def bench_serving(rate, seconds, slots, max_requests):
     """Poisson load of measurement solves (L in {8,12,16,24}, T in {L,2L}, Ls in
     {8,16,32,48}, beta in [0.2,0.5], am in {0.005..0.05}, 1-12 sources, FP64
@@ -275,6 +311,7 @@ def main():
     ap.add_argument("--no-solver", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-multi-rhs", action="store_true")
+    ap.add_argument("--no-force", action="store_true")
     ap.add_argument("--serve-rate", type=float, default=2.0)
     ap.add_argument("--serve-seconds", type=float, default=15.0)
     ap.add_argument("--serve-slots", type=int, default=4)
@@ -352,6 +389,7 @@ def main():
                                   legs_wanted=tuple(args.solver_legs.split(",")))
         solver["lattice"] = list(sd)
         solver["am"] = args.solver_am
+    force = None if args.no_force else bench_force(G)
     serving = None
     if not args.no_serving:
         serving = bench_serving(args.serve_rate, args.serve_seconds, args.serve_slots, 60)
@@ -393,6 +431,7 @@ def main():
         "dslash_other_precisions": extra,
         "dslash_multi_rhs": multi,
         "multishift": solver,
+        "force": force,
         "serving": serving,
         "bench_wall_s": round(wall, 2),
     }

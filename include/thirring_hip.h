@@ -117,7 +117,12 @@ int thr_derivs(const double *r, const double *x2, double anum, int iflag, const 
 /* qmrherm_module.F90:20-194  qmrherm(Phi, x, res, itercg, am, imass, anum,
  * aden, ndiagq, iflag, use_sp, cg_returns).  x1 (Ls,Lx,Ly,Lt,4,ndiagq) and
  * phi0 (haloed, 25 slots) are the module arrays; dsdpi accumulates for
- * iflag 2/3; max_qmr_iters as params.F90:56. */
+ * iflag 2/3; max_qmr_iters as params.F90:56.  use_sp 0: the reference's FP64
+ * multishift; 1: its FP32 variant (multishift_module.F90:289-534); 2 (an
+ * addition, Shamir kernel): the batched solver with FP64-then-FP32 Krylov
+ * storage and FP64 defect correction, every shift to true residual
+ * |Phi - (M^+M + aden_i) x_i| <= res |Phi|.  The Wilson kernel solves in FP64
+ * for every use_sp, as the reference does. */
 int thr_qmrherm(const double *phi, double *x, float res, int *itercg, float am, int imass,
                 float am3, const double *anum, const double *aden, int ndiagq, int iflag,
                 int use_sp, const double *u, const int dims[4], int max_qmr_iters, int kernel,
@@ -128,7 +133,8 @@ int thr_qmrherm(const double *phi, double *x, float res, int *itercg, float am, 
  * (anum4g/aden4g, bnum2g/bden2g, anum2g/aden2g, remezg), the antiperiodic
  * flip of dSdpi(:,:,Lt,3) when ibound == -1, plus beta*Nf*theta.  theta is
  * trial::theta (Lx,Ly,Lt,3) float; dsdpi is overwritten; itercg[4] (may be
- * NULL) receives the four qmrherm iteration counts.  kernel 0 = Shamir, 1 =
+ * NULL) receives the four qmrherm iteration counts; use_sp as for
+ * thr_qmrherm (the reference's spmd switch is use_sp = 1).  kernel 0 = Shamir, 1 =
  * Wilson kernel (the GENERATE_WITH_* switch; single precision is not used with
  * the Wilson kernel, qmrherm_module.F90:54-63). */
 int thr_force(const double *phi, float res1, float am, int imass, float am3, float beta,

@@ -237,3 +237,43 @@ def test_e2e_acceptance_and_exp_dH(G):
     print("acceptance", acc / n, "<exp-dH>", yav, "+/-", err, flush=True)
     assert acc / n >= 0.8
     assert abs(1.0 - yav) <= err
+
+
+def test_force_mixed_precision(G):
+    """force with the batched mixed-precision solver (use_sp = "mixed": FP32
+    Krylov storage below 1e-4 and FP64 defect correction of every shift to
+    true residual res1) against the oracle's FP64 force."""
+    dims = (4, 4, 4, 8)
+    th, pp, u, phi = fields(dims, 1)
+    coeffs = _md_coeffs()
+    for res1 in (1e-8, 1e-10):
+        fg, its = G.force(phi, res1, 0.05, 3, 0.3, th, u, dims, coeffs, use_sp="mixed")
+        fo = O.force(phi, res1, 0.05, 3, 0.3, th, u, dims, coeffs)
+        e = rel(fg, fo)
+        print("res1", res1, "force rel error vs FP64 oracle", e, "itercg", list(its), flush=True)
+        assert e <= 100 * res1, e
+
+
+@pytest.mark.parametrize("iflag", [0, 1, 2, 3])
+def test_qmrherm_mixed_precision(G, iflag):
+    """qmrherm (iflag 0-3) with the mixed-precision solver: x, the shifted
+    solutions x1 and the force terms within 100 res of the oracle's FP64."""
+    dims = (4, 4, 4, 8)
+    lx, ly, lt, ls = dims
+    th, pp, u, phi = fields(dims, 7)
+    (a4, d4), _ = remez_file("remez4g")
+    res = 1e-9
+    phi0 = None
+    if iflag == 3:  # the Phi0 register comes from an iflag 1 call
+        _, _, _, phi0, _, _ = O.qmrherm(phi, res, 0.05, 3, a4, d4, 1, u, dims)
+    xg, _, x1g, _, dg, _ = G.qmrherm(phi, res, 0.05, 3, a4, d4, iflag, u, dims,
+                                     phi0=None if phi0 is None else phi0.copy(order="F"),
+                                     use_sp="mixed")
+    xo, _, x1o, _, do, _ = O.qmrherm(phi, res, 0.05, 3, a4, d4, iflag, u, dims,
+                                     phi0=None if phi0 is None else phi0.copy(order="F"))
+    if iflag < 2:
+        e = max(rel(O.interior(xg), O.interior(xo)), rel(x1g, x1o))
+    else:
+        e = rel(dg, do)
+    print("iflag", iflag, "rel error", e, flush=True)
+    assert e <= 100 * res, e

@@ -8,7 +8,10 @@ int derivs_dispatch(const Dims &d, int kernel, const double2 *R, const double2 *
                     int iflag, float am, int imass, double *f, hipStream_t s);
 
 // qmrherm_module.F90:20-194 on device fields: phi, x [4 V5]; x1 [n][4 V5];
-// phi0 [n][4 V5] (the Phi0 register, halo-free); dsdpi [3 V4]
+// phi0 [n][4 V5] (the Phi0 register, halo-free); dsdpi [3 V4].  use_sp: 0
+// the reference's FP64 multishift, 1 its FP32 variant, 2 the batched solver
+// with mixed-precision storage and FP64 defect correction to true residual
+// res |phi| (Shamir kernel; the Wilson kernel always solves in FP64)
 int qmrherm_device(const OpDesc &op, const double2 *phi, double2 *x, float res, int *itercg,
                    const double *anum, const double *aden, int n, int iflag, int use_sp,
                    int max_qmr_iters, double2 *x1, double2 *phi0, double *dsdpi,

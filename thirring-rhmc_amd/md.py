@@ -66,14 +66,18 @@ def initialise_phi_1flavour(rng, u, dims, am, imass, rescga, coeffs, am3=1.0, op
 
 
 def trajectory(theta, rng, dims, am, imass, beta, iterl, dt, rescga, rescgg, coeffs=None,
-               ibound=-1, am3=1.0, max_qmr_iters=50000, ops=G, kernel="shamir"):
+               ibound=-1, am3=1.0, max_qmr_iters=50000, ops=G, kernel="shamir",
+               use_sp=False):
     """One sweep of dwf3d_lib.F90:263-318.  theta (Lx,Ly,Lt,3) float32 is not
     modified; returns a dict with the new theta (the trial theta if accepted,
     else a copy of the old one), accepted, dH = H0 - H1, dS = S0 - S1, the MD
     step count, and H0/H1 (h, hg, hp, s, hf).  `ops` supplies coef, qmrherm,
     hamilton and evolve_theta_pp with this module's signatures (the GPU
     library by default); kernel "shamir" or "wilson" is the generation kernel
-    (GENERATE_WITH_SHAMIR / GENERATE_WITH_WILSON)."""
+    (GENERATE_WITH_SHAMIR / GENERATE_WITH_WILSON); use_sp selects the force's
+    solver (False: FP64 as the reference, True: its spmd FP32 solver,
+    "mixed": the batched mixed-precision solver with FP64 correction).  The
+    Hamiltonians that feed the Metropolis test always solve in FP64."""
     c = coefficient_sets() if coeffs is None else coeffs
     thetat = np.array(theta, dtype=np.float32, order="F")
     u = ops.coef(thetat, dims, ibound)
@@ -84,7 +88,8 @@ def trajectory(theta, rng, dims, am, imass, beta, iterl, dt, rescga, rescgg, coe
                          max_qmr_iters, kernel=kernel)
     _, steps = ops.evolve_theta_pp(iterl, np.float32(dt), pp, phi, rescgg, am, imass, beta,
                                    thetat, u, dims, c["force"], lambda: rng.rano_at(0, 0, 0),
-                                   ibound, am3, max_qmr_iters=max_qmr_iters, kernel=kernel)
+                                   ibound, am3, use_sp=use_sp, max_qmr_iters=max_qmr_iters,
+                                   kernel=kernel)
     h1, _ = ops.hamilton(phi, rescga, am, imass, beta, thetat, pp, u, dims, c["hamilton"], am3,
                          max_qmr_iters, kernel=kernel)
     dH = h0["h"] - h1["h"]

@@ -258,7 +258,7 @@ def qmrherm(phi, res, am, imass, anum, aden, iflag, u, dims, x1=None, phi0=None,
     it = ctypes.c_int()
     cgrs = np.zeros(n, dtype=np.int32)
     _check(lib().thr_qmrherm(_p(phi), _p(x), res, ctypes.byref(it), am, int(imass), am3,
-                             _p(anum), _p(aden), n, int(iflag), int(bool(use_sp)), _p(u),
+                             _p(anum), _p(aden), n, int(iflag), _sp(use_sp), _p(u),
                              _dims(dims), int(max_qmr_iters), KERNEL[kernel], _p(x1), _p(phi0),
                              _p(dsdpi), _p(cgrs)), "qmrherm")
     return x, it.value, x1, phi0, dsdpi, cgrs
@@ -275,9 +275,17 @@ def force(phi, res1, am, imass, beta, theta, u, dims, coeffs_g, ibound=-1, am3=1
     its = np.zeros(4, dtype=np.int32)
     _check(lib().thr_force(_p(phi), res1, am, int(imass), am3, beta, int(ibound), _p(th), _p(u),
                            _dims(dims), _p(a4), _p(d4), _p(b2), _p(e2), _p(a2), _p(d2), len(d4),
-                           int(bool(use_sp)), int(max_qmr_iters), KERNEL[kernel], _p(dsdpi),
+                           _sp(use_sp), int(max_qmr_iters), KERNEL[kernel], _p(dsdpi),
                            _p(its)), "force")
     return dsdpi, its
+
+
+def _sp(use_sp):
+    """use_sp argument: False/0 FP64, True/1 the reference's FP32 multishift,
+    "mixed"/2 the batched mixed-precision solver with FP64 correction."""
+    if isinstance(use_sp, str):
+        return {"fp64": 0, "sp": 1, "mixed": 2}[use_sp]
+    return int(use_sp)
 
 
 def _uniform_source(ytest):
@@ -317,7 +325,7 @@ def evolve_theta_pp(iterl, dt, pp, phi, res, am, imass, beta, theta, u, dims, co
     rc = lib().thr_evolve_theta_pp(int(iterl), dt, _p(pp), _p(phi), res, am, int(imass), am3,
                                    beta, int(ibound), _p(theta), _p(u), _dims(dims), _p(a4),
                                    _p(d4), _p(b2), _p(e2), _p(a2), _p(d2), len(d4),
-                                   int(bool(use_sp)), int(max_qmr_iters), KERNEL[kernel], fn,
+                                   _sp(use_sp), int(max_qmr_iters), KERNEL[kernel], fn,
                                    None, _p(dsdpi), ctypes.byref(itot))
     if err:
         raise err[0]
@@ -338,7 +346,7 @@ def evolve_theta_pp_dev(iterl, dt, pp, phi, res, am, imass, beta, theta, links64
     rc = lib().thr_evolve_theta_pp_dev(int(iterl), dt, _p(pp), _p(phi), res, am, int(imass), am3,
                                        beta, int(ibound), _p(theta), _p(links64), _dims(dims),
                                        _p(a4), _p(d4), _p(b2), _p(e2), _p(a2), _p(d2), len(d4),
-                                       int(bool(use_sp)), int(max_qmr_iters), KERNEL[kernel], fn,
+                                       _sp(use_sp), int(max_qmr_iters), KERNEL[kernel], fn,
                                        None, _p(dsdpi), ctypes.byref(itot), _stream(stream))
     if err:
         raise err[0]
