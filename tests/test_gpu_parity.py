@@ -243,7 +243,7 @@ def test_device_precisions(G):
     check(got.astype(np.float64), rc, 4e-3)
 
 @pytest.mark.parametrize("dims", [(4, 6, 8, 8), (6, 4, 6, 12), (4, 4, 3, 64), (4, 2, 8, 16),
-                                  (8, 4, 4, 64), (4, 4, 4, 48)])
+                                  (8, 4, 4, 64), (4, 4, 4, 48), (5, 3, 7, 8), (3, 5, 4, 24)])
 def test_dslash_launch_shapes(G, dims):
     """Every launch shape (t-chunk marched per thread, XCD remap or XCD boxes,
     scalar links, lane-read walls and LDS x exchange when Ls = 64; s hops by
