@@ -238,6 +238,32 @@ template <typename C> struct SLinks {
   }
 };
 
+// Ls = 32: a wave holds two s-lines (lanes 0-31 and 32-63), each with its
+// own links: two scalar loads per link and a per-lane select.  Lines are
+// valid or invalid as a whole and the second never precedes the first, so
+// the first active lane belongs to the first line, and the second line's
+// index is read only when lane 32 is active.
+template <typename C> struct SLinks2 {
+  const void *p;
+  __device__ inline SLinks2(const void *q, unsigned) : p(q) {}
+  __device__ inline cx<C> operator()(unsigned i) const {
+    const unsigned u0 = __builtin_amdgcn_readfirstlane(i);
+    const bool second = (__builtin_amdgcn_read_exec() >> 32) & 1;
+    const unsigned u1 = second ? (unsigned)__builtin_amdgcn_readlane((int)i, 32) : u0;
+    cx<C> a, b;
+    if constexpr (sizeof(C) == 8) {
+      ConstF64 *q = (ConstF64 *)p;
+      a = {q[2 * u0], q[2 * u0 + 1]};
+      b = {q[2 * u1], q[2 * u1 + 1]};
+    } else {
+      ConstF32 *q = (ConstF32 *)p;
+      a = {q[2 * u0], q[2 * u0 + 1]};
+      b = {q[2 * u1], q[2 * u1 + 1]};
+    }
+    return (threadIdx.x & 32) ? b : a;
+  }
+};
+
 // value of lane k of the wave (a wave-uniform broadcast)
 __device__ inline float lane_value(float v, int k) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
@@ -327,9 +353,10 @@ __device__ inline void hop(int mu, cx<C> o[4], cx<C> uf, const cx<C> rf[4], cx<C
 // t neighbour at t+2, so the t hops come from registers.  x and y hops are
 // L1/L2 hits (neighbouring lanes and waves of the same and nearby blocks).
 // With SHUF (Ls divides 64) the s hops are DPP lane shifts of the centre.
-// With UL (Ls = 64: a wave is one s-line) links are scalar loads and the
+// With UL = 1 (Ls = 64: a wave is one s-line) links are scalar loads and the
 // wall couplings s = 0 <-> Ls-1 are lane reads of the centre; with XL the
 // inner x neighbours come from the other waves of the block through LDS.
+// UL = 2 (Ls = 32: two s-lines per wave) loads both lines' links as scalars.
 // Hardware block b runs on XCD b % 8, and each XCD has its own L2, so the
 // tiling decides which neighbours are L2 hits:
 //   linear  blocks walk the s-x-y slab of a t-chunk in order;
@@ -340,7 +367,7 @@ __device__ inline void hop(int mu, cx<C> o[4], cx<C> uf, const cx<C> rf[4], cx<C
 //           are fetched across XCDs.
 // Partial sums go to partial[rhs * pstride + blockIdx.x]; block 0 zero-fills
 // [gridDim.x, pstride) so consumers can always sum pstride entries.
-template <int PREC, bool DAG, bool SHUF, int MODE, bool UL, bool XL, bool NT>
+template <int PREC, bool DAG, bool SHUF, int MODE, int UL, bool XL, bool NT>
 __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, const Tiling TL) {
   using C = typename Store<PREC>::C;
   using T = typename Store<PREC>::T;
@@ -392,7 +419,8 @@ __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, const Tili
     const unsigned lxb = xb + lx * y, lyb = x + lx * yb;
     const long roff = (long)rhs * 4 * v5;
     using In = BAcc<PREC>;
-    using Lk = std::conditional_t<UL, SLinks<C>, BLinks<C>>;
+    using Lk = std::conditional_t<UL == 1, SLinks<C>,
+                                  std::conditional_t<UL == 2, SLinks2<C>, BLinks<C>>>;
     const In in((const T *)L.in + roff, L.in_norm ? L.in_norm + (long)rhs * v5 : nullptr, v5);
     const Lk lk(L.links, 3 * v4);
     const C diag = L.wilson4d ? (C)(3.0f + (-L.am3)) : (C)((3.0f - L.am3) + 1.0f);
@@ -469,7 +497,7 @@ __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, const Tili
       if constexpr (SHUF) {
         cx<C> u0 = lane_shift<true>(cur[up0]), u1 = lane_shift<true>(cur[up0 + 1]);
         cx<C> d0 = lane_shift<false>(cur[dn0]), d1 = lane_shift<false>(cur[dn0 + 1]);
-        if (64 % ls != 0) {
+        if (UL == 0 && 64 % ls != 0) {
           // s-lines cross wave boundaries: the neighbour of the last lane
           // (s+1) and of the first lane (s-1) sit in the next/previous wave
           const unsigned ln = threadIdx.x & 63;
@@ -505,7 +533,7 @@ __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, const Tili
       if (imass == 1 || imass == 3) {
         // top: s = Ls couples to s = 1 on the up0 spins; bottom: s = 1 couples to s = Ls
         cx<C> w0[2], wL[2];
-        if constexpr (UL) {
+        if constexpr (UL == 1) {
           // the wave is the s-line: s = 0 is lane 0, s = Ls-1 is lane 63
 #pragma unroll
           for (int k = 0; k < 2; k++) {
@@ -516,14 +544,14 @@ __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, const Tili
         if (s == ls - 1) {
 #pragma unroll
           for (int k = 0; k < 2; k++) {
-            const cx<C> r0 = UL ? w0[k] : in.load1(col, up0 + k);
+            const cx<C> r0 = UL == 1 ? w0[k] : in.load1(col, up0 + k);
             o[up0 + k] = o[up0 + k] + (imass == 1 ? m * r0 : m * muli(r0));
           }
         }
         if (s == 0) {
 #pragma unroll
           for (int k = 0; k < 2; k++) {
-            const cx<C> rL = UL ? wL[k] : in.load1(col + ls - 1, dn0 + k);
+            const cx<C> rL = UL == 1 ? wL[k] : in.load1(col + ls - 1, dn0 + k);
             o[dn0 + k] = o[dn0 + k] + (imass == 1 ? m * rL : (-m) * muli(rL));
           }
         }
@@ -670,6 +698,7 @@ DslashConfig default_config(const Dims &d) {
   // lane shifts pay when every wave holds whole s-lines; when lines cross
   // waves the edge-lane loads cost as much as they save (measured at Ls = 48)
   c.laneshift = (64 % d.ls) == 0 ? 1 : 0;
+  c.uniform_links = (d.ls == 64 || d.ls == 32) ? 1 : 0;
   if (const char *e = std::getenv("THR_DSLASH_TCHUNK")) c.tchunk = std::max(1, std::atoi(e));
   if (const char *e = std::getenv("THR_DSLASH_REMAP")) c.remap = std::atoi(e) ? 1 : 0;
   if (const char *e = std::getenv("THR_DSLASH_UNIFORM")) c.uniform_links = std::atoi(e) ? 1 : 0;
@@ -742,9 +771,10 @@ int launch_dslash_cfg(const DslashLaunch &L, DslashConfig c, hipStream_t s) {
   dim3 grid(nblocks, (unsigned)L.nrhs);
   dim3 block(BLK);
   const bool shuf = c.laneshift != 0;
-  // a wave is exactly one s-line when Ls = 64 (blocks are whole waves and
-  // every tiling starts a wave at s = 0)
+  // a wave is exactly one s-line when Ls = 64, two when Ls = 32 (blocks are
+  // whole waves and every tiling starts a wave at s = 0)
   const bool ul = c.uniform_links && shuf && L.d.ls == 64;
+  const bool ul2 = c.uniform_links && shuf && L.d.ls == 32;
   // LDS x exchange: every block must be 4 full waves at 4 consecutive x of
   // one (y, t) (all waves then take the same barriers)
   const bool xl = ul && c.lds_x && L.d.lx % 4 == 0 &&
@@ -755,11 +785,13 @@ int launch_dslash_cfg(const DslashLaunch &L, DslashConfig c, hipStream_t s) {
 #define THR_LAUNCH_NT(P, D, S, M, NTV)                                                        \
   do {                                                                                        \
     if (xl)                                                                                   \
-      hipLaunchKernelGGL((k_dslash<P, D, true, M, true, true, NTV>), grid, block, 0, s, L, TL);  \
+      hipLaunchKernelGGL((k_dslash<P, D, true, M, 1, true, NTV>), grid, block, 0, s, L, TL);     \
     else if (ul)                                                                              \
-      hipLaunchKernelGGL((k_dslash<P, D, true, M, true, false, NTV>), grid, block, 0, s, L, TL); \
+      hipLaunchKernelGGL((k_dslash<P, D, true, M, 1, false, NTV>), grid, block, 0, s, L, TL);    \
+    else if (ul2)                                                                             \
+      hipLaunchKernelGGL((k_dslash<P, D, true, M, 2, false, NTV>), grid, block, 0, s, L, TL);    \
     else                                                                                      \
-      hipLaunchKernelGGL((k_dslash<P, D, S, M, false, false, NTV>), grid, block, 0, s, L, TL);   \
+      hipLaunchKernelGGL((k_dslash<P, D, S, M, 0, false, NTV>), grid, block, 0, s, L, TL);       \
   } while (0)
 #define THR_LAUNCH(P, D, S, M)                 \
   do {                                         \
@@ -864,8 +896,9 @@ int dslash_autotune(const Dims &d, int prec, const void *links, DslashConfig *be
   for (int tc : tcs) {
     if (tc > d.lt) continue;
     for (int v = 0; v < 64 && !rc; v++) {
-      if ((v & 2) && (!(v & 4) || d.ls != 64)) continue;  // uniform links need Ls = 64
-      if ((v & 16) && !(v & 2)) continue;          // the LDS x exchange needs them
+      if ((v & 2) && (!(v & 4) || (d.ls != 64 && d.ls != 32)))
+        continue;  // uniform links need Ls = 64 or 32
+      if ((v & 16) && (!(v & 2) || d.ls != 64)) continue;  // LDS x: one s-line per wave
       if ((v & 8) && (v & 1)) continue;            // boxes replace the remap
       DslashConfig c;
       c.tchunk = tc;
