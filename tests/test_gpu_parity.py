@@ -244,11 +244,11 @@ def test_device_precisions(G):
 
 @pytest.mark.parametrize("dims", [(4, 6, 8, 8), (6, 4, 6, 12), (4, 4, 3, 64), (4, 2, 8, 16),
                                   (8, 4, 4, 64), (4, 4, 4, 48), (5, 3, 7, 8), (3, 5, 4, 24),
-                                  (4, 4, 6, 32), (3, 5, 4, 32)])
+                                  (4, 4, 6, 32), (3, 5, 4, 32), (8, 4, 4, 32), (16, 4, 2, 32)])
 def test_dslash_launch_shapes(G, dims):
     """Every launch shape (t-chunk marched per thread, XCD remap or XCD boxes,
-    scalar links when Ls = 64 or 32 (two s-lines per wave), lane-read walls
-    and LDS x exchange when Ls = 64; s hops by
+    scalar links and LDS x exchange when Ls = 64 or 32 (two s-lines per
+    wave), lane-read walls when Ls = 64; s hops by
     lane shifts, with loads at the wave edges when an s-line crosses waves
     (Ls = 12, 48), or by loads) gives the same operator,
     equal to the oracle's to 1e-13: bit-identical across shapes, except the

@@ -452,10 +452,11 @@ __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, const Tili
 #pragma unroll
       for (int a = 0; a < 4; a++) o[a] = diag * cur[a];
       if constexpr (XL) {
-        // the block's 4 waves are 4 consecutive x at one (y, t): the x
-        // neighbours of the inner waves are the centre registers of the
-        // next/previous wave, exchanged through LDS (double-buffered by step
-        // parity, one barrier per step); the outer waves load theirs
+        // the block's 4 waves are 4 consecutive x at one (y, t) (8 when
+        // Ls = 32): the x neighbours inside the block are the centre
+        // registers of the next/previous line, exchanged through LDS
+        // (double-buffered by step parity, one barrier per step); the outer
+        // lines load theirs
         __shared__ C xbuf[2][4][4][64][2];
         const unsigned w = threadIdx.x >> 6, ln = threadIdx.x & 63, pb = (t - t0) & 1;
 #pragma unroll
@@ -465,17 +466,39 @@ __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, const Tili
         }
         __syncthreads();
         cx<C> rf[4], rb[4];
-        if (w < 3) {
+        if constexpr (UL == 2) {
+          // Ls = 32: the block is 8 consecutive x, wave w holding x = 2w (lanes
+          // 0-31) and 2w + 1 (lanes 32-63); position p = 2w + half
+          const unsigned p = 2 * w + (ln >> 5), sl = ln & 31;
+          if (p < 7) {
+            const unsigned ww = (p + 1) >> 1, ll = ((p + 1) & 1) * 32 + sl;
 #pragma unroll
-          for (int a = 0; a < 4; a++) rf[a] = {xbuf[pb][w + 1][a][ln][0], xbuf[pb][w + 1][a][ln][1]};
-        } else {
-          in.load4(jxf + v3 * t, rf);
-        }
-        if (w > 0) {
+            for (int a = 0; a < 4; a++) rf[a] = {xbuf[pb][ww][a][ll][0], xbuf[pb][ww][a][ll][1]};
+          } else {
+            in.load4(jxf + v3 * t, rf);
+          }
+          if (p > 0) {
+            const unsigned ww = (p - 1) >> 1, ll = ((p - 1) & 1) * 32 + sl;
 #pragma unroll
-          for (int a = 0; a < 4; a++) rb[a] = {xbuf[pb][w - 1][a][ln][0], xbuf[pb][w - 1][a][ln][1]};
+            for (int a = 0; a < 4; a++) rb[a] = {xbuf[pb][ww][a][ll][0], xbuf[pb][ww][a][ll][1]};
+          } else {
+            in.load4(jxb + v3 * t, rb);
+          }
         } else {
-          in.load4(jxb + v3 * t, rb);
+          if (w < 3) {
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+              rf[a] = {xbuf[pb][w + 1][a][ln][0], xbuf[pb][w + 1][a][ln][1]};
+          } else {
+            in.load4(jxf + v3 * t, rf);
+          }
+          if (w > 0) {
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+              rb[a] = {xbuf[pb][w - 1][a][ln][0], xbuf[pb][w - 1][a][ln][1]};
+          } else {
+            in.load4(jxb + v3 * t, rb);
+          }
         }
         hop<C, DAG>(0, o, lk(xy + l4), rf, conj(lk(lxb + l4)), rb);
       } else {
@@ -699,6 +722,7 @@ DslashConfig default_config(const Dims &d) {
   // waves the edge-lane loads cost as much as they save (measured at Ls = 48)
   c.laneshift = (64 % d.ls) == 0 ? 1 : 0;
   c.uniform_links = (d.ls == 64 || d.ls == 32) ? 1 : 0;
+  c.lds_x = c.uniform_links;
   if (const char *e = std::getenv("THR_DSLASH_TCHUNK")) c.tchunk = std::max(1, std::atoi(e));
   if (const char *e = std::getenv("THR_DSLASH_REMAP")) c.remap = std::atoi(e) ? 1 : 0;
   if (const char *e = std::getenv("THR_DSLASH_UNIFORM")) c.uniform_links = std::atoi(e) ? 1 : 0;
@@ -775,10 +799,13 @@ int launch_dslash_cfg(const DslashLaunch &L, DslashConfig c, hipStream_t s) {
   // whole waves and every tiling starts a wave at s = 0)
   const bool ul = c.uniform_links && shuf && L.d.ls == 64;
   const bool ul2 = c.uniform_links && shuf && L.d.ls == 32;
-  // LDS x exchange: every block must be 4 full waves at 4 consecutive x of
-  // one (y, t) (all waves then take the same barriers)
+  // LDS x exchange: every block must be 4 full waves at 4 (Ls = 64) or 8
+  // (Ls = 32) consecutive x of one (y, t) (all waves then take the same
+  // barriers)
   const bool xl = ul && c.lds_x && L.d.lx % 4 == 0 &&
                   (TL.mode != TILE_BOXES || TL.bx % 4 == 0);
+  const bool xl2 = ul2 && c.lds_x && L.d.lx % 8 == 0 &&
+                   (TL.mode != TILE_BOXES || TL.bx % 8 == 0);
   const bool nt = c.nt_stores != 0;
   THR_CHECK(4 * L.d.v5() * Store<P64>::bytes < (1L << 32), THR_ERR_SIZE,
             "spinor too large for 32-bit buffer offsets");
@@ -788,6 +815,8 @@ int launch_dslash_cfg(const DslashLaunch &L, DslashConfig c, hipStream_t s) {
       hipLaunchKernelGGL((k_dslash<P, D, true, M, 1, true, NTV>), grid, block, 0, s, L, TL);     \
     else if (ul)                                                                              \
       hipLaunchKernelGGL((k_dslash<P, D, true, M, 1, false, NTV>), grid, block, 0, s, L, TL);    \
+    else if (xl2)                                                                             \
+      hipLaunchKernelGGL((k_dslash<P, D, true, M, 2, true, NTV>), grid, block, 0, s, L, TL);     \
     else if (ul2)                                                                             \
       hipLaunchKernelGGL((k_dslash<P, D, true, M, 2, false, NTV>), grid, block, 0, s, L, TL);    \
     else                                                                                      \
@@ -898,7 +927,7 @@ int dslash_autotune(const Dims &d, int prec, const void *links, DslashConfig *be
     for (int v = 0; v < 64 && !rc; v++) {
       if ((v & 2) && (!(v & 4) || (d.ls != 64 && d.ls != 32)))
         continue;  // uniform links need Ls = 64 or 32
-      if ((v & 16) && (!(v & 2) || d.ls != 64)) continue;  // LDS x: one s-line per wave
+      if ((v & 16) && !(v & 2)) continue;  // the LDS x exchange needs uniform links
       if ((v & 8) && (v & 1)) continue;            // boxes replace the remap
       DslashConfig c;
       c.tchunk = tc;
