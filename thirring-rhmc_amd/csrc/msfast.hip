@@ -229,8 +229,12 @@ __global__ __launch_bounds__(BLK) void k_msf_flush(long v5, int n, TT<P> *const 
   }
   __syncthreads();
   const long ns = 4 * v5;
-  const long e = (long)blockIdx.x * BLK + threadIdx.x;
-  if (e >= ns) return;
+  const long tau = (long)blockIdx.x * BLK + threadIdx.x;
+  if (tau >= ns) return;
+  // element e = spin * V5 + site of the FP64 x_i, p_i; FP16 residuals keep a
+  // site's 4 spins together, so there consecutive threads take the 4 spins of
+  // one site (the ring reads stay contiguous, the FP64 planes 16 lanes wide)
+  const long e = P == P16 ? (tau & 3) * v5 + (tau >> 2) : tau;
   const long off = (long)src * ns;
   cx<double> R[MSB];
 #pragma unroll
