@@ -201,16 +201,16 @@ int thr_dslash_dev(void *out, float *out_norm, const void *in, const float *in_n
  *   THR_SHAPE_XCD_REMAP      give every XCD a contiguous range of blocks
  *   THR_SHAPE_UNIFORM_LINKS  Ls = 64 (a wave is one s-line): links through
  *                            scalar loads, wall couplings as lane reads;
- *                            Ls = 32 (two s-lines per wave): two scalar loads
- *                            per link (needs lane shifts)
+ *                            Ls = 32 / 16 (two / four s-lines per wave): two /
+ *                            four scalar loads per link (needs lane shifts)
  *   THR_SHAPE_LANESHIFT      s hops as DPP lane shifts (any Ls: when s-lines
  *                            cross waves the wave-edge lanes load theirs)
  *   THR_SHAPE_XCD_BOXES      every XCD owns one box of a split of (x, y, t)
  *                            into 8 (used when such a split divides the
  *                            lattice; the remap bit is then ignored)
- *   THR_SHAPE_LDS_X          with uniform links and Lx % 4 == 0 (Ls = 64) or
- *                            Lx % 8 == 0 (Ls = 32): the inner x neighbours
- *                            of a block come through LDS
+ *   THR_SHAPE_LDS_X          with uniform links and Lx % 4, 8, 16 == 0 (Ls =
+ *                            64, 32, 16): the inner x neighbours of a block
+ *                            come through LDS
  *   THR_SHAPE_NT_STORES      non-temporal output stores
  * No counterpart in the reference (a tuning knob of this library).
  * thr_dslash_autotune times the candidates (tchunk 1..16 x flags) on scratch

@@ -244,15 +244,16 @@ def test_device_precisions(G):
 
 @pytest.mark.parametrize("dims", [(4, 6, 8, 8), (6, 4, 6, 12), (4, 4, 3, 64), (4, 2, 8, 16),
                                   (8, 4, 4, 64), (4, 4, 4, 48), (5, 3, 7, 8), (3, 5, 4, 24),
-                                  (4, 4, 6, 32), (3, 5, 4, 32), (8, 4, 4, 32), (16, 4, 2, 32)])
+                                  (4, 4, 6, 32), (3, 5, 4, 32), (8, 4, 4, 32), (16, 4, 2, 32),
+                                  (16, 2, 3, 16), (3, 5, 4, 16)])
 def test_dslash_launch_shapes(G, dims):
     """Every launch shape (t-chunk marched per thread, XCD remap or XCD boxes,
-    scalar links and LDS x exchange when Ls = 64 or 32 (two s-lines per
-    wave), lane-read walls when Ls = 64; s hops by
+    scalar links and LDS x exchange when Ls = 64, 32 or 16 (one, two or four
+    s-lines per wave), lane-read walls when Ls = 64; s hops by
     lane shifts, with loads at the wave edges when an s-line crosses waves
     (Ls = 12, 48), or by loads) gives the same operator,
     equal to the oracle's to 1e-13: bit-identical across shapes, except the
-    scalar-link variants (Ls = 64, 32), whose compiled code places fused
+    scalar-link variants (Ls = 64, 32, 16), whose compiled code places fused
     multiply-adds differently (last-bit differences)."""
     from hiprt import DeviceArray
     lx, ly, lt, ls = dims
@@ -287,7 +288,7 @@ def test_dslash_launch_shapes(G, dims):
                                 first = h
                                 check(h[:, 0] + 1j * h[:, 1], o,
                                       1e-13 if prec == "f64" else 2e-6)
-                            elif ls in (32, 64) and flags & 2 and flags & 4:
+                            elif ls in (16, 32, 64) and flags & 2 and flags & 4:
                                 # scalar links: same arithmetic, different placement
                                 # of fused multiply-adds
                                 e = rel(h, first)

@@ -238,29 +238,40 @@ template <typename C> struct SLinks {
   }
 };
 
-// Ls = 32: a wave holds two s-lines (lanes 0-31 and 32-63), each with its
-// own links: two scalar loads per link and a per-lane select.  Lines are
-// valid or invalid as a whole and the second never precedes the first, so
-// the first active lane belongs to the first line, and the second line's
-// index is read only when lane 32 is active.
-template <typename C> struct SLinks2 {
+// Ls = 32 or 16: a wave holds NL = 2 or 4 s-lines of 64/NL lanes, each with
+// its own links: NL scalar loads per link and a per-lane select.  Lines are
+// valid or invalid as a whole and a later line never precedes an earlier
+// one, so the first active lane belongs to the first line, and a later
+// line's index is read only when its first lane is active.
+template <typename C> __device__ inline cx<C> slink(const void *p, unsigned u) {
+  if constexpr (sizeof(C) == 8) {
+    ConstF64 *q = (ConstF64 *)p;
+    return {q[2 * u], q[2 * u + 1]};
+  } else {
+    ConstF32 *q = (ConstF32 *)p;
+    return {q[2 * u], q[2 * u + 1]};
+  }
+}
+template <typename C, int NL> struct SLinksN {
   const void *p;
-  __device__ inline SLinks2(const void *q, unsigned) : p(q) {}
+  __device__ inline SLinksN(const void *q, unsigned) : p(q) {}
   __device__ inline cx<C> operator()(unsigned i) const {
+    const unsigned long long ex = __builtin_amdgcn_read_exec();
     const unsigned u0 = __builtin_amdgcn_readfirstlane(i);
-    const bool second = (__builtin_amdgcn_read_exec() >> 32) & 1;
-    const unsigned u1 = second ? (unsigned)__builtin_amdgcn_readlane((int)i, 32) : u0;
-    cx<C> a, b;
-    if constexpr (sizeof(C) == 8) {
-      ConstF64 *q = (ConstF64 *)p;
-      a = {q[2 * u0], q[2 * u0 + 1]};
-      b = {q[2 * u1], q[2 * u1 + 1]};
+    if constexpr (NL == 2) {
+      const unsigned u1 = ((ex >> 32) & 1) ? (unsigned)__builtin_amdgcn_readlane((int)i, 32) : u0;
+      const cx<C> a = slink<C>(p, u0), b = slink<C>(p, u1);
+      return (threadIdx.x & 32) ? b : a;
     } else {
-      ConstF32 *q = (ConstF32 *)p;
-      a = {q[2 * u0], q[2 * u0 + 1]};
-      b = {q[2 * u1], q[2 * u1 + 1]};
+      const unsigned u1 = ((ex >> 16) & 1) ? (unsigned)__builtin_amdgcn_readlane((int)i, 16) : u0;
+      const unsigned u2 = ((ex >> 32) & 1) ? (unsigned)__builtin_amdgcn_readlane((int)i, 32) : u0;
+      const unsigned u3 = ((ex >> 48) & 1) ? (unsigned)__builtin_amdgcn_readlane((int)i, 48) : u0;
+      const cx<C> a = slink<C>(p, u0), b = slink<C>(p, u1), c = slink<C>(p, u2),
+                  e = slink<C>(p, u3);
+      const unsigned line = (threadIdx.x >> 4) & 3;
+      const cx<C> lo = (line & 1) ? b : a, hi = (line & 1) ? e : c;
+      return (line & 2) ? hi : lo;
     }
-    return (threadIdx.x & 32) ? b : a;
   }
 };
 
@@ -420,7 +431,8 @@ __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, const Tili
     const long roff = (long)rhs * 4 * v5;
     using In = BAcc<PREC>;
     using Lk = std::conditional_t<UL == 1, SLinks<C>,
-                                  std::conditional_t<UL == 2, SLinks2<C>, BLinks<C>>>;
+                                  std::conditional_t<(UL > 1), SLinksN<C, (UL > 1 ? UL : 2)>,
+                                                     BLinks<C>>>;
     const In in((const T *)L.in + roff, L.in_norm ? L.in_norm + (long)rhs * v5 : nullptr, v5);
     const Lk lk(L.links, 3 * v4);
     const C diag = L.wilson4d ? (C)(3.0f + (-L.am3)) : (C)((3.0f - L.am3) + 1.0f);
@@ -466,19 +478,20 @@ __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, const Tili
         }
         __syncthreads();
         cx<C> rf[4], rb[4];
-        if constexpr (UL == 2) {
-          // Ls = 32: the block is 8 consecutive x, wave w holding x = 2w (lanes
-          // 0-31) and 2w + 1 (lanes 32-63); position p = 2w + half
-          const unsigned p = 2 * w + (ln >> 5), sl = ln & 31;
-          if (p < 7) {
-            const unsigned ww = (p + 1) >> 1, ll = ((p + 1) & 1) * 32 + sl;
+        if constexpr (UL > 1) {
+          // Ls = 64 / UL: the block is 4 UL consecutive x, wave w holding
+          // x = UL w + k in lanes [64 k / UL, 64 (k + 1) / UL); position p = UL w + k
+          constexpr unsigned NL = UL > 1 ? UL : 2, W = 64 / NL;
+          const unsigned p = NL * w + ln / W, sl = ln % W;
+          if (p < 4 * NL - 1) {
+            const unsigned ww = (p + 1) / NL, ll = ((p + 1) % NL) * W + sl;
 #pragma unroll
             for (int a = 0; a < 4; a++) rf[a] = {xbuf[pb][ww][a][ll][0], xbuf[pb][ww][a][ll][1]};
           } else {
             in.load4(jxf + v3 * t, rf);
           }
           if (p > 0) {
-            const unsigned ww = (p - 1) >> 1, ll = ((p - 1) & 1) * 32 + sl;
+            const unsigned ww = (p - 1) / NL, ll = ((p - 1) % NL) * W + sl;
 #pragma unroll
             for (int a = 0; a < 4; a++) rb[a] = {xbuf[pb][ww][a][ll][0], xbuf[pb][ww][a][ll][1]};
           } else {
@@ -795,10 +808,11 @@ int launch_dslash_cfg(const DslashLaunch &L, DslashConfig c, hipStream_t s) {
   dim3 grid(nblocks, (unsigned)L.nrhs);
   dim3 block(BLK);
   const bool shuf = c.laneshift != 0;
-  // a wave is exactly one s-line when Ls = 64, two when Ls = 32 (blocks are
-  // whole waves and every tiling starts a wave at s = 0)
+  // a wave is exactly one s-line when Ls = 64, two when Ls = 32, four when
+  // Ls = 16 (blocks are whole waves and every tiling starts a wave at s = 0)
   const bool ul = c.uniform_links && shuf && L.d.ls == 64;
   const bool ul2 = c.uniform_links && shuf && L.d.ls == 32;
+  const bool ul4 = c.uniform_links && shuf && L.d.ls == 16;
   // LDS x exchange: every block must be 4 full waves at 4 (Ls = 64) or 8
   // (Ls = 32) consecutive x of one (y, t) (all waves then take the same
   // barriers)
@@ -806,6 +820,8 @@ int launch_dslash_cfg(const DslashLaunch &L, DslashConfig c, hipStream_t s) {
                   (TL.mode != TILE_BOXES || TL.bx % 4 == 0);
   const bool xl2 = ul2 && c.lds_x && L.d.lx % 8 == 0 &&
                    (TL.mode != TILE_BOXES || TL.bx % 8 == 0);
+  const bool xl4 = ul4 && c.lds_x && L.d.lx % 16 == 0 &&
+                   (TL.mode != TILE_BOXES || TL.bx % 16 == 0);
   const bool nt = c.nt_stores != 0;
   THR_CHECK(4 * L.d.v5() * Store<P64>::bytes < (1L << 32), THR_ERR_SIZE,
             "spinor too large for 32-bit buffer offsets");
@@ -819,6 +835,10 @@ int launch_dslash_cfg(const DslashLaunch &L, DslashConfig c, hipStream_t s) {
       hipLaunchKernelGGL((k_dslash<P, D, true, M, 2, true, NTV>), grid, block, 0, s, L, TL);     \
     else if (ul2)                                                                             \
       hipLaunchKernelGGL((k_dslash<P, D, true, M, 2, false, NTV>), grid, block, 0, s, L, TL);    \
+    else if (xl4)                                                                             \
+      hipLaunchKernelGGL((k_dslash<P, D, true, M, 4, true, NTV>), grid, block, 0, s, L, TL);     \
+    else if (ul4)                                                                             \
+      hipLaunchKernelGGL((k_dslash<P, D, true, M, 4, false, NTV>), grid, block, 0, s, L, TL);    \
     else                                                                                      \
       hipLaunchKernelGGL((k_dslash<P, D, S, M, 0, false, NTV>), grid, block, 0, s, L, TL);       \
   } while (0)
@@ -939,8 +959,8 @@ int dslash_autotune(const Dims &d, int prec, const void *links, DslashConfig *be
   for (int tc : tcs) {
     if (tc > d.lt) continue;
     for (int v = 0; v < 64 && !rc; v++) {
-      if ((v & 2) && (!(v & 4) || (d.ls != 64 && d.ls != 32)))
-        continue;  // uniform links need Ls = 64 or 32
+      if ((v & 2) && (!(v & 4) || (d.ls != 64 && d.ls != 32 && d.ls != 16)))
+        continue;  // uniform links need Ls = 64, 32 or 16
       if ((v & 16) && !(v & 2)) continue;  // the LDS x exchange needs uniform links
       if ((v & 8) && (v & 1)) continue;            // boxes replace the remap
       DslashConfig c;

@@ -54,7 +54,7 @@ struct DslashConfig {
   int tchunk = 8;
   int remap = 1;
   int uniform_links = 1;  // scalar link loads: Ls = 64 (also lane-read wall couplings),
-                          // Ls = 32 (two lines per wave)
+                          // Ls = 32 / 16 (two / four lines per wave; off by default at 16)
   int laneshift = 1;      // s hops as DPP lane shifts (wave-edge lanes load theirs when
                           // Ls does not divide 64; default on only when it does)
   int box = 1;            // XCD boxes: each XCD owns one box of an (x, y, t) split
