@@ -53,10 +53,11 @@ template <int P> struct VAcc {
   const TT<P> *p;
   const float *nrm;
   long v5;
-  __device__ inline cx<CC<P>> ld(long e) const {
+  // element e = spin * V5 + site (FP64/FP32 spin planes); FP16 stores the
+  // spins of a site together and is addressed by site and spin
+  __device__ inline cx<CC<P>> ld_at(long e, long site, int spin) const {
     if constexpr (P == P16) {
-      const long site = e % v5;
-      const float2 f = __half22float2(p[4 * site + e / v5]);
+      const float2 f = __half22float2(p[4 * site + spin]);
       const float sc = nrm[site];
       return {f.x * sc, f.y * sc};
     } else {
@@ -234,14 +235,16 @@ __global__ __launch_bounds__(BLK) void k_msf_flush(long v5, int n, TT<P> *const 
   // element e = spin * V5 + site of the FP64 x_i, p_i; FP16 residuals keep a
   // site's 4 spins together, so there consecutive threads take the 4 spins of
   // one site (the ring reads stay contiguous, the FP64 planes 16 lanes wide)
-  const long e = P == P16 ? (tau & 3) * v5 + (tau >> 2) : tau;
+  const long site = tau >> 2;
+  const int spin = (int)(tau & 3);
+  const long e = P == P16 ? spin * v5 + site : tau;
   const long off = (long)src * ns;
   cx<double> R[MSB];
 #pragma unroll
   for (int q = 0; q < MSB; q++) {
     if (q < jc) {
       VAcc<P> a{ring[q] + off, ring_n ? ring_n[q] + (long)src * v5 : nullptr, v5};
-      const cx<CC<P>> v = a.ld(e);
+      const cx<CC<P>> v = a.ld_at(e, site, spin);
       R[q] = {(double)v.r, (double)v.i};
     } else {
       R[q] = {0.0, 0.0};
