@@ -122,7 +122,7 @@ def time_dslash(G, hr, dims, links, prec, steps, warmup, nbuf, stream, nrhs=1, a
     return float(np.mean(per)), float(t0.elapsed_ms(t1) / steps)
 
 
-def multi_rhs_legs(G, hr, steps, warmup, stream, pinned=None):
+def multi_rhs_legs(G, hr, steps, warmup, stream, pinned=None, tuned=()):
     """FP64 Dslash over several right-hand sides per launch (links read once
     per launch): 4 sources at 32^3 x Ls=64, and 16 sources with masses drawn
     from {0.005, 0.01, 0.02, 0.05} at 24^3 x Ls=32."""
@@ -134,8 +134,8 @@ def multi_rhs_legs(G, hr, steps, warmup, stream, pinned=None):
         _, l64, l32 = make_fields(G, hr.DeviceArray, dims, 0.3, 99)
         if pinned:
             G.dslash_set_config(dims, "f64", *pinned)
-        else:
-            G.dslash_autotune(l64, dims, "f64", stream=stream)
+        elif dims not in tuned:  # lattices the single-RHS leg tuned keep that shape
+            G.dslash_autotune(l64, dims, "f64", stream=stream, nrhs=nrhs)
         tc, fl = G.dslash_get_config(dims, "f64")
         am = rng.choice([0.005, 0.01, 0.02, 0.05], size=nrhs).astype(np.float32)
         nb = max(2, int(np.ceil(4 * 256e6 / (2 * 16 * 4 * v5 * nrhs))))
@@ -377,7 +377,8 @@ def main():
                        "GB_s": round(dslash_bytes(v4, v5, prec) / (ms * 1e-3) / 1e9, 1)}
     pinned = (args.tchunk, args.shape_flags) if args.tchunk > 0 else None
     multi = None if args.no_multi_rhs else multi_rhs_legs(G, hr, max(20, args.steps // 4),
-                                                           args.warmup, stream, pinned)
+                                                           args.warmup, stream, pinned,
+                                                           tuned=(dims,))
     progress("device copy rate")
     copy_gbs = copy_rate(G, hr, stream)
     solver = None

@@ -226,6 +226,11 @@ int thr_dslash_dev(void *out, float *out_norm, const void *in, const float *in_n
 #define THR_SHAPE_NT_STORES 32
 int thr_dslash_autotune(const void *links, const int dims[4], int prec, int *tchunk, int *flags,
                         void *stream);
+/* thr_dslash_autotune timing launches over nrhs right-hand sides (1..32), for
+ * callers that apply the operator to batches: the pinned shape applies to
+ * every nrhs of these dims and precision (no counterpart in the reference) */
+int thr_dslash_autotune_nrhs(const void *links, const int dims[4], int prec, int nrhs,
+                             int *tchunk, int *flags, void *stream);
 int thr_dslash_set_config(const int dims[4], int prec, int tchunk, int flags);
 int thr_dslash_get_config(const int dims[4], int prec, int *tchunk, int *flags);
 

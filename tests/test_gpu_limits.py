@@ -49,3 +49,35 @@ def test_autotune_config_limits(G):
     assert _status(G, G.dslash_set_config, dims, "f64", 0, 0) == 1
     assert _status(G, G.dslash_set_config, dims, "f64", 9, 0) == 1
     assert _status(G, G.dslash_set_config, dims, "f64", 1, 64) == 1
+    from hiprt import DeviceArray
+    buf = DeviceArray((3 * 4 * 4 * 4, 2), np.float64)
+    assert _status(G, G.dslash_autotune, buf, dims, "f64", None, 0) == 1
+    assert _status(G, G.dslash_autotune, buf, dims, "f64", None, 33) == 1
+
+
+def test_autotune_over_rhs_batches(G):
+    """thr_dslash_autotune_nrhs times shapes on batches of right-hand sides and
+    pins a valid shape; the operator is unchanged by it."""
+    from hiprt import DeviceArray
+    dims = (8, 8, 4, 32)
+    lx, ly, lt, ls = dims
+    rng = np.random.default_rng(3)
+    th = (rng.standard_normal((lx, ly, lt, 3)) / np.sqrt(0.3)).astype(np.float32)
+    theta = DeviceArray.from_host(th.reshape(-1, order="F"))
+    l64 = DeviceArray((lx * ly * lt * 3, 2), np.float64)
+    G.coef_dev(l64, theta, dims)
+    nsp = 4 * lx * ly * lt * ls
+    xh = rng.standard_normal((4, nsp, 2))
+    x = DeviceArray.from_host(xh)
+    out = DeviceArray((4, nsp, 2), np.float64)
+    before = G.dslash_get_config(dims, "f64")
+    G.dslash_dev(out, x, l64, dims, 0.05, imass=3, nrhs=4)
+    ref = out.to_host()
+    tc, fl = G.dslash_autotune(l64, dims, "f64", nrhs=4)
+    assert (tc, fl) == G.dslash_get_config(dims, "f64")
+    assert 1 <= tc <= lt and 0 <= fl < 64
+    G.dslash_dev(out, x, l64, dims, 0.05, imass=3, nrhs=4)
+    got = out.to_host()
+    # same operator (scalar-link shapes may place fused multiply-adds differently)
+    assert np.linalg.norm(got - ref) <= 1e-15 * np.linalg.norm(ref) * 10
+    G.dslash_set_config(dims, "f64", *before)

@@ -602,6 +602,19 @@ int thr_dslash_autotune(const void *links, const int dims[4], int prec, int *tch
   return THR_OK;
 }
 
+int thr_dslash_autotune_nrhs(const void *links, const int dims[4], int prec, int nrhs,
+                             int *tchunk, int *flags, void *stream) {
+  TRY(check_dims(dims));
+  THR_CHECK(links, THR_ERR_ARG, "NULL argument");
+  THR_CHECK(prec >= 0 && prec <= 2, THR_ERR_ARG, "prec must be THR_F64, THR_F32 or THR_F16");
+  THR_CHECK(nrhs >= 1 && nrhs <= MAXRHS, THR_ERR_ARG, "nrhs out of range (1..32)");
+  DslashConfig c;
+  TRY(dslash_autotune(mkdims(dims), prec, links, &c, pick(stream), nrhs));
+  if (tchunk) *tchunk = c.tchunk;
+  if (flags) *flags = shape_flags(c);
+  return THR_OK;
+}
+
 int thr_dslash_set_config(const int dims[4], int prec, int tchunk, int flags) {
   TRY(check_dims(dims));
   THR_CHECK(prec >= 0 && prec <= 2, THR_ERR_ARG, "prec must be THR_F64, THR_F32 or THR_F16");

@@ -879,16 +879,18 @@ int launch_dslash(const DslashLaunch &L, hipStream_t s) {
 }
 
 int dslash_autotune(const Dims &d, int prec, const void *links, DslashConfig *best,
-                    hipStream_t s) {
+                    hipStream_t s, int nrhs) {
   THR_CHECK(links, THR_ERR_ARG, "links required");
+  THR_CHECK(nrhs >= 1 && nrhs <= MAXRHS, THR_ERR_ARG, "nrhs out of range");
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   THR_HIP(hipStreamIsCapturing(s, &cs));
   THR_CHECK(cs == hipStreamCaptureStatusNone, THR_ERR_STATE, "cannot autotune while capturing");
   const size_t es = prec == P64 ? 16 : prec == P32 ? 8 : 4;
-  const size_t nsp = (size_t)4 * d.v5();
+  const size_t nsp = (size_t)4 * d.v5() * nrhs;
+  const size_t nsite = (size_t)d.v5() * nrhs;
   // rotate over enough input/output pairs to exceed the 256 MB Infinity
   // Cache, so every candidate is timed on cold fields as in a solver
-  const size_t pair = 2 * (es * nsp + sizeof(float) * d.v5());
+  const size_t pair = 2 * (es * nsp + sizeof(float) * nsite);
   const int nbuf = (int)std::min<size_t>(8, std::max<size_t>(2, (768u << 20) / pair + 1));
   std::vector<void *> in(nbuf, nullptr), out(nbuf, nullptr);
   std::vector<float *> inn(nbuf, nullptr), outn(nbuf, nullptr);
@@ -908,8 +910,8 @@ int dslash_autotune(const Dims &d, int prec, const void *links, DslashConfig *be
   bool ok = hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
   for (int k = 0; k < nbuf && ok; k++)
     ok = hipMalloc(&in[k], es * nsp) == hipSuccess && hipMalloc(&out[k], es * nsp) == hipSuccess &&
-         hipMalloc((void **)&inn[k], sizeof(float) * d.v5()) == hipSuccess &&
-         hipMalloc((void **)&outn[k], sizeof(float) * d.v5()) == hipSuccess;
+         hipMalloc((void **)&inn[k], sizeof(float) * nsite) == hipSuccess &&
+         hipMalloc((void **)&outn[k], sizeof(float) * nsite) == hipSuccess;
   if (!ok) {
     cleanup();
     set_error("dslash_autotune: allocation failed");
@@ -917,15 +919,15 @@ int dslash_autotune(const Dims &d, int prec, const void *links, DslashConfig *be
   }
   for (int k = 0; k < nbuf; k++) {
     (void)hipMemsetAsync(in[k], 0, es * nsp, s);
-    (void)hipMemsetAsync(inn[k], 0, sizeof(float) * d.v5(), s);
+    (void)hipMemsetAsync(inn[k], 0, sizeof(float) * nsite, s);
   }
   DslashLaunch L{};
   L.d = d;
-  L.nrhs = 1;
+  L.nrhs = nrhs;
   L.prec = prec;
   L.imass = 3;
   L.am3 = 1.0f;
-  L.am[0] = 0.01f;
+  for (int k = 0; k < nrhs; k++) L.am[k] = 0.01f;
   L.links = links;
   L.mode = EPI_STORE;
   L.sc_stride = 1;

@@ -81,9 +81,10 @@ long dslash_blocks(const Dims &d);
 long dslash_grid_blocks(const Dims &d, const DslashConfig &c);
 DslashConfig dslash_config(const Dims &d, int prec);
 void dslash_set_config(const Dims &d, int prec, DslashConfig c);
-// times the launch shapes on scratch fields and pins the fastest for (d, prec)
+// times the launch shapes on scratch fields of nrhs right-hand sides and pins
+// the fastest for (d, prec) (for every nrhs)
 int dslash_autotune(const Dims &d, int prec, const void *links, DslashConfig *best,
-                    hipStream_t s);
+                    hipStream_t s, int nrhs = 1);
 
 // Wilson-kernel operator on device (FP64): slice-composed, imass 1 or 3
 int launch_dslash_wilson(const Dims &d, int nrhs, int dagger, int imass, float am, float am3,

@@ -55,6 +55,7 @@ _SIGS = {
                                  P, P, P, I32, I32, I32, I32, UFN, P, P, IP, P], I32),
     "thr_coef_dev": ([P, P, P, I32, P], I32),
     "thr_dslash_autotune": ([P, P, I32, IP, IP, P], I32),
+    "thr_dslash_autotune_nrhs": ([P, P, I32, I32, IP, IP, P], I32),
     "thr_dslash_set_config": ([P, I32, I32, I32], I32),
     "thr_dslash_get_config": ([P, I32, IP, IP], I32),
     "thr_links_to_f32_dev": ([P, P, P, P], I32),
@@ -411,12 +412,18 @@ SHAPE_XCD_REMAP, SHAPE_UNIFORM_LINKS, SHAPE_LANESHIFT, SHAPE_XCD_BOXES, SHAPE_LD
 SHAPE_NT_STORES = 32
 
 
-def dslash_autotune(links, dims, prec="f64", stream=None):
-    """Times the Dslash launch shapes and pins the fastest; returns (tchunk, flags)."""
+def dslash_autotune(links, dims, prec="f64", stream=None, nrhs=1):
+    """Times the Dslash launch shapes (over `nrhs` right-hand sides per launch)
+    and pins the fastest for these dims and precision; returns (tchunk, flags)."""
     init()
     tc, fl = ctypes.c_int(), ctypes.c_int()
-    _check(lib().thr_dslash_autotune(_p(links), _dims(dims), PREC[prec], ctypes.byref(tc),
-                                     ctypes.byref(fl), _stream(stream)), "dslash_autotune")
+    if nrhs == 1:
+        rc = lib().thr_dslash_autotune(_p(links), _dims(dims), PREC[prec], ctypes.byref(tc),
+                                       ctypes.byref(fl), _stream(stream))
+    else:
+        rc = lib().thr_dslash_autotune_nrhs(_p(links), _dims(dims), PREC[prec], int(nrhs),
+                                            ctypes.byref(tc), ctypes.byref(fl), _stream(stream))
+    _check(rc, "dslash_autotune")
     return tc.value, fl.value
 
 
