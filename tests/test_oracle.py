@@ -604,3 +604,32 @@ def test_site_random_single_site_stream():
     nc = c.rano()
     nc[0, 0, 0] = 0
     assert np.array_equal(na, nc)
+
+
+def test_trajectory_composition_on_the_oracle():
+    """thirring-rhmc_amd/md.py (dwf3d_lib.F90:263-318) run over the oracle on
+    the reference's end-to-end sample (con_4x4x4, midout, Ls = 8): the sweep
+    is reproducible from the seed (to the oracle's OpenMP summation order),
+    conserves energy to |dH| < 0.5, and accepts whenever dH >= 0."""
+    import os
+    import confio
+    import md
+    from md_oracle_ops import OracleOps
+    from site_random import SiteRandom
+    gold = os.path.join(os.path.dirname(__file__), "golden")
+    th, seed, _ = confio.read_con(os.path.join(gold, "con_4x4x4_serial"), 4, 4, 4)
+    dims = (4, 4, 4, 8)
+    runs = []
+    for _ in range(2):
+        rng = SiteRandom(4, 4, 4, seed)
+        r = md.trajectory(th, rng, dims, 0.01, 3, 0.4, 7, 0.14, np.float32(1e-8),
+                          np.float32(1e-6), ops=OracleOps)
+        runs.append(r)
+    a, b = runs
+    assert a["steps"] == b["steps"] and a["accepted"] == b["accepted"]
+    assert abs(a["dH"] - b["dH"]) < 1e-6
+    assert np.max(np.abs(a["theta"] - b["theta"])) < 1e-5
+    assert 1 <= a["steps"] <= 28 and abs(a["dH"]) < 0.5
+    if a["dH"] >= 0:
+        assert a["accepted"]
+    assert a["H0"]["hg"] > 0 and a["H0"]["hp"] > 0
