@@ -47,6 +47,17 @@ extern "C" {
 #define THR_ERR_NOCONV 3
 #define THR_ERR_SIZE 4
 #define THR_ERR_STATE 5
+#define THR_ERR_NONFINITE 6
+/* Failure reporting.  THR_ERR_NOCONV: a solver with an FP64 true-residual
+ * contract (thr_multishift_fast_dev, qmrherm/force with use_sp = 2) left a
+ * (source, shift) pair above its target after its correction rounds or at
+ * maxcg; outputs hold the last iterates.  THR_ERR_NONFINITE: a dot product
+ * of a solver came out NaN or Inf (a non-finite source, guess or link field);
+ * the solve stops at that iteration instead of running to its cap.  The
+ * reference-faithful solvers (congrad, multishift, qmrherm use_sp 0/1) keep
+ * the reference's contract for the cap: they return THR_OK with the
+ * iteration count equal to the cap, which the caller checks
+ * (measure_module.F90:124-135, dwf3d_lib.F90:860-881). */
 
 /* precisions of device spinors */
 #define THR_F64 0
@@ -74,6 +85,10 @@ int thr_context_destroy(void *ctx);
 int thr_context_bind(void *ctx);
 /* device bytes currently held by the bound context's scratch buffers */
 int thr_context_bytes(unsigned long long *bytes);
+/* release the bound context's device scratch buffers once its stream is idle
+ * (later calls allocate them again); lets a long-lived worker give back the
+ * scratch of a large request instead of holding its high-water mark */
+int thr_context_trim(void);
 
 /* ---- (1) host-array drop-in entry points -------------------------------- */
 
@@ -145,12 +160,15 @@ int thr_force(const double *phi, float res1, float am, int imass, float am3, flo
 
 /* dwf3d_lib.F90:511-616  hamilton(Phi, h, hg, hp, s, res2, am, imass) for
  * Nf = 1 (rational branch); out = {h, hg, hp, s, hf}; itercg[2] (may be NULL);
- * kernel as for thr_force */
+ * kernel as for thr_force.  use_sp selects the solver of its two qmrherm
+ * calls as for thr_qmrherm: 0 is the reference (FP64); 2 solves with
+ * mixed-precision storage and FP64 correction of every shift to true
+ * residual res2 |Phi|, the action sum staying FP64. */
 int thr_hamilton(const double *phi, float res2, float am, int imass, float am3, float beta,
                  const float *theta, const float *pp, const double *u, const int dims[4],
                  const double *anum4, const double *aden4, const double *bnum2,
-                 const double *bden2, int ndiag, int max_qmr_iters, int kernel, double *out,
-                 int *itercg);
+                 const double *bden2, int ndiag, int use_sp, int max_qmr_iters, int kernel,
+                 double *out, int *itercg);
 
 /* Source of the uniform deviates that end a molecular-dynamics trajectory:
  * returns the caller's next rano(yran, idum, 1, 1, 1) (evolution.F90:159-161). */
@@ -188,6 +206,21 @@ int thr_links_to_f32_dev(void *links32, const void *links64, const int dims[4], 
  * ceiling next to the Dslash, and a field copy that avoids the runtime's
  * blit path. */
 int thr_device_copy(void *dst, const void *src, unsigned long long bytes, void *stream);
+
+/* Synthetic fields on the device (no counterpart in the reference, whose
+ * noise comes from host generators): n real values, each sigma * N(0, 1),
+ * written as double (prec THR_F64) or float (THR_F32).  Counter-based:
+ * element i depends only on (seed, i), so a field is the same for any stream
+ * or launch.  Used for seeded sources (sigma = 1/sqrt(2) per component gives
+ * unit complex variance) and theta (sigma = 1/sqrt(beta)) without host
+ * staging. */
+int thr_gaussian_dev(void *out, unsigned long long n, int prec, double sigma,
+                     unsigned long long seed, void *stream);
+
+/* norms[k] = |x_k| (FP64 L2 norm) of nvec complex128 vectors of n elements
+ * each, stored one after another; deterministic summation order; returns
+ * with the norms on the host (no counterpart in the reference). */
+int thr_norms_dev(const void *x, int nvec, unsigned long long n, double *norms, void *stream);
 
 /* out = M in (dagger = 0) or M^dagger in (dagger = 1) for nrhs right-hand
  * sides with per-rhs masses am[nrhs]; prec THR_F64/F32/F16 (links must be
@@ -266,14 +299,20 @@ int thr_multishift_plain_dev(const void *links64, const int dims[4], float am, i
  * still meets tol in FP64.
  * iterations: iterations run; cg_return[nsrc]: per-source iterations;
  * corr_iterations (optional): iterations spent in corrections; relres
- * (optional, nsrc*nshift): final true FP64 residuals |b - (M^+M + sigma_i) x_i| / |b|.
+ * (optional, nsrc*nshift): final true FP64 residuals |b - (M^+M + sigma_i) x_i| / |b|;
+ * switches (optional, 3): iteration at which the Krylov storage entered FP64,
+ * FP32, FP16 (-1: never).
+ * Status: THR_ERR_NOCONV when a true residual (computed whenever relres is
+ * asked for or corrections run) stays above tol, or, without them, when a
+ * shift is still above its iterated target at maxcg; THR_ERR_NONFINITE when a
+ * source's recurrences produce NaN/Inf (that source stops at once).
  * Same recurrences as multishift_module.F90:103-282 (see thr_multishift). */
 int thr_multishift_fast_dev(const void *links64, const void *links32, const int dims[4],
                             int nsrc, const float *am, int imass, float am3, int nshift,
                             const double *aden, const void *b, void *x, double tol, int maxcg,
                             double th32, double th16, int max_corrections, int *iterations,
                             int *cg_return, int *corr_iterations, double *relres,
-                            void *stream);
+                            int *switches, void *stream);
 
 /* True FP64 residuals relres[src*nshift + i] = |b - (M^+M + aden_i) x_i| / |b|
  * for solutions laid out as thr_multishift_fast_dev's x (check of any solver) */

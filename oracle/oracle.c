@@ -25,10 +25,16 @@
  * (evolution.F90:113-179) is pinned by reversibility and by the dt^2 scaling
  * of the energy error.
  *
- * Parity unpinned: the absolute sign of the imass = 1 and 5 wall-mass terms
- * (adjointness fixes M against M^+, not the sign itself; a restatement of
- * dirac.F90:119-176, 356-402), and the exact iteration counts of the FP32
- * multishift (multishift_module.F90:289-534).
+ * The imass = 1 wall term is pinned by the reference's own operator
+ * definition (notes/fermions-on-lattice.tex:196-199: M(m) - M(0) is m times
+ * the chirally projected wall coupling, and M(-1) is periodic in s); the
+ * imass = 5 term by the equality of its condensate with imass = 3's on every
+ * gauge field (the two twisted masses are related by a chiral rotation), so
+ * with imass = 3's sign pinned a flip of the imass = 5 term fails.
+ *
+ * Parity unpinned: the exact iteration counts of the FP32 multishift
+ * (multishift_module.F90:289-534); its converged solutions are pinned
+ * against dense solves.
  */
 #include "oracle.h"
 

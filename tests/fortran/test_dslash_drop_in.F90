@@ -7,6 +7,7 @@ program test_dslash_drop_in
   use dirac
   use comms
   use comms5
+  use inverter_utils, only: dirac_op_shifted, dirac_operator
   use test_state
   implicit none
   complex(dp), allocatable :: u(:, :, :, :), Phi(:, :, :, :, :), R(:, :, :, :, :)
@@ -36,5 +37,15 @@ program test_dslash_drop_in
               reshape(Phi(1:kthird_l, 1:ksizex_l, 1:ksizey_l, 1:ksizet_l, :), [kthird_l*kvol*4]), &
               kthird_l*kvol*4)
   end do
+  ! inverter_utils.F90:9-54 call pattern: M^+ M R + shift R, and shift = 0
+  call starting_state(Phi, u, R=R)
+  call dirac_op_shifted(Phi, R, u, am, 3, 0.37)
+  call dump(trim(outdir)//'/dirac_op_shifted.bin', &
+            reshape(Phi(1:kthird_l, 1:ksizex_l, 1:ksizey_l, 1:ksizet_l, :), [kthird_l*kvol*4]), &
+            kthird_l*kvol*4)
+  call dirac_operator(Phi, R, u, am, 3)
+  call dump(trim(outdir)//'/dirac_operator.bin', &
+            reshape(Phi(1:kthird_l, 1:ksizex_l, 1:ksizey_l, 1:ksizet_l, :), [kthird_l*kvol*4]), &
+            kthird_l*kvol*4)
   print *, 'test_dslash_drop_in done'
 end program test_dslash_drop_in

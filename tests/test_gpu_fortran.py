@@ -58,6 +58,12 @@ def test_fortran_dslash(tmp_path):
         g = load(tmp_path / f"dslash_Md_{imass}.bin", shape)
         o = O.interior(O.dslashd(st["r"], st["u"], DIMS, 0.05, imass))
         check(g, o, 1e-13)
+    # inverter_utils (dirac_op_shifted with the default-real shift 0.37, and
+    # dirac_operator) through the Fortran module
+    for name, shift in (("dirac_op_shifted", float(np.float32(0.37))), ("dirac_operator", 0.0)):
+        g = load(tmp_path / f"{name}.bin", shape)
+        o = O.interior(O.dirac_op_shifted(st["r"], st["u"], DIMS, 0.05, 3, shift=shift))
+        check(g, o, 1e-13)
 def test_fortran_congrad(tmp_path):
     out = run("test_congrad_drop_in", tmp_path)
     assert "PASS itercg" in out and "27" in out, out

@@ -51,6 +51,23 @@ def test_dslash_shamir(G, dims, imass):
         assert rel(g, o) <= 1e-13  # periodic halo as update_halo_5 leaves it
 
 
+@pytest.mark.parametrize("kernel,imass", [("shamir", 1), ("shamir", 3), ("shamir", 5),
+                                          ("wilson", 1), ("wilson", 3)])
+def test_dirac_op_shifted(G, kernel, imass):
+    """inverter_utils.F90:9-54: xout = M^+ M xin + shift xin with the
+    periodic halo (dirac_operator is shift = 0); shift is default real."""
+    dims = (4, 4, 4, 8)
+    rng = np.random.default_rng(90 + 10 * imass + (kernel == "wilson"))
+    th = (rng.standard_normal((4, 4, 4, 3)) / np.sqrt(0.3)).astype(np.float32)
+    u = O.coef(th, dims)
+    x = D.rand_spinor(dims, rng)
+    for shift in (0.0, float(np.float32(0.37))):
+        g = G.dirac_op_shifted(x, u, dims, 0.05, imass, shift=shift, kernel=kernel)
+        o = O.dirac_op_shifted(x, u, dims, 0.05, imass, shift=shift, kernel=kernel)
+        check(O.interior(g), O.interior(o), 1e-13)
+        assert rel(g, refstate.halo5(np.array(o, order="F"))) <= 1e-13
+
+
 @pytest.mark.parametrize("imass", [1, 3])
 def test_dslash_wilson(G, imass):
     dims = (4, 4, 4, 8)

@@ -8,6 +8,10 @@ plausible slip (sign, index, conjugation, projector, coefficient) fails:
   * reference known answer: congrad itercg == 27 (tests/test_congrad.F90:47)
   * Clifford algebra of the gamma tables (gammamatrices.F90:27-125)
   * dense brute-force assembly of M from those tables vs. the oracle
+  * imass = 1 wall term vs. the reference's operator definition
+    (notes/fermions-on-lattice.tex:196-199); imass = 5 condensate == imass = 3
+    condensate on every gauge field (chiral rotation), with imass = 3's sign
+    fixed by the psibarpsi known answer
   * adjointness <a, M b> = <M^+ a, b> (Shamir and Wilson, imass 1/3/5)
   * U(1) gauge covariance (link direction and conjugation)
   * Wilson-kernel DWF == Shamir DWF dressed with (1 - D_W) on every s-coupling
@@ -131,6 +135,108 @@ def test_dense_assembly(imass):
     Mo = D.dense(lambda v: O.dslash(v, u, dims, 0.3, imass), dims)
     Md = D.dense(lambda v: O.dslashd(v, u, dims, 0.3, imass), dims)
     assert np.abs(Md - Mo.conj().T).max() < 1e-14
+
+
+def _s_shift(a):
+    """Cyclic shift of a haloed spinor in s: (S a)(s) = a(s + 1 mod Ls)."""
+    b = np.array(a, copy=True, order="F")
+    b[1:-1] = np.roll(a[1:-1], -1, axis=0)
+    return refstate.halo5(b)
+
+
+def test_imass1_wall_term_matches_notes():
+    """The imass = 1 mass term against the reference's own definition of the
+    domain-wall operator (notes/fermions-on-lattice.tex:196-199):
+        D_{d+1}(s;r) = delta_sr - (1 - delta_{s,Ls-1}) P- delta_{s+1,r}
+                       - (1 - delta_{s0}) P+ delta_{s-1,r}
+                       + m (P- delta_{s,Ls-1} delta_{0r} + P+ delta_{s0} delta_{Ls-1,r}),
+    with the domain-wall chiral projectors P+- = (1 +- gamma_4)/2 of the
+    reference's gamma table (gammamatrices.F90:27-125).  Two checks:
+    M(m) - M(0) is m times that wall term (built here from the table, not
+    from spin ranges), and at m = -1 the wall term is exactly the missing
+    wrap of the s hops, so M(-1) commutes with the cyclic shift in s.  A
+    flipped sign (planted on the oracle's output) fails the second check."""
+    rng = np.random.default_rng(21)
+    dims = TINY
+    lx, ly, lt, ls = dims
+    u = D.rand_links(dims, rng)
+    G = D.gamma_tables()
+    Pp = 0.5 * (np.eye(4) + G[3])
+    Pm = 0.5 * (np.eye(4) - G[3])
+    n4 = lx * ly * lt
+    w_lo = np.zeros((ls, ls)); w_lo[ls - 1, 0] = 1.0   # s = Ls-1 <- r = 0
+    w_hi = np.zeros((ls, ls)); w_hi[0, ls - 1] = 1.0   # s = 0 <- r = Ls-1
+    W = np.kron(Pm, np.kron(np.eye(n4), w_lo)) + np.kron(Pp, np.kron(np.eye(n4), w_hi))
+    M0 = D.dense(lambda v: O.dslash(v, u, dims, 0.0, 1), dims)
+    m = 0.25  # exact in float32
+    Mm = D.dense(lambda v: O.dslash(v, u, dims, m, 1), dims)
+    assert np.abs((Mm - M0) - m * W).max() < 1e-15
+    r = D.rand_spinor(dims, rng)
+
+    def commutator(op):
+        return np.linalg.norm(D.interior_vec(op(_s_shift(r)) - _s_shift(op(r))))
+    good = commutator(lambda v: O.dslash(v, u, dims, -1.0, 1))
+    assert good < 1e-13 * np.linalg.norm(D.interior_vec(r))
+    # the same operator with the wall term's sign flipped is antiperiodic in s
+    flipped = commutator(lambda v: 2 * O.dslash(v, u, dims, 0.0, 1) - O.dslash(v, u, dims, -1.0, 1))
+    assert flipped > 0.1 * np.linalg.norm(D.interior_vec(r))
+
+
+def _exact_condensate(M, dims, imass):
+    """The noise-free value of measure_shamir's psibarpsi for one gauge field:
+    its wall-to-wall contractions (measure_module.F90:240-347) with the noise
+    average taken exactly (sum over every wall site and spin of the inverse),
+    finished as at measure_module.F90:368-380."""
+    lx, ly, lt, ls = dims
+    kvol = lx * ly * lt
+    Mi = np.linalg.inv(M)
+
+    def ix(s, x, y, t, a):  # interior vector index, Fortran order (s, x, y, t, spin)
+        return s + ls * (x + lx * (y + ly * (t + lt * a)))
+    p1 = p2 = 0j
+    for t in range(lt):
+        for y in range(ly):
+            for x in range(lx):
+                for ids in (0, 1):
+                    if imass != 5:
+                        # source spin ids on s = 1 read at s = Ls; spin ids+2 on s = Ls read at s = 1
+                        p1 += Mi[ix(ls - 1, x, y, t, ids), ix(0, x, y, t, ids)]
+                        p2 += Mi[ix(0, x, y, t, ids + 2), ix(ls - 1, x, y, t, ids + 2)]
+                    else:
+                        # xi(1, 1:2) = -eta placed from spins 3:4, read at s = 1 spin ids+2
+                        # (245-257, 281-291); xi(Ls, 3:4) = -eta, read at s = Ls spin ids
+                        # (302-315, 341-347)
+                        p1 -= Mi[ix(0, x, y, t, ids + 2), ix(0, x, y, t, ids)]
+                        p2 -= Mi[ix(ls - 1, x, y, t, ids), ix(ls - 1, x, y, t, ids + 2)]
+    if imass == 1:
+        return (p1 / kvol + p2 / kvol).real
+    if imass == 3:
+        return (-1j * p1 / kvol + 1j * p2 / kvol).real
+    return (-1j * p1 / kvol - 1j * p2 / kvol).real
+
+
+def test_imass5_condensate_equals_imass3():
+    """imass = 3 and imass = 5 are the two twisted masses of 2+1-D domain-wall
+    fermions, related by a chiral rotation: on every gauge field their
+    condensates, as measure_shamir contracts them, are equal (the spectra of
+    M and M^+M are equal too, but those are the same for m and -m, so they
+    cannot fix a sign).  imass = 3's sign is pinned by the reference's
+    psibarpsi known answer (test_measure_shamir_known_answer), so this pins
+    the sign of the imass = 5 wall term of dirac.F90:158-176 together with the
+    reference's imass = 5 contraction: with that term flipped (planted on the
+    oracle's output) the condensate changes sign."""
+    rng = np.random.default_rng(22)
+    dims = TINY
+    u = D.rand_links(dims, rng, kind="theta")
+    m = 0.25
+    M3 = D.dense(lambda v: O.dslash(v, u, dims, m, 3), dims)
+    M5 = D.dense(lambda v: O.dslash(v, u, dims, m, 5), dims)
+    c3, c5 = _exact_condensate(M3, dims, 3), _exact_condensate(M5, dims, 5)
+    assert abs(c3) > 0.05
+    assert abs(c5 - c3) < 1e-12 * abs(c3), (c3, c5)
+    M50 = D.dense(lambda v: O.dslash(v, u, dims, 0.0, 5), dims)
+    c5_flipped = _exact_condensate(2 * M50 - M5, dims, 5)
+    assert abs(c5_flipped + c3) < 1e-12 * abs(c3)
 
 
 @pytest.mark.parametrize("kernel,imass", [("shamir", 1), ("shamir", 3), ("shamir", 5),

@@ -171,6 +171,13 @@ int thr_context_bytes(unsigned long long *bytes) {
   return THR_OK;
 }
 
+int thr_context_trim(void) {
+  // the context's own stream may still read the buffers: wait for it first
+  THR_HIP(hipStreamSynchronize(default_stream()));
+  release_workspace();
+  return THR_OK;
+}
+
 static int links_to_host(const Dims &d, const double2 *u64, double *u, hipStream_t s);
 
 int thr_coef(double *u, const float *theta, const int dims[4], int ibound) {
@@ -506,8 +513,8 @@ int thr_evolve_theta_pp_dev(int iterl, float dt, float *pp, const void *phi, flo
 int thr_hamilton(const double *phi, float res2, float am, int imass, float am3, float beta,
                  const float *theta, const float *pp, const double *u, const int dims[4],
                  const double *anum4, const double *aden4, const double *bnum2,
-                 const double *bden2, int ndiag, int max_qmr_iters, int kernel, double *out,
-                 int *itercg) {
+                 const double *bden2, int ndiag, int use_sp, int max_qmr_iters, int kernel,
+                 double *out, int *itercg) {
   TRY(check_dims(dims));
   TRY(check_op(kernel, imass));
   THR_CHECK(phi && theta && pp && u && anum4 && aden4 && bnum2 && bden2 && out, THR_ERR_ARG,
@@ -527,6 +534,7 @@ int thr_hamilton(const double *phi, float res2, float am, int imass, float am3, 
   A.res2 = res2;
   A.beta = beta;
   A.ndiag = ndiag;
+  A.use_sp = use_sp;
   A.max_qmr_iters = max_qmr_iters;
   A.anum4 = anum4;
   A.aden4 = aden4;
@@ -688,7 +696,7 @@ int thr_multishift_fast_dev(const void *links64, const void *links32, const int 
                             const double *aden, const void *b, void *x, double tol, int maxcg,
                             double th32, double th16, int max_corrections, int *iterations,
                             int *cg_return, int *corr_iterations, double *relres,
-                            void *stream) {
+                            int *switches, void *stream) {
   TRY(check_dims(dims));
   TRY(check_op(0, imass));
   THR_CHECK(links64 && links32 && am && aden && b && x, THR_ERR_ARG, "NULL argument");
@@ -719,15 +727,37 @@ int thr_multishift_fast_dev(const void *links64, const void *links32, const int 
   if (iterations) *iterations = R.iterations;
   if (cg_return)
     for (int k = 0; k < nsrc; k++) cg_return[k] = R.cg_return[k];
+  if (switches)
+    for (int k = 0; k < 3; k++) switches[k] = R.switches[k];
   if (corr_iterations) *corr_iterations = 0;
-  if (max_corrections > 0) {
+  THR_CHECK(!R.nonfinite, THR_ERR_NONFINITE,
+            "multishift: non-finite dot product (NaN or Inf in a source or the links)");
+  if (max_corrections > 0 || relres) {
     std::vector<double> rr((size_t)nsrc * nshift);
-    TRY(msfast_correct(A, max_corrections, tol, rr.data(), corr_iterations, s));
+    if (max_corrections > 0)
+      TRY(msfast_correct(A, max_corrections, tol, rr.data(), corr_iterations, s));
+    else
+      TRY(msfast_true_residuals(A, rr.data(), s));
     if (relres) std::copy(rr.begin(), rr.end(), relres);
-  } else if (relres) {
-    TRY(msfast_true_residuals(A, relres, s));
+    THR_HIP(hipStreamSynchronize(s));
+    // the contract is the FP64 true residual: report any pair above tol
+    // (NaN included) after the correction rounds, as the reference aborts
+    // when its solver hits the iteration cap (dwf3d_lib.F90:860-881)
+    int bad = 0;
+    double worst = 0.0;
+    for (double v : rr) {
+      if (!(v <= tol)) bad++;
+      if (!(v <= worst)) worst = v;
+    }
+    THR_CHECK(bad == 0, THR_ERR_NOCONV,
+              "multishift: " + std::to_string(bad) + " (source, shift) pairs above the true-residual "
+              "target (worst " + std::to_string(worst) + ")");
+    return THR_OK;
   }
   THR_HIP(hipStreamSynchronize(s));
+  THR_CHECK(R.unconverged == 0, THR_ERR_NOCONV,
+            "multishift: " + std::to_string(R.unconverged) +
+                " (source, shift) pairs still above the iterated target at maxcg");
   return THR_OK;
 }
 

@@ -145,3 +145,4 @@ __device__ inline double block_sum(double v) {
 #define THR_ERR_NOCONV 3
 #define THR_ERR_SIZE 4
 #define THR_ERR_STATE 5
+#define THR_ERR_NONFINITE 6

@@ -14,6 +14,7 @@ struct MsfState {
   double cb[MAXSHIFT][MSB], cd[MAXSHIFT][MSB];
   int active[MAXSHIFT], touched[MAXSHIFT], cg_returns[MAXSHIFT];
   int n, mini, maxi, jcount, cg_return, maxcg, stop, first_block;
+  int nonfinite;  // alpha or lambda came out NaN/Inf: this source stopped there
 };
 
 struct MsfArgs {
@@ -39,7 +40,10 @@ struct MsfResult {
   int iterations, blocks;
   int cg_return[MAXRHS];
   int cg_returns[MAXRHS * MAXSHIFT];
+  // iteration at which the Krylov storage entered FP64 / FP32 / FP16 (-1: never)
   int switches[3];
+  int nonfinite;    // sources whose recurrences produced NaN/Inf
+  int unconverged;  // (source, shift) pairs still above the iterated target at maxcg
 };
 
 int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s);

@@ -85,7 +85,7 @@ __global__ __launch_bounds__(BLK) void k_msf_init(MsfState *st, int n, const dou
       S->zeta_i[i] = 1.0;
       S->zeta_ii[i] = 1.0;
       S->zeta_iii[i] = 1.0;
-      S->active[i] = 1;
+      S->active[i] = b2 != 0.0 ? 1 : 0;  // x = 0 solves b = 0 exactly
       S->touched[i] = 0;
       S->cg_returns[i] = 0;
       S->ca[i] = 1.0;
@@ -98,7 +98,8 @@ __global__ __launch_bounds__(BLK) void k_msf_init(MsfState *st, int n, const dou
     S->cg_return = 0;
     S->maxcg = maxcg;
     S->first_block = 1;
-    S->stop = (maxcg <= 0 || b2 == 0.0) ? 1 : 0;
+    S->nonfinite = isfinite(b2) ? 0 : 1;
+    S->stop = (maxcg <= 0 || b2 == 0.0 || S->nonfinite) ? 1 : 0;
   }
 }
 
@@ -117,6 +118,16 @@ __global__ __launch_bounds__(BLK) void k_msf_fin1(MsfState *st, const double *ad
   __syncthreads();
   const double os = S->omega, gg = S->gammag;
   const double om = -S->delta / s_al;
+  if (!isfinite(om)) {
+    // a non-finite alpha can never converge: stop this source before any of
+    // its vectors is updated (its later launches exit on the stop flag)
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      S->nonfinite = 1;
+      S->stop = 1;
+    }
+    return;
+  }
   const int i = threadIdx.x;
   if (i >= S->mini - 1 && i < S->maxi && S->active[i]) {
     S->zeta_iii[i] = (S->zeta_i[i] * S->zeta_ii[i] * os) /
@@ -149,6 +160,14 @@ __global__ __launch_bounds__(BLK) void k_msf_fin2(MsfState *st, const double *pa
   }
   __syncthreads();
   const double L = s_lam;
+  if (!isfinite(L)) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      S->nonfinite = 1;
+      S->stop = 1;
+    }
+    return;
+  }
   const double gam = L / S->delta;
   const double omega = S->omega, tol2 = S->tol2;
   const int j = S->jcount, cgr = S->cg_return;
@@ -483,6 +502,10 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
   }
   int prec = P64;
   if (A.start_prec > 0) prec = A.start_prec;
+  if (res) {
+    res->switches[0] = res->switches[1] = res->switches[2] = -1;
+    res->switches[prec] = 0;
+  }
   THR_CHECK(prec == P64 || (prec == P32 && A.th32 > 0) || (prec == P16 && A.th16 > 0), THR_ERR_ARG,
             "start precision needs its buffers (set th32/th16)");
   // r0 and p in the starting precision
@@ -664,9 +687,14 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
   if (res) {
     res->iterations = (int)t;
     res->blocks = blocks;
+    res->nonfinite = 0;
+    res->unconverged = 0;
     for (int k = 0; k < ns_; k++) {
       res->cg_return[k] = hs[k].cg_return;
       for (int i = 0; i < n; i++) res->cg_returns[k * MAXSHIFT + i] = hs[k].cg_returns[i];
+      res->nonfinite += hs[k].nonfinite ? 1 : 0;
+      // a shift leaves the active set only once it meets its target
+      for (int i = 0; i < n; i++) res->unconverged += hs[k].active[i] ? 1 : 0;
     }
   }
   return rc;
@@ -802,6 +830,8 @@ int msfast_correct(const MsfArgs &A, int max_rounds, double tol_true, double *re
       B.start_prec = P32;
       MsfResult R{};
       TRY_RC(msfast_solve(B, &R, s));
+      THR_CHECK(!R.nonfinite, THR_ERR_NONFINITE,
+                "multishift correction: non-finite residual recurrence");
       total += R.iterations;
       for (int j = 0; j < g; j++) {
         hipLaunchKernelGGL(k_addvec, dim3(nblk(nsp)), dim3(BLK), 0, s, nsp,

@@ -35,6 +35,7 @@ struct HamArgs {
   const float *pp;     // host momenta (3 V4)
   float res2, beta;
   int ndiag, max_qmr_iters;
+  int use_sp;  // solver of the two qmrherm calls, as qmrherm_device's use_sp
   const double *anum4, *aden4, *bnum2, *bden2;  // host coefficients
 };
 // dwf3d_lib.F90:511-616 (rational branch, Nf = 1): out = {h, hg, hp, s, hf}

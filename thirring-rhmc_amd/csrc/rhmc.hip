@@ -126,12 +126,20 @@ int multishift_mixed(const OpDesc &op, int n, const double *aden, const double2 
   A.th16 = 0.0;
   MsfResult R{};
   TRY_RC(msfast_solve(A, &R, s));
+  THR_CHECK(!R.nonfinite, THR_ERR_NONFINITE,
+            "qmrherm (mixed): non-finite dot product (NaN or Inf in Phi or the links)");
   std::vector<double> rr((size_t)n);
   int corr = 0;
   TRY_RC(msfast_correct(A, 4, (double)res, rr.data(), &corr, s));
   if (itercg) *itercg = R.iterations;
   if (cg_returns)
     for (int i = 0; i < n; i++) cg_returns[i] = R.cg_returns[i];
+  // every shift must meet the FP64 true residual after the corrections, or
+  // the force / Hamiltonian built on it would go on with unconverged shifts
+  for (int i = 0; i < n; i++)
+    THR_CHECK(rr[i] <= (double)res, THR_ERR_NOCONV,
+              "qmrherm (mixed): shift " + std::to_string(i + 1) + " true residual " +
+                  std::to_string(rr[i]) + " above res after the correction rounds");
   return THR_OK;
 }
 
@@ -247,10 +255,10 @@ int hamilton_device(const HamArgs &A, double out[5], int itercg[2], hipStream_t 
   one.imass = 1;
   int it1 = 0, it2 = 0;
   THR_HIP(hipMemcpyAsync(R, A.phi, sizeof(double2) * ns, hipMemcpyDeviceToDevice, s));
-  TRY_RC(qmrherm_device(one, R, Xr, A.res2, &it1, A.anum4, A.aden4, A.ndiag, 0, 0,
+  TRY_RC(qmrherm_device(one, R, Xr, A.res2, &it1, A.anum4, A.aden4, A.ndiag, 0, A.use_sp,
                         A.max_qmr_iters, x1, nullptr, nullptr, nullptr, s));  // (584)
   THR_HIP(hipMemcpyAsync(R, Xr, sizeof(double2) * ns, hipMemcpyDeviceToDevice, s));
-  TRY_RC(qmrherm_device(A.op, R, Xr, A.res2, &it2, A.bnum2, A.bden2, A.ndiag, 0, 0,
+  TRY_RC(qmrherm_device(A.op, R, Xr, A.res2, &it2, A.bnum2, A.bden2, A.ndiag, 0, A.use_sp,
                         A.max_qmr_iters, x1, nullptr, nullptr, nullptr, s));  // (590)
   hipLaunchKernelGGL(k_redot, dim3(nb), dim3(BLK), 0, s, ns, R, Xr, partial);
   hipLaunchKernelGGL(k_sum, dim3(1), dim3(BLK), 0, s, partial, nb, dev);

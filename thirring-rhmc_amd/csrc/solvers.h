@@ -38,6 +38,22 @@ bool context_graphs();
 int cg_reference(const OpDesc &op, const double2 *phi, double2 *x, float res, int niterc,
                  int *itercg, int *iterations, hipStream_t s);
 
+// the same CG on caller-provided device buffers: three spinors, the solver
+// state (cg_state_bytes()) and dslash_blocks(d) partial sums.  None needs
+// clearing (every one is written before it is read).  host_flag (optional)
+// is the host int the stop flag is polled into; given one, nothing is
+// allocated (then only the Shamir kernel is allowed: the Wilson kernel's
+// unfused epilogue needs library scratch).
+struct CgBuffers {
+  double2 *x1, *r, *p;
+  void *state;
+  double *partial;
+};
+size_t cg_state_bytes();
+int cg_reference_bufs(const OpDesc &op, const double2 *phi, double2 *x, float res, int niterc,
+                      int *itercg, int *iterations, hipStream_t s, const CgBuffers &B,
+                      int *host_flag);
+
 // reference-faithful multishift CG (multishift_module.F90:13-284), FP64 vectors.
 // out: [n][4*V5] shift solutions.
 // plain = 1: corr_i = 1 and the threshold is res*|b| (stop when the shifted

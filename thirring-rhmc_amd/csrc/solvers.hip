@@ -262,6 +262,12 @@ __global__ __launch_bounds__(BLK) void k_cg_alpha(CgState *st, const double *par
     } else {
       st->alpha_x = 0.0;
     }
+    // a non-finite |M p|^2 (NaN or Inf in the source, the guess or the links)
+    // can never meet the stopping rule: end the solve here instead of at niterc
+    if (!isfinite(alphad) || !isfinite(st->alpha)) {
+      st->stop = 3;
+      st->iterations = st->nx;
+    }
   }
 }
 
@@ -274,7 +280,10 @@ __global__ __launch_bounds__(BLK) void k_cg_beta(CgState *st, const double *part
     st->betacgd = bn;
     st->alphan = bn;
     if (st->nx == 1) st->betacg = 0.0;
-    if (bn < (double)st->resid) {
+    if (!isfinite(bn)) {
+      st->stop = 3;
+      st->iterations = st->nx;
+    } else if (bn < (double)st->resid) {
       st->stop = 1;
       st->iterations = st->nx;
     } else if (st->nx >= st->niterc) {
@@ -285,10 +294,11 @@ __global__ __launch_bounds__(BLK) void k_cg_beta(CgState *st, const double *part
   }
 }
 
-// the p update of the final capped iteration must still run: stop == 2 lets it
+// the p update of the final capped iteration must still run: stop == 2 lets
+// it (stop 1: converged, 3: non-finite scalars)
 __global__ __launch_bounds__(BLK) void k_cg_p(long n, const double2 *r, double2 *p,
                                               const CgState *S) {
-  if (S->stop == 1) return;
+  if (S->stop == 1 || S->stop == 3) return;
   const long i = (long)blockIdx.x * BLK + threadIdx.x;
   if (i >= n) return;
   st(p, i, ldc(r, i) + S->betacg * ldc(p, i));
@@ -303,6 +313,7 @@ struct MsState {
       gammas[MAXSHIFT], corr[MAXSHIFT], aden[MAXSHIFT];
   int flags[MAXSHIFT], cg_returns[MAXSHIFT];
   int n, mini, maxi, umin, umax, cg_return, maxcg, stop, done, plain;
+  int nonfinite;  // a dot product came out NaN/Inf: the solve stopped there
   float res;
   double resd;  // threshold used by the convergence test
 };
@@ -326,7 +337,8 @@ __global__ __launch_bounds__(BLK) void k_ms_init(MsState *st, const double *part
     st->mini = 1;
     st->umin = 1;
     st->umax = 0;
-    st->stop = (st->maxcg <= 0 || st->n <= 0) ? 1 : 0;
+    st->nonfinite = isfinite(d) ? 0 : 1;
+    st->stop = (st->maxcg <= 0 || st->n <= 0 || st->nonfinite) ? 1 : 0;
     st->done = st->stop;
     // reference rule: sqrt(delta zeta^2) corr_i < res; plain rule (corr_i = 1):
     // shifted residual below res |b|
@@ -345,6 +357,14 @@ __global__ void k_ms_alpha(MsState *st, const double *partial, int nb) {
     st->alpha = al;
     st->omega_save = st->omega;
     st->omega = -st->delta / al;
+    if (!isfinite(st->omega)) {
+      // nothing downstream of a non-finite omega is meaningful: stop before
+      // any vector of this iteration is updated
+      st->nonfinite = 1;
+      st->stop = 1;
+      st->done = 1;
+      return;
+    }
     const double om = st->omega, os = st->omega_save, gg = st->gammag;
     for (int i = st->mini - 1; i < st->maxi; i++) {
       st->zeta_iii[i] = (st->zeta_i[i] * st->zeta_ii[i] * os) /
@@ -359,6 +379,12 @@ __global__ void k_ms_lambda(MsState *st, const double *partial, int nb) {
   if (st->stop) return;
   const double lam = fin_sum(partial, nb);
   if (threadIdx.x == 0) {
+    if (!isfinite(lam)) {
+      st->nonfinite = 1;
+      st->stop = 1;
+      st->done = 1;
+      return;
+    }
     st->lambda = lam;
     st->gammag = lam / st->delta;
     for (int i = st->mini - 1; i < st->maxi; i++) {
@@ -529,11 +555,13 @@ static int apply_op_epi(const OpDesc &op, int dagger, const double2 *in, double2
   return launch_dslash(L, s);
 }
 
-// poll a device int every `batch` iterations; batches grow to bound host syncs
+// poll a device int every `batch` iterations; batches grow to bound host syncs.
+// host_flag: where the flag is read back (NULL: a pinned int allocated here)
 template <typename F>
-static int run_batched(F &&one_iteration, const int *dev_flag, hipStream_t s, long max_iters) {
-  int *h = nullptr;
-  THR_HIP(hipHostMalloc((void **)&h, sizeof(int), hipHostMallocDefault));
+static int run_batched(F &&one_iteration, const int *dev_flag, hipStream_t s, long max_iters,
+                       int *host_flag = nullptr) {
+  int *h = host_flag;
+  if (!h) THR_HIP(hipHostMalloc((void **)&h, sizeof(int), hipHostMallocDefault));
   *h = 0;
   int batch = 4;
   long done_iters = 0;
@@ -559,21 +587,39 @@ static int run_batched(F &&one_iteration, const int *dev_flag, hipStream_t s, lo
     }
     if (batch < 64) batch *= 2;
   }
-  (void)hipHostFree(h);
+  if (!host_flag) (void)hipHostFree(h);
   return rc;
 }
+
+size_t cg_state_bytes() { return sizeof(CgState); }
 
 int cg_reference(const OpDesc &op, const double2 *phi, double2 *x, float res, int niterc,
                  int *itercg, int *iterations, hipStream_t s) {
   const Dims &d = op.d;
   const long n = d.nspinor();
+  CgBuffers B{};
+  B.x1 = (double2 *)workspace(10, sizeof(double2) * n);
+  B.r = (double2 *)workspace(11, sizeof(double2) * n);
+  B.p = (double2 *)workspace(12, sizeof(double2) * n);
+  B.state = workspace(13, sizeof(CgState));
+  B.partial = partial_buffer((int)dslash_blocks(d));
+  THR_CHECK(B.x1 && B.r && B.p && B.state && B.partial, THR_ERR_HIP,
+            "workspace allocation failed");
+  return cg_reference_bufs(op, phi, x, res, niterc, itercg, iterations, s, B, nullptr);
+}
+
+int cg_reference_bufs(const OpDesc &op, const double2 *phi, double2 *x, float res, int niterc,
+                      int *itercg, int *iterations, hipStream_t s, const CgBuffers &B,
+                      int *host_flag) {
+  const Dims &d = op.d;
+  const long n = d.nspinor();
   const int nb = (int)dslash_blocks(d);
-  double2 *x1 = (double2 *)workspace(10, sizeof(double2) * n);
-  double2 *r = (double2 *)workspace(11, sizeof(double2) * n);
-  double2 *p = (double2 *)workspace(12, sizeof(double2) * n);
-  CgState *st = (CgState *)workspace(13, sizeof(CgState));
-  double *partial = partial_buffer(nb);
-  THR_CHECK(x1 && r && p && st && partial, THR_ERR_HIP, "workspace allocation failed");
+  double2 *x1 = B.x1, *r = B.r, *p = B.p;
+  CgState *st = (CgState *)B.state;
+  double *partial = B.partial;
+  THR_CHECK(x1 && r && p && st && partial, THR_ERR_ARG, "congrad: missing buffers");
+  THR_CHECK(op.kernel == 0 || !host_flag, THR_ERR_ARG,
+            "congrad on caller buffers runs the Shamir kernel only");
   // resid = 4*ksize*ksize*ksizet*kthird*res*res in default real (measure_module.F90:41)
   const long nsite4 = 4L * d.lx * d.ly * d.lt * d.ls;
   const float resid = ((float)nsite4 * res) * res;
@@ -594,13 +640,15 @@ int cg_reference(const OpDesc &op, const double2 *phi, double2 *x, float res, in
     THR_HIP(hipGetLastError());
     return THR_OK;
   };
-  int rc = run_batched(iter, stop, s, niterc);
+  int rc = run_batched(iter, stop, s, niterc, host_flag);
   if (rc) return rc;
   CgState h;
   THR_HIP(hipMemcpyAsync(&h, st, sizeof(CgState), hipMemcpyDeviceToHost, s));
   THR_HIP(hipStreamSynchronize(s));
   *itercg = h.itercg;
   if (iterations) *iterations = h.iterations;
+  THR_CHECK(h.stop != 3, THR_ERR_NONFINITE,
+            "congrad: non-finite dot product (NaN or Inf in the source, guess or links)");
   return THR_OK;
 }
 
@@ -676,6 +724,8 @@ int multishift_reference(const OpDesc &op, int n, const double *aden, const doub
   *cg_return = hs.cg_return;
   if (cg_returns)
     for (int i = 0; i < n; i++) cg_returns[i] = hs.cg_returns[i];
+  THR_CHECK(!hs.nonfinite, THR_ERR_NONFINITE,
+            "multishift: non-finite dot product (NaN or Inf in the source or links)");
   return THR_OK;
 }
 
@@ -754,6 +804,8 @@ int multishift_reference_sp(const OpDesc &op, int n, const double *aden, const d
   *cg_return = hs.cg_return;
   if (cg_returns)
     for (int i = 0; i < n; i++) cg_returns[i] = hs.cg_returns[i];
+  THR_CHECK(!hs.nonfinite, THR_ERR_NONFINITE,
+            "multishift: non-finite dot product (NaN or Inf in the source or links)");
   return THR_OK;
 }
 

@@ -36,6 +36,7 @@ _SIGS = {
     "thr_context_destroy": ([P], I32),
     "thr_context_bind": ([P], I32),
     "thr_context_bytes": ([ctypes.POINTER(ctypes.c_ulonglong)], I32),
+    "thr_context_trim": ([], I32),
     "thr_coef": ([P, P, P, I32], I32),
     "thr_dslash": ([P, P, P, P, F32, I32, F32, I32], I32),
     "thr_dslashd": ([P, P, P, P, F32, I32, F32, I32], I32),
@@ -47,8 +48,8 @@ _SIGS = {
                      P], I32),
     "thr_force": ([P, F32, F32, I32, F32, F32, I32, P, P, P, P, P, P, P, P, P, I32, I32, I32, I32,
                    P, P], I32),
-    "thr_hamilton": ([P, F32, F32, I32, F32, F32, P, P, P, P, P, P, P, P, I32, I32, I32, P, P],
-                     I32),
+    "thr_hamilton": ([P, F32, F32, I32, F32, F32, P, P, P, P, P, P, P, P, I32, I32, I32, I32, P,
+                      P], I32),
     "thr_evolve_theta_pp": ([I32, F32, P, P, F32, F32, I32, F32, F32, I32, P, P, P, P, P, P, P,
                              P, P, I32, I32, I32, I32, UFN, P, P, IP], I32),
     "thr_evolve_theta_pp_dev": ([I32, F32, P, P, F32, F32, I32, F32, F32, I32, P, P, P, P, P, P,
@@ -60,6 +61,8 @@ _SIGS = {
     "thr_dslash_get_config": ([P, I32, IP, IP], I32),
     "thr_links_to_f32_dev": ([P, P, P, P], I32),
     "thr_device_copy": ([P, P, ctypes.c_ulonglong, P], I32),
+    "thr_gaussian_dev": ([P, ctypes.c_ulonglong, I32, F64, ctypes.c_ulonglong, P], I32),
+    "thr_norms_dev": ([P, I32, ctypes.c_ulonglong, P, P], I32),
     "thr_dslash_dev": ([P, P, P, P, P, P, I32, P, I32, F32, I32, I32, P], I32),
     "thr_congrad_dev": ([P, P, P, P, F32, F32, I32, F32, I32, I32, IP, IP, P], I32),
     "thr_multishift_dev": ([P, P, F32, I32, F32, I32, P, P, P, P, F32, I32, I32, I32, IP, P, P],
@@ -67,14 +70,29 @@ _SIGS = {
     "thr_true_residuals_dev": ([P, P, I32, P, I32, F32, I32, P, P, P, P, P], I32),
     "thr_multishift_plain_dev": ([P, P, F32, I32, F32, I32, P, P, P, F32, I32, IP, P, P], I32),
     "thr_multishift_fast_dev": ([P, P, P, I32, P, I32, F32, I32, P, P, P, F64, I32, F64, F64, I32,
-                                 IP, P, IP, P, P], I32),
+                                 IP, P, IP, P, P, P], I32),
 }
 
 _lib = None
 
 
 class ThrError(RuntimeError):
-    pass
+    """A library call returned a THR_ERR_* status (`status`)."""
+
+    def __init__(self, msg, status=None):
+        super().__init__(msg)
+        self.status = status
+
+
+class ThrNoConvergence(ThrError):
+    """THR_ERR_NOCONV: a solver left a residual above its target."""
+
+
+class ThrNonFinite(ThrError):
+    """THR_ERR_NONFINITE: a solver's dot products came out NaN or Inf."""
+
+
+STATUS = {"ok": 0, "arg": 1, "hip": 2, "noconv": 3, "size": 4, "state": 5, "nonfinite": 6}
 
 
 def lib():
@@ -109,7 +127,9 @@ def init(device=0):
 def _check(rc, what):
     if rc != 0:
         msg = lib().thr_last_error()
-        raise ThrError(f"{what} failed (status {rc}): {msg.decode() if msg else ''}")
+        cls = {STATUS["noconv"]: ThrNoConvergence, STATUS["nonfinite"]: ThrNonFinite}.get(rc,
+                                                                                         ThrError)
+        raise cls(f"{what} failed (status {rc}): {msg.decode() if msg else ''}", rc)
 
 
 def _p(a):
@@ -162,6 +182,10 @@ class Context:
         v = ctypes.c_ulonglong()
         _check(lib().thr_context_bytes(ctypes.byref(v)), "thr_context_bytes")
         return int(v.value)
+
+    def trim(self):
+        """Releases this context's scratch buffers (it must be the bound one)."""
+        _check(lib().thr_context_trim(), "thr_context_trim")
 
     def destroy(self):
         if self.handle:
@@ -356,8 +380,9 @@ def evolve_theta_pp_dev(iterl, dt, pp, phi, res, am, imass, beta, theta, links64
 
 
 def hamilton(phi, res2, am, imass, beta, theta, pp, u, dims, coeffs, am3=1.0,
-             max_qmr_iters=50000, kernel="shamir"):
-    """dwf3d_lib.F90:511-616; coeffs = (anum4, aden4, bnum2, bden2)."""
+             max_qmr_iters=50000, kernel="shamir", use_sp=False):
+    """dwf3d_lib.F90:511-616; coeffs = (anum4, aden4, bnum2, bden2); use_sp
+    as for qmrherm (the reference's hamilton solves in FP64)."""
     init()
     a4, d4, b2, e2 = [np.ascontiguousarray(c, dtype=np.float64) for c in coeffs]
     th = np.asfortranarray(theta, dtype=np.float32)
@@ -365,7 +390,7 @@ def hamilton(phi, res2, am, imass, beta, theta, pp, u, dims, coeffs, am3=1.0,
     out = np.zeros(5)
     its = np.zeros(2, dtype=np.int32)
     _check(lib().thr_hamilton(_p(phi), res2, am, int(imass), am3, beta, _p(th), _p(ppf), _p(u),
-                              _dims(dims), _p(a4), _p(d4), _p(b2), _p(e2), len(d4),
+                              _dims(dims), _p(a4), _p(d4), _p(b2), _p(e2), len(d4), _sp(use_sp),
                               int(max_qmr_iters), KERNEL[kernel], _p(out), _p(its)), "hamilton")
     keys = ("h", "hg", "hp", "s", "hf")
     return dict(zip(keys, out.tolist())), (int(its[0]), int(its[1]))
@@ -397,6 +422,23 @@ def device_copy(dst, src, nbytes, stream=None):
     """Streaming device-to-device copy (nbytes a multiple of 16)."""
     init()
     _check(lib().thr_device_copy(_p(dst), _p(src), int(nbytes), _stream(stream)), "device_copy")
+
+
+def gaussian_dev(out, n, sigma, seed, prec="f64", stream=None):
+    """Fills out with n reals sigma*N(0,1) (counter-based: element i depends
+    only on (seed, i)); prec "f64" (double) or "f32" (float)."""
+    init()
+    _check(lib().thr_gaussian_dev(_p(out), int(n), PREC[prec], float(sigma),
+                                  int(seed) & 0xFFFFFFFFFFFFFFFF, _stream(stream)),
+           "gaussian_dev")
+
+
+def norms_dev(x, nvec, n, stream=None):
+    """FP64 L2 norms of nvec complex128 vectors of n elements each."""
+    init()
+    out = np.zeros(int(nvec), dtype=np.float64)
+    _check(lib().thr_norms_dev(_p(x), int(nvec), int(n), _p(out), _stream(stream)), "norms_dev")
+    return out
 
 
 def dslash_dev(out, inp, links, dims, am, imass=3, am3=1.0, dagger=False, prec="f64",
@@ -492,7 +534,10 @@ def multishift_fast_dev(links64, links32, dims, am, imass, aden, b, x, tol, maxc
 
     corrections: rounds of FP64 defect correction (default: 4 when th32 or th16
     lowers the Krylov precision, else 0); stats (dict, optional) receives
-    "corr_iterations"."""
+    "corr_iterations" and "switches" (iteration at which the Krylov storage
+    entered FP64, FP32, FP16; -1 never).  Raises ThrNoConvergence when a true
+    residual stays above tol (ThrNonFinite for NaN/Inf); the partial results
+    are then still in x and in the exception's `result`."""
     init()
     n = len(aden)
     if corrections is None:
@@ -502,15 +547,22 @@ def multishift_fast_dev(links64, links32, dims, am, imass, aden, b, x, tol, maxc
     it, ci = ctypes.c_int(), ctypes.c_int()
     cgr = np.zeros(nsrc, dtype=np.int32)
     rel = np.zeros(nsrc * n, dtype=np.float64) if (true_residuals or corrections) else None
-    _check(lib().thr_multishift_fast_dev(_p(links64), _p(links32), _dims(dims), int(nsrc),
-                                         _p(ams), int(imass), am3, n, _p(aden), _p(b), _p(x),
-                                         float(tol), int(maxcg), float(th32), float(th16),
-                                         int(corrections), ctypes.byref(it), _p(cgr),
-                                         ctypes.byref(ci), _p(rel), _stream(stream)),
-           "multishift_fast_dev")
+    sw = np.full(3, -1, dtype=np.int32)
+    rc = lib().thr_multishift_fast_dev(_p(links64), _p(links32), _dims(dims), int(nsrc), _p(ams),
+                                       int(imass), am3, n, _p(aden), _p(b), _p(x), float(tol),
+                                       int(maxcg), float(th32), float(th16), int(corrections),
+                                       ctypes.byref(it), _p(cgr), ctypes.byref(ci), _p(rel),
+                                       _p(sw), _stream(stream))
     if stats is not None:
         stats["corr_iterations"] = ci.value
-    return it.value, cgr, (rel.reshape(nsrc, n) if rel is not None else None)
+        stats["switches"] = [int(v) for v in sw]
+    result = (it.value, cgr, (rel.reshape(nsrc, n) if rel is not None else None))
+    try:
+        _check(rc, "multishift_fast_dev")
+    except ThrError as e:
+        e.result = result
+        raise
+    return result
 
 
 def true_residuals_dev(links64, dims, am, imass, aden, b, x, nsrc=1, am3=1.0, stream=None):
