@@ -116,6 +116,18 @@ int thr_congrad(const double *phi, double *x, const double *u, const int dims[4]
                 float am, int imass, float am3, int niterc, int kernel, int *itercg,
                 int *iterations);
 
+/* measure_module.F90:170-423  one noise vector of measure_shamir: the four
+ * wall-source solves of the inner loop (measure_module.F90:210-372).  ps and
+ * pt are gauss0's haloed real32 fields ps(0:Lx+1, 0:Ly+1, 0:Lt+1, 2) (only
+ * the interior is read).  Each source xi goes through dslashd_shamir, then
+ * congrad (operator `kernel`, initial guess xi, cap niterc) solves
+ * M^dagger M xi = M^dagger xi; pbp1 / pbp2 receive the unnormalised sums
+ * psibarpsi1 / psibarpsi2 (re, im) of the reference before its division by
+ * kvol, iters the sum of the four itercg.  imass 1, 3 and 5. */
+int thr_measure_shamir_noise(const float *ps, const float *pt, const double *u, const int dims[4],
+                             float res, float am, int imass, float am3, int niterc, int kernel,
+                             double pbp1[2], double pbp2[2], int *iters);
+
 /* multishift_module.F90:13-284 (use_sp = 0) and 289-534 (use_sp = 1)
  * multishift_solver(u, am, imass, ndiagq, aden, anum, output, input, res,
  *                   maxcg, cg_return, cg_returns) */

@@ -111,3 +111,25 @@ def test_fortran_qmrherm_and_derivs(tmp_path):
     O.derivs(R, X, TAU, 0, DIMS, ds)
     gd = load(tmp_path / "derivs_shamir.bin", (12, 12, 12, 3), np.float64)
     check(gd - st["dsdpi"], ds - st["dsdpi"], 1e-12)
+
+def test_fortran_measure(tmp_path):
+    """measure_shamir and measure (measure_module.F90:142-423) through the
+    drop-in measure_module: the reference's known answer psibarpsi =
+    2.504295E-04, aviter = 5 (tests/test_measure_shamir.F90:45-46) with the
+    reference's noise (SITE_RANDOM streams, tests/fortran/test_random.F90),
+    the same result through `measure` with isweep_total (dwf3d_lib.F90:355),
+    and the reference's output records: knoise lines in fort.100 per
+    measurement, one in fort.200."""
+    exe = os.path.join(BIN, "test_measure_drop_in")
+    assert os.path.exists(exe), f"{exe} not built (run __graft_entry__.build())"
+    print("case: Fortran measure_shamir 12^3 x 24 imass 3", flush=True)
+    p = subprocess.run([exe], capture_output=True, text=True, timeout=300, cwd=tmp_path)
+    print(p.stdout, flush=True)
+    assert p.returncode == 0, p.stdout + p.stderr
+    assert "PASS measure_shamir" in p.stdout
+    f100 = (tmp_path / "fort.100").read_text().splitlines()
+    f200 = (tmp_path / "fort.200").read_text().splitlines()
+    assert len(f100) == 20 and len(f200) == 2
+    assert len(f100[0].split()) == 4 and len(f100[10].split()) == 5  # isweep_total on the second
+    assert f200[1].split()[0] == "7"
+    assert abs(float(f200[0].split()[0]) - 2.504295e-04) <= 1e-3 * 2.504295e-04

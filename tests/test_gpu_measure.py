@@ -44,3 +44,27 @@ def test_measure_batched_matches_reference_order(G, imass, knoise):
     assert info["sources"] == 4 * knoise
     assert info["max_true_residual"] <= 1e-10
     assert abs(got - ref) <= 1e-6 * abs(ref)
+
+
+@pytest.mark.parametrize("imass", [1, 3, 5])
+def test_measure_noise_entry_point_matches_oracle(G, imass):
+    """thr_measure_shamir_noise (one noise vector's four wall solves on the
+    device) against the same measurement composed from the oracle's dslashd
+    and congrad (measure_module.F90:210-372): equal psibarpsi and equal
+    iteration counts, for every imass (imass 5 reads at the source wall)."""
+    import measure
+    from site_random import SiteRandom
+    dims = (4, 4, 8, 8)
+    lx, ly, lt, ls = dims
+    print("case: measure_shamir_noise imass", imass, "4x4x8 Ls 8 seed 31", flush=True)
+    rng = np.random.default_rng(31)
+    th = (rng.standard_normal((lx, ly, lt, 3)) / np.sqrt(0.4)).astype(np.float32)
+    u = O.coef(th, dims)
+    res, am = np.float32(1e-8), np.float32(0.05)
+    got, itg = measure.measure_shamir(u, dims, res, am, imass, SiteRandom(lx, ly, lt, 9.0),
+                                      knoise=2)
+    ref, ito = measure.measure_shamir(u, dims, res, am, imass, SiteRandom(lx, ly, lt, 9.0),
+                                      knoise=2, ops=O)
+    print("psibarpsi", got, ref, "aviter", itg, ito, flush=True)
+    assert abs(itg - ito) <= 0.25  # at most one iteration apart per solve
+    assert abs(got - ref) <= 1e-6 * abs(ref)

@@ -239,6 +239,48 @@ def test_imass5_condensate_equals_imass3():
     assert abs(c5_flipped + c3) < 1e-12 * abs(c3)
 
 
+@pytest.mark.parametrize("imass", [1, 3, 5])
+def test_measure_wall_table_matches_dense_condensate(imass):
+    """The wall sources and reads of measure_shamir's noise loop
+    (thirring-rhmc_amd/measure.py, the table the GPU entry point
+    thr_measure_shamir_noise follows) against the dense contraction of
+    _exact_condensate, for one noise vector: the solved composition (oracle
+    dslashd + congrad to res 1e-9) equals sum over wall sites of
+    conj(eta) M^-1 xi built from the dense inverse, for every imass.
+    _exact_condensate's imass 5 table is itself pinned by the imass 3 = 5
+    condensate equality above."""
+    from site_random import SiteRandom
+    rng = np.random.default_rng(23)
+    dims = TINY
+    lx, ly, lt, ls = dims
+    u = D.rand_links(dims, rng, kind="theta")
+    m = np.float32(0.25)
+    Mi = np.linalg.inv(D.dense(lambda v: O.dslash(v, u, dims, m, imass), dims))
+    gen = SiteRandom(lx, ly, lt, 5.0)
+    got, _ = measure.measure_shamir(u, dims, np.float32(1e-9), m, imass,
+                                    SiteRandom(lx, ly, lt, 5.0), knoise=1, ops=O, niterc=1000)
+    ps, pt = gen.gauss0(), gen.gauss0()
+    e1 = (ps[..., 0] + 1j * ps[..., 1].astype(np.float64)).ravel(order="F")  # (x, y, t)
+    e2 = (pt[..., 0] + 1j * pt[..., 1].astype(np.float64)).ravel(order="F")
+    sites = np.arange(lx * ly * lt)
+
+    def wall(s, a):  # interior indices of (s, x, y, t, spin a) over the 4-D sites
+        return s + ls * (sites + lx * ly * lt * a)
+
+    def contract(eta, read, src):  # sum_{x, x'} conj(eta(x)) M^-1[read(x), src(x')] eta(x')
+        return np.conj(eta) @ Mi[np.ix_(read, src)] @ eta
+    p1 = p2 = 0j
+    for ids in (0, 1):
+        if imass != 5:
+            p1 += contract(e1, wall(ls - 1, ids), wall(0, ids))
+            p2 += contract(e2, wall(0, ids + 2), wall(ls - 1, ids + 2))
+        else:
+            p1 -= contract(e1, wall(0, ids + 2), wall(0, ids))
+            p2 -= contract(e2, wall(ls - 1, ids), wall(ls - 1, ids + 2))
+    want = float(np.float32(measure._finish(p1, p2, imass, lx * ly * lt)))
+    assert abs(got - want) <= 1e-6 * abs(want), (got, want)
+
+
 @pytest.mark.parametrize("kernel,imass", [("shamir", 1), ("shamir", 3), ("shamir", 5),
                                           ("wilson", 1), ("wilson", 3)])
 def test_adjoint(kernel, imass):

@@ -276,6 +276,76 @@ int thr_congrad(const double *phi, double *x, const double *u, const int dims[4]
   return THR_OK;
 }
 
+int thr_measure_shamir_noise(const float *ps, const float *pt, const double *u, const int dims[4],
+                             float res, float am, int imass, float am3, int niterc, int kernel,
+                             double pbp1[2], double pbp2[2], int *iters) {
+  TRY(check_dims(dims));
+  TRY(check_op(kernel, imass));
+  THR_CHECK(ps && pt && u && pbp1 && pbp2 && iters, THR_ERR_ARG, "NULL argument");
+  const Dims d = mkdims(dims);
+  hipStream_t s = default_stream();
+  double2 *u64;
+  TRY(stage_links(d, u, &u64, s));
+  const long n = d.nspinor();
+  const long nh4 = (long)(d.lx + 2) * (d.ly + 2) * (d.lt + 2) * 2;
+  const long nbd = wall_dot_blocks(d);
+  double2 *xi = (double2 *)workspace(W_A, sizeof(double2) * n);
+  double2 *phi = (double2 *)workspace(W_B, sizeof(double2) * n);
+  float *noise = (float *)workspace(W_P, sizeof(float) * 2 * nh4);
+  double *part = (double *)workspace(W_T, sizeof(double) * 2 * nbd);
+  double *hpart = (double *)host_workspace(W_T, sizeof(double) * 2 * nbd);
+  THR_CHECK(xi && phi && noise && part && hpart, THR_ERR_HIP, "workspace allocation failed");
+  THR_HIP(hipMemcpyAsync(noise, ps, sizeof(float) * nh4, hipMemcpyHostToDevice, s));
+  THR_HIP(hipMemcpyAsync(noise + nh4, pt, sizeof(float) * nh4, hipMemcpyHostToDevice, s));
+  // the source is dslashd_shamir's input whatever operator congrad uses
+  // (measure_module.F90:245,325); congrad's guess is the source (vector::X)
+  const OpDesc shamir = mkop(d, 0, imass, am, am3, u64);
+  const OpDesc op = mkop(d, kernel, imass, am, am3, u64);
+  const int wall0 = 0, wallL = d.ls - 1;
+  double acc[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+  int total = 0;
+  for (int id = 0; id < 2; id++) {
+    for (int side = 0; side < 2; side++) {
+      // side 0: source on s = 1 from ps (psibarpsi1); side 1: on s = Ls from pt (psibarpsi2).
+      // imass 1, 3: eta on spin id (side 0) / id + 2 (side 1), read at the other wall on the
+      // same spin; imass 5: -eta on the chirality-flipped spin, read at the same wall
+      // (measure_module.F90:224-234, 255-266, 284-293, 312-324, 334-348)
+      const float *eta = noise + side * nh4;
+      const int s_src = side == 0 ? wall0 : wallL;
+      int spin_src, spin_read, s_read;
+      double sign;
+      if (imass != 5) {
+        spin_src = spin_read = side == 0 ? id : id + 2;
+        s_read = side == 0 ? wallL : wall0;
+        sign = 1.0;
+      } else {
+        spin_src = side == 0 ? id : id + 2;
+        spin_read = side == 0 ? id + 2 : id;
+        s_read = s_src;
+        sign = -1.0;
+      }
+      TRY(launch_wall_source(d, eta, spin_src, s_src, sign, xi, s));
+      TRY(apply_op(shamir, 1, xi, phi, s));
+      int itercg = 0;
+      TRY(cg_reference(op, phi, xi, res, niterc, &itercg, nullptr, s));
+      total += itercg;
+      TRY(launch_wall_dot(d, eta, xi, spin_read, s_read, part, s));
+      THR_HIP(hipMemcpyAsync(hpart, part, sizeof(double) * 2 * nbd, hipMemcpyDeviceToHost, s));
+      THR_HIP(hipStreamSynchronize(s));
+      for (long b = 0; b < nbd; b++) {
+        acc[side][0] += hpart[2 * b];
+        acc[side][1] += hpart[2 * b + 1];
+      }
+    }
+  }
+  pbp1[0] = acc[0][0];
+  pbp1[1] = acc[0][1];
+  pbp2[0] = acc[1][0];
+  pbp2[1] = acc[1][1];
+  *iters = total;
+  return THR_OK;
+}
+
 int thr_multishift(const double *u, const int dims[4], float am, int imass, float am3,
                    int ndiagq, const double *aden, const double *anum, double *output,
                    const double *input, float res, int maxcg, int use_sp, int kernel,

@@ -73,6 +73,16 @@ int convert_d2f(long n, const double2 *a, float2 *b, hipStream_t s);
 // streaming device-to-device copy (bytes % 16 == 0, 16-byte aligned pointers)
 int stream_copy(const void *src, void *dst, size_t bytes, hipStream_t s);
 
+// psibarpsi measurement (measure.hip): wall source sign * eta on (spin, s_wall)
+// from a haloed real32 noise field ps(0:Lx+1, 0:Ly+1, 0:Lt+1, 2), and
+// per-block partial sums (re, im pairs, wall_dot_blocks(d) of them) of
+// sum_x conj(eta(x)) X(s_read, x, spin)
+int launch_wall_source(const Dims &d, const float *ps, int spin, int s_wall, double sign,
+                       double2 *out, hipStream_t s);
+long wall_dot_blocks(const Dims &d);
+int launch_wall_dot(const Dims &d, const float *ps, const double2 *X, int spin, int s_read,
+                    double *partial, hipStream_t s);
+
 // apply M or M^+ (FP64) into out
 int apply_op(const OpDesc &op, int dagger, const double2 *in, double2 *out, hipStream_t s);
 

@@ -13,6 +13,7 @@
 // Force, derivs_module.F90:99-196: for every link (x, mu) the sum over s and
 // spin of the Wilson and Dirac parts of d/dtheta Re<R, M^(+) X2>, with the
 // float32 coefficient tzi_real = 2*real(anum).
+#include <algorithm>
 #include <vector>
 
 #include "kernels.h"
@@ -91,6 +92,106 @@ __constant__ double kGre[3][4] = {{0, 0, 0, 0}, {-1, 1, 1, -1}, {0, 0, 0, 0}};
 __constant__ double kGim[3][4] = {{-1, -1, 1, 1}, {0, 0, 0, 0}, {-1, 1, 1, -1}};
 __constant__ int kGin[3][4] = {{3, 2, 1, 0}, {3, 2, 1, 0}, {2, 3, 0, 1}};
 
+// Shamir force, streaming form.  A block holds nl whole s-lines (columns
+// (x, y), Ls consecutive threads each, s fastest: every load of a spin plane
+// is one contiguous run) and marches them through a chunk of t.  At each step
+// a thread forms the three link terms of its 5-D site, whose neighbours are
+// the site itself and x+mu: the t neighbour loaded at step t is the centre of
+// step t+1, so per step a thread loads R and X2 at (x+1), (y+1) and (t+1) only.
+// The Ls terms of each link are summed in LDS in increasing s (a fixed order,
+// so results are identical run to run) and added into dSdpi.
+//   link term (x, mu) = w1 sum_a Im(R0_a^* X1_a - R1_a^* X0_a)
+//                     + w2 sum_a Im(g_mu,a (R0_a^* X1_b + R1_a^* X0_b)),  b = gamin(mu, a)
+// with (R0, X0) at x and (R1, X1) at x+mu, w1 = fl32(tzi/2), w2 = +-tzi
+// (derivs_module.F90:131-187).
+template <int MU>
+__device__ inline double derivs_term(const cx<double> R0[4], const cx<double> X0[4],
+                                     const cx<double> R1[4], const cx<double> X1[4], double w1,
+                                     double w2) {
+  // gamval(mu, a) / akappa (gammamatrices.F90:27-75) and gamin(mu, a) - 1
+  constexpr double gre[3][4] = {{0, 0, 0, 0}, {-1, 1, 1, -1}, {0, 0, 0, 0}};
+  constexpr double gim[3][4] = {{-1, -1, 1, 1}, {0, 0, 0, 0}, {-1, 1, 1, -1}};
+  constexpr int gin[3][4] = {{3, 2, 1, 0}, {3, 2, 1, 0}, {2, 3, 0, 1}};
+  double sw = 0.0, sd = 0.0;
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+    const int b = gin[MU][a];
+    sw += cjmul(R0[a], X1[a]).i - cjmul(R1[a], X0[a]).i;
+    const cx<double> g{0.5 * gre[MU][a], 0.5 * gim[MU][a]};
+    sd += cmul(g, cjmul(R0[a], X1[b]) + cjmul(R1[a], X0[b])).i;
+  }
+  return w1 * sw + w2 * sd;
+}
+
+__device__ inline void ld_spinor(const double2 *__restrict__ p, unsigned v5, unsigned site,
+                                 cx<double> v[4]) {
+#pragma unroll
+  for (int a = 0; a < 4; a++) v[a] = ldc(p, (long)a * v5 + site);
+}
+
+__global__ __launch_bounds__(BLK) void k_derivs_march(const Dims d, const double2 *__restrict__ R,
+                                                      const double2 *__restrict__ X2, float tzi,
+                                                      int iflag, unsigned nl, unsigned tchunk,
+                                                      double *ds) {
+  __shared__ double terms[2][3][BLK];
+  const unsigned lx = d.lx, ly = d.ly, lt = d.lt, ls = d.ls;
+  const unsigned v2 = lx * ly, v3 = v2 * ls, v4 = v2 * lt, v5 = v4 * ls;
+  const unsigned nb2 = (v2 + nl - 1) / nl;  // blocks per t-chunk
+  const unsigned chunk = blockIdx.x / nb2, lb = blockIdx.x - chunk * nb2;
+  const unsigned line = threadIdx.x / ls, s = threadIdx.x - line * ls;
+  const unsigned col = lb * nl + line;  // 2-D site x + lx y
+  const unsigned t0 = chunk * tchunk, t1 = min(t0 + tchunk, lt);
+  const bool valid = line < nl && col < v2;
+  const double w1 = (double)(tzi * 0.5f);  // tzi_real*real(akappa)
+  const double w2 = (double)tzi * (iflag == 0 ? 1.0 : -1.0);
+  unsigned x = 0, y = 0;
+  if (valid) {
+    x = col % lx;
+    y = col / lx;
+  }
+  const unsigned xf = (x + 1 == lx) ? 0 : x + 1, yf = (y + 1 == ly) ? 0 : y + 1;
+  const unsigned j = s + ls * col, jx = s + ls * (xf + lx * y), jy = s + ls * (x + lx * yf);
+  cx<double> R0[4], X0[4];
+  if (valid) {
+    ld_spinor(R, v5, j + v3 * t0, R0);
+    ld_spinor(X2, v5, j + v3 * t0, X0);
+  }
+  for (unsigned t = t0; t < t1; t++) {
+    const unsigned pb = (t - t0) & 1;
+    if (valid) {
+      const unsigned tf = (t + 1 == lt) ? 0 : t + 1;
+      cx<double> R1[4], X1[4], Rt[4], Xt[4];
+      ld_spinor(R, v5, jx + v3 * t, R1);
+      ld_spinor(X2, v5, jx + v3 * t, X1);
+      terms[pb][0][threadIdx.x] = derivs_term<0>(R0, X0, R1, X1, w1, w2);
+      ld_spinor(R, v5, jy + v3 * t, R1);
+      ld_spinor(X2, v5, jy + v3 * t, X1);
+      terms[pb][1][threadIdx.x] = derivs_term<1>(R0, X0, R1, X1, w1, w2);
+      ld_spinor(R, v5, j + v3 * tf, Rt);
+      ld_spinor(X2, v5, j + v3 * tf, Xt);
+      terms[pb][2][threadIdx.x] = derivs_term<2>(R0, X0, Rt, Xt, w1, w2);
+#pragma unroll
+      for (int a = 0; a < 4; a++) {
+        R0[a] = Rt[a];
+        X0[a] = Xt[a];
+      }
+    }
+    // one barrier per step: buffer pb is written again two steps later, after
+    // the next step's barrier, by when its readers below are done
+    __syncthreads();
+    for (unsigned r = threadIdx.x; r < 3 * nl; r += blockDim.x) {  // 3 nl > blockDim when Ls < 3
+      const unsigned mu = r / nl, ln = r - mu * nl, c = lb * nl + ln;
+      if (c < v2) {
+        double sum = 0.0;
+        for (unsigned k = 0; k < ls; k++) sum += terms[pb][mu][ln * ls + k];
+        ds[(long)mu * v4 + c + v2 * t] += sum;
+      }
+    }
+  }
+}
+
+// any Ls: one thread per link, serial over s and spin (used when an s-line
+// does not fit one block)
 __global__ __launch_bounds__(BLK) void k_derivs(const Dims d, const double2 *__restrict__ R,
                                                 const double2 *__restrict__ X2, float tzi,
                                                 int iflag, double *ds) {
@@ -268,6 +369,20 @@ int launch_dslash_wilson(const Dims &d, int nrhs, int dagger, int imass, float a
 int launch_derivs(const Dims &d, const double2 *R, const double2 *X2, double anum, int iflag,
                   double *dsdpi, hipStream_t s) {
   const float tzi = 2.0f * (float)anum;  // real :: tzi_real = 2*real(anum)
+  THR_CHECK(d.v5() * 4 < (1L << 31), THR_ERR_SIZE, "derivs: 4*V5 must be below 2^31");
+  if (d.ls <= BLK) {
+    // whole s-lines per block; t-chunks long enough to reuse the t neighbour
+    // but short enough to leave ~2^18 threads in flight
+    const unsigned nl = BLK / d.ls, v2 = (unsigned)d.lx * d.ly;
+    const long v5 = d.v5();
+    const unsigned tchunk = (unsigned)std::max(1L, std::min<long>(d.lt, v5 >> 18));
+    const unsigned nchunk = (d.lt + tchunk - 1) / tchunk;
+    const unsigned nb = ((v2 + nl - 1) / nl) * nchunk;
+    hipLaunchKernelGGL(k_derivs_march, dim3(nb), dim3(nl * d.ls), 0, s, d, R, X2, tzi, iflag, nl,
+                       tchunk, dsdpi);
+    THR_HIP(hipGetLastError());
+    return THR_OK;
+  }
   const unsigned nb = (unsigned)((d.v4() * 3 + BLK - 1) / BLK);
   hipLaunchKernelGGL(k_derivs, dim3(nb), dim3(BLK), 0, s, d, R, X2, tzi, iflag, dsdpi);
   THR_HIP(hipGetLastError());

@@ -1,7 +1,7 @@
 ! Fortran drop-in modules for j-ostmeyer/thirring-rhmc's hot path.
 !
 ! Same module names, procedure names, argument lists and array shapes as the
-! reference (dirac.F90, measure_module.F90 congrad, multishift_module.F90,
+! reference (dirac.F90, multishift_module.F90,
 ! qmrherm_module.F90, derivs_module.F90, gauge.F90, inverter_utils.F90 and the
 ! state modules trial / vector / gforce / dum1 / gammamatrices), compiled
 ! against the caller's own `params` module (built with -DMPI and
@@ -9,7 +9,9 @@
 ! numerical work runs on the GPU through libthirring_hip.so; halo exchange is
 ! a periodic host copy and MPI requests are inert.
 !
-! Module order follows the dependencies.
+! Module order follows the dependencies.  `measure_module` (congrad, measure,
+! measure_shamir) is in measure_shim.F90, compiled after the caller's
+! `random` and `gaussian` modules.
 
 module comms_common
   implicit none
@@ -376,43 +378,6 @@ contains
     call dirac_op_shifted(xout, xin, u, am, imass, 0.0)
   end subroutine dirac_operator
 end module inverter_utils
-
-module measure_module
-  implicit none
-contains
-  ! measure_module.F90:9-140: CG on M^dagger M, initial guess and solution in vector::X
-  subroutine congrad(Phi, res, itercg, am, imass, iterations)
-    use trial, only: u
-    use vector
-    use params
-    use thr_capi
-    complex(dp), intent(in), target :: Phi(0:kthird_l + 1, 0:ksizex_l + 1, 0:ksizey_l + 1, &
-                                           0:ksizet_l + 1, 4)
-    real, intent(in) :: res, am
-    integer, intent(out) :: itercg
-    integer, intent(in) :: imass
-    integer, intent(out), optional :: iterations
-    integer(c_int) :: d(4), it, nx
-    call thr_ensure_init()
-    d = [int(ksizex_l, c_int), int(ksizey_l, c_int), int(ksizet_l, c_int), int(kthird_l, c_int)]
-    call thr_check(thr_congrad(c_loc(Phi), c_loc(X), c_loc(u), d, real(res, c_float), &
-                               real(am, c_float), int(imass, c_int), real(am3, c_float), &
-                               int(niterc, c_int), generation_kernel(), it, nx), 'congrad')
-    itercg = it
-    if (nx > niterc) then
-      if (ip_global_zero()) then
-        write (7, '(a)') ' # iterations of congrad exceeds niterc'
-        write (7, *) "res, Iterations:", res, nx
-      end if
-    end if
-    if (present(iterations)) iterations = nx
-  end subroutine congrad
-
-  logical function ip_global_zero()
-    use comms_common, only: ip_global
-    ip_global_zero = ip_global == 0
-  end function ip_global_zero
-end module measure_module
 
 module multishift_module
   implicit none

@@ -1,15 +1,18 @@
 """Noisy psibarpsi measurement with domain-wall wall sources
-(measure_shamir, measure_module.F90:170-423), imass 1 or 3, on one GPU.
+(measure_shamir, measure_module.F90:170-423) on one GPU.
 
 Per noise vector the reference draws two Gaussian fields (gauss0) and solves
-four systems M^dagger M x = M^dagger xi: sources on the wall s = 1 (spins 1,
-2), read at s = Ls, and on s = Ls (spins 3, 4), read at s = 1.  psibarpsi is
-the average over `knoise` vectors of the wall-to-wall contractions.
+four systems M^dagger M x = M^dagger xi: for imass 1 and 3, sources on the
+wall s = 1 (spins 1, 2), read at s = Ls, and on s = Ls (spins 3, 4), read at
+s = 1; for imass 5 the sources sit on the chirality-flipped spins and are read
+at their own wall.  psibarpsi is the average over `knoise` vectors of the
+wall-to-wall contractions.
 
 Two ways to run it:
   * `measure_shamir` — the reference's order: one congrad per source,
-    starting from the source (so iteration counts, hence `aviter`, match);
-  * `measure_shamir_batched` — all 4 knoise sources of the measurement
+    starting from the source (so iteration counts, hence `aviter`, match),
+    imass 1, 3, 5; on the GPU one library call per noise vector;
+  * `measure_shamir_batched` (imass 1, 3) — all 4 knoise sources of the measurement
     solved together in multi-RHS launches of the batched solver to an FP64
     true residual, the fields staying on the device.  Same noise, same
     contractions; psibarpsi agrees with a converged `measure_shamir`.
@@ -36,7 +39,9 @@ def _finish(p1, p2, imass, kvol):
     """psibarpsi of one noise vector (measure_module.F90:394-405)."""
     if imass == 1:
         return (p1 / kvol + p2 / kvol).real
-    return (-1j * p1 / kvol + 1j * p2 / kvol).real
+    if imass == 3:
+        return (-1j * p1 / kvol + 1j * p2 / kvol).real
+    return (-1j * p1 / kvol - 1j * p2 / kvol).real
 
 
 def _average(pbp, iters, knoise):
@@ -44,42 +49,74 @@ def _average(pbp, iters, knoise):
     return float(np.float32(np.sum(pbp)) / knoise), iters / (4 * knoise)
 
 
-def _sources(rng, dims, knoise):
-    """The 4 knoise wall sources in the reference's draw order: per noise
-    vector ps, pt = gauss0(), gauss0(); for ids in (1, 2) the source on s = 1
-    (spin ids, noise ps, read at s = Ls) then on s = Ls (spin ids + 2, noise
-    pt, read at s = 1).  Yields (k, wall, spin, read, eta)."""
-    lx, ly, lt, ls = dims
-    for k in range(knoise):
-        ps, pt = rng.gauss0(), rng.gauss0()
-        for ids in (0, 1):
-            yield k, 1, ids, ls, _noise(ps)
-            yield k, ls, ids + 2, 1, _noise(pt)
+def _walls(dims, imass):
+    """The four wall solves of one noise vector in the reference's order: for
+    ids in (1, 2), side 0 (source on s = 1 from ps, summed into psibarpsi1)
+    then side 1 (source on s = Ls from pt, into psibarpsi2).  Yields
+    (side, src_s, src_spin, sign, read_s, read_spin), s 1-based, spins
+    0-based.  imass 1, 3: eta on spin ids (side 0) or ids + 2 (side 1), read
+    on the same spin at the other wall; imass 5: -eta on the chirality-flipped
+    spin, read at the same wall (measure_module.F90:224-348)."""
+    ls = dims[3]
+    for ids in (0, 1):
+        if imass != 5:
+            yield 0, 1, ids, 1.0, ls, ids
+            yield 1, ls, ids + 2, 1.0, 1, ids + 2
+        else:
+            yield 0, 1, ids, -1.0, 1, ids + 2
+            yield 1, ls, ids + 2, -1.0, ls, ids
 
 
-def measure_shamir(u, dims, res, am, imass, rng, knoise=10, am3=1.0, ops=G):
+def _haloed_noise(ps):
+    """gauss0's field with its periodic halo, (Lx+2, Ly+2, Lt+2, 2) float32."""
+    return np.asfortranarray(np.pad(ps, ((1, 1), (1, 1), (1, 1), (0, 0)), mode="wrap"))
+
+
+def measure_shamir(u, dims, res, am, imass, rng, knoise=10, am3=1.0, ops=G, niterc=None):
     """measure_shamir (measure_module.F90:170-423) on one process: returns
     (psibarpsi, aviter).  u is the haloed link array, rng a
-    site_random.SiteRandom; `ops` supplies dslashd and congrad with the
-    signatures of thirring_hip (the GPU library by default)."""
-    if imass not in (1, 3):
-        raise ValueError("measure_shamir supports imass 1 and 3")
+    site_random.SiteRandom (draws ps, pt = gauss0(), gauss0() per noise
+    vector, as the reference does).  With `ops` the GPU library (the default)
+    each noise vector's four solves run in one call of
+    thr_measure_shamir_noise, fields staying on the device; any other `ops`
+    supplies dslashd and congrad with the signatures of thirring_hip and the
+    solves run one by one through it (the oracle, in the tests).  niterc is
+    congrad's iteration cap (default Ls V4, params.F90:57)."""
+    if imass not in (1, 3, 5):
+        raise ValueError("imass must be 1, 3 or 5")
     lx, ly, lt, ls = dims
-    p1 = np.zeros(knoise, dtype=np.complex128)
-    p2 = np.zeros(knoise, dtype=np.complex128)
-    iters = 0
-    for k, wall, spin, read, eta in _sources(rng, dims, knoise):
-        xi = _wall_source(dims, wall, spin, eta)
-        phi = G.halo5(ops.dslashd(xi, u, dims, am, imass, am3=am3))
-        it, _ = ops.congrad(phi, xi, u, dims, res, am, imass, am3=am3)  # x starts at xi
-        iters += it
-        acc = np.sum(np.conj(eta) * xi[read, 1:-1, 1:-1, 1:-1, spin])
-        if wall == 1:
-            p1[k] += acc
+    pbp, iters = [], 0
+    for _ in range(knoise):
+        ps, pt = rng.gauss0(), rng.gauss0()
+        if ops is G:
+            p1, p2, it = G.measure_shamir_noise(_haloed_noise(ps), _haloed_noise(pt), u, dims,
+                                                res, am, imass, am3=am3, niterc=niterc)
+            iters += it
         else:
-            p2[k] += acc
-    pbp = [_finish(p1[k], p2[k], imass, lx * ly * lt) for k in range(knoise)]
+            p1 = p2 = 0j
+            for side, src_s, src_spin, sign, read_s, read_spin in _walls(dims, imass):
+                eta = _noise(ps if side == 0 else pt)
+                xi = _wall_source(dims, src_s, src_spin, sign * eta)
+                phi = G.halo5(ops.dslashd(xi, u, dims, am, imass, am3=am3))
+                # x starts at xi
+                it, _ = ops.congrad(phi, xi, u, dims, res, am, imass, am3=am3, niterc=niterc)
+                iters += it
+                acc = np.sum(np.conj(eta) * xi[read_s, 1:-1, 1:-1, 1:-1, read_spin])
+                if side == 0:
+                    p1 += acc
+                else:
+                    p2 += acc
+        pbp.append(_finish(p1, p2, imass, lx * ly * lt))
     return _average(pbp, iters, knoise)
+
+
+def _sources(rng, dims, knoise):
+    """The 4 knoise wall sources (imass 1, 3) in the reference's draw order,
+    for the batched form: yields (k, wall, spin, read, eta)."""
+    for k in range(knoise):
+        ps, pt = rng.gauss0(), rng.gauss0()
+        for side, src_s, src_spin, _, read_s, _ in _walls(dims, 3):
+            yield k, src_s, src_spin, read_s, _noise(ps if side == 0 else pt)
 
 
 def measure_shamir_batched(links64, links32, dims, am, imass, rng, knoise=10, tol=1e-10,

@@ -42,6 +42,7 @@ _SIGS = {
     "thr_dslashd": ([P, P, P, P, F32, I32, F32, I32], I32),
     "thr_dirac_op_shifted": ([P, P, P, P, F32, I32, F32, F64, I32], I32),
     "thr_congrad": ([P, P, P, P, F32, F32, I32, F32, I32, I32, IP, IP], I32),
+    "thr_measure_shamir_noise": ([P, P, P, P, F32, F32, I32, F32, I32, I32, P, P, IP], I32),
     "thr_multishift": ([P, P, F32, I32, F32, I32, P, P, P, P, F32, I32, I32, I32, IP, P], I32),
     "thr_derivs": ([P, P, F64, I32, P, F32, I32, F32, I32, P], I32),
     "thr_qmrherm": ([P, P, F32, IP, F32, I32, F32, P, P, I32, I32, I32, P, P, I32, I32, P, P, P,
@@ -239,6 +240,30 @@ def congrad(phi, x, u, dims, res, am, imass, am3=1.0, niterc=None, kernel="shami
                              int(niterc), KERNEL[kernel], ctypes.byref(it), ctypes.byref(nx)),
            "congrad")
     return it.value, nx.value
+
+
+def measure_shamir_noise(ps, pt, u, dims, res, am, imass, am3=1.0, niterc=None,
+                         kernel="shamir"):
+    """One noise vector of measure_shamir (measure_module.F90:210-372) on the
+    GPU: ps, pt are gauss0 fields, float32 haloed (Lx+2, Ly+2, Lt+2, 2) in
+    Fortran order.  Returns (psibarpsi1, psibarpsi2, iters): the complex sums
+    before the division by kvol and the total congrad iterations of the four
+    solves."""
+    init()
+    lx, ly, lt, ls = dims
+    if niterc is None:
+        niterc = ls * lx * ly * lt
+    for a in (ps, pt):
+        if a.dtype != np.float32 or a.shape != (lx + 2, ly + 2, lt + 2, 2) or \
+                not a.flags.f_contiguous:
+            raise ValueError("noise fields must be float32 (Lx+2, Ly+2, Lt+2, 2) in Fortran order")
+    p1 = np.zeros(2)
+    p2 = np.zeros(2)
+    it = ctypes.c_int()
+    _check(lib().thr_measure_shamir_noise(_p(ps), _p(pt), _p(u), _dims(dims), res, am, int(imass),
+                                          am3, int(niterc), KERNEL[kernel], _p(p1), _p(p2),
+                                          ctypes.byref(it)), "measure_shamir_noise")
+    return complex(p1[0], p1[1]), complex(p2[0], p2[1]), it.value
 
 
 def multishift(u, dims, am, imass, aden, anum, inp, res, maxcg, am3=1.0, sp=False,
