@@ -17,13 +17,18 @@ Also reported in the same JSON line:
   * FP32 / FP16 operator bandwidth, and the FP64 operator over 4 and 16
     right-hand sides per launch;
   * multishift solves/s to FP64 true residual 1e-10 for all 25 poles of the
-    x^(-1/4) approximation (remez4 block 2): the plain FP64 GPU path (the
-    reference's recurrences) against the batched, blocked-update,
-    mixed-precision path, with the true residuals of both;
+    x^(-1/4) approximation (remez4 block 2) at 32^3 x Ls=64: the plain FP64
+    GPU path (the reference's recurrences) against the batched,
+    blocked-update, mixed-precision path, with the true residuals of both and
+    the per-pole difference of their solutions;
+  * the batched multi-RHS workload (16 sources x 25 shifts at 24^3 x Ls=32,
+    per-source masses): solves/s of the batched solver against the plain
+    FP64 path;
   * the RHMC force at 16^3 x Ls=48: the FP64 solver against the
     mixed-precision solver with FP64 correction (time and difference);
-  * serving: p50/p99 latency and solves/s of a Poisson load of measurement
-    solves with continuous batching (thirring-rhmc_amd/serving.py);
+  * mixed against FP64 at 32^3 x Ls=64: force and dH differences;
+  * serving: a sweep of Poisson arrival rates, each with p50/p99 latency and
+    solves/s under continuous batching (thirring-rhmc_amd/serving.py);
   * the CPU baseline: the FP64 C oracle's Dslash on the host cores.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
@@ -162,8 +167,10 @@ def bench_multishift(G, hr, dims, l64, l32, am, nsrc, stream, tol=1e-10, maxcg=5
     bh = rng.standard_normal((nsrc, nsp, 2)) / np.sqrt(2)
     b = hr.DeviceArray.from_host(bh)
     out = {}
+    plain_x = None
     if "plain_fp64" in legs_wanted:
-        out.update(_plain_leg(G, hr, dims, l64, am, bh, sig, n, nsp, tol, maxcg, stream))
+        leg, plain_x = _plain_leg(G, hr, dims, l64, am, bh, sig, n, nsp, tol, maxcg, stream)
+        out.update(leg)
     # both batched legs verify every (source, shift) in FP64 inside the timed
     # region and correct the ones above tol (defect correction)
     legs = (("blocked_fp64", 0.0, 0.0, 2), ("blocked_mixed", mixed_th32, mixed_th16, 4))
@@ -186,7 +193,13 @@ def bench_multishift(G, hr, dims, l64, l32, am, nsrc, stream, tol=1e-10, maxcg=5
                       "correction_iterations": int(st.get("corr_iterations", 0)),
                       "th32": th32, "th16": th16,
                       "max_true_residual": float(rel.max())}
-        del x
+        if plain_x is not None:
+            # source 0 against the plain FP64 solution, per pole
+            x0 = x.head_to_host(1)[0]
+            out[label]["max_pole_rel_diff_vs_plain_fp64"] = max(
+                float(np.linalg.norm(x0[i] - plain_x[i]) / np.linalg.norm(plain_x[i]))
+                for i in range(n))
+        x.free()
     for label in ("blocked_fp64", "blocked_mixed"):
         if label in out and "plain_fp64" in out:
             out[f"speedup_{label}_vs_plain_fp64"] = round(out[label]["solves_per_s"] /
@@ -208,10 +221,115 @@ def _plain_leg(G, hr, dims, l64, am, bh, sig, n, nsp, tol, maxcg, stream):
     hr.synchronize()
     t_plain = time.perf_counter() - t
     rel_plain = G.true_residuals_dev(l64, dims, am, 3, sig, b0, xp, stream=stream)
-    del xp
+    xh = xp.to_host()[0]
+    xp.free()
+    b0.free()
     return {"plain_fp64": {"solves_per_s": round(1.0 / t_plain, 4), "seconds": round(t_plain, 3),
                            "iterations": int(cg_plain),
-                           "max_true_residual": float(rel_plain.max())}}
+                           "max_true_residual": float(rel_plain.max())}}, xh
+
+
+def bench_batched_config4(G, hr, stream, dims=(24, 24, 24, 32), nsrc=16, tol=1e-10,
+                          maxcg=50000, seed=24):
+    """The batched multi-RHS workload: 16 sources x 25 shifts of x^(-1/4)
+    (remez4 block 2) at 24^3 x Ls=32, one link field, per-source masses drawn
+    from {0.005, 0.01, 0.02, 0.05}: the batched solver (FP32 Krylov storage
+    from the start, hipGraph-captured blocks, FP64 defect correction) against
+    the plain FP64 path (the reference's recurrences, one source at a time,
+    each with its own mass); solves/s to FP64 true residual `tol` for both."""
+    lx, ly, lt, ls = dims
+    nsp = 4 * lx * ly * lt * ls
+    _, l64, l32 = make_fields(G, hr.DeviceArray, dims, 0.3, seed)
+    a0, a, sig = remez_block("remez4", 1)
+    n = len(sig)
+    rng = np.random.default_rng(seed)
+    masses = rng.choice([0.005, 0.01, 0.02, 0.05], size=nsrc).astype(np.float32)
+    bh = rng.standard_normal((nsrc, nsp, 2)) / np.sqrt(2)
+    b = hr.DeviceArray.from_host(bh)
+    out = {"lattice": list(dims), "sources": nsrc, "shifts": n, "masses": masses.tolist()}
+    x = hr.DeviceArray((nsrc, n, nsp, 2), np.float64)
+    progress(f"config-4 batched solve {dims}: {nsrc} sources x {n} shifts")
+    hr.synchronize()
+    st = {}
+    t = time.perf_counter()
+    it, _, _ = G.multishift_fast_dev(l64, l32, dims, masses, 3, sig, b, x, tol=tol, maxcg=maxcg,
+                                     nsrc=nsrc, th32=1.0, true_residuals=False, corrections=4,
+                                     stats=st, stream=stream)
+    hr.synchronize()
+    dt = time.perf_counter() - t
+    rel = G.true_residuals_dev(l64, dims, masses, 3, sig, b, x, nsrc=nsrc, stream=stream)
+    out["batched_mixed"] = {"solves_per_s": round(nsrc / dt, 4), "seconds": round(dt, 3),
+                            "iterations": int(it),
+                            "correction_iterations": int(st.get("corr_iterations", 0)),
+                            "max_true_residual": float(rel.max())}
+    x.free()
+    progress(f"config-4 plain FP64, {nsrc} sources one at a time")
+    xp = hr.DeviceArray((1, n, nsp, 2), np.float64)
+    worst, total, its = 0.0, 0.0, []
+    for k in range(nsrc):
+        bk = hr.DeviceArray.from_host(bh[k:k + 1])
+        hr.synchronize()
+        t = time.perf_counter()
+        cg, _ = G.multishift_plain_dev(l64, dims, masses[k], 3, sig, bk, xp, tol=0.75 * tol,
+                                       maxcg=maxcg, stream=stream)
+        hr.synchronize()
+        total += time.perf_counter() - t
+        its.append(int(cg))
+        worst = max(worst, float(G.true_residuals_dev(l64, dims, masses[k], 3, sig, bk, xp,
+                                                      stream=stream).max()))
+        bk.free()
+    xp.free()
+    out["plain_fp64"] = {"solves_per_s": round(nsrc / total, 4), "seconds": round(total, 3),
+                         "iterations": its, "max_true_residual": worst}
+    out["speedup_batched_mixed_vs_plain_fp64"] = round(out["batched_mixed"]["solves_per_s"]
+                                                       / out["plain_fp64"]["solves_per_s"], 3)
+    b.free()
+    l64.free()
+    l32.free()
+    return out
+
+
+def bench_config3_errors(G, dims=(32, 32, 32, 64), am=0.005, beta=0.3, seed=303):
+    """Mixed precision against FP64 at 32^3 x Ls=64 (am 0.005) for the RHMC
+    force (force_module.F90:9-113, res1 = 1e-8) and the Hamiltonian
+    (dwf3d_lib.F90:511-616, res2 = 1e-10) at two gauge fields a small MD
+    step apart: relative L2 difference of dSdpi, relative difference of the
+    fermion action hf and the absolute difference of dH, with times."""
+    import md
+    lx, ly, lt, ls = dims
+    rng = np.random.default_rng(seed)
+    th0 = np.asfortranarray((rng.standard_normal((lx, ly, lt, 3)) / np.sqrt(beta))
+                            .astype(np.float32))
+    phi = np.zeros((ls + 2, lx + 2, ly + 2, lt + 2, 4), dtype=np.complex128, order="F")
+    phi[1:-1, 1:-1, 1:-1, 1:-1] = (rng.standard_normal((ls, lx, ly, lt, 4))
+                                   + 1j * rng.standard_normal((ls, lx, ly, lt, 4))) / np.sqrt(2)
+    G.halo5(phi)
+    pp = np.asfortranarray(rng.standard_normal(th0.shape).astype(np.float32))
+    th1 = np.asfortranarray((th0 + np.float32(0.02) * pp).astype(np.float32))
+    u0, u1 = G.coef(th0, dims), G.coef(th1, dims)
+    cs = md.coefficient_sets()
+    out = {"lattice": list(dims), "am": am, "res1": 1e-8, "res2": 1e-10}
+    forces, ham = {}, {}
+    for name, sp in (("fp64", False), ("mixed", "mixed")):
+        progress(f"config-3 force and hamilton, {name}, {dims}")
+        t = time.perf_counter()
+        forces[name], _ = G.force(phi, 1e-8, am, 3, beta, th0, u0, dims, cs["force"], use_sp=sp)
+        tf = time.perf_counter() - t
+        t = time.perf_counter()
+        ham[name] = [G.hamilton(phi, 1e-10, am, 3, beta, th, pp, u, dims, cs["hamilton"],
+                                use_sp=sp)[0] for th, u in ((th0, u0), (th1, u1))]
+        out[name] = {"force_seconds": round(tf, 3),
+                     "hamilton_seconds": round((time.perf_counter() - t) / 2, 3)}
+    f64 = forces["fp64"]
+    out["force_mixed_vs_fp64_rel_l2"] = float(np.linalg.norm(forces["mixed"] - f64)
+                                              / np.linalg.norm(f64))
+    out["hf_mixed_vs_fp64_rel"] = max(abs(ham["mixed"][k]["hf"] / ham["fp64"][k]["hf"] - 1)
+                                      for k in (0, 1))
+    dh = {k: v[0]["h"] - v[1]["h"] for k, v in ham.items()}
+    out["dH_fp64"] = dh["fp64"]
+    out["dH_mixed"] = dh["mixed"]
+    out["dH_abs_diff"] = abs(dh["mixed"] - dh["fp64"])
+    return out
 
 
 def bench_force(G, dims=(16, 16, 16, 48), am=0.01, beta=0.3, res1=1e-8, seed=77):
@@ -247,13 +365,43 @@ def bench_force(G, dims=(16, 16, 16, 48), am=0.01, beta=0.3, res1=1e-8, seed=77)
     return out
 
 
-def bench_serving(rate, seconds, slots, max_requests):
+def bench_serving(rates, seconds, slots):
     """Poisson load of measurement solves (L in {8,12,16,24}, T in {L,2L}, Ls in
     {8,16,32,48}, beta in [0.2,0.5], am in {0.005..0.05}, 1-12 sources, FP64
-    true residual 1e-10), continuous batching over `slots` contexts."""
+    true residual 1e-10), continuous batching over `slots` contexts, at each
+    rate of the sweep: p50/p99 latency, solves/s, and the requests shed when
+    the arrival window closed; the highest sustainable rate is the largest one
+    that shed and failed nothing."""
     import serving as S
-    progress(f"serving: Poisson rate {rate}/s for {seconds} s, {slots} slots")
-    return S.run_load(rate=rate, duration=seconds, seed=3, slots=slots, max_requests=max_requests)
+    return S.rate_sweep(rates, seconds, seed=3, slots=slots, progress=progress)
+
+
+KERNEL_SOURCES = ("thirring-rhmc_amd/csrc/dslash.hip", "thirring-rhmc_amd/csrc/kernels.h",
+                  "thirring-rhmc_amd/csrc/common.h")
+
+
+def kernel_source_digest():
+    """sha256 over the Dslash kernel's sources (what the counters measured)."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()
+
+
+def measured_traffic():
+    """HBM bytes per FP64 Dslash launch from the committed rocprofv3 counter
+    summary, only if it was taken on the current kernel sources (else null:
+    a changed kernel needs new counters)."""
+    tpath = os.path.join(ROOT, "profiles", "dslash_f64_32x32x32x64_traffic.json")
+    if not os.path.exists(tpath):
+        return None, "no counter summary"
+    with open(tpath) as fh:
+        rec = json.load(fh)
+    if rec.get("kernel_source_sha256") != kernel_source_digest():
+        return None, "counter summary predates the current Dslash sources"
+    return rec.get("hbm_bytes_per_launch"), rec.get("source")
 
 
 def copy_rate(G, hr, stream, nbytes=1 << 30, reps=20):
@@ -312,8 +460,12 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-multi-rhs", action="store_true")
     ap.add_argument("--no-force", action="store_true")
-    ap.add_argument("--serve-rate", type=float, default=2.0)
-    ap.add_argument("--serve-seconds", type=float, default=15.0)
+    ap.add_argument("--no-config4", action="store_true")
+    ap.add_argument("--no-config3-errors", action="store_true")
+    ap.add_argument("--serve-rates", default="1,2,5,10,20,50,100,200",
+                    help="Poisson arrival rates (per s) of the serving sweep")
+    ap.add_argument("--serve-seconds", type=float, default=6.0,
+                    help="arrival window per rate (requests still queued at its end are shed)")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--solver-lattice", default="32,32,32,64")
     ap.add_argument("--solver-am", type=float, default=0.005)
@@ -390,17 +542,16 @@ def main():
                                   legs_wanted=tuple(args.solver_legs.split(",")))
         solver["lattice"] = list(sd)
         solver["am"] = args.solver_am
+    config4 = None if args.no_config4 else bench_batched_config4(G, hr, stream)
     force = None if args.no_force else bench_force(G)
+    config3 = None if args.no_config3_errors else bench_config3_errors(G)
     serving = None
     if not args.no_serving:
-        serving = bench_serving(args.serve_rate, args.serve_seconds, args.serve_slots, 60)
+        serving = bench_serving([float(r) for r in args.serve_rates.split(",")],
+                                args.serve_seconds, args.serve_slots)
     wall = time.time() - t_wall0
 
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "dslash_f64_32x32x32x64_traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as fh:
-            traffic = json.load(fh).get("hbm_bytes_per_launch")
+    traffic, traffic_note = measured_traffic()
 
     progress("cpu baseline")
     cpu = None if args.no_cpu else cpu_baseline(dims, args.cpu_seconds)
@@ -425,14 +576,17 @@ def main():
                    "launch_shape": shapes,
                    "region_ms_per_step": round(region64, 5)},
         "roofline": {"bound": "hbm", "achieved": round(gbs64, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(gbs64 / HBM_PEAK_GBS, 4), "traffic": traffic},
+                     "unit": "GB/s", "frac": round(gbs64 / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": traffic_note},
         "hbm_copy_GB_s": round(copy_gbs, 1),
         "frac_of_copy": round(gbs64 / copy_gbs, 4),
         "cpu_baseline": cpu,
         "dslash_other_precisions": extra,
         "dslash_multi_rhs": multi,
         "multishift": solver,
+        "batched_multi_rhs_config4": config4,
         "force": force,
+        "mixed_vs_fp64_32x32x32x64": config3,
         "serving": serving,
         "bench_wall_s": round(wall, 2),
     }

@@ -130,6 +130,13 @@ def test_fortran_measure(tmp_path):
     f100 = (tmp_path / "fort.100").read_text().splitlines()
     f200 = (tmp_path / "fort.200").read_text().splitlines()
     assert len(f100) == 20 and len(f200) == 2
-    assert len(f100[0].split()) == 4 and len(f100[10].split()) == 5  # isweep_total on the second
-    assert f200[1].split()[0] == "7"
-    assert abs(float(f200[0].split()[0]) - 2.504295e-04) <= 1e-3 * 2.504295e-04
+
+    def fields(line, width, lead):  # fixed-width records: optional I5, then E fields
+        body = line[5:] if lead else line
+        return [float(body[k:k + width]) for k in range(0, len(body), width)]
+    # '(4E17.9E3)' then '(I5,4E17.9E3)' with isweep_total; '(2E15.7E3)', '(I5,2E15.7E3)'
+    assert len(f100[0]) == 68 and len(fields(f100[0], 17, False)) == 4
+    assert len(f100[10]) == 73 and int(f100[10][:5]) == 7 and len(fields(f100[10], 17, True)) == 4
+    assert len(f200[0]) == 30 and int(f200[1][:5]) == 7
+    pbp0, pbp1 = fields(f200[0], 15, False)[0], fields(f200[1], 15, True)[0]
+    assert abs(pbp0 - 2.504295e-04) <= 1e-3 * 2.504295e-04 and pbp1 == pbp0

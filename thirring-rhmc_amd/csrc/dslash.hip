@@ -198,6 +198,79 @@ template <int PREC> struct BAcc {
   __device__ inline cx<C> load1(unsigned site, unsigned a) const { return ld(site, a, scale(site)); }
 };
 
+// A 5-D site's four spins as the kernel carries them: FP64 and FP32 as
+// complex values; FP16 as the stored half2 words with the site's scale,
+// expanded only inside the arithmetic, where v_fma_mix_f32 reads a half
+// operand directly (no separate conversion or scaling instructions).
+template <int PREC> struct Site {
+  cx<typename Store<PREC>::C> v[4];
+};
+template <> struct Site<P16> {
+  unsigned h[4];
+  float sc;
+};
+// one spin of a site (s hops, wall terms)
+template <int PREC> struct Spin {
+  cx<typename Store<PREC>::C> v;
+};
+template <> struct Spin<P16> {
+  unsigned h;
+  float sc;
+};
+
+__device__ inline float lo16(unsigned w) { return __low2float(__builtin_bit_cast(__half2, w)); }
+__device__ inline float hi16(unsigned w) { return __high2float(__builtin_bit_cast(__half2, w)); }
+// sc * (half2 word), the conversion folded into the multiply (fma with a zero
+// addend: hipcc then emits v_fma_mix_f32 with the half operand)
+__device__ inline cx<float> expand(unsigned w, float sc) {
+  return {fmaf(sc, lo16(w), 0.0f), fmaf(sc, hi16(w), 0.0f)};
+}
+
+template <int PREC> __device__ inline cx<typename Store<PREC>::C> spin_value(const Spin<PREC> &p) {
+  if constexpr (PREC == P16)
+    return expand(p.h, p.sc);
+  else
+    return p.v;
+}
+template <int PREC> __device__ inline Spin<PREC> spin_of(const Site<PREC> &S, int a) {
+  if constexpr (PREC == P16)
+    return {S.h[a], S.sc};
+  else
+    return {S.v[a]};
+}
+// o -= value of p
+template <int PREC, typename C> __device__ inline void sub_spin(cx<C> &o, const Spin<PREC> &p) {
+  if constexpr (PREC == P16) {
+    o.r = fmaf(-p.sc, lo16(p.h), o.r);
+    o.i = fmaf(-p.sc, hi16(p.h), o.i);
+  } else {
+    o = o - p.v;
+  }
+}
+
+template <int PREC> __device__ inline Site<PREC> load_site(const BAcc<PREC> &in, unsigned site) {
+  Site<PREC> S;
+  if constexpr (PREC == P16) {  // the site's 4 spins in one 128-bit load
+    const auto q = __builtin_amdgcn_raw_buffer_load_b128(in.rs, site * 16u, 0, 0);
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+      const unsigned e = q[a];  // the element as a value first (see BAcc::load4)
+      S.h[a] = e;
+    }
+    S.sc = in.scale(site);
+  } else {
+    in.load4(site, S.v);
+  }
+  return S;
+}
+template <int PREC>
+__device__ inline Spin<PREC> load_spin(const BAcc<PREC> &in, unsigned site, unsigned a) {
+  if constexpr (PREC == P16)
+    return {__builtin_amdgcn_raw_buffer_load_b32(in.rs, site * 16u + a * 4u, 0, 0), in.scale(site)};
+  else
+    return {in.ld(site, a, 1.0f)};
+}
+
 template <typename C> struct BLinks {
   __amdgpu_buffer_rsrc_t r;
   __device__ inline BLinks(const void *p, unsigned n) {
@@ -299,25 +372,18 @@ template <typename C> __device__ inline cx<C> link(const void *links, long i) {
   }
 }
 
-// o[a] += 1/2 gamval(mu,a) D[gamin(mu,a)]  (gammamatrices.F90:27-75, kappa = 1/2)
-template <typename C> __device__ inline void add_gamma(int mu, cx<C> o[4], const cx<C> D[4]) {
-  const C h = C(0.5);
-  if (mu == 0) {  // gamma_1: (-i,-i,i,i) at (4,3,2,1)
-    o[0] = o[0] - h * muli(D[3]);
-    o[1] = o[1] - h * muli(D[2]);
-    o[2] = o[2] + h * muli(D[1]);
-    o[3] = o[3] + h * muli(D[0]);
-  } else if (mu == 1) {  // gamma_2: (-1,1,1,-1) at (4,3,2,1)
-    o[0] = o[0] - h * D[3];
-    o[1] = o[1] + h * D[2];
-    o[2] = o[2] + h * D[1];
-    o[3] = o[3] - h * D[0];
-  } else {  // gamma_3: (-i,i,i,-i) at (3,4,1,2)
-    o[0] = o[0] - h * muli(D[2]);
-    o[1] = o[1] + h * muli(D[3]);
-    o[2] = o[2] + h * muli(D[0]);
-    o[3] = o[3] - h * muli(D[1]);
-  }
+// gamma_mu (gammamatrices.F90:27-75, without kappa): (gamma_mu v)[a] =
+// e(mu, a) v[p(mu, a)], p an involution pairing {0, 1} with {2, 3} and
+// e(mu, a) e(mu, p(a)) = 1 (gamma_mu^2 = 1).  Phases: 0 -> +1, 1 -> -1,
+// 2 -> +i, 3 -> -i.
+__device__ constexpr int kGamP[3][4] = {{3, 2, 1, 0}, {3, 2, 1, 0}, {2, 3, 0, 1}};
+__device__ constexpr int kGamE[3][4] = {{3, 3, 2, 2}, {1, 0, 0, 1}, {3, 2, 2, 3}};
+
+template <int E, typename C> __device__ inline cx<C> phase(cx<C> v) {
+  if constexpr (E == 0) return v;
+  else if constexpr (E == 1) return {-v.r, -v.i};
+  else if constexpr (E == 2) return {-v.i, v.r};
+  else return {v.i, -v.r};
 }
 
 // value of v held by lane+1 (NEXT) or lane-1 (PREV) of the wave: DPP wave
@@ -340,24 +406,111 @@ template <bool NEXT, typename C> __device__ inline cx<C> lane_shift(cx<C> v) {
   return {lane_shift<NEXT>(v.r), lane_shift<NEXT>(v.i)};
 }
 
-// 4-D hop in direction mu: o -= 1/2 (A + B), o += 1/2 gamma_mu (A - B)
-// with A = U_mu(x) R(x+mu), B = U_mu(x-mu)^* R(x-mu) (roles of A, B swapped
-// in the gamma term for M^dagger)
-template <typename C, bool DAG>
-__device__ inline void hop(int mu, cx<C> o[4], cx<C> uf, const cx<C> rf[4], cx<C> ub,
-                           const cx<C> rb[4]) {
-  cx<C> W[4], D[4];
-#pragma unroll
-  for (int a = 0; a < 4; a++) {
-    const cx<C> A = cmul(uf, rf[a]);
-    const cx<C> B = cmul(ub, rb[a]);
-    W[a] = A + B;
-    D[a] = DAG ? (B - A) : (A - B);
+// One side of a 4-D hop: o -= 1/2 (1 + SIG gamma_mu) U v.  (1 + SIG gamma_mu)
+// has rank 2: its upper components are c_a = v_a + SIG e_a v_p(a) (a = 0, 1)
+// and its lower ones SIG e_b c_p(b) (b = 2, 3), so the link multiplies the two
+// components of c only (Wilson spin projection).  For FP16 sites c is formed
+// from the half words with the site scale folded in.
+template <int MU, int SIG, int PREC, typename C>
+__device__ inline void half_hop(cx<C> o[4], cx<C> u, const Site<PREC> &S) {
+  constexpr int p0 = kGamP[MU][0], p1 = kGamP[MU][1];
+  cx<C> c0, c1;
+  if constexpr (PREC == P16) {
+    const float sp = S.sc, sq = SIG > 0 ? S.sc : -S.sc;
+    const cx<float> q0 = phase<kGamE[MU][0]>(cx<float>{lo16(S.h[p0]), hi16(S.h[p0])});
+    const cx<float> q1 = phase<kGamE[MU][1]>(cx<float>{lo16(S.h[p1]), hi16(S.h[p1])});
+    c0 = {fmaf(sp, lo16(S.h[0]), fmaf(sq, q0.r, 0.0f)), fmaf(sp, hi16(S.h[0]), fmaf(sq, q0.i, 0.0f))};
+    c1 = {fmaf(sp, lo16(S.h[1]), fmaf(sq, q1.r, 0.0f)), fmaf(sp, hi16(S.h[1]), fmaf(sq, q1.i, 0.0f))};
+  } else {
+    const cx<C> q0 = phase<kGamE[MU][0]>(S.v[p0]), q1 = phase<kGamE[MU][1]>(S.v[p1]);
+    c0 = SIG > 0 ? S.v[0] + q0 : S.v[0] - q0;
+    c1 = SIG > 0 ? S.v[1] + q1 : S.v[1] - q1;
   }
-#pragma unroll
-  for (int a = 0; a < 4; a++) o[a] = o[a] - C(0.5) * W[a];
-  add_gamma<C>(mu, o, D);
+  const cx<C> w0 = cmul(u, c0), w1 = cmul(u, c1);
+  const C h = C(0.5), hs = SIG > 0 ? C(0.5) : C(-0.5);
+  o[0] = o[0] - h * w0;
+  o[1] = o[1] - h * w1;
+  // lower components: b = p(a) for a = 0, 1
+  o[p0] = o[p0] - hs * phase<kGamE[MU][p0]>(w0);
+  o[p1] = o[p1] - hs * phase<kGamE[MU][p1]>(w1);
 }
+
+// 4-D hop in direction mu, dirac.F90:49-117 (M) and 330-355 (M^dagger):
+//   M:   o -= 1/2 (1 - gamma_mu) U_mu(x) R(x+mu) + 1/2 (1 + gamma_mu) U_mu(x-mu)^* R(x-mu)
+//   M^+: the projector signs swapped
+// uf = U_mu(x), rf = R(x+mu), ub = U_mu(x-mu)^*, rb = R(x-mu)
+template <int MU, bool DAG, int PREC, typename C>
+__device__ inline void hop(cx<C> o[4], cx<C> uf, const Site<PREC> &rf, cx<C> ub,
+                           const Site<PREC> &rb) {
+  half_hop<MU, DAG ? 1 : -1>(o, uf, rf);
+  half_hop<MU, DAG ? -1 : 1>(o, ub, rb);
+}
+
+// o = diag * S
+template <int PREC, typename C>
+__device__ inline void centre(cx<C> o[4], C diag, const Site<PREC> &S) {
+  if constexpr (PREC == P16) {
+    const float d = diag * S.sc;
+#pragma unroll
+    for (int a = 0; a < 4; a++) o[a] = expand(S.h[a], d);
+  } else {
+#pragma unroll
+    for (int a = 0; a < 4; a++) o[a] = diag * S.v[a];
+  }
+}
+
+// one spin of the site held by lane+1 (NEXT) or lane-1 of the wave
+template <bool NEXT> __device__ inline unsigned lane_shift_u(unsigned v) {
+  return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, NEXT ? 0x130 : 0x138, 0xF, 0xF, false);
+}
+template <bool NEXT, int PREC> __device__ inline Spin<PREC> shift_spin(const Site<PREC> &S, int a) {
+  if constexpr (PREC == P16)
+    return {lane_shift_u<NEXT>(S.h[a]), lane_shift<NEXT>(S.sc)};
+  else
+    return {lane_shift<NEXT>(S.v[a])};
+}
+// one spin of the site held by lane k of the wave
+template <int PREC> __device__ inline Spin<PREC> lane_spin(const Site<PREC> &S, int a, int k) {
+  if constexpr (PREC == P16)
+    return {(unsigned)__builtin_amdgcn_readlane((int)S.h[a], k), lane_value(S.sc, k)};
+  else
+    return {lane_value(S.v[a], k)};
+}
+
+// x-neighbour exchange through LDS (one buffer per step parity): FP64/FP32
+// sites as re/im values, FP16 sites as their 4 half2 words and scale
+template <int PREC> struct XBuf {
+  using C = typename Store<PREC>::C;
+  C (*b)[4][4][64][2];
+  __device__ inline void put(unsigned pb, unsigned w, unsigned ln, const Site<PREC> &S) const {
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+      b[pb][w][a][ln][0] = S.v[a].r;
+      b[pb][w][a][ln][1] = S.v[a].i;
+    }
+  }
+  __device__ inline Site<PREC> get(unsigned pb, unsigned w, unsigned ln) const {
+    Site<PREC> S;
+#pragma unroll
+    for (int a = 0; a < 4; a++) S.v[a] = {b[pb][w][a][ln][0], b[pb][w][a][ln][1]};
+    return S;
+  }
+};
+template <> struct XBuf<P16> {
+  unsigned (*b)[4][5][64];
+  __device__ inline void put(unsigned pb, unsigned w, unsigned ln, const Site<P16> &S) const {
+#pragma unroll
+    for (int a = 0; a < 4; a++) b[pb][w][a][ln] = S.h[a];
+    b[pb][w][4][ln] = __float_as_uint(S.sc);
+  }
+  __device__ inline Site<P16> get(unsigned pb, unsigned w, unsigned ln) const {
+    Site<P16> S;
+#pragma unroll
+    for (int a = 0; a < 4; a++) S.h[a] = b[pb][w][a][ln];
+    S.sc = __uint_as_float(b[pb][w][4][ln]);
+    return S;
+  }
+};
 
 // One thread owns the s-x-y site (s fastest) of one t-chunk and marches
 // through it: R(t+1) loaded at step t is the centre at t+1 and the backward
@@ -449,145 +602,130 @@ __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, const Tili
       eb = (C)L.b_ptr[(long)rhs * L.sc_stride];
     }
 
-    cx<C> prv[4], cur[4], nxt[4];
+    Site<PREC> prv, cur, nxt;
     const unsigned tb0 = (t0 == 0) ? lt - 1 : t0 - 1;
-    in.load4(j + v3 * tb0, prv);
+    prv = load_site(in, j + v3 * tb0);
     cx<C> ubt = conj(lk(2 * v4 + xy + v2 * tb0));
-    in.load4(j + v3 * t0, cur);
+    cur = load_site(in, j + v3 * t0);
     for (unsigned t = t0; t < t1; t++) {
       const unsigned tf = (t + 1 == lt) ? 0 : t + 1;
-      in.load4(j + v3 * tf, nxt);
+      nxt = load_site(in, j + v3 * tf);
       const cx<C> uft = lk(2 * v4 + xy + v2 * t);
       const unsigned i = j + v3 * t;
       const unsigned l4 = v2 * t;
       cx<C> o[4];
-#pragma unroll
-      for (int a = 0; a < 4; a++) o[a] = diag * cur[a];
+      centre(o, diag, cur);
       if constexpr (XL) {
         // the block's 4 waves are 4 consecutive x at one (y, t) (8 when
         // Ls = 32): the x neighbours inside the block are the centre
         // registers of the next/previous line, exchanged through LDS
         // (double-buffered by step parity, one barrier per step); the outer
         // lines load theirs
-        __shared__ C xbuf[2][4][4][64][2];
+        using XB = std::conditional_t<PREC == P16, unsigned[2][4][5][64], C[2][4][4][64][2]>;
+        __shared__ XB xraw;
+        const XBuf<PREC> xbuf{xraw};
         const unsigned w = threadIdx.x >> 6, ln = threadIdx.x & 63, pb = (t - t0) & 1;
-#pragma unroll
-        for (int a = 0; a < 4; a++) {
-          xbuf[pb][w][a][ln][0] = cur[a].r;
-          xbuf[pb][w][a][ln][1] = cur[a].i;
-        }
+        xbuf.put(pb, w, ln, cur);
         __syncthreads();
-        cx<C> rf[4], rb[4];
+        Site<PREC> rf, rb;
         if constexpr (UL > 1) {
           // Ls = 64 / UL: the block is 4 UL consecutive x, wave w holding
           // x = UL w + k in lanes [64 k / UL, 64 (k + 1) / UL); position p = UL w + k
           constexpr unsigned NL = UL > 1 ? UL : 2, W = 64 / NL;
           const unsigned p = NL * w + ln / W, sl = ln % W;
           if (p < 4 * NL - 1) {
-            const unsigned ww = (p + 1) / NL, ll = ((p + 1) % NL) * W + sl;
-#pragma unroll
-            for (int a = 0; a < 4; a++) rf[a] = {xbuf[pb][ww][a][ll][0], xbuf[pb][ww][a][ll][1]};
+            rf = xbuf.get(pb, (p + 1) / NL, ((p + 1) % NL) * W + sl);
           } else {
-            in.load4(jxf + v3 * t, rf);
+            rf = load_site(in, jxf + v3 * t);
           }
           if (p > 0) {
-            const unsigned ww = (p - 1) / NL, ll = ((p - 1) % NL) * W + sl;
-#pragma unroll
-            for (int a = 0; a < 4; a++) rb[a] = {xbuf[pb][ww][a][ll][0], xbuf[pb][ww][a][ll][1]};
+            rb = xbuf.get(pb, (p - 1) / NL, ((p - 1) % NL) * W + sl);
           } else {
-            in.load4(jxb + v3 * t, rb);
+            rb = load_site(in, jxb + v3 * t);
           }
         } else {
           if (w < 3) {
-#pragma unroll
-            for (int a = 0; a < 4; a++)
-              rf[a] = {xbuf[pb][w + 1][a][ln][0], xbuf[pb][w + 1][a][ln][1]};
+            rf = xbuf.get(pb, w + 1, ln);
           } else {
-            in.load4(jxf + v3 * t, rf);
+            rf = load_site(in, jxf + v3 * t);
           }
           if (w > 0) {
-#pragma unroll
-            for (int a = 0; a < 4; a++)
-              rb[a] = {xbuf[pb][w - 1][a][ln][0], xbuf[pb][w - 1][a][ln][1]};
+            rb = xbuf.get(pb, w - 1, ln);
           } else {
-            in.load4(jxb + v3 * t, rb);
+            rb = load_site(in, jxb + v3 * t);
           }
         }
-        hop<C, DAG>(0, o, lk(xy + l4), rf, conj(lk(lxb + l4)), rb);
+        hop<0, DAG>(o, lk(xy + l4), rf, conj(lk(lxb + l4)), rb);
       } else {
-        cx<C> rf[4], rb[4];
-        in.load4(jxf + v3 * t, rf);
-        in.load4(jxb + v3 * t, rb);
-        hop<C, DAG>(0, o, lk(xy + l4), rf, conj(lk(lxb + l4)), rb);
+        const Site<PREC> rf = load_site(in, jxf + v3 * t), rb = load_site(in, jxb + v3 * t);
+        hop<0, DAG>(o, lk(xy + l4), rf, conj(lk(lxb + l4)), rb);
       }
       {
-        cx<C> rf[4], rb[4];
-        in.load4(jyf + v3 * t, rf);
-        in.load4(jyb + v3 * t, rb);
-        hop<C, DAG>(1, o, lk(v4 + xy + l4), rf, conj(lk(v4 + lyb + l4)), rb);
+        const Site<PREC> rf = load_site(in, jyf + v3 * t), rb = load_site(in, jyb + v3 * t);
+        hop<1, DAG>(o, lk(v4 + xy + l4), rf, conj(lk(v4 + lyb + l4)), rb);
       }
-      hop<C, DAG>(2, o, uft, nxt, ubt, prv);
+      hop<2, DAG>(o, uft, nxt, ubt, prv);
 
       // hops along s (dirac.F90:80-87 for M, 317-324 for M^dagger): spins that
       // hop from s+1 are {3,4} for M and {1,2} for M^dagger; the others from s-1.
       if constexpr (SHUF) {
-        cx<C> u0 = lane_shift<true>(cur[up0]), u1 = lane_shift<true>(cur[up0 + 1]);
-        cx<C> d0 = lane_shift<false>(cur[dn0]), d1 = lane_shift<false>(cur[dn0 + 1]);
+        Spin<PREC> u0 = shift_spin<true>(cur, up0), u1 = shift_spin<true>(cur, up0 + 1);
+        Spin<PREC> d0 = shift_spin<false>(cur, dn0), d1 = shift_spin<false>(cur, dn0 + 1);
         if (UL == 0 && 64 % ls != 0) {
           // s-lines cross wave boundaries: the neighbour of the last lane
           // (s+1) and of the first lane (s-1) sit in the next/previous wave
           const unsigned ln = threadIdx.x & 63;
           if (ln == 63 && s + 1 < ls) {
-            u0 = in.load1(i + 1, up0);
-            u1 = in.load1(i + 1, up0 + 1);
+            u0 = load_spin(in, i + 1, up0);
+            u1 = load_spin(in, i + 1, up0 + 1);
           }
           if (ln == 0 && s > 0) {
-            d0 = in.load1(i - 1, dn0);
-            d1 = in.load1(i - 1, dn0 + 1);
+            d0 = load_spin(in, i - 1, dn0);
+            d1 = load_spin(in, i - 1, dn0 + 1);
           }
         }
         if (shops && s + 1 < ls) {
-          o[up0] = o[up0] - u0;
-          o[up0 + 1] = o[up0 + 1] - u1;
+          sub_spin(o[up0], u0);
+          sub_spin(o[up0 + 1], u1);
         }
         if (shops && s > 0) {
-          o[dn0] = o[dn0] - d0;
-          o[dn0 + 1] = o[dn0 + 1] - d1;
+          sub_spin(o[dn0], d0);
+          sub_spin(o[dn0 + 1], d1);
         }
       } else {
         if (shops && s + 1 < ls) {
-          o[up0] = o[up0] - in.load1(i + 1, up0);
-          o[up0 + 1] = o[up0 + 1] - in.load1(i + 1, up0 + 1);
+          sub_spin(o[up0], load_spin(in, i + 1, up0));
+          sub_spin(o[up0 + 1], load_spin(in, i + 1, up0 + 1));
         }
         if (shops && s > 0) {
-          o[dn0] = o[dn0] - in.load1(i - 1, dn0);
-          o[dn0 + 1] = o[dn0 + 1] - in.load1(i - 1, dn0 + 1);
+          sub_spin(o[dn0], load_spin(in, i - 1, dn0));
+          sub_spin(o[dn0 + 1], load_spin(in, i - 1, dn0 + 1));
         }
       }
       // wall mass (dirac.F90:119-176 for M, 356-402 for M^dagger)
       const unsigned col = i - s;  // s = 0 of this (x,y,t)
       if (imass == 1 || imass == 3) {
         // top: s = Ls couples to s = 1 on the up0 spins; bottom: s = 1 couples to s = Ls
-        cx<C> w0[2], wL[2];
+        Spin<PREC> w0[2], wL[2];
         if constexpr (UL == 1) {
           // the wave is the s-line: s = 0 is lane 0, s = Ls-1 is lane 63
 #pragma unroll
           for (int k = 0; k < 2; k++) {
-            w0[k] = lane_value(cur[up0 + k], 0);
-            wL[k] = lane_value(cur[dn0 + k], 63);
+            w0[k] = lane_spin(cur, up0 + k, 0);
+            wL[k] = lane_spin(cur, dn0 + k, 63);
           }
         }
         if (s == ls - 1) {
 #pragma unroll
           for (int k = 0; k < 2; k++) {
-            const cx<C> r0 = UL == 1 ? w0[k] : in.load1(col, up0 + k);
+            const cx<C> r0 = spin_value(UL == 1 ? w0[k] : load_spin(in, col, up0 + k));
             o[up0 + k] = o[up0 + k] + (imass == 1 ? m * r0 : m * muli(r0));
           }
         }
         if (s == 0) {
 #pragma unroll
           for (int k = 0; k < 2; k++) {
-            const cx<C> rL = UL == 1 ? wL[k] : in.load1(col + ls - 1, dn0 + k);
+            const cx<C> rL = spin_value(UL == 1 ? wL[k] : load_spin(in, col + ls - 1, dn0 + k));
             o[dn0 + k] = o[dn0 + k] + (imass == 1 ? m * rL : (-m) * muli(rL));
           }
         }
@@ -597,12 +735,12 @@ __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, const Tili
         if (s == ls - 1) {
 #pragma unroll
           for (int k = 0; k < 2; k++)
-            o[up0 + k] = o[up0 + k] + (sg * m) * muli(cur[dn0 + k]);
+            o[up0 + k] = o[up0 + k] + (sg * m) * muli(spin_value(spin_of(cur, dn0 + k)));
         }
         if (s == 0) {
 #pragma unroll
           for (int k = 0; k < 2; k++)
-            o[dn0 + k] = o[dn0 + k] + (sg * m) * muli(cur[up0 + k]);
+            o[dn0 + k] = o[dn0 + k] + (sg * m) * muli(spin_value(spin_of(cur, up0 + k)));
         }
       }
 
@@ -661,11 +799,8 @@ __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, const Tili
           for (int k = 0; k < 4; k++) red += (double)(wv[k].r * o[k].r + wv[k].i * o[k].i);
         }
       }
-#pragma unroll
-      for (int a = 0; a < 4; a++) {
-        prv[a] = cur[a];
-        cur[a] = nxt[a];
-      }
+      prv = cur;
+      cur = nxt;
       ubt = conj(uft);
     }
   }

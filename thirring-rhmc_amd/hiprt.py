@@ -90,6 +90,15 @@ class DeviceArray:
                               self.nbytes, D2H), "hipMemcpy D2H")
         return a
 
+    def head_to_host(self, k):
+        """The first k entries along axis 0, copied to the host."""
+        k = int(k)
+        assert 0 <= k <= self.shape[0]
+        a = np.empty((k,) + self.shape[1:], dtype=self.dtype)
+        check(hip().hipMemcpy(a.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(self.ptr),
+                              a.nbytes, D2H), "hipMemcpy D2H")
+        return a
+
     def copy_from(self, other):
         """Device-to-device copy, complete on return."""
         assert other.nbytes == self.nbytes
