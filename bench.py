@@ -16,6 +16,7 @@ stream the kernels run on.
 Also reported in the same JSON line:
   * FP32 / FP16 operator bandwidth, and the FP64 operator over 4 and 16
     right-hand sides per launch;
+  * the fermion force kernel derivs at 16^3 x Ls=48 and 32^3 x Ls=64;
   * multishift solves/s to FP64 true residual 1e-10 for all 25 poles of the
     x^(-1/4) approximation (remez4 block 2) at 32^3 x Ls=64: the plain FP64
     GPU path (the reference's recurrences) against the batched,
@@ -227,6 +228,42 @@ def _plain_leg(G, hr, dims, l64, am, bh, sig, n, nsp, tol, maxcg, stream):
     return {"plain_fp64": {"solves_per_s": round(1.0 / t_plain, 4), "seconds": round(t_plain, 3),
                            "iterations": int(cg_plain),
                            "max_true_residual": float(rel_plain.max())}}, xh
+
+
+def bench_derivs(G, hr, stream, steps=50, warmup=5):
+    """The Shamir fermion force derivs (derivs_module.F90:99-196) on device
+    fields at 16^3 x Ls=48 and 32^3 x Ls=64: mean kernel time from HIP
+    events around each launch on the stream, and GB/s of algorithmic bytes
+    (R and X2 read once, dSdpi read and written once: 2 V5 64 + 2 3 V4 8)."""
+    out = {}
+    rng = np.random.default_rng(11)
+    for dims in ((16, 16, 16, 48), (32, 32, 32, 64)):
+        lx, ly, lt, ls = dims
+        v4, v5 = lx * ly * lt, lx * ly * lt * ls
+        nb = max(2, int(np.ceil(4 * 256e6 / (2 * 64 * v5))))  # rotate past the Infinity Cache
+        rs = [hr.DeviceArray.from_host(rng.standard_normal((4 * v5, 2))) for _ in range(nb)]
+        xs = [hr.DeviceArray.from_host(rng.standard_normal((4 * v5, 2))) for _ in range(nb)]
+        ds = hr.DeviceArray((3 * v4,), np.float64)
+        ds.zero()
+        progress(f"derivs {dims}")
+        for k in range(warmup):
+            G.derivs_dev(rs[k % nb], xs[k % nb], 0.37, 0, dims, ds, stream=stream)
+        stream.synchronize()
+        ev = [(hr.Event(), hr.Event()) for _ in range(steps)]
+        for k in range(steps):
+            ev[k][0].record(stream)
+            G.derivs_dev(rs[k % nb], xs[k % nb], 0.37, k & 1, dims, ds, stream=stream)
+            ev[k][1].record(stream)
+        stream.synchronize()
+        ms = float(np.mean([a.elapsed_ms(b) for a, b in ev]))
+        nbytes = 2 * v5 * 64 + 2 * 3 * v4 * 8
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        out["x".join(map(str, dims))] = {"ms": round(ms, 5), "GB_s": round(gbs, 1),
+                                         "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4),
+                                         "algorithmic_bytes": nbytes}
+        for a in rs + xs + [ds]:
+            a.free()
+    return out
 
 
 def bench_batched_config4(G, hr, stream, dims=(24, 24, 24, 32), nsrc=16, tol=1e-10,
@@ -461,6 +498,7 @@ def main():
     ap.add_argument("--no-multi-rhs", action="store_true")
     ap.add_argument("--no-force", action="store_true")
     ap.add_argument("--no-config4", action="store_true")
+    ap.add_argument("--no-derivs", action="store_true")
     ap.add_argument("--no-config3-errors", action="store_true")
     ap.add_argument("--serve-rates", default="1,2,5,10,20,50,100,200",
                     help="Poisson arrival rates (per s) of the serving sweep")
@@ -542,6 +580,7 @@ def main():
                                   legs_wanted=tuple(args.solver_legs.split(",")))
         solver["lattice"] = list(sd)
         solver["am"] = args.solver_am
+    derivs = None if args.no_derivs else bench_derivs(G, hr, stream)
     config4 = None if args.no_config4 else bench_batched_config4(G, hr, stream)
     force = None if args.no_force else bench_force(G)
     config3 = None if args.no_config3_errors else bench_config3_errors(G)
@@ -583,6 +622,7 @@ def main():
         "cpu_baseline": cpu,
         "dslash_other_precisions": extra,
         "dslash_multi_rhs": multi,
+        "derivs": derivs,
         "multishift": solver,
         "batched_multi_rhs_config4": config4,
         "force": force,

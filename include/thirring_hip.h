@@ -284,6 +284,11 @@ int thr_congrad_dev(const void *phi, void *x, const void *links64, const int dim
                     float am, int imass, float am3, int niterc, int kernel, int *itercg,
                     int *iterations, void *stream);
 
+/* derivs_module.F90:8-26, 99-196 on device fields: r, x2 are spinors,
+ * dsdpi [mu][t][y][x] float64 accumulates */
+int thr_derivs_dev(const void *r, const void *x2, double anum, int iflag, const int dims[4],
+                   float am, int imass, int kernel, void *dsdpi, void *stream);
+
 /* reference-faithful multishift on device fields; out [ndiagq][4*V5] complex128 */
 int thr_multishift_dev(const void *links64, const int dims[4], float am, int imass, float am3,
                        int ndiagq, const double *aden, const double *anum, void *out,

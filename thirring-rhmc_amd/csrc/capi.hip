@@ -731,6 +731,17 @@ int thr_congrad_dev(const void *phi, void *x, const void *links64, const int dim
                       pick(stream));
 }
 
+int thr_derivs_dev(const void *r, const void *x2, double anum, int iflag, const int dims[4],
+                   float am, int imass, int kernel, void *dsdpi, void *stream) {
+  TRY(check_dims(dims));
+  TRY(check_op(kernel, imass));
+  THR_CHECK(r && x2 && dsdpi, THR_ERR_ARG, "NULL argument");
+  hipStream_t s = pick(stream);
+  return done_if_default(stream, derivs_dispatch(mkdims(dims), kernel, (const double2 *)r,
+                                                 (const double2 *)x2, anum, iflag, am, imass,
+                                                 (double *)dsdpi, s));
+}
+
 int thr_multishift_dev(const void *links64, const int dims[4], float am, int imass, float am3,
                        int ndiagq, const double *aden, const double *anum, void *out,
                        const void *input, float res, int maxcg, int use_sp, int kernel,

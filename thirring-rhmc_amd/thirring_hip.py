@@ -42,6 +42,7 @@ _SIGS = {
     "thr_dslashd": ([P, P, P, P, F32, I32, F32, I32], I32),
     "thr_dirac_op_shifted": ([P, P, P, P, F32, I32, F32, F64, I32], I32),
     "thr_congrad": ([P, P, P, P, F32, F32, I32, F32, I32, I32, IP, IP], I32),
+    "thr_derivs_dev": ([P, P, F64, I32, P, F32, I32, I32, P, P], I32),
     "thr_measure_shamir_noise": ([P, P, P, P, F32, F32, I32, F32, I32, I32, P, P, IP], I32),
     "thr_multishift": ([P, P, F32, I32, F32, I32, P, P, P, P, F32, I32, I32, I32, IP, P], I32),
     "thr_derivs": ([P, P, F64, I32, P, F32, I32, F32, I32, P], I32),
@@ -520,6 +521,16 @@ def congrad_dev(phi, x, links64, dims, res, am, imass, am3=1.0, niterc=None, ker
                                  am3, int(niterc), KERNEL[kernel], ctypes.byref(it),
                                  ctypes.byref(nx), _stream(stream)), "congrad_dev")
     return it.value, nx.value
+
+
+def derivs_dev(r, x2, anum, iflag, dims, dsdpi, am=0.0, imass=3, kernel="shamir", stream=None):
+    """derivs on device fields (R, X2 spinors, dsdpi [mu][t][y][x] float64,
+    accumulated in place)."""
+    init()
+    _check(lib().thr_derivs_dev(_p(r), _p(x2), float(anum), int(iflag), _dims(dims), am,
+                                int(imass), KERNEL[kernel], _p(dsdpi), _stream(stream)),
+           "derivs_dev")
+    return dsdpi
 
 
 def multishift_dev(links64, dims, am, imass, aden, anum, out, inp, res, maxcg, am3=1.0, sp=False,
