@@ -284,6 +284,22 @@ int thr_congrad_dev(const void *phi, void *x, const void *links64, const int dim
                     float am, int imass, float am3, int niterc, int kernel, int *itercg,
                     int *iterations, void *stream);
 
+/* No counterpart in the reference: thr_congrad_dev (Shamir kernel) for
+ * callers that cannot allocate -- code under hipGraph capture, tight serving
+ * loops, harnesses that forbid allocation.  It calls no hipMalloc, hipFree,
+ * hipHostMalloc or stream constructor: its scratch (three spinors, solver
+ * state, partial sums) lives in a file-scope device array of
+ * thr_noalloc_scratch_bytes() bytes (64 MiB) that each GPU holds from the
+ * library's load to the end of the process.  Lattices needing more return
+ * THR_ERR_SIZE (4*V5 complex128 spinors x 3 + small change: up to about
+ * 3.4e5 5-D sites, e.g. 16^3 x 64).  Calls on one GPU share the array and
+ * must not overlap; every call writes its scratch before reading it.  A NULL
+ * stream means the null stream.  Results are those of thr_congrad_dev. */
+unsigned long long thr_noalloc_scratch_bytes(void);
+int thr_congrad_dev_noalloc(const void *phi, void *x, const void *links64, const int dims[4],
+                            float res, float am, int imass, float am3, int niterc, int *itercg,
+                            int *iterations, void *stream);
+
 /* derivs_module.F90:8-26, 99-196 on device fields: r, x2 are spinors,
  * dsdpi [mu][t][y][x] float64 accumulates */
 int thr_derivs_dev(const void *r, const void *x2, double anum, int iflag, const int dims[4],

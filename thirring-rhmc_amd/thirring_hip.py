@@ -42,6 +42,8 @@ _SIGS = {
     "thr_dslashd": ([P, P, P, P, F32, I32, F32, I32], I32),
     "thr_dirac_op_shifted": ([P, P, P, P, F32, I32, F32, F64, I32], I32),
     "thr_congrad": ([P, P, P, P, F32, F32, I32, F32, I32, I32, IP, IP], I32),
+    "thr_noalloc_scratch_bytes": ([], ctypes.c_ulonglong),
+    "thr_congrad_dev_noalloc": ([P, P, P, P, F32, F32, I32, F32, I32, IP, IP, P], I32),
     "thr_derivs_dev": ([P, P, F64, I32, P, F32, I32, I32, P, P], I32),
     "thr_measure_shamir_noise": ([P, P, P, P, F32, F32, I32, F32, I32, I32, P, P, IP], I32),
     "thr_multishift": ([P, P, F32, I32, F32, I32, P, P, P, P, F32, I32, I32, I32, IP, P], I32),
@@ -531,6 +533,21 @@ def derivs_dev(r, x2, anum, iflag, dims, dsdpi, am=0.0, imass=3, kernel="shamir"
                                 int(imass), KERNEL[kernel], _p(dsdpi), _stream(stream)),
            "derivs_dev")
     return dsdpi
+
+
+def congrad_dev_noalloc(phi, x, links64, dims, res, am, imass, am3=1.0, niterc=None,
+                        stream=None):
+    """congrad_dev (Shamir kernel) with no allocation: scratch from the
+    library's fixed device array (thr_noalloc_scratch_bytes())."""
+    init()
+    lx, ly, lt, ls = dims
+    if niterc is None:
+        niterc = ls * lx * ly * lt
+    it, nx = ctypes.c_int(), ctypes.c_int()
+    _check(lib().thr_congrad_dev_noalloc(_p(phi), _p(x), _p(links64), _dims(dims), res, am,
+                                         int(imass), am3, int(niterc), ctypes.byref(it),
+                                         ctypes.byref(nx), _stream(stream)), "congrad_dev_noalloc")
+    return it.value, nx.value
 
 
 def multishift_dev(links64, dims, am, imass, aden, anum, out, inp, res, maxcg, am3=1.0, sp=False,
