@@ -257,6 +257,8 @@ int thr_dslash_dev(void *out, float *out_norm, const void *in, const float *in_n
  *                            64, 32, 16): the inner x neighbours of a block
  *                            come through LDS
  *   THR_SHAPE_NT_STORES      non-temporal output stores
+ *   k << THR_SHAPE_OCC_SHIFT occupancy cap, k = 1..3: at most k + 1 resident
+ *                            blocks per CU (LDS padding; 0 = no cap)
  * No counterpart in the reference (a tuning knob of this library).
  * thr_dslash_autotune times the candidates (tchunk 1..16 x flags) on scratch
  * fields with the given links (complex128 for F64, complex64 otherwise) and
@@ -269,6 +271,7 @@ int thr_dslash_dev(void *out, float *out_norm, const void *in, const float *in_n
 #define THR_SHAPE_XCD_BOXES 8
 #define THR_SHAPE_LDS_X 16
 #define THR_SHAPE_NT_STORES 32
+#define THR_SHAPE_OCC_SHIFT 6
 int thr_dslash_autotune(const void *links, const int dims[4], int prec, int *tchunk, int *flags,
                         void *stream);
 /* thr_dslash_autotune timing launches over nrhs right-hand sides (1..32), for

@@ -174,12 +174,18 @@ def test_multishift_maxcg_cap(G):
     oo, co, _ = O.multishift(u, dims, 0.1, 1, sig, an, inp, res=1e-14, maxcg=7)
     assert cg == co == 7
     check(og, oo, 1e-11)
+# Ls <= 64: s-lines in power-of-two lane segments of a wave (Ls = 8, 12, 24,
+# 64); 64 < Ls <= 256: whole s-lines per block summed in LDS (Ls = 80); Ls >
+# 256: one thread per link (Ls = 260); odd extents exercise the partial blocks
+@pytest.mark.parametrize("dims", [(4, 4, 6, 8), (3, 5, 4, 12), (5, 3, 4, 24), (4, 4, 4, 64),
+                                  (3, 4, 5, 80), (2, 3, 2, 260)])
 @pytest.mark.parametrize("iflag", [0, 1])
-def test_derivs(G, iflag):
-    dims = (4, 4, 6, 8)
+def test_derivs(G, iflag, dims):
+    lx, ly, lt, ls = dims
+    print("case: derivs", dims, "iflag", iflag, flush=True)
     rng = np.random.default_rng(8 + iflag)
     R, X2 = D.rand_spinor(dims, rng), D.rand_spinor(dims, rng)
-    base = np.asfortranarray(rng.standard_normal((4, 4, 6, 3)))
+    base = np.asfortranarray(rng.standard_normal((lx, ly, lt, 3)))
     dg = G.derivs(R, X2, 0.37, iflag, dims, base.copy(order="F"))
     do = O.derivs(R, X2, 0.37, iflag, dims, base.copy(order="F"))
     check(dg - base, do - base, 1e-12)
@@ -268,7 +274,8 @@ def test_dslash_launch_shapes(G, dims):
     scalar links and LDS x exchange when Ls = 64, 32 or 16 (one, two or four
     s-lines per wave), lane-read walls when Ls = 64; s hops by
     lane shifts, with loads at the wave edges when an s-line crosses waves
-    (Ls = 12, 48), or by loads) gives the same operator,
+    (Ls = 12, 48), or by loads; resident blocks capped per CU) gives the same
+    operator,
     equal to the oracle's to 1e-13: bit-identical across shapes, except the
     scalar-link variants (Ls = 64, 32, 16), whose compiled code places fused
     multiply-adds differently (last-bit differences)."""
@@ -295,7 +302,9 @@ def test_dslash_launch_shapes(G, dims):
                     x = DeviceArray.from_host(xh.astype(dt))
                     first = None
                     for tc in sorted({1, 2, 3, 4, lt}):
-                        for flags in range(64):
+                        # every flag set, and occupancy caps (LDS padding) on three of them
+                        caps = [f | (k << 6) for f in (0, 7, 55) for k in (1, 2, 3)]
+                        for flags in list(range(64)) + caps:
                             G.dslash_set_config(dims, prec, min(tc, lt), flags)
                             out = DeviceArray((4 * v5, 2), dt)
                             G.dslash_dev(out, x, links, dims, 0.05, imass=imass, dagger=dagger,

@@ -878,6 +878,7 @@ DslashConfig default_config(const Dims &d) {
   if (const char *e = std::getenv("THR_DSLASH_BOXES")) c.box = std::atoi(e) ? 1 : 0;
   if (const char *e = std::getenv("THR_DSLASH_LDSX")) c.lds_x = std::atoi(e) ? 1 : 0;
   if (const char *e = std::getenv("THR_DSLASH_NT")) c.nt_stores = std::atoi(e) ? 1 : 0;
+  if (const char *e = std::getenv("THR_DSLASH_OCC")) c.occ_cap = std::max(0, std::min(3, std::atoi(e)));
   return c;
 }
 }  // namespace
@@ -925,6 +926,41 @@ long dslash_grid_blocks(const Dims &d, const DslashConfig &c) {
   return nb;
 }
 
+// Dynamic LDS that caps a kernel's resident blocks per CU at cap + 1
+// (occupancy cap of a launch shape): the padding is launch-time only, so
+// results do not change.  The kernel's static LDS and the device's LDS per CU
+// and per block are queried once.
+static std::mutex g_pad_mu;
+static std::map<const void *, size_t> g_static_lds;
+static size_t lds_pad(const void *kern, int cap) {
+  if (cap <= 0) return 0;
+  static int lds_cu = -1, lds_blk = -1;
+  std::lock_guard<std::mutex> lk(g_pad_mu);
+  if (lds_cu < 0) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) !=
+            hipSuccess ||
+        hipDeviceGetAttribute(&lds_blk, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+            hipSuccess) {
+      lds_cu = lds_blk = 0;
+    }
+  }
+  if (lds_cu <= 0 || lds_blk <= 0) return 0;
+  auto it = g_static_lds.find(kern);
+  if (it == g_static_lds.end()) {
+    hipFuncAttributes a{};
+    const size_t st = hipFuncGetAttributes(&a, kern) == hipSuccess ? a.sharedSizeBytes : 0;
+    it = g_static_lds.emplace(kern, st).first;
+  }
+  const size_t blocks = (size_t)cap + 1;
+  size_t target = ((size_t)lds_cu / blocks) & ~(size_t)511;
+  target = std::min(target, (size_t)lds_blk);
+  // a cap that the per-block limit cannot enforce is not applied
+  if ((blocks + 1) * target <= (size_t)lds_cu || target <= it->second) return 0;
+  return target - it->second;
+}
+
 int launch_dslash_cfg(const DslashLaunch &L, DslashConfig c, hipStream_t s) {
   THR_CHECK(L.nrhs >= 1 && L.nrhs <= MAXRHS, THR_ERR_ARG, "nrhs out of range");
   THR_CHECK(L.d.lx > 0 && L.d.ly > 0 && L.d.lt > 0 && L.d.ls > 0, THR_ERR_ARG, "bad dims");
@@ -960,22 +996,27 @@ int launch_dslash_cfg(const DslashLaunch &L, DslashConfig c, hipStream_t s) {
   const bool nt = c.nt_stores != 0;
   THR_CHECK(4 * L.d.v5() * Store<P64>::bytes < (1L << 32), THR_ERR_SIZE,
             "spinor too large for 32-bit buffer offsets");
-#define THR_LAUNCH_NT(P, D, S, M, NTV)                                                        \
-  do {                                                                                        \
-    if (xl)                                                                                   \
-      hipLaunchKernelGGL((k_dslash<P, D, true, M, 1, true, NTV>), grid, block, 0, s, L, TL);     \
-    else if (ul)                                                                              \
-      hipLaunchKernelGGL((k_dslash<P, D, true, M, 1, false, NTV>), grid, block, 0, s, L, TL);    \
-    else if (xl2)                                                                             \
-      hipLaunchKernelGGL((k_dslash<P, D, true, M, 2, true, NTV>), grid, block, 0, s, L, TL);     \
-    else if (ul2)                                                                             \
-      hipLaunchKernelGGL((k_dslash<P, D, true, M, 2, false, NTV>), grid, block, 0, s, L, TL);    \
-    else if (xl4)                                                                             \
-      hipLaunchKernelGGL((k_dslash<P, D, true, M, 4, true, NTV>), grid, block, 0, s, L, TL);     \
-    else if (ul4)                                                                             \
-      hipLaunchKernelGGL((k_dslash<P, D, true, M, 4, false, NTV>), grid, block, 0, s, L, TL);    \
-    else                                                                                      \
-      hipLaunchKernelGGL((k_dslash<P, D, S, M, 0, false, NTV>), grid, block, 0, s, L, TL);       \
+#define THR_GO(K)                                                                       \
+  do {                                                                                  \
+    const void *kf = (const void *)(K);                                                 \
+    hipLaunchKernelGGL(K, grid, block, lds_pad(kf, c.occ_cap), s, L, TL);               \
+  } while (0)
+#define THR_LAUNCH_NT(P, D, S, M, NTV)                                 \
+  do {                                                                 \
+    if (xl)                                                            \
+      THR_GO((k_dslash<P, D, true, M, 1, true, NTV>));                 \
+    else if (ul)                                                       \
+      THR_GO((k_dslash<P, D, true, M, 1, false, NTV>));                \
+    else if (xl2)                                                      \
+      THR_GO((k_dslash<P, D, true, M, 2, true, NTV>));                 \
+    else if (ul2)                                                      \
+      THR_GO((k_dslash<P, D, true, M, 2, false, NTV>));                \
+    else if (xl4)                                                      \
+      THR_GO((k_dslash<P, D, true, M, 4, true, NTV>));                 \
+    else if (ul4)                                                      \
+      THR_GO((k_dslash<P, D, true, M, 4, false, NTV>));                \
+    else                                                               \
+      THR_GO((k_dslash<P, D, S, M, 0, false, NTV>));                   \
   } while (0)
 #define THR_LAUNCH(P, D, S, M)                 \
   do {                                         \
@@ -1005,6 +1046,7 @@ int launch_dslash_cfg(const DslashLaunch &L, DslashConfig c, hipStream_t s) {
 #undef THR_LAUNCH1
 #undef THR_LAUNCH
 #undef THR_LAUNCH_NT
+#undef THR_GO
   THR_HIP(hipGetLastError());
   return THR_OK;
 }
@@ -1120,18 +1162,27 @@ int dslash_autotune(const Dims &d, int prec, const void *links, DslashConfig *be
   if (!rc && !first.empty()) {
     std::sort(first.begin(), first.end(),
               [](const Cand &a, const Cand &b) { return a.ms < b.ms; });
-    const size_t nk = std::min<size_t>(4, first.size());
+    // the best four, each also with its resident blocks capped at 2, 3, 4
+    // per CU (more resident waves hide latency but share each XCD's L2)
+    std::vector<DslashConfig> fin;
+    for (size_t k = 0; k < std::min<size_t>(4, first.size()); k++)
+      for (int cap = 0; cap <= 3; cap++) {
+        DslashConfig c = first[k].c;
+        c.occ_cap = cap;
+        fin.push_back(c);
+      }
+    const size_t nk = fin.size();
     std::vector<float> tot(nk, 0.f);
     for (int round = 0; round < 6 && !rc; round++)
       for (size_t k = 0; k < nk && !rc; k++) {
         float ms = 0.f;
-        rc = timed(first[k].c, 2 * nbuf, &ms);
+        rc = timed(fin[k], 2 * nbuf, &ms);
         tot[k] += ms;
       }
     size_t kb = 0;
     for (size_t k = 1; k < nk; k++)
       if (tot[k] < tot[kb]) kb = k;
-    pick = first[kb].c;
+    pick = fin[kb];
   }
   cleanup();
   if (rc) return rc;

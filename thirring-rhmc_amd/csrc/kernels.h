@@ -60,6 +60,8 @@ struct DslashConfig {
   int box = 1;            // XCD boxes: each XCD owns one box of an (x, y, t) split
   int lds_x = 1;          // with uniform links: inner x neighbours through LDS
   int nt_stores = 0;      // non-temporal output stores
+  int occ_cap = 0;        // 0: no cap; k = 1..3: at most k + 1 blocks per CU (k + 1 waves per
+                          // SIMD for 4-wave blocks), by padding each block's LDS
 };
 
 // block -> lattice mapping of one Dslash launch (see dslash.hip)

@@ -129,6 +129,70 @@ __device__ inline void ld_spinor(const double2 *__restrict__ p, unsigned v5, uns
   for (int a = 0; a < 4; a++) v[a] = ldc(p, (long)a * v5 + site);
 }
 
+// Ls <= 64: each s-line takes a segment of W lanes of one wave (W the
+// smallest power of two >= Ls, lanes s >= Ls idle), so the sum over s is a
+// butterfly of wave shuffles inside the segment (a fixed order, no LDS and no
+// block barrier); segment lane 0 adds the three link terms into dSdpi.  The
+// t march and loads are those of k_derivs_march below.
+__global__ __launch_bounds__(BLK) void k_derivs_seg(const Dims d, const double2 *__restrict__ R,
+                                                    const double2 *__restrict__ X2, float tzi,
+                                                    int iflag, unsigned wseg, unsigned tchunk,
+                                                    double *ds) {
+  const unsigned lx = d.lx, ly = d.ly, lt = d.lt, ls = d.ls;
+  const unsigned v2 = lx * ly, v3 = v2 * ls, v4 = v2 * lt, v5 = v4 * ls;
+  const unsigned nl = BLK / wseg;           // lines per block
+  const unsigned nb2 = (v2 + nl - 1) / nl;  // blocks per t-chunk
+  const unsigned chunk = blockIdx.x / nb2, lb = blockIdx.x - chunk * nb2;
+  const unsigned line = threadIdx.x / wseg, s = threadIdx.x - line * wseg;
+  const unsigned col = lb * nl + line;  // 2-D site x + lx y
+  const unsigned t0 = chunk * tchunk, t1 = min(t0 + tchunk, lt);
+  const bool valid = s < ls && col < v2;
+  const double w1 = (double)(tzi * 0.5f);  // tzi_real*real(akappa)
+  const double w2 = (double)tzi * (iflag == 0 ? 1.0 : -1.0);
+  unsigned x = 0, y = 0;
+  if (col < v2) {
+    x = col % lx;
+    y = col / lx;
+  }
+  const unsigned xf = (x + 1 == lx) ? 0 : x + 1, yf = (y + 1 == ly) ? 0 : y + 1;
+  const unsigned j = s + ls * col, jx = s + ls * (xf + lx * y), jy = s + ls * (x + lx * yf);
+  cx<double> R0[4], X0[4];
+  if (valid) {
+    ld_spinor(R, v5, j + v3 * t0, R0);
+    ld_spinor(X2, v5, j + v3 * t0, X0);
+  }
+  for (unsigned t = t0; t < t1; t++) {
+    double v[3] = {0.0, 0.0, 0.0};
+    if (valid) {
+      const unsigned tf = (t + 1 == lt) ? 0 : t + 1;
+      cx<double> R1[4], X1[4], Rt[4], Xt[4];
+      ld_spinor(R, v5, jx + v3 * t, R1);
+      ld_spinor(X2, v5, jx + v3 * t, X1);
+      v[0] = derivs_term<0>(R0, X0, R1, X1, w1, w2);
+      ld_spinor(R, v5, jy + v3 * t, R1);
+      ld_spinor(X2, v5, jy + v3 * t, X1);
+      v[1] = derivs_term<1>(R0, X0, R1, X1, w1, w2);
+      ld_spinor(R, v5, j + v3 * tf, Rt);
+      ld_spinor(X2, v5, j + v3 * tf, Xt);
+      v[2] = derivs_term<2>(R0, X0, Rt, Xt, w1, w2);
+#pragma unroll
+      for (int a = 0; a < 4; a++) {
+        R0[a] = Rt[a];
+        X0[a] = Xt[a];
+      }
+    }
+    // every lane of the wave takes part in the shuffles (idle lanes add 0)
+    for (unsigned off = wseg >> 1; off > 0; off >>= 1) {
+#pragma unroll
+      for (int mu = 0; mu < 3; mu++) v[mu] += __shfl_xor(v[mu], (int)off, 64);
+    }
+    if (s == 0 && col < v2) {
+#pragma unroll
+      for (int mu = 0; mu < 3; mu++) ds[(long)mu * v4 + col + v2 * t] += v[mu];
+    }
+  }
+}
+
 __global__ __launch_bounds__(BLK) void k_derivs_march(const Dims d, const double2 *__restrict__ R,
                                                       const double2 *__restrict__ X2, float tzi,
                                                       int iflag, unsigned nl, unsigned tchunk,
@@ -370,13 +434,25 @@ int launch_derivs(const Dims &d, const double2 *R, const double2 *X2, double anu
                   double *dsdpi, hipStream_t s) {
   const float tzi = 2.0f * (float)anum;  // real :: tzi_real = 2*real(anum)
   THR_CHECK(d.v5() * 4 < (1L << 31), THR_ERR_SIZE, "derivs: 4*V5 must be below 2^31");
+  // t-chunks long enough to reuse the t neighbour but short enough to leave
+  // ~2^18 threads in flight
+  const long v5 = d.v5();
+  const unsigned tchunk = (unsigned)std::max(1L, std::min<long>(d.lt, v5 >> 18));
+  const unsigned nchunk = (d.lt + tchunk - 1) / tchunk;
+  const unsigned v2 = (unsigned)d.lx * d.ly;
+  if (d.ls <= 64) {
+    unsigned w = 1;
+    while (w < (unsigned)d.ls) w <<= 1;
+    const unsigned nl = BLK / w;
+    const unsigned nb = ((v2 + nl - 1) / nl) * nchunk;
+    hipLaunchKernelGGL(k_derivs_seg, dim3(nb), dim3(BLK), 0, s, d, R, X2, tzi, iflag, w, tchunk,
+                       dsdpi);
+    THR_HIP(hipGetLastError());
+    return THR_OK;
+  }
   if (d.ls <= BLK) {
-    // whole s-lines per block; t-chunks long enough to reuse the t neighbour
-    // but short enough to leave ~2^18 threads in flight
-    const unsigned nl = BLK / d.ls, v2 = (unsigned)d.lx * d.ly;
-    const long v5 = d.v5();
-    const unsigned tchunk = (unsigned)std::max(1L, std::min<long>(d.lt, v5 >> 18));
-    const unsigned nchunk = (d.lt + tchunk - 1) / tchunk;
+    // whole s-lines per block, summed in LDS
+    const unsigned nl = BLK / d.ls;
     const unsigned nb = ((v2 + nl - 1) / nl) * nchunk;
     hipLaunchKernelGGL(k_derivs_march, dim3(nb), dim3(nl * d.ls), 0, s, d, R, X2, tzi, iflag, nl,
                        tchunk, dsdpi);
