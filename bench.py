@@ -10,8 +10,10 @@ inputs rotate over enough buffer sets to exceed the 256 MB Infinity Cache.
     algorithmic bytes = 2 * V5 * 4 * 16 + 3 * V4 * 16   (inputs read once,
                                                           output written once)
 
-Kernel durations come from HIP events recorded around every launch on the
-stream the kernels run on.
+Kernel durations come from HIP events on the stream the kernels run on,
+around K launches issued back to back (the time per application of a stream
+of Dslash applications); the line also gives the mean of events recorded
+around each launch, which adds the launch gaps.
 
 Also reported in the same JSON line:
   * FP32 / FP16 operator bandwidth, and the FP64 operator over 4 and 16
@@ -28,6 +30,8 @@ Also reported in the same JSON line:
   * the RHMC force at 16^3 x Ls=48: the FP64 solver against the
     mixed-precision solver with FP64 correction (time and difference);
   * mixed against FP64 at 32^3 x Ls=64: force and dH differences;
+  * solve time per iteration at launch-bound lattices (8^3 x 8, 12^2 x 24 x
+    16) with hipGraph replay and with eager launches;
   * serving: a sweep of Poisson arrival rates, each with p50/p99 latency and
     solves/s under continuous batching (thirring-rhmc_amd/serving.py);
   * the CPU baseline: the FP64 C oracle's Dslash on the host cores.
@@ -82,8 +86,9 @@ def make_fields(G, DA, dims, beta, seed):
 
 
 def time_dslash(G, hr, dims, links, prec, steps, warmup, nbuf, stream, nrhs=1, am=0.005):
-    """Mean per-launch time (HIP events around each launch on `stream`) and the
-    whole region's time per launch; `nrhs` spinors per launch, masses `am`."""
+    """Per-launch times on `stream`: of K launches back to back (HIP events
+    around the whole stream), the mean of events around each launch, and the
+    event-bracketed region's; `nrhs` spinors per launch, masses `am`."""
     lx, ly, lt, ls = dims
     v5 = lx * ly * lt * ls
     rng = np.random.default_rng(1)
@@ -122,10 +127,18 @@ def time_dslash(G, hr, dims, links, prec, steps, warmup, nbuf, stream, nrhs=1, a
     t1.record(stream)
     stream.synchronize()
     per = np.array([a.elapsed_ms(b) for a, b in ev])
+    # the same launches back to back, events only around the whole stream:
+    # the time per launch of a stream of applications (what a solver sees)
+    b0, b1 = hr.Event(), hr.Event()
+    b0.record(stream)
+    for k in range(steps):
+        launch(k)
+    b1.record(stream)
+    stream.synchronize()
     for a in ins + outs + inn + outn:
         if a is not None:
             a.free()
-    return float(np.mean(per)), float(t0.elapsed_ms(t1) / steps)
+    return float(b0.elapsed_ms(b1) / steps), float(np.mean(per)), float(t0.elapsed_ms(t1) / steps)
 
 
 def multi_rhs_legs(G, hr, steps, warmup, stream, pinned=None, tuned=()):
@@ -146,7 +159,8 @@ def multi_rhs_legs(G, hr, steps, warmup, stream, pinned=None, tuned=()):
         am = rng.choice([0.005, 0.01, 0.02, 0.05], size=nrhs).astype(np.float32)
         nb = max(2, int(np.ceil(4 * 256e6 / (2 * 16 * 4 * v5 * nrhs))))
         progress(f"dslash f64 {dims} nrhs={nrhs}")
-        ms, _ = time_dslash(G, hr, dims, l64, "f64", steps, warmup, nb, stream, nrhs=nrhs, am=am)
+        ms, _, _ = time_dslash(G, hr, dims, l64, "f64", steps, warmup, nb, stream, nrhs=nrhs,
+                               am=am)
         gbs = dslash_bytes(v4, v5, "f64", nrhs) / (ms * 1e-3) / 1e9
         out["x".join(map(str, dims)) + f"_nrhs{nrhs}"] = {
             "ms": round(ms, 5), "GB_s": round(gbs, 1), "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4),
@@ -333,6 +347,50 @@ def bench_batched_config4(G, hr, stream, dims=(24, 24, 24, 32), nsrc=16, tol=1e-
     return out
 
 
+def bench_small_lattices(G, hr, stream, nsrc=4, tol=1e-10):
+    """Launch-bound sizes of the serving mix: the batched mixed solve of
+    (M^+M) x = b (the serving request's system, 4 sources, am 0.05) at
+    8^3 x Ls=8 and 12^2 x 24 x Ls=16, with the steady-state blocks replayed
+    from hipGraphs and launched eagerly: wall time per solve and per
+    iteration."""
+    out = {}
+    saved = os.environ.get("THR_GRAPHS")
+    rng = np.random.default_rng(5)
+    try:
+        for dims in ((8, 8, 8, 8), (12, 12, 24, 16)):
+            lx, ly, lt, ls = dims
+            nsp = 4 * lx * ly * lt * ls
+            _, l64, l32 = make_fields(G, hr.DeviceArray, dims, 0.3, 17)
+            b = hr.DeviceArray.from_host(rng.standard_normal((nsrc, nsp, 2)) / np.sqrt(2))
+            x = hr.DeviceArray((nsrc, 1, nsp, 2), np.float64)
+            row = {}
+            modes = (("eager", "0"),) if under_profiler() else (("graphs", "1"), ("eager", "0"))
+            for label, flag in modes:
+                os.environ["THR_GRAPHS"] = flag
+                progress(f"small lattice {dims} {label}")
+                G.multishift_fast_dev(l64, l32, dims, 0.05, 3, np.zeros(1), b, x, tol=tol,
+                                      maxcg=100000, nsrc=nsrc, th32=1e-4, true_residuals=False,
+                                      stream=stream)  # warm: workspace, graph capture
+                hr.synchronize()
+                t = time.perf_counter()
+                it, _, _ = G.multishift_fast_dev(l64, l32, dims, 0.05, 3, np.zeros(1), b, x,
+                                                 tol=tol, maxcg=100000, nsrc=nsrc, th32=1e-4,
+                                                 true_residuals=False, stream=stream)
+                hr.synchronize()
+                dt = time.perf_counter() - t
+                row[label] = {"seconds": round(dt, 5), "iterations": int(it),
+                              "us_per_iteration": round(1e6 * dt / max(1, int(it)), 2)}
+            out["x".join(map(str, dims))] = row
+            for a in (b, x, l64, l32):
+                a.free()
+    finally:
+        if saved is None:
+            os.environ.pop("THR_GRAPHS", None)
+        else:
+            os.environ["THR_GRAPHS"] = saved
+    return out
+
+
 def bench_config3_errors(G, dims=(32, 32, 32, 64), am=0.005, beta=0.3, seed=303):
     """Mixed precision against FP64 at 32^3 x Ls=64 (am 0.005) for the RHMC
     force (force_module.F90:9-113, res1 = 1e-8) and the Hamiltonian
@@ -492,6 +550,12 @@ def cpu_baseline(dims, seconds=12.0):
                       f"{lx}x{ly}x{lt}x{ls}, {dt * 1e3:.1f} ms each"}
 
 
+def under_profiler():
+    """True when a rocprofv3 tool library is preloaded into this process."""
+    pre = os.environ.get("LD_PRELOAD", "")
+    return "rocprof" in pre or any(k.startswith("ROCPROF") for k in os.environ)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -506,6 +570,7 @@ def main():
     ap.add_argument("--no-force", action="store_true")
     ap.add_argument("--no-config4", action="store_true")
     ap.add_argument("--no-derivs", action="store_true")
+    ap.add_argument("--no-small", action="store_true")
     ap.add_argument("--no-config3-errors", action="store_true")
     ap.add_argument("--serve-rates", default="1,2,5,10,20,50,100,200",
                     help="Poisson arrival rates (per s) of the serving sweep")
@@ -525,6 +590,9 @@ def main():
                     help="with --tchunk: 1 XCD remap | 2 uniform links | 4 lane-shift s hops | "
                          "8 XCD boxes | 16 LDS x exchange | 32 non-temporal stores | "
                          "k << 6 at most k + 1 resident blocks per CU")
+    ap.add_argument("--shapes", default="",
+                    help="pin per-precision shapes instead of autotuning: "
+                         "prec:tchunk:flags,... (e.g. f64:16:55,f32:2:55,f16:2:39)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run N device copies of 1 GiB (scales the HBM counters)")
     args = ap.parse_args()
@@ -534,6 +602,10 @@ def main():
               "shard one process per GPU)", file=sys.stderr)
         sys.exit(2)
 
+    if under_profiler():
+        # graph replays are never profiled (the profiler's interception of
+        # hipGraph launches once aborted the host process): launch eagerly
+        os.environ["THR_GRAPHS"] = "0"
     import hiprt as hr
     import thirring_hip as G
     G.init(0)
@@ -554,8 +626,14 @@ def main():
 
     t_wall0 = time.time()
     shapes = {}
+    pins = {}
+    for item in filter(None, args.shapes.split(",")):
+        p, tc, fl = item.split(":")
+        pins[p] = (int(tc), int(fl))
     for prec, lk in (("f64", l64), ("f32", l32), ("f16", l32)):
-        if args.tchunk > 0:
+        if prec in pins:
+            G.dslash_set_config(dims, prec, *pins[prec])
+        elif args.tchunk > 0:
             G.dslash_set_config(dims, prec, args.tchunk, args.shape_flags)
         else:
             progress(f"autotune dslash {prec}")
@@ -564,13 +642,14 @@ def main():
         shapes[prec] = {"tchunk": tc, "flags": fl}
     nbuf64 = max(2, int(np.ceil(4 * 256e6 / (2 * 16 * 4 * v5))))
     progress(f"dslash f64 {dims} x{args.steps}")
-    ms64, region64 = time_dslash(G, hr, dims, l64, "f64", args.steps, args.warmup, nbuf64, stream)
+    ms64, ev64, region64 = time_dslash(G, hr, dims, l64, "f64", args.steps, args.warmup, nbuf64,
+                                       stream)
     gbs64 = dslash_bytes(v4, v5, "f64") / (ms64 * 1e-3) / 1e9
     extra = {}
     for prec in ("f32", "f16"):
         nb = max(2, int(np.ceil(4 * 256e6 / (2 * {"f32": 8, "f16": 5}[prec] * 4 * v5))))
         progress(f"dslash {prec}")
-        ms, _ = time_dslash(G, hr, dims, l32, prec, args.steps, args.warmup, nb, stream)
+        ms, _, _ = time_dslash(G, hr, dims, l32, prec, args.steps, args.warmup, nb, stream)
         extra[prec] = {"ms": round(ms, 5),
                        "GB_s": round(dslash_bytes(v4, v5, prec) / (ms * 1e-3) / 1e9, 1)}
     pinned = (args.tchunk, args.shape_flags) if args.tchunk > 0 else None
@@ -590,6 +669,7 @@ def main():
         solver["am"] = args.solver_am
     derivs = None if args.no_derivs else bench_derivs(G, hr, stream)
     config4 = None if args.no_config4 else bench_batched_config4(G, hr, stream)
+    small = None if args.no_small else bench_small_lattices(G, hr, stream)
     force = None if args.no_force else bench_force(G)
     config3 = None if args.no_config3_errors else bench_config3_errors(G)
     serving = None
@@ -621,7 +701,8 @@ def main():
                                "M^dagger alternating), rotating buffers beyond the Infinity Cache",
                    "lattice": list(dims), "rotating_buffer_sets": nbuf64,
                    "launch_shape": shapes,
-                   "region_ms_per_step": round(region64, 5)},
+                   "per_launch_event_ms": round(ev64, 5),
+                   "event_bracketed_region_ms_per_step": round(region64, 5)},
         "roofline": {"bound": "hbm", "achieved": round(gbs64, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(gbs64 / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_source": traffic_note},
@@ -633,6 +714,7 @@ def main():
         "derivs": derivs,
         "multishift": solver,
         "batched_multi_rhs_config4": config4,
+        "small_lattice_solves": small,
         "force": force,
         "mixed_vs_fp64_32x32x32x64": config3,
         "serving": serving,

@@ -89,6 +89,18 @@ int thr_context_bytes(unsigned long long *bytes);
  * (later calls allocate them again); lets a long-lived worker give back the
  * scratch of a large request instead of holding its high-water mark */
 int thr_context_trim(void);
+/* Reliable updates in the bound context's batched mixed-precision multishift
+ * (thr_multishift_fast_dev and the mixed qmrherm/force): while the Krylov
+ * storage is FP32 or FP16, each time the iterated residual has fallen by the
+ * factor delta since the last update, the base residual is replaced by the
+ * FP64 true residual of the smallest active shift (scaled by 1/zeta of that
+ * shift), so that shift's solution does not inherit the low-precision drift.
+ * delta in [0, 1); 0 disables; default 0.1.  No counterpart in the reference,
+ * whose FP32 multishift (multishift_module.F90:289-534) has no correction. */
+int thr_context_set_reliable(double delta);
+/* reliable updates taken by the bound context's last multishift solve
+ * (its main solve; corrections not counted) */
+int thr_context_reliable_updates(int *count);
 
 /* ---- (1) host-array drop-in entry points -------------------------------- */
 
@@ -125,6 +137,18 @@ int thr_congrad(const double *phi, double *x, const double *u, const int dims[4]
  * psibarpsi1 / psibarpsi2 (re, im) of the reference before its division by
  * kvol, iters the sum of the four itercg.  imass 1, 3 and 5. */
 int thr_measure_shamir_noise(const float *ps, const float *pt, const double *u, const int dims[4],
+                             float res, float am, int imass, float am3, int niterc, int kernel,
+                             double pbp1[2], double pbp2[2], int *iters);
+
+/* measure_module.F90:431-640  one noise vector of measure_wilson (the
+ * Wilson-kernel measurement, imass 1 and 3): sources on s = 1 for all four
+ * spins from ps, then on s = Ls from pt; each goes through dslashd_wilson,
+ * then congrad (operator `kernel`, initial guess the source); the read slice
+ * (s = Ls through spins 1, 2; s = 1 through spins 3, 4) is replaced by
+ * DWilson(islice, -am3) - islice before the contraction
+ * (measure_module.F90:521-531, 594-604).  Outputs as thr_measure_shamir_noise
+ * (eight solves' iterations in iters). */
+int thr_measure_wilson_noise(const float *ps, const float *pt, const double *u, const int dims[4],
                              float res, float am, int imass, float am3, int niterc, int kernel,
                              double pbp1[2], double pbp2[2], int *iters);
 

@@ -118,8 +118,10 @@ def test_fortran_measure(tmp_path):
     2.504295E-04, aviter = 5 (tests/test_measure_shamir.F90:45-46) with the
     reference's noise (SITE_RANDOM streams, tests/fortran/test_random.F90),
     the same result through `measure` with isweep_total (dwf3d_lib.F90:355),
-    and the reference's output records: knoise lines in fort.100 per
-    measurement, one in fort.200."""
+    the reference's output records (knoise lines in fort.100 per
+    measurement, one in fort.200), and measure_wilson's known answer
+    |psibarpsi| = 5.4226685E-02, aviter = 10.9 (tests/test_measure_wilson.F90:
+    46-47; the reference's negative value predates its imass = 3 phase swap)."""
     exe = os.path.join(BIN, "test_measure_drop_in")
     assert os.path.exists(exe), f"{exe} not built (run __graft_entry__.build())"
     print("case: Fortran measure_shamir 12^3 x 24 imass 3", flush=True)
@@ -129,7 +131,8 @@ def test_fortran_measure(tmp_path):
     assert "PASS measure_shamir" in p.stdout
     f100 = (tmp_path / "fort.100").read_text().splitlines()
     f200 = (tmp_path / "fort.200").read_text().splitlines()
-    assert len(f100) == 20 and len(f200) == 2
+    assert len(f100) == 30 and len(f200) == 3
+    assert "Wilson measure" in p.stdout
 
     def fields(line, width, lead):  # fixed-width records: optional I5, then E fields
         body = line[5:] if lead else line

@@ -72,3 +72,34 @@ def test_measure_noise_entry_point_matches_oracle(G, imass):
     print("psibarpsi", got, ref, "aviter", itg, ito, flush=True)
     assert abs(itg - ito) <= max(1.0, 0.01 * ito)  # aviter: iterations per solve
     assert abs(got - ref) <= 1e-6 * abs(ref)
+
+
+@pytest.mark.parametrize("imass", [1, 3])
+def test_measure_wilson_entry_point_matches_oracle(G, imass):
+    """thr_measure_wilson_noise (measure_wilson's eight wall solves per noise
+    vector with the DWilson projection of the read slices, on the device)
+    against the same measurement composed from the oracle's dslashd (Wilson
+    kernel), congrad and DWilson slice (tests/measure_ref.py,
+    measure_module.F90:472-606): equal psibarpsi, iteration counts at most
+    one apart per solve on average."""
+    import measure
+    import measure_ref
+    from site_random import SiteRandom
+    dims = (4, 4, 8, 8)
+    lx, ly, lt, ls = dims
+    print("case: measure_wilson_noise imass", imass, "4x4x8 Ls 8 seed 33", flush=True)
+    rng = np.random.default_rng(33)
+    th = (rng.standard_normal((lx, ly, lt, 3)) / np.sqrt(4.0)).astype(np.float32)
+    u = O.coef(th, dims)
+    res, am = np.float32(1e-8), np.float32(0.05)
+    got, itg = measure.measure_wilson(u, dims, res, am, imass, SiteRandom(lx, ly, lt, 9.0),
+                                      knoise=2)
+
+    def dslashd_w(r, uu, d, m, im):
+        return O.dslashd(r, uu, d, m, im, kernel="wilson")
+
+    ref, ito = measure_ref.measure_wilson(dslashd_w, O.congrad, O.dwilson_slice, u, dims, res, am,
+                                          imass, SiteRandom(lx, ly, lt, 9.0), knoise=2)
+    print("psibarpsi", got, ref, "aviter", itg, ito, flush=True)
+    assert abs(itg - ito) <= max(1.0, 0.01 * ito)
+    assert abs(got - ref) <= 1e-6 * abs(ref)

@@ -76,6 +76,53 @@ def test_fast_multishift_true_residual(G, th32, th16):
         assert err <= 1e-8, (i, err)
 
 
+@pytest.mark.parametrize("th16", [0.0, 1e-3])
+def test_reliable_updates_low_precision_from_start(G, th16):
+    """Krylov storage in FP32 (and FP16 below 1e-3) from the first block on.
+    Without reliable updates the smallest shift (the one that needs every
+    iteration) keeps the drift of the low-precision residual recurrence, far
+    above the target; with them (the default, factor 0.1) its FP64 true
+    residual meets 1e-10 with no correction at all, and with the default
+    correction rounds every shift does."""
+    from hiprt import DeviceArray
+    dims = (4, 4, 8, 8)
+    sig = _remez4()
+    th, l64, l32, bh, b = _setup(dims, 5, 2)
+    n, nsp = len(sig), bh.shape[1]
+    imin = int(np.argmin(sig))
+    masses = [0.01, 0.03]
+
+    def solve(delta, corrections):
+        G.set_reliable_updates(delta)
+        x = DeviceArray((2, n, nsp, 2), np.float64)
+        st = {}
+        try:
+            it, cgr, rel = G.multishift_fast_dev(l64, l32, dims, masses, 3, sig, b, x, tol=1e-10,
+                                                 maxcg=20000, nsrc=2, th32=1.0, th16=th16,
+                                                 corrections=corrections, stats=st)
+        except G.ThrNoConvergence as e:
+            it, cgr, rel = e.result
+            st = dict(st, noconv=True)
+        return it, rel, st
+
+    try:
+        it0, rel0, st0 = solve(0.0, 0)
+        it1, rel1, st1 = solve(0.1, 0)
+        it2, rel2, st2 = solve(0.1, 4)
+    finally:
+        G.set_reliable_updates(0.1)
+    print("no RU: it", it0, "smallest shift", rel0[:, imin], "max", rel0.max(), flush=True)
+    print("RU: it", it1, "updates", st1["reliable"], "smallest shift", rel1[:, imin], "max",
+          rel1.max(), flush=True)
+    print("RU + corrections: corr iterations", st2["corr_iterations"], "max", rel2.max(),
+          flush=True)
+    assert st0["reliable"] == 0 and st1["reliable"] > 0
+    assert st1["switches"][1] == 32
+    assert rel1[:, imin].max() <= 1e-10, rel1[:, imin]
+    assert rel0[:, imin].max() > 10 * rel1[:, imin].max()
+    assert rel2.max() <= 1e-10
+
+
 def test_fast_multishift_batch_equals_alone(G):
     """Sources with different masses give the same answer alone and batched."""
     from hiprt import DeviceArray

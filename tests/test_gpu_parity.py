@@ -345,15 +345,14 @@ def test_device_copy(G):
 
 
 def test_measure_known_answers(G):
-    """The reference's psibarpsi known answers on the GPU path (host-array
-    dslashd and congrad; tests/test_measure_shamir.F90:45-46 and
+    """The reference's psibarpsi known answers on the GPU path
+    (tests/test_measure_shamir.F90:45-46 and
     tests/test_measure_wilson.F90:46-47, 12^3 x Ls 24, SITE_RANDOM noise):
     Shamir 2.504295e-04 with 5 iterations per solve; Wilson kernel
     |psibarpsi| 5.4226685e-02 with 10.9 (the reference's value is negative:
-    it predates the swap of the imass = 3 phases in measure_wilson).  The
-    DWilson wall projection of the Wilson measurement runs on the oracle."""
+    it predates the swap of the imass = 3 phases in measure_wilson).  Both
+    run through the library's per-noise-vector entry points."""
     import measure
-    import measure_ref
     from site_random import SiteRandom
     dims = (12, 12, 12, 24)
     st = refstate.starting_state(dims, want=("u",))
@@ -363,10 +362,9 @@ def test_measure_known_answers(G):
     assert abs(pbp - 2.504295e-04) <= 1e-3 * 2.504295e-04, pbp
     assert abs(aviter - 5.0) <= 1e-3 * 5.0, aviter
 
-    def dslashd_w(r, u, d, m, imass):
-        return G.dslashd(r, u, d, m, imass, kernel="wilson")
-
-    pbp, aviter = measure_ref.measure_wilson(dslashd_w, G.congrad, O.dwilson_slice, st["u"], dims,
-                                             res, am, 3, SiteRandom(12, 12, 12, seed))
+    # the Wilson-kernel measurement on the device (thr_measure_wilson_noise:
+    # dslashd_wilson, congrad with the Shamir operator the reference's test
+    # was generated with, DWilson projection of the read slices)
+    pbp, aviter = measure.measure_wilson(st["u"], dims, res, am, 3, SiteRandom(12, 12, 12, seed))
     assert abs(pbp - 5.4226685e-02) <= 1e-3 * 5.4226685e-02, pbp
     assert abs(aviter - 10.9) <= 1e-3 * 10.9, aviter

@@ -178,6 +178,18 @@ int thr_context_trim(void) {
   return THR_OK;
 }
 
+int thr_context_set_reliable(double delta) {
+  THR_CHECK(delta >= 0.0 && delta < 1.0, THR_ERR_ARG, "reliable-update factor out of range [0, 1)");
+  context_set_reliable(delta);
+  return THR_OK;
+}
+
+int thr_context_reliable_updates(int *count) {
+  THR_CHECK(count, THR_ERR_ARG, "NULL argument");
+  *count = context_last_reliable();
+  return THR_OK;
+}
+
 static int links_to_host(const Dims &d, const double2 *u64, double *u, hipStream_t s);
 
 int thr_coef(double *u, const float *theta, const int dims[4], int ibound) {
@@ -329,7 +341,84 @@ int thr_measure_shamir_noise(const float *ps, const float *pt, const double *u, 
       int itercg = 0;
       TRY(cg_reference(op, phi, xi, res, niterc, &itercg, nullptr, s));
       total += itercg;
-      TRY(launch_wall_dot(d, eta, xi, spin_read, s_read, part, s));
+      TRY(launch_wall_dot(d, eta, xi, nullptr, spin_read, s_read, part, s));
+      THR_HIP(hipMemcpyAsync(hpart, part, sizeof(double) * 2 * nbd, hipMemcpyDeviceToHost, s));
+      THR_HIP(hipStreamSynchronize(s));
+      for (long b = 0; b < nbd; b++) {
+        acc[side][0] += hpart[2 * b];
+        acc[side][1] += hpart[2 * b + 1];
+      }
+    }
+  }
+  pbp1[0] = acc[0][0];
+  pbp1[1] = acc[0][1];
+  pbp2[0] = acc[1][0];
+  pbp2[1] = acc[1][1];
+  *iters = total;
+  return THR_OK;
+}
+
+int thr_measure_wilson_noise(const float *ps, const float *pt, const double *u, const int dims[4],
+                             float res, float am, int imass, float am3, int niterc, int kernel,
+                             double pbp1[2], double pbp2[2], int *iters) {
+  TRY(check_dims(dims));
+  TRY(check_op(kernel, imass));
+  THR_CHECK(imass == 1 || imass == 3, THR_ERR_ARG, "the Wilson-kernel measurement supports imass 1 and 3");
+  THR_CHECK(ps && pt && u && pbp1 && pbp2 && iters, THR_ERR_ARG, "NULL argument");
+  const Dims d = mkdims(dims);
+  const Dims d4{d.lx, d.ly, d.lt, 1};
+  hipStream_t s = default_stream();
+  double2 *u64;
+  TRY(stage_links(d, u, &u64, s));
+  const long n = d.nspinor();
+  const long nh4 = (long)(d.lx + 2) * (d.ly + 2) * (d.lt + 2) * 2;
+  const long nbd = wall_dot_blocks(d);
+  double2 *xi = (double2 *)workspace(W_A, sizeof(double2) * n);
+  double2 *phi = (double2 *)workspace(W_B, sizeof(double2) * n);
+  double2 *isl = (double2 *)workspace(W_C, sizeof(double2) * 4 * d.v4());
+  double2 *osl = (double2 *)workspace(W_D, sizeof(double2) * 4 * d.v4());
+  float *noise = (float *)workspace(W_P, sizeof(float) * 2 * nh4);
+  double *part = (double *)workspace(W_T, sizeof(double) * 2 * nbd);
+  double *hpart = (double *)host_workspace(W_T, sizeof(double) * 2 * nbd);
+  THR_CHECK(xi && phi && isl && osl && noise && part && hpart, THR_ERR_HIP,
+            "workspace allocation failed");
+  THR_HIP(hipMemcpyAsync(noise, ps, sizeof(float) * nh4, hipMemcpyHostToDevice, s));
+  THR_HIP(hipMemcpyAsync(noise + nh4, pt, sizeof(float) * nh4, hipMemcpyHostToDevice, s));
+  // the source goes through dslashd_wilson whatever operator congrad uses
+  // (measure_module.F90:509-533, 582-598)
+  const OpDesc wilson = mkop(d, 1, imass, am, am3, u64);
+  const OpDesc op = mkop(d, kernel, imass, am, am3, u64);
+  // DWilson(islice, -am3): the 4-D Wilson slice operator, diag = 3 - am3 (dirac.F90:181-224)
+  DslashLaunch W{};
+  W.d = d4;
+  W.nrhs = 1;
+  W.prec = P64;
+  W.imass = 3;
+  W.am3 = am3;
+  W.links = u64;
+  W.mode = EPI_STORE;
+  W.sc_stride = 1;
+  W.done_stride = 1;
+  W.wilson4d = 1;
+  W.in = isl;
+  W.out = osl;
+  double acc[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+  int total = 0;
+  for (int side = 0; side < 2; side++) {
+    // side 0: sources on s = 1 from ps, read at s = Ls through P+ (spins 1, 2);
+    // side 1: sources on s = Ls from pt, read at s = 1 through P- (spins 3, 4)
+    const float *eta = noise + side * nh4;
+    const int s_src = side == 0 ? 0 : d.ls - 1, s_read = side == 0 ? d.ls - 1 : 0;
+    const int lo = side == 0 ? 0 : 2;
+    for (int id = 0; id < 4; id++) {
+      TRY(launch_wall_source(d, eta, id, s_src, 1.0, xi, s));
+      TRY(apply_op(wilson, 1, xi, phi, s));
+      int itercg = 0;
+      TRY(cg_reference(op, phi, xi, res, niterc, &itercg, nullptr, s));
+      total += itercg;
+      TRY(launch_slice_project(d, xi, s_read, lo, isl, s));
+      TRY(launch_dslash(W, s));
+      TRY(launch_wall_dot(d4, eta, osl, isl, id, 0, part, s));
       THR_HIP(hipMemcpyAsync(hpart, part, sizeof(double) * 2 * nbd, hipMemcpyDeviceToHost, s));
       THR_HIP(hipStreamSynchronize(s));
       for (long b = 0; b < nbd; b++) {

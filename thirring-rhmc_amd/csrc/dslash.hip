@@ -1134,9 +1134,15 @@ int dslash_autotune(const Dims &d, int prec, const void *links, DslashConfig *be
     if (!r && hipEventElapsedTime(ms, e0, e1) != hipSuccess) r = THR_ERR_HIP;
     return r;
   };
-  const int tcs[] = {1, 2, 4, 8, 16};
+  // t-chunks: powers of two and the intermediate lengths that split Lt into
+  // 3, 5, 6 or 12 chunks (so the grid fills the GPU in whole rounds of
+  // resident blocks at more sizes)
+  std::vector<int> tcs = {1, 2, 4, 8, 16};
+  for (int k : {3, 5, 6, 12}) tcs.push_back((d.lt + k - 1) / k);
+  std::sort(tcs.begin(), tcs.end());
+  tcs.erase(std::unique(tcs.begin(), tcs.end()), tcs.end());
   for (int tc : tcs) {
-    if (tc > d.lt) continue;
+    if (tc > d.lt || tc < 1) continue;
     for (int v = 0; v < 64 && !rc; v++) {
       if ((v & 2) && (!(v & 4) || (d.ls != 64 && d.ls != 32 && d.ls != 16)))
         continue;  // uniform links need Ls = 64, 32 or 16
@@ -1162,10 +1168,11 @@ int dslash_autotune(const Dims &d, int prec, const void *links, DslashConfig *be
   if (!rc && !first.empty()) {
     std::sort(first.begin(), first.end(),
               [](const Cand &a, const Cand &b) { return a.ms < b.ms; });
-    // the best four, each also with its resident blocks capped at 2, 3, 4
-    // per CU (more resident waves hide latency but share each XCD's L2)
+    // the best six, each also with its resident blocks capped at 2, 3, 4
+    // per CU (more resident waves hide latency but share each XCD's L2),
+    // timed in ten interleaved rounds of at least 16 launches each
     std::vector<DslashConfig> fin;
-    for (size_t k = 0; k < std::min<size_t>(4, first.size()); k++)
+    for (size_t k = 0; k < std::min<size_t>(6, first.size()); k++)
       for (int cap = 0; cap <= 3; cap++) {
         DslashConfig c = first[k].c;
         c.occ_cap = cap;
@@ -1173,10 +1180,11 @@ int dslash_autotune(const Dims &d, int prec, const void *links, DslashConfig *be
       }
     const size_t nk = fin.size();
     std::vector<float> tot(nk, 0.f);
-    for (int round = 0; round < 6 && !rc; round++)
+    const int reps = std::max(16, 2 * nbuf);
+    for (int round = 0; round < 10 && !rc; round++)
       for (size_t k = 0; k < nk && !rc; k++) {
         float ms = 0.f;
-        rc = timed(fin[k], 2 * nbuf, &ms);
+        rc = timed(fin[k], reps, &ms);
         tot[k] += ms;
       }
     size_t kb = 0;

@@ -5,7 +5,9 @@
 // (real32, haloed; only the interior is read), the complex noise
 // eta = cmplx(ps(..,1), ps(..,2)).  A wall source puts sign * eta on one spin
 // at one s and zero everywhere else; a wall contraction is
-// sum_x conj(eta(x)) X(s, x, spin) over the 4-D sites.
+// sum_x conj(eta(x)) X(s, x, spin) over the 4-D sites.  The Wilson-kernel
+// measurement also projects the read slice: islice keeps two spins of X at
+// that s and the contraction uses DWilson(islice, -am3) - islice.
 #include "solvers.h"
 
 namespace thr {
@@ -40,16 +42,23 @@ __global__ __launch_bounds__(BLK) void k_wall_source(const Dims d, const float *
   out[i] = v;
 }
 
-// per-block partial sums of conj(eta) X(s_read, ., spin): partial[2 b], partial[2 b + 1]
+// per-block partial sums of conj(eta) (X - Xs)(s_read, ., spin) (Xs optional):
+// partial[2 b], partial[2 b + 1]
 __global__ __launch_bounds__(BLK) void k_wall_dot(const Dims d, const float *ps, const double2 *X,
-                                                  int spin, int s_read, double *partial) {
+                                                  const double2 *Xs, int spin, int s_read,
+                                                  double *partial) {
   const long v4 = d.v4(), v5 = d.v5();
   const long c = (long)blockIdx.x * BLK + threadIdx.x;
   double re = 0.0, im = 0.0;
   if (c < v4) {
     const int x = (int)(c % d.lx), y = (int)((c / d.lx) % d.ly), t = (int)(c / ((long)d.lx * d.ly));
     const double er = noise_at(ps, d, x, y, t, 0), ei = noise_at(ps, d, x, y, t, 1);
-    const double2 v = X[spin * v5 + c * d.ls + s_read];
+    const long e = spin * v5 + c * d.ls + s_read;
+    double2 v = X[e];
+    if (Xs) {
+      const double2 w = Xs[e];
+      v = make_double2(v.x - w.x, v.y - w.y);
+    }
     re = er * v.x + ei * v.y;  // conj(eta) * v
     im = er * v.y - ei * v.x;
   }
@@ -61,7 +70,27 @@ __global__ __launch_bounds__(BLK) void k_wall_dot(const Dims d, const float *ps,
   }
 }
 
+// slice s_read of X, spins [lo, lo + 2) kept and the others zero, as a 4-D
+// spinor [spin][t][y][x] (the islice of measure_module.F90:524-525, 597-598)
+__global__ __launch_bounds__(BLK) void k_slice_project(const Dims d, const double2 *X, int s_read,
+                                                       int lo, double2 *out) {
+  const long v4 = d.v4(), v5 = d.v5();
+  const long i = (long)blockIdx.x * BLK + threadIdx.x;
+  if (i >= 4 * v4) return;
+  const int a = (int)(i / v4);
+  const long c = i - a * v4;
+  out[i] = (a >= lo && a < lo + 2) ? X[a * v5 + c * d.ls + s_read] : make_double2(0.0, 0.0);
+}
+
 }  // namespace
+
+int launch_slice_project(const Dims &d, const double2 *X, int s_read, int lo, double2 *out,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(k_slice_project, dim3(nblk(4 * d.v4())), dim3(BLK), 0, s, d, X, s_read, lo,
+                     out);
+  THR_HIP(hipGetLastError());
+  return THR_OK;
+}
 
 int launch_wall_source(const Dims &d, const float *ps, int spin, int s_wall, double sign,
                        double2 *out, hipStream_t s) {
@@ -73,9 +102,9 @@ int launch_wall_source(const Dims &d, const float *ps, int spin, int s_wall, dou
 
 long wall_dot_blocks(const Dims &d) { return nblk(d.v4()); }
 
-int launch_wall_dot(const Dims &d, const float *ps, const double2 *X, int spin, int s_read,
-                    double *partial, hipStream_t s) {
-  hipLaunchKernelGGL(k_wall_dot, dim3(nblk(d.v4())), dim3(BLK), 0, s, d, ps, X, spin, s_read,
+int launch_wall_dot(const Dims &d, const float *ps, const double2 *X, const double2 *Xsub, int spin,
+                    int s_read, double *partial, hipStream_t s) {
+  hipLaunchKernelGGL(k_wall_dot, dim3(nblk(d.v4())), dim3(BLK), 0, s, d, ps, X, Xsub, spin, s_read,
                      partial);
   THR_HIP(hipGetLastError());
   return THR_OK;

@@ -44,6 +44,7 @@ struct MsfResult {
   int switches[3];
   int nonfinite;    // sources whose recurrences produced NaN/Inf
   int unconverged;  // (source, shift) pairs still above the iterated target at maxcg
+  int reliable;     // reliable updates of the base residual taken
 };
 
 int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s);

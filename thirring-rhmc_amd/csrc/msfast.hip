@@ -327,6 +327,77 @@ __global__ __launch_bounds__(BLK) void k_true_res(long v5, const double2 *b, con
   if (threadIdx.x == 0) partial[(long)src * gridDim.x + blockIdx.x] = tot;
 }
 
+// Reliable update of the base residual (the replacement of the iterated
+// residual by a true one, Sleijpen and van der Vorst): for each source with
+// rel[src] != 0, r = (b - (M^+M + sigma) x) / zeta from the FP64 true
+// residual of one shift (t holds M^+M x, x that shift's solution, zeta its
+// coefficient r_i = zeta r), stored in the Krylov precision P over the
+// residual the next iteration reads.  rel: [nsrc] flags, [nsrc] sigma,
+// [nsrc] 1/zeta.  partial[src * gridDim.x + block] = |r|^2 of the stored values.
+template <int P>
+__global__ __launch_bounds__(BLK) void k_msf_reliable(long v5, const double2 *b, const double2 *t,
+                                                      const double2 *x, const double *rel,
+                                                      TT<P> *r, float *rn, double *partial) {
+  const int src = blockIdx.y, nsrc = gridDim.y;
+  const bool on = rel[src] != 0.0;
+  const long i = (long)blockIdx.x * BLK + threadIdx.x;
+  double red = 0.0;
+  if (on && i < v5) {
+    const double sig = rel[nsrc + src], iz = rel[2 * nsrc + src];
+    const long off = (long)src * 4 * v5;
+    cx<double> v[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const long e = off + k * v5 + i;
+      v[k] = iz * (ldc(b, e) - ldc(t, e) - sig * ldc(x, e));
+    }
+    if constexpr (P == P16) {
+      float m = 0.f;
+#pragma unroll
+      for (int k = 0; k < 4; k++) m = fmaxf(m, fmaxf(fabsf((float)v[k].r), fabsf((float)v[k].i)));
+      const float inv = m > 0.f ? 1.f / m : 0.f;
+      rn[(long)src * v5 + i] = m;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const __half2 h = __floats2half2_rn((float)v[k].r * inv, (float)v[k].i * inv);
+        r[off + 4 * i + k] = h;
+        const float2 f = __half22float2(h);
+        red += norm2(cx<double>{(double)(f.x * m), (double)(f.y * m)});
+      }
+    } else if constexpr (P == P32) {
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const cx<float> f{(float)v[k].r, (float)v[k].i};
+        st(r, off + k * v5 + i, f);
+        red += norm2(cx<double>{(double)f.r, (double)f.i});
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        st(r, off + k * v5 + i, v[k]);
+        red += norm2(v[k]);
+      }
+    }
+  }
+  const double tot = block_sum(red);
+  if (threadIdx.x == 0) partial[(long)src * gridDim.x + blockIdx.x] = tot;
+}
+
+// delta = lambda = |r|^2 of the replaced residual (rel as in k_msf_reliable)
+__global__ __launch_bounds__(BLK) void k_msf_reliable_fin(MsfState *st, const double *rel,
+                                                          const double *partial, int nb) {
+  MsfState *S = st + blockIdx.x;
+  const double r2 = fin_sum(partial + (long)blockIdx.x * nb, nb);
+  if (threadIdx.x == 0 && rel[blockIdx.x] != 0.0) {
+    S->delta = r2;
+    S->lambda = r2;
+    if (!isfinite(r2)) {
+      S->nonfinite = 1;
+      S->stop = 1;
+    }
+  }
+}
+
 __global__ __launch_bounds__(BLK) void k_sum_rows(const double *partial, int nb, double *out) {
   const double v = fin_sum(partial + (long)blockIdx.x * nb, nb);
   if (threadIdx.x == 0) out[blockIdx.x] = v;
@@ -444,7 +515,9 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
   void *hbuf[3], *r0[3];
   float *hn = nullptr, *r0n = nullptr;
   for (int pr = 0; pr < 3; pr++) {
-    const bool need = (pr == P64) || (pr == P32 && A.th32 > 0) || (pr == P16 && A.th16 > 0);
+    // precision only falls during a solve: FP64 buffers only for an FP64 start
+    const bool need = (pr == P64 && A.start_prec == P64) || (pr == P32 && A.th32 > 0) ||
+                      (pr == P16 && A.th16 > 0);
     for (int q = 0; q < MSB; q++)
       ring[pr][q] = need ? workspace(w_ring(pr, q), bytes_p[pr] * nsb) : nullptr;
     hbuf[pr] = need ? workspace(w_h(pr), bytes_p[pr] * nsb) : nullptr;
@@ -527,6 +600,82 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
   long t = 0;  // completed iterations
   int rc = THR_OK;
   int blocks = 0;
+  // reliable updates while the Krylov storage is FP32/FP16: each time the
+  // worst iterated residual has fallen by rdelta since the last one, the base
+  // residual of every running source is replaced by the FP64 true residual of
+  // its smallest active shift divided by that shift's zeta.  The smallest
+  // shift needs the most iterations and is the costliest to correct later;
+  // the heavier shifts keep their drift for the FP64 correction.
+  const double rdelta = context_reliable();
+  double r_ref = 1.0;  // worst |r|/|b| at the last reliable update
+  int n_reliable = 0;
+  auto reliable_update = [&]() -> int {
+    double *hrel = (double *)host_workspace(6, sizeof(double) * 3 * ns_);
+    int *istar = (int *)host_workspace(7, sizeof(int) * ns_);
+    THR_CHECK(hrel && istar, THR_ERR_HIP, "host workspace allocation failed");
+    int any = 0;
+    for (int k = 0; k < ns_; k++) {
+      const double *ak = A.aden_src ? A.aden_src + (long)k * n : A.aden;
+      int best = -1;
+      if (!hs[k].stop)
+        for (int i = 0; i < n; i++)
+          if (hs[k].active[i] && (best < 0 || ak[i] < ak[best])) best = i;
+      const double z = best >= 0 ? hs[k].zeta_ii[best] : 0.0;
+      const bool on = best >= 0 && z != 0.0 && std::isfinite(1.0 / z);
+      istar[k] = best;
+      hrel[k] = on ? 1.0 : 0.0;
+      hrel[ns_ + k] = on ? ak[best] : 0.0;
+      hrel[2 * ns_ + k] = on ? 1.0 / z : 0.0;
+      any += on ? 1 : 0;
+    }
+    if (!any) return THR_OK;
+    double2 *xg = (double2 *)workspace(380, sizeof(double2) * nsb);
+    double2 *t1 = (double2 *)workspace(381, sizeof(double2) * nsb);
+    double2 *t2 = (double2 *)workspace(382, sizeof(double2) * nsb);
+    double *drel = (double *)workspace(383, sizeof(double) * 3 * ns_);
+    const int nbr = (int)nblk(v5);
+    double *rpart = (double *)workspace(384, sizeof(double) * nbr * ns_);
+    THR_CHECK(xg && t1 && t2 && drel && rpart, THR_ERR_HIP, "workspace allocation failed");
+    for (int k = 0; k < ns_; k++)
+      if (hrel[k] != 0.0)
+        THR_HIP(hipMemcpyAsync(xg + k * nsp, A.x + ((long)k * n + istar[k]) * nsp,
+                               sizeof(double2) * nsp, hipMemcpyDeviceToDevice, s));
+    THR_HIP(hipMemcpyAsync(drel, hrel, sizeof(double) * 3 * ns_, hipMemcpyHostToDevice, s));
+    DslashLaunch a{};
+    a.d = d;
+    a.nrhs = ns_;
+    a.prec = P64;
+    a.imass = A.imass;
+    a.am3 = A.am3;
+    for (int k = 0; k < ns_; k++) a.am[k] = A.am[k];
+    a.links = A.u64;
+    a.mode = EPI_STORE;
+    a.sc_stride = 1;
+    a.done_stride = 1;
+    a.in = xg;
+    a.out = t1;
+    TRY_RC(launch_dslash(a, s));
+    DslashLaunch c = a;
+    c.dagger = 1;
+    c.in = t1;
+    c.out = t2;
+    TRY_RC(launch_dslash(c, s));
+    const long slot = (t - 1) % MSB;
+    dim3 g(nbr, ns_);
+    if (prec == P32)
+      hipLaunchKernelGGL(k_msf_reliable<P32>, g, dim3(BLK), 0, s, v5, A.b, t2, xg, drel,
+                         (float2 *)ring[P32][slot], nullptr, rpart);
+    else if (prec == P16)
+      hipLaunchKernelGGL(k_msf_reliable<P16>, g, dim3(BLK), 0, s, v5, A.b, t2, xg, drel,
+                         (__half2 *)ring[P16][slot], ringn[slot], rpart);
+    else
+      hipLaunchKernelGGL(k_msf_reliable<P64>, g, dim3(BLK), 0, s, v5, A.b, t2, xg, drel,
+                         (double2 *)ring[P64][slot], nullptr, rpart);
+    hipLaunchKernelGGL(k_msf_reliable_fin, dim3(ns_), dim3(BLK), 0, s, st, drel, rpart, nbr);
+    THR_HIP(hipGetLastError());
+    n_reliable++;
+    return THR_OK;
+  };
   // one block = MSB iterations + flush + reset; every block after the first
   // issues identical launches, so it can be captured once per precision as a
   // hipGraph and replayed (THR_GRAPHS=0 disables)
@@ -663,6 +812,15 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
       set_error("multishift: stop flag not raised");
       return THR_ERR_STATE;
     }
+    if (rdelta > 0 && prec != P64 && worst < rdelta * r_ref) {
+      rc = reliable_update();
+      if (rc) {
+        free_graphs();
+        return rc;
+      }
+      r_ref = worst;
+    }
+    if (prec == P64) r_ref = worst;  // drift starts when the storage precision drops
     // lower the Krylov precision as the residual falls
     int want = prec;
     if (A.th32 > 0 && worst < A.th32 && prec == P64) want = P32;
@@ -684,7 +842,9 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
     }
   }
   free_graphs();
+  context_note_reliable(n_reliable);
   if (res) {
+    res->reliable = n_reliable;
     res->iterations = (int)t;
     res->blocks = blocks;
     res->nonfinite = 0;
@@ -762,9 +922,12 @@ int msfast_correct(const MsfArgs &A, int max_rounds, double tol_true, double *re
   int total = 0;
   for (int round = 0;; round++) {
     TRY_RC(msfast_true_residuals(A, rr.data(), s));
-    std::vector<int> fail;  // k * n + i
-    for (int q = 0; q < ns_ * n; q++)
-      if (!(rr[q] <= tol_true)) fail.push_back(q);
+    // failing pairs q = k * n + i, grouped by shift so that each batched
+    // correction holds pairs of similar conditioning (similar iteration counts)
+    std::vector<int> fail;
+    for (int i = 0; i < n; i++)
+      for (int k = 0; k < ns_; k++)
+        if (!(rr[k * n + i] <= tol_true)) fail.push_back(k * n + i);
     if (fail.empty() || round >= max_rounds) break;
     // buffers for one group of up to ns_ (source, shift) pairs
     double2 *xs = (double2 *)workspace(372, sizeof(double2) * nsp * ns_);

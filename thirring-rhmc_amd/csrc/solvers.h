@@ -33,6 +33,12 @@ double *partial_buffer(int nrhs_blocks);
 hipStream_t default_stream();
 // whether solvers may capture their blocks as hipGraphs in the bound context
 bool context_graphs();
+// reliable-update factor of the mixed multishift in the bound context (0: off)
+// and the number of reliable updates its last solve took
+double context_reliable();
+void context_set_reliable(double delta);
+void context_note_reliable(int n);
+int context_last_reliable();
 
 // reference-faithful CG on M^+M (measure_module.F90:9-140); x in/out (FP64 device)
 int cg_reference(const OpDesc &op, const double2 *phi, double2 *x, float res, int niterc,
@@ -76,12 +82,15 @@ int stream_copy(const void *src, void *dst, size_t bytes, hipStream_t s);
 // psibarpsi measurement (measure.hip): wall source sign * eta on (spin, s_wall)
 // from a haloed real32 noise field ps(0:Lx+1, 0:Ly+1, 0:Lt+1, 2), and
 // per-block partial sums (re, im pairs, wall_dot_blocks(d) of them) of
-// sum_x conj(eta(x)) X(s_read, x, spin)
+// sum_x conj(eta(x)) (X - Xsub)(s_read, x, spin), Xsub optional
 int launch_wall_source(const Dims &d, const float *ps, int spin, int s_wall, double sign,
                        double2 *out, hipStream_t s);
 long wall_dot_blocks(const Dims &d);
-int launch_wall_dot(const Dims &d, const float *ps, const double2 *X, int spin, int s_read,
-                    double *partial, hipStream_t s);
+int launch_wall_dot(const Dims &d, const float *ps, const double2 *X, const double2 *Xsub, int spin,
+                    int s_read, double *partial, hipStream_t s);
+// X(s_read) with spins [lo, lo + 2) kept, as a 4-D spinor [spin][t][y][x]
+int launch_slice_project(const Dims &d, const double2 *X, int s_read, int lo, double2 *out,
+                         hipStream_t s);
 
 // apply M or M^+ (FP64) into out
 int apply_op(const OpDesc &op, int dagger, const double2 *in, double2 *out, hipStream_t s);

@@ -41,6 +41,11 @@ struct Context {
   // next to other threads' allocations and copies (a stream capture is
   // invalidated by device-synchronising calls from any thread)
   bool graphs = true;
+  // reliable updates of the mixed multishift's base residual: taken each time
+  // the residual falls by this factor while the Krylov storage is FP32/FP16
+  // (0: never); and how many the last solve in this context took
+  double reliable = 0.1;
+  int last_reliable = 0;
 };
 
 namespace {
@@ -56,6 +61,10 @@ Context *context_create() {
 }
 
 bool context_graphs() { return cur().graphs; }
+double context_reliable() { return cur().reliable; }
+void context_set_reliable(double delta) { cur().reliable = delta; }
+void context_note_reliable(int n) { cur().last_reliable = n; }
+int context_last_reliable() { return cur().last_reliable; }
 
 void context_destroy(Context *c) {
   if (!c) return;

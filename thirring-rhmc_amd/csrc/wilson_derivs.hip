@@ -14,6 +14,7 @@
 // spin of the Wilson and Dirac parts of d/dtheta Re<R, M^(+) X2>, with the
 // float32 coefficient tzi_real = 2*real(anum).
 #include <algorithm>
+#include <cstdlib>
 #include <vector>
 
 #include "kernels.h"
@@ -134,7 +135,7 @@ __device__ inline void ld_spinor(const double2 *__restrict__ p, unsigned v5, uns
 // butterfly of wave shuffles inside the segment (a fixed order, no LDS and no
 // block barrier); segment lane 0 adds the three link terms into dSdpi.  The
 // t march and loads are those of k_derivs_march below.
-__global__ __launch_bounds__(BLK) void k_derivs_seg(const Dims d, const double2 *__restrict__ R,
+__global__ __launch_bounds__(BLK, 3) void k_derivs_seg(const Dims d, const double2 *__restrict__ R,
                                                     const double2 *__restrict__ X2, float tzi,
                                                     int iflag, unsigned wseg, unsigned tchunk,
                                                     double *ds) {
@@ -164,16 +165,17 @@ __global__ __launch_bounds__(BLK) void k_derivs_seg(const Dims d, const double2 
   for (unsigned t = t0; t < t1; t++) {
     double v[3] = {0.0, 0.0, 0.0};
     if (valid) {
+      // all six neighbour spinors of the step in flight together
       const unsigned tf = (t + 1 == lt) ? 0 : t + 1;
-      cx<double> R1[4], X1[4], Rt[4], Xt[4];
-      ld_spinor(R, v5, jx + v3 * t, R1);
-      ld_spinor(X2, v5, jx + v3 * t, X1);
-      v[0] = derivs_term<0>(R0, X0, R1, X1, w1, w2);
-      ld_spinor(R, v5, jy + v3 * t, R1);
-      ld_spinor(X2, v5, jy + v3 * t, X1);
-      v[1] = derivs_term<1>(R0, X0, R1, X1, w1, w2);
+      cx<double> Rx[4], Xx[4], Ry[4], Xy[4], Rt[4], Xt[4];
+      ld_spinor(R, v5, jx + v3 * t, Rx);
+      ld_spinor(X2, v5, jx + v3 * t, Xx);
+      ld_spinor(R, v5, jy + v3 * t, Ry);
+      ld_spinor(X2, v5, jy + v3 * t, Xy);
       ld_spinor(R, v5, j + v3 * tf, Rt);
       ld_spinor(X2, v5, j + v3 * tf, Xt);
+      v[0] = derivs_term<0>(R0, X0, Rx, Xx, w1, w2);
+      v[1] = derivs_term<1>(R0, X0, Ry, Xy, w1, w2);
       v[2] = derivs_term<2>(R0, X0, Rt, Xt, w1, w2);
 #pragma unroll
       for (int a = 0; a < 4; a++) {
@@ -434,10 +436,21 @@ int launch_derivs(const Dims &d, const double2 *R, const double2 *X2, double anu
                   double *dsdpi, hipStream_t s) {
   const float tzi = 2.0f * (float)anum;  // real :: tzi_real = 2*real(anum)
   THR_CHECK(d.v5() * 4 < (1L << 31), THR_ERR_SIZE, "derivs: 4*V5 must be below 2^31");
-  // t-chunks long enough to reuse the t neighbour but short enough to leave
-  // ~2^18 threads in flight
-  const long v5 = d.v5();
-  const unsigned tchunk = (unsigned)std::max(1L, std::min<long>(d.lt, v5 >> 18));
+  // one t step per thread: marching longer t-chunks reuses the t neighbour
+  // from registers but leaves fewer threads in flight, and measured slower
+  // (32^3 x 64: 76-80 us at one step, 82-90 us at 2-16; 16^3 x 48: 12-13
+  // against 14-44 us) (THR_DERIVS_TCHUNK overrides)
+  unsigned tchunk = 1;
+  if (const char *e = std::getenv("THR_DERIVS_TCHUNK"))
+    tchunk = (unsigned)std::max(1, std::min(d.lt, std::atoi(e)));
+  // resident blocks per CU capped by LDS padding (THR_DERIVS_OCC = k: k + 1
+  // blocks): fewer resident s-lines per XCD keep the x+1 / y+1 neighbours in L2
+  size_t pad = 0;
+  if (const char *e = std::getenv("THR_DERIVS_OCC")) {
+    const int k = std::max(0, std::min(3, std::atoi(e)));
+    if (k > 0) pad = (size_t)(160 * 1024 / (k + 1)) & ~(size_t)511;
+    if (pad > 65536) pad = 65536;
+  }
   const unsigned nchunk = (d.lt + tchunk - 1) / tchunk;
   const unsigned v2 = (unsigned)d.lx * d.ly;
   if (d.ls <= 64) {
@@ -445,7 +458,7 @@ int launch_derivs(const Dims &d, const double2 *R, const double2 *X2, double anu
     while (w < (unsigned)d.ls) w <<= 1;
     const unsigned nl = BLK / w;
     const unsigned nb = ((v2 + nl - 1) / nl) * nchunk;
-    hipLaunchKernelGGL(k_derivs_seg, dim3(nb), dim3(BLK), 0, s, d, R, X2, tzi, iflag, w, tchunk,
+    hipLaunchKernelGGL(k_derivs_seg, dim3(nb), dim3(BLK), pad, s, d, R, X2, tzi, iflag, w, tchunk,
                        dsdpi);
     THR_HIP(hipGetLastError());
     return THR_OK;

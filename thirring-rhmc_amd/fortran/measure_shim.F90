@@ -1,12 +1,15 @@
-! Fortran drop-in `measure_module` (measure_module.F90): congrad, measure and
-! measure_shamir with the reference's names, argument lists and outputs.
+! Fortran drop-in `measure_module` (measure_module.F90): congrad, measure,
+! measure_shamir and measure_wilson with the reference's names, argument
+! lists and outputs.
 !
 ! Compiled after shims.F90 and after the caller's own `random` and `gaussian`
 ! modules: the noise of the psibarpsi measurement comes from the caller's
 ! gauss0, drawn in the reference's order (two fields per noise vector), so a
-! run reproduces the reference's numbers.  The four wall-source solves of each
-! noise vector (dslashd_shamir, then congrad from the source) run on the GPU
-! in one call of thr_measure_shamir_noise; the normalisation, the
+! run reproduces the reference's numbers.  The wall-source solves of each
+! noise vector (dslashd_shamir or dslashd_wilson, then congrad from the
+! source, and for the Wilson kernel the DWilson projection of the read slices)
+! run on the GPU in one call of thr_measure_shamir_noise or
+! thr_measure_wilson_noise; the normalisation, the
 ! susceptibility and the fort.100 / fort.200 records are the reference's.
 
 module measure_module
@@ -58,8 +61,7 @@ contains
     call measure_shamir(psibarpsi, res, aviter, am, imass, isweep_total)
 #endif
 #ifdef MEASURE_WILSON
-    print *, 'measure: the Wilson-kernel measurement (MEASURE_WILSON) is not provided by this build'
-    call exit(1)
+    call measure_wilson(psibarpsi, res, aviter, am, imass, isweep_total)
 #endif
 #ifdef MEASURE_OWILSON
     call measure_wilsonKDDW(psibarpsi, res, aviter, am, imass, isweep_total)
@@ -68,11 +70,34 @@ contains
 
   ! measure_module.F90:170-423: noisy psibarpsi with wall sources
   subroutine measure_shamir(psibarpsi, res, aviter, am, imass, isweep_total)
+    real, intent(out) :: psibarpsi, aviter
+    real, intent(in) :: res, am
+    integer, intent(in) :: imass
+    integer, intent(in), optional :: isweep_total
+    call noisy_measurement(0, psibarpsi, res, aviter, am, imass, isweep_total)
+  end subroutine measure_shamir
+
+  ! measure_module.F90:431-640: the same with Wilson-kernel wall sources and
+  ! the DWilson projection of the read slices (imass 1 and 3)
+  subroutine measure_wilson(psibarpsi, res, aviter, am, imass, isweep_total)
+    real, intent(out) :: psibarpsi, aviter
+    real, intent(in) :: res, am
+    integer, intent(in) :: imass
+    integer, intent(in), optional :: isweep_total
+    print *, "Wilson measure"
+    call noisy_measurement(1, psibarpsi, res, aviter, am, imass, isweep_total)
+  end subroutine measure_wilson
+
+  ! the noise loop both measurements share: noise drawn with the caller's
+  ! gauss0 in the reference's order, one library call per noise vector for
+  ! the wall-source solves, the reference's normalisation and records
+  subroutine noisy_measurement(which, psibarpsi, res, aviter, am, imass, isweep_total)
     use trial, only: u
     use params
     use comms_common, only: ip_global
     use gaussian
     use thr_capi
+    integer, intent(in) :: which
     real, intent(out) :: psibarpsi, aviter
     real, intent(in) :: res, am
     integer, intent(in) :: imass
@@ -82,7 +107,7 @@ contains
     real(c_double) :: s1(2), s2(2)
     complex(dp) :: psibarpsi1, psibarpsi2
     real(dp) :: pbp(knoise)
-    integer(c_int) :: d(4), itn
+    integer(c_int) :: d(4), itn, rc
     integer :: inoise, iter
     real :: susclsing
 #ifdef MPI
@@ -101,15 +126,21 @@ contains
       call gauss0(ps)
       call gauss0(pt)
 #endif
-      call thr_check(thr_measure_shamir_noise(c_loc(ps), c_loc(pt), c_loc(u), d, &
-                                              real(res, c_float), real(am, c_float), &
-                                              int(imass, c_int), real(am3, c_float), &
-                                              int(niterc, c_int), generation_kernel(), s1, s2, &
-                                              itn), 'measure_shamir')
+      if (which == 0) then
+        rc = thr_measure_shamir_noise(c_loc(ps), c_loc(pt), c_loc(u), d, real(res, c_float), &
+                                      real(am, c_float), int(imass, c_int), real(am3, c_float), &
+                                      int(niterc, c_int), generation_kernel(), s1, s2, itn)
+        call thr_check(rc, 'measure_shamir')
+      else
+        rc = thr_measure_wilson_noise(c_loc(ps), c_loc(pt), c_loc(u), d, real(res, c_float), &
+                                      real(am, c_float), int(imass, c_int), real(am3, c_float), &
+                                      int(niterc, c_int), generation_kernel(), s1, s2, itn)
+        call thr_check(rc, 'measure_wilson')
+      end if
       iter = iter + itn
       psibarpsi1 = cmplx(s1(1), s1(2), dp)
       psibarpsi2 = cmplx(s2(1), s2(2), dp)
-      ! measure_module.F90:394-405
+      ! measure_module.F90:394-405, 608-620
       if (imass .eq. 1) then
         psibarpsi1 = psibarpsi1/kvol
         psibarpsi2 = psibarpsi2/kvol
@@ -151,5 +182,5 @@ contains
       close (200)
     end if
     aviter = float(iter)/(4*knoise)
-  end subroutine measure_shamir
+  end subroutine noisy_measurement
 end module measure_module

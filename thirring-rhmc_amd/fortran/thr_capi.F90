@@ -73,6 +73,17 @@ module thr_capi
       integer(c_int), intent(out) :: iters
     end function
 
+    integer(c_int) function thr_measure_wilson_noise(ps, pt, u, dims, res, am, imass, am3, niterc, &
+        kernel, pbp1, pbp2, iters) bind(C, name="thr_measure_wilson_noise")
+      import :: c_int, c_ptr, c_float, c_double
+      type(c_ptr), value :: ps, pt, u
+      integer(c_int), intent(in) :: dims(4)
+      real(c_float), value :: res, am, am3
+      integer(c_int), value :: imass, niterc, kernel
+      real(c_double), intent(out) :: pbp1(2), pbp2(2)
+      integer(c_int), intent(out) :: iters
+    end function
+
     integer(c_int) function thr_multishift(u, dims, am, imass, am3, ndiagq, aden, anum, output, &
         input, res, maxcg, use_sp, kernel, cg_return, cg_returns) bind(C, name="thr_multishift")
       import :: c_int, c_ptr, c_float

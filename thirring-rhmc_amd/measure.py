@@ -8,7 +8,10 @@ s = 1; for imass 5 the sources sit on the chirality-flipped spins and are read
 at their own wall.  psibarpsi is the average over `knoise` vectors of the
 wall-to-wall contractions.
 
-Two ways to run it:
+The Wilson-kernel measurement (measure_wilson) runs through the library in
+the same way (`measure_wilson`).
+
+Two ways to run measure_shamir:
   * `measure_shamir` — the reference's order: one congrad per source,
     starting from the source (so iteration counts, hence `aviter`, match),
     imass 1, 3, 5; on the GPU one library call per noise vector;
@@ -106,6 +109,27 @@ def measure_shamir(u, dims, res, am, imass, rng, knoise=10, am3=1.0, ops=G, nite
                     p1 += acc
                 else:
                     p2 += acc
+        pbp.append(_finish(p1, p2, imass, lx * ly * lt))
+    return _average(pbp, iters, knoise)
+
+
+def measure_wilson(u, dims, res, am, imass, rng, knoise=10, am3=1.0, niterc=None,
+                   kernel="shamir"):
+    """measure_wilson (measure_module.F90:431-640), the Wilson-kernel
+    psibarpsi, imass 1 or 3: returns (psibarpsi, aviter).  Per noise vector
+    (ps, pt = gauss0(), gauss0()) one library call runs the eight wall-source
+    solves on the device: dslashd_wilson, congrad with `kernel` (the
+    generation kernel the caller's congrad uses) and the DWilson projection
+    of the read slice."""
+    if imass not in (1, 3):
+        raise ValueError("measure_wilson supports imass 1 and 3")
+    lx, ly, lt, ls = dims
+    pbp, iters = [], 0
+    for _ in range(knoise):
+        ps, pt = rng.gauss0(), rng.gauss0()
+        p1, p2, it = G.measure_wilson_noise(_haloed_noise(ps), _haloed_noise(pt), u, dims, res,
+                                            am, imass, am3=am3, niterc=niterc, kernel=kernel)
+        iters += it
         pbp.append(_finish(p1, p2, imass, lx * ly * lt))
     return _average(pbp, iters, knoise)
 

@@ -37,6 +37,8 @@ _SIGS = {
     "thr_context_bind": ([P], I32),
     "thr_context_bytes": ([ctypes.POINTER(ctypes.c_ulonglong)], I32),
     "thr_context_trim": ([], I32),
+    "thr_context_set_reliable": ([ctypes.c_double], I32),
+    "thr_context_reliable_updates": ([ctypes.POINTER(ctypes.c_int)], I32),
     "thr_coef": ([P, P, P, I32], I32),
     "thr_dslash": ([P, P, P, P, F32, I32, F32, I32], I32),
     "thr_dslashd": ([P, P, P, P, F32, I32, F32, I32], I32),
@@ -46,6 +48,7 @@ _SIGS = {
     "thr_congrad_dev_noalloc": ([P, P, P, P, F32, F32, I32, F32, I32, IP, IP, P], I32),
     "thr_derivs_dev": ([P, P, F64, I32, P, F32, I32, I32, P, P], I32),
     "thr_measure_shamir_noise": ([P, P, P, P, F32, F32, I32, F32, I32, I32, P, P, IP], I32),
+    "thr_measure_wilson_noise": ([P, P, P, P, F32, F32, I32, F32, I32, I32, P, P, IP], I32),
     "thr_multishift": ([P, P, F32, I32, F32, I32, P, P, P, P, F32, I32, I32, I32, IP, P], I32),
     "thr_derivs": ([P, P, F64, I32, P, F32, I32, F32, I32, P], I32),
     "thr_qmrherm": ([P, P, F32, IP, F32, I32, F32, P, P, I32, I32, I32, P, P, I32, I32, P, P, P,
@@ -166,6 +169,23 @@ def halo5(a):
     return a
 
 
+def set_reliable_updates(delta):
+    """Reliable-update factor of the mixed multishift in the bound context
+    (0 disables, default 0.1): while the Krylov storage is FP32/FP16 the base
+    residual is replaced by the FP64 true residual of the smallest active
+    shift each time the residual has fallen by this factor."""
+    init()
+    _check(lib().thr_context_set_reliable(float(delta)), "thr_context_set_reliable")
+
+
+def reliable_updates():
+    """Reliable updates taken by the bound context's last multishift solve."""
+    init()
+    v = ctypes.c_int()
+    _check(lib().thr_context_reliable_updates(ctypes.byref(v)), "thr_context_reliable_updates")
+    return int(v.value)
+
+
 class Context:
     """Private scratch buffers + stream for one thread of work."""
 
@@ -245,13 +265,7 @@ def congrad(phi, x, u, dims, res, am, imass, am3=1.0, niterc=None, kernel="shami
     return it.value, nx.value
 
 
-def measure_shamir_noise(ps, pt, u, dims, res, am, imass, am3=1.0, niterc=None,
-                         kernel="shamir"):
-    """One noise vector of measure_shamir (measure_module.F90:210-372) on the
-    GPU: ps, pt are gauss0 fields, float32 haloed (Lx+2, Ly+2, Lt+2, 2) in
-    Fortran order.  Returns (psibarpsi1, psibarpsi2, iters): the complex sums
-    before the division by kvol and the total congrad iterations of the four
-    solves."""
+def _measure_noise(fn, what, ps, pt, u, dims, res, am, imass, am3, niterc, kernel):
     init()
     lx, ly, lt, ls = dims
     if niterc is None:
@@ -263,10 +277,30 @@ def measure_shamir_noise(ps, pt, u, dims, res, am, imass, am3=1.0, niterc=None,
     p1 = np.zeros(2)
     p2 = np.zeros(2)
     it = ctypes.c_int()
-    _check(lib().thr_measure_shamir_noise(_p(ps), _p(pt), _p(u), _dims(dims), res, am, int(imass),
-                                          am3, int(niterc), KERNEL[kernel], _p(p1), _p(p2),
-                                          ctypes.byref(it)), "measure_shamir_noise")
+    _check(fn(_p(ps), _p(pt), _p(u), _dims(dims), res, am, int(imass), am3, int(niterc),
+              KERNEL[kernel], _p(p1), _p(p2), ctypes.byref(it)), what)
     return complex(p1[0], p1[1]), complex(p2[0], p2[1]), it.value
+
+
+def measure_shamir_noise(ps, pt, u, dims, res, am, imass, am3=1.0, niterc=None,
+                         kernel="shamir"):
+    """One noise vector of measure_shamir (measure_module.F90:210-372) on the
+    GPU: ps, pt are gauss0 fields, float32 haloed (Lx+2, Ly+2, Lt+2, 2) in
+    Fortran order.  Returns (psibarpsi1, psibarpsi2, iters): the complex sums
+    before the division by kvol and the total congrad iterations of the four
+    solves."""
+    return _measure_noise(lib().thr_measure_shamir_noise, "measure_shamir_noise", ps, pt, u,
+                          dims, res, am, imass, am3, niterc, kernel)
+
+
+def measure_wilson_noise(ps, pt, u, dims, res, am, imass, am3=1.0, niterc=None,
+                         kernel="shamir"):
+    """One noise vector of measure_wilson (measure_module.F90:472-606) on the
+    GPU: eight wall-source solves (dslashd_wilson, congrad with `kernel`) and
+    the DWilson projection of the read slices; returns (psibarpsi1,
+    psibarpsi2, iters) as measure_shamir_noise."""
+    return _measure_noise(lib().thr_measure_wilson_noise, "measure_wilson_noise", ps, pt, u,
+                          dims, res, am, imass, am3, niterc, kernel)
 
 
 def multishift(u, dims, am, imass, aden, anum, inp, res, maxcg, am3=1.0, sp=False,
@@ -587,8 +621,9 @@ def multishift_fast_dev(links64, links32, dims, am, imass, aden, b, x, tol, maxc
 
     corrections: rounds of FP64 defect correction (default: 4 when th32 or th16
     lowers the Krylov precision, else 0); stats (dict, optional) receives
-    "corr_iterations" and "switches" (iteration at which the Krylov storage
-    entered FP64, FP32, FP16; -1 never).  Raises ThrNoConvergence when a true
+    "corr_iterations", "switches" (iteration at which the Krylov storage
+    entered FP64, FP32, FP16; -1 never) and "reliable" (reliable updates of
+    the base residual taken, see set_reliable_updates).  Raises ThrNoConvergence when a true
     residual stays above tol (ThrNonFinite for NaN/Inf); the partial results
     are then still in x and in the exception's `result`."""
     init()
@@ -609,6 +644,7 @@ def multishift_fast_dev(links64, links32, dims, am, imass, aden, b, x, tol, maxc
     if stats is not None:
         stats["corr_iterations"] = ci.value
         stats["switches"] = [int(v) for v in sw]
+        stats["reliable"] = reliable_updates()
     result = (it.value, cgr, (rel.reshape(nsrc, n) if rel is not None else None))
     try:
         _check(rc, "multishift_fast_dev")
