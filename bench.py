@@ -31,7 +31,8 @@ Also reported in the same JSON line:
     mixed-precision solver with FP64 correction (time and difference);
   * mixed against FP64 at 32^3 x Ls=64: force and dH differences;
   * solve time per iteration at launch-bound lattices (8^3 x 8, 12^2 x 24 x
-    16) with hipGraph replay and with eager launches;
+    16): the persistent cooperative solver against the streaming one with
+    hipGraph replay and with eager launches;
   * serving: a sweep of Poisson arrival rates, each with p50/p99 latency and
     solves/s under continuous batching (thirring-rhmc_amd/serving.py);
   * the CPU baseline: the FP64 C oracle's Dslash on the host cores.
@@ -171,7 +172,7 @@ def multi_rhs_legs(G, hr, steps, warmup, stream, pinned=None, tuned=()):
 
 
 def bench_multishift(G, hr, dims, l64, l32, am, nsrc, stream, tol=1e-10, maxcg=50000,
-                     mixed_th32=1e-4, mixed_th16=1e-7,
+                     mixed_th32=1e-2, mixed_th16=1e-6,
                      legs_wanted=("plain_fp64", "blocked_fp64", "blocked_mixed")):
     """Plain FP64 path (one source at a time) vs the batched mixed path."""
     lx, ly, lt, ls = dims
@@ -206,7 +207,7 @@ def bench_multishift(G, hr, dims, l64, l32, am, nsrc, stream, tol=1e-10, maxcg=5
         out[label] = {"solves_per_s": round(nsrc / dt, 4), "seconds": round(dt, 3),
                       "sources": nsrc, "iterations": int(it),
                       "correction_iterations": int(st.get("corr_iterations", 0)),
-                      "th32": th32, "th16": th16,
+                      "th32": th32, "th16": th16, "reliable_updates": st.get("reliable"),
                       "max_true_residual": float(rel.max())}
         if plain_x is not None:
             # source 0 against the plain FP64 solution, per pole
@@ -285,8 +286,8 @@ def bench_batched_config4(G, hr, stream, dims=(24, 24, 24, 32), nsrc=16, tol=1e-
     """The batched multi-RHS workload: 16 sources x 25 shifts of x^(-1/4)
     (remez4 block 2) at 24^3 x Ls=32, one link field, per-source masses drawn
     from {0.005, 0.01, 0.02, 0.05}: the batched solver (FP32 Krylov storage
-    from the start, or after |r|/|b| < 1e-4; hipGraph-captured blocks, FP64
-    defect correction) against
+    from the start, or after |r|/|b| < 1e-2 or 1e-4; reliable updates,
+    hipGraph-captured blocks, FP64 defect correction) against
     the plain FP64 path (the reference's recurrences, one source at a time,
     each with its own mass); solves/s to FP64 true residual `tol` for both."""
     lx, ly, lt, ls = dims
@@ -299,10 +300,13 @@ def bench_batched_config4(G, hr, stream, dims=(24, 24, 24, 32), nsrc=16, tol=1e-
     bh = rng.standard_normal((nsrc, nsp, 2)) / np.sqrt(2)
     b = hr.DeviceArray.from_host(bh)
     out = {"lattice": list(dims), "sources": nsrc, "shifts": n, "masses": masses.tolist()}
-    # two precision schedules of the Krylov storage: FP32 from the first
-    # iteration, and FP64 until |r|/|b| < 1e-4 then FP32; both end with the
-    # FP64 defect correction of every (source, shift) above tol
-    for label, th32 in (("batched_fp32_from_start", 1.0), ("batched_fp64_then_fp32", 1e-4)):
+    # precision schedules of the Krylov storage: FP32 from the first block,
+    # and FP64 until |r|/|b| < 1e-2 or 1e-4 then FP32; all with reliable
+    # updates of the base residual (the library default) and the FP64 defect
+    # correction of every (source, shift) above tol
+    scheds = (("batched_fp32_from_start", 1.0), ("batched_fp64_then_fp32", 1e-2),
+              ("batched_fp64_to_1e-4_then_fp32", 1e-4))
+    for label, th32 in scheds:
         x = hr.DeviceArray((nsrc, n, nsp, 2), np.float64)
         progress(f"config-4 batched solve {dims}: {nsrc} sources x {n} shifts, th32={th32}")
         hr.synchronize()
@@ -318,6 +322,7 @@ def bench_batched_config4(G, hr, stream, dims=(24, 24, 24, 32), nsrc=16, tol=1e-
         out[label] = {"solves_per_s": round(nsrc / dt, 4), "seconds": round(dt, 3),
                       "th32": th32, "iterations": int(it),
                       "correction_iterations": int(st.get("corr_iterations", 0)),
+                      "reliable_updates": st.get("reliable"),
                       "max_true_residual": float(rel.max())}
         x.free()
     progress(f"config-4 plain FP64, {nsrc} sources one at a time")
@@ -338,7 +343,7 @@ def bench_batched_config4(G, hr, stream, dims=(24, 24, 24, 32), nsrc=16, tol=1e-
     xp.free()
     out["plain_fp64"] = {"solves_per_s": round(nsrc / total, 4), "seconds": round(total, 3),
                          "iterations": its, "max_true_residual": worst}
-    for label in ("batched_fp32_from_start", "batched_fp64_then_fp32"):
+    for label, _ in scheds:
         out[f"speedup_{label}_vs_plain_fp64"] = round(out[label]["solves_per_s"]
                                                       / out["plain_fp64"]["solves_per_s"], 3)
     b.free()
@@ -348,14 +353,21 @@ def bench_batched_config4(G, hr, stream, dims=(24, 24, 24, 32), nsrc=16, tol=1e-
 
 
 def bench_small_lattices(G, hr, stream, nsrc=4, tol=1e-10):
-    """Launch-bound sizes of the serving mix: the batched mixed solve of
+    """Launch-bound sizes of the serving mix: the batched solve of
     (M^+M) x = b (the serving request's system, 4 sources, am 0.05) at
-    8^3 x Ls=8 and 12^2 x 24 x Ls=16, with the steady-state blocks replayed
-    from hipGraphs and launched eagerly: wall time per solve and per
-    iteration."""
+    8^3 x Ls=8 and 12^2 x 24 x Ls=16 by the persistent cooperative kernel
+    (THR_PERSIST=1, an option) and by the streaming solver (the default) with
+    its steady-state blocks replayed from hipGraphs and launched eagerly:
+    wall time per solve and per iteration."""
     out = {}
-    saved = os.environ.get("THR_GRAPHS")
+    keys = ("THR_GRAPHS", "THR_PERSIST")
+    saved = {k: os.environ.get(k) for k in keys}
     rng = np.random.default_rng(5)
+    modes = [("persistent", {"THR_PERSIST": "1"}),
+             ("graphs", {"THR_PERSIST": "0", "THR_GRAPHS": "1"}),
+             ("eager", {"THR_PERSIST": "0", "THR_GRAPHS": "0"})]
+    if under_profiler():  # no graph launches under the profiler
+        modes = [m for m in modes if m[0] != "graphs"]
     try:
         for dims in ((8, 8, 8, 8), (12, 12, 24, 16)):
             lx, ly, lt, ls = dims
@@ -364,30 +376,37 @@ def bench_small_lattices(G, hr, stream, nsrc=4, tol=1e-10):
             b = hr.DeviceArray.from_host(rng.standard_normal((nsrc, nsp, 2)) / np.sqrt(2))
             x = hr.DeviceArray((nsrc, 1, nsp, 2), np.float64)
             row = {}
-            modes = (("eager", "0"),) if under_profiler() else (("graphs", "1"), ("eager", "0"))
-            for label, flag in modes:
-                os.environ["THR_GRAPHS"] = flag
+            for label, env in modes:
+                os.environ.update(env)
                 progress(f"small lattice {dims} {label}")
                 G.multishift_fast_dev(l64, l32, dims, 0.05, 3, np.zeros(1), b, x, tol=tol,
                                       maxcg=100000, nsrc=nsrc, th32=1e-4, true_residuals=False,
                                       stream=stream)  # warm: workspace, graph capture
                 hr.synchronize()
                 t = time.perf_counter()
-                it, _, _ = G.multishift_fast_dev(l64, l32, dims, 0.05, 3, np.zeros(1), b, x,
-                                                 tol=tol, maxcg=100000, nsrc=nsrc, th32=1e-4,
-                                                 true_residuals=False, stream=stream)
+                _, cgr, _ = G.multishift_fast_dev(l64, l32, dims, 0.05, 3, np.zeros(1), b, x,
+                                                  tol=tol, maxcg=100000, nsrc=nsrc, th32=1e-4,
+                                                  true_residuals=False, stream=stream)
                 hr.synchronize()
                 dt = time.perf_counter() - t
-                row[label] = {"seconds": round(dt, 5), "iterations": int(it),
-                              "us_per_iteration": round(1e6 * dt / max(1, int(it)), 2)}
+                it = int(max(cgr))
+                rel = G.true_residuals_dev(l64, dims, 0.05, 3, np.zeros(1), b, x, nsrc=nsrc,
+                                           stream=stream)
+                row[label] = {"seconds": round(dt, 5), "iterations": it,
+                              "us_per_iteration": round(1e6 * dt / max(1, it), 2),
+                              "max_true_residual": float(rel.max())}
+            if "graphs" in row:
+                row["speedup_persistent_vs_graphs"] = round(row["graphs"]["seconds"]
+                                                            / row["persistent"]["seconds"], 3)
             out["x".join(map(str, dims))] = row
             for a in (b, x, l64, l32):
                 a.free()
     finally:
-        if saved is None:
-            os.environ.pop("THR_GRAPHS", None)
-        else:
-            os.environ["THR_GRAPHS"] = saved
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
     return out
 
 
@@ -580,8 +599,8 @@ def main():
     ap.add_argument("--solver-lattice", default="32,32,32,64")
     ap.add_argument("--solver-am", type=float, default=0.005)
     ap.add_argument("--solver-sources", type=int, default=4)
-    ap.add_argument("--mixed-th32", type=float, default=1e-4)
-    ap.add_argument("--mixed-th16", type=float, default=1e-7)
+    ap.add_argument("--mixed-th32", type=float, default=1e-2)
+    ap.add_argument("--mixed-th16", type=float, default=1e-6)
     ap.add_argument("--solver-legs", default="plain_fp64,blocked_fp64,blocked_mixed",
                     help="comma-separated subset of the multishift legs to run")
     ap.add_argument("--tchunk", type=int, default=0,

@@ -91,10 +91,11 @@ int thr_context_bytes(unsigned long long *bytes);
 int thr_context_trim(void);
 /* Reliable updates in the bound context's batched mixed-precision multishift
  * (thr_multishift_fast_dev and the mixed qmrherm/force): while the Krylov
- * storage is FP32 or FP16, each time the iterated residual has fallen by the
- * factor delta since the last update, the base residual is replaced by the
- * FP64 true residual of the smallest active shift (scaled by 1/zeta of that
- * shift), so that shift's solution does not inherit the low-precision drift.
+ * storage is FP32 or FP16, each time the iterated residual of the smallest
+ * active shift has fallen by the factor delta since the last update, the base
+ * residual is replaced by the FP64 true residual of that shift (scaled by
+ * 1/zeta of that shift) and the shift directions are moved with it, so that
+ * shift's solution does not inherit the low-precision drift.
  * delta in [0, 1); 0 disables; default 0.1.  No counterpart in the reference,
  * whose FP32 multishift (multishift_module.F90:289-534) has no correction. */
 int thr_context_set_reliable(double delta);

@@ -43,9 +43,24 @@ def _sigmas():
     return np.array([1e-4, 1e-3, 1e-2, 1e-1, 1.0])
 
 
-def test_nan_source_fast_multishift_stops_within_one_block(G):
+@pytest.fixture(params=["0", "1"], ids=["streaming", "persistent"])
+def solver_kind(request):
+    """The batched multishift as streaming kernels (THR_PERSIST=0) and as the
+    persistent cooperative kernel (THR_PERSIST=1)."""
+    import os
+    saved = os.environ.get("THR_PERSIST")
+    os.environ["THR_PERSIST"] = request.param
+    yield request.param
+    if saved is None:
+        os.environ.pop("THR_PERSIST", None)
+    else:
+        os.environ["THR_PERSIST"] = saved
+
+
+def test_nan_source_fast_multishift_stops_within_one_block(G, solver_kind):
     from hiprt import DeviceArray
-    print("case: NaN source, batched multishift (2 sources, one poisoned)", flush=True)
+    print("case: NaN source, batched multishift (2 sources, one poisoned), THR_PERSIST",
+          solver_kind, flush=True)
     dims = (4, 4, 4, 8)
     _, l64, l32, bh = _fields(dims, 11, nsrc=2)
     bh[1, 17, 0] = np.nan
@@ -62,11 +77,11 @@ def test_nan_source_fast_multishift_stops_within_one_block(G):
     assert it < 50000
 
 
-def test_nan_link_fast_multishift(G):
+def test_nan_link_fast_multishift(G, solver_kind):
     """A non-finite link makes alpha NaN in the first iteration: the solve
     ends within its first 32-iteration block."""
     from hiprt import DeviceArray
-    print("case: NaN link, batched multishift", flush=True)
+    print("case: NaN link, batched multishift, THR_PERSIST", solver_kind, flush=True)
     dims = (4, 4, 4, 8)
     th, l64, l32, bh = _fields(dims, 12)
     lh = l64.to_host()

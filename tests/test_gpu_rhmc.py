@@ -16,6 +16,21 @@ import oracle as O
 
 pytestmark = pytest.mark.gpu
 
+
+@pytest.fixture(autouse=True)
+def streaming_solver():
+    """THR_PERSIST=0: the mixed-precision force and qmrherm tests exercise the
+    streaming blocked solver, which the library replaces by the persistent
+    FP64 kernel on small lattices."""
+    import os
+    saved = os.environ.get("THR_PERSIST")
+    os.environ["THR_PERSIST"] = "0"
+    yield
+    if saved is None:
+        os.environ.pop("THR_PERSIST", None)
+    else:
+        os.environ["THR_PERSIST"] = saved
+
 DATA = os.path.join(os.path.dirname(__file__), "..", "thirring-rhmc_amd", "data")
 
 

@@ -34,6 +34,7 @@ struct MsfArgs {
   int start_prec;      // 0 FP64 (default), 1 FP32, 2 FP16
   const double *aden_src = nullptr;  // host [nsrc][nshift]: per-source shifts (overrides aden)
   const double *tol_src = nullptr;   // host [nsrc]: per-source tolerances (overrides tol)
+  bool note_reliable = true;         // record the reliable-update count in the context
 };
 
 struct MsfResult {
@@ -48,6 +49,11 @@ struct MsfResult {
 };
 
 int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s);
+// the same solve as one persistent cooperative kernel (FP64 storage, two grid
+// barriers per iteration); msfast_solve takes it when msfast_persistent_fits
+// (THR_PERSIST=1)
+bool msfast_persistent_fits(const MsfArgs &A);
+int msfast_persistent(const MsfArgs &A, MsfResult *res, hipStream_t s);
 // relres[src * nshift + i] = |b - (M^+M + sigma_i) x_i| / |b| in FP64
 int msfast_true_residuals(const MsfArgs &A, double *relres, hipStream_t s);
 // FP64 defect correction after msfast_solve: every (source, shift) whose true

@@ -329,53 +329,79 @@ __global__ __launch_bounds__(BLK) void k_true_res(long v5, const double2 *b, con
 
 // Reliable update of the base residual (the replacement of the iterated
 // residual by a true one, Sleijpen and van der Vorst): for each source with
-// rel[src] != 0, r = (b - (M^+M + sigma) x) / zeta from the FP64 true
+// rel[src] != 0, r' = (b - (M^+M + sigma) x) / zeta from the FP64 true
 // residual of one shift (t holds M^+M x, x that shift's solution, zeta its
 // coefficient r_i = zeta r), stored in the Krylov precision P over the
-// residual the next iteration reads.  rel: [nsrc] flags, [nsrc] sigma,
-// [nsrc] 1/zeta.  partial[src * gridDim.x + block] = |r|^2 of the stored values.
+// residual r the next iteration reads.  Every active shift direction holds
+// zeta_j r (p_j = gamma_j p_j + zeta_j r at the last iteration), so it gets
+// zeta_j (r' - r) added (Pv: [nsrc][n][4 V5]); without that the shifted
+// recurrences no longer match the base one and the light shifts stall.
+// rel: [nsrc] flags, [nsrc] sigma, [nsrc] 1/zeta, [nsrc][MAXSHIFT] zeta_j of
+// the active shifts (0 for the others).
+// partial[src * gridDim.x + block] = |r'|^2 of the stored values.
 template <int P>
-__global__ __launch_bounds__(BLK) void k_msf_reliable(long v5, const double2 *b, const double2 *t,
-                                                      const double2 *x, const double *rel,
-                                                      TT<P> *r, float *rn, double *partial) {
+__global__ __launch_bounds__(BLK) void k_msf_reliable(long v5, int n, const double2 *b,
+                                                      const double2 *t, const double2 *x,
+                                                      const double *rel, TT<P> *r, float *rn,
+                                                      double2 *Pv, double *partial) {
   const int src = blockIdx.y, nsrc = gridDim.y;
   const bool on = rel[src] != 0.0;
   const long i = (long)blockIdx.x * BLK + threadIdx.x;
   double red = 0.0;
   if (on && i < v5) {
     const double sig = rel[nsrc + src], iz = rel[2 * nsrc + src];
+    const double *zj = rel + 3 * nsrc + (long)src * MAXSHIFT;
     const long off = (long)src * 4 * v5;
-    cx<double> v[4];
+    cx<double> v[4], d[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
       const long e = off + k * v5 + i;
       v[k] = iz * (ldc(b, e) - ldc(t, e) - sig * ldc(x, e));
     }
+    // new and old residual as stored (the process reads the stored values)
     if constexpr (P == P16) {
+      const float mo = rn[(long)src * v5 + i];
       float m = 0.f;
 #pragma unroll
       for (int k = 0; k < 4; k++) m = fmaxf(m, fmaxf(fabsf((float)v[k].r), fabsf((float)v[k].i)));
       const float inv = m > 0.f ? 1.f / m : 0.f;
-      rn[(long)src * v5 + i] = m;
 #pragma unroll
       for (int k = 0; k < 4; k++) {
+        const float2 fo = __half22float2(r[off + 4 * i + k]);
         const __half2 h = __floats2half2_rn((float)v[k].r * inv, (float)v[k].i * inv);
         r[off + 4 * i + k] = h;
         const float2 f = __half22float2(h);
-        red += norm2(cx<double>{(double)(f.x * m), (double)(f.y * m)});
+        v[k] = {(double)(f.x * m), (double)(f.y * m)};
+        d[k] = v[k] - cx<double>{(double)(fo.x * mo), (double)(fo.y * mo)};
       }
+      rn[(long)src * v5 + i] = m;
     } else if constexpr (P == P32) {
 #pragma unroll
       for (int k = 0; k < 4; k++) {
+        const long e = off + k * v5 + i;
+        const cx<float> fo = ldc(r, e);
         const cx<float> f{(float)v[k].r, (float)v[k].i};
-        st(r, off + k * v5 + i, f);
-        red += norm2(cx<double>{(double)f.r, (double)f.i});
+        st(r, e, f);
+        v[k] = {(double)f.r, (double)f.i};
+        d[k] = v[k] - cx<double>{(double)fo.r, (double)fo.i};
       }
     } else {
 #pragma unroll
       for (int k = 0; k < 4; k++) {
-        st(r, off + k * v5 + i, v[k]);
-        red += norm2(v[k]);
+        const long e = off + k * v5 + i;
+        d[k] = v[k] - ldc(r, e);
+        st(r, e, v[k]);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) red += norm2(v[k]);
+    for (int j = 0; j < n; j++) {
+      const double z = zj[j];
+      if (z == 0.0) continue;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const long e = ((long)src * n + j) * 4 * v5 + k * v5 + i;
+        st(Pv, e, ldc(Pv, e) + z * d[k]);
       }
     }
   }
@@ -501,6 +527,7 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
   THR_CHECK(ns_ >= 1 && ns_ <= MAXRHS, THR_ERR_ARG, "number of sources out of range (1..32)");
   THR_CHECK(A.imass == 1 || A.imass == 3 || A.imass == 5, THR_ERR_ARG, "imass must be 1, 3 or 5");
   THR_CHECK(A.u64 && A.u32 && A.b && A.x, THR_ERR_ARG, "NULL argument");
+  if (msfast_persistent_fits(A)) return msfast_persistent(A, res, s);
   const long v5 = d.v5(), nsp = 4 * v5, nsb = nsp * ns_;
   const int nb = (int)dslash_blocks(d);
   // workspace slots of the Krylov buffers per precision pr: ring slot q,
@@ -601,16 +628,21 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
   int rc = THR_OK;
   int blocks = 0;
   // reliable updates while the Krylov storage is FP32/FP16: each time the
-  // worst iterated residual has fallen by rdelta since the last one, the base
+  // iterated residual of the smallest active shift has fallen by rdelta since
+  // the last one (worst over the sources), the base
   // residual of every running source is replaced by the FP64 true residual of
-  // its smallest active shift divided by that shift's zeta.  The smallest
-  // shift needs the most iterations and is the costliest to correct later;
-  // the heavier shifts keep their drift for the FP64 correction.
+  // its smallest active shift divided by that shift's zeta, and the active
+  // shift directions are moved with it.  The smallest shift needs the most
+  // iterations and is the costliest to correct later; the heavier shifts keep
+  // the drift of the low-precision iterations before they converged (the
+  // residual gap of a recurrence is the sum of its local rounding errors, so
+  // it is set by |r| when the storage precision drops) for the FP64 correction.
   const double rdelta = context_reliable();
-  double r_ref = 1.0;  // worst |r|/|b| at the last reliable update
+  double r_ref = 1.0;  // smallest shift's worst |r_i|/|b| at the last reliable update
   int n_reliable = 0;
   auto reliable_update = [&]() -> int {
-    double *hrel = (double *)host_workspace(6, sizeof(double) * 3 * ns_);
+    const size_t nrel = (size_t)(3 + MAXSHIFT) * ns_;
+    double *hrel = (double *)host_workspace(6, sizeof(double) * nrel);
     int *istar = (int *)host_workspace(7, sizeof(int) * ns_);
     THR_CHECK(hrel && istar, THR_ERR_HIP, "host workspace allocation failed");
     int any = 0;
@@ -626,13 +658,16 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
       hrel[k] = on ? 1.0 : 0.0;
       hrel[ns_ + k] = on ? ak[best] : 0.0;
       hrel[2 * ns_ + k] = on ? 1.0 / z : 0.0;
+      for (int j = 0; j < MAXSHIFT; j++)
+        hrel[3 * ns_ + k * MAXSHIFT + j] =
+            (on && j < n && hs[k].active[j]) ? hs[k].zeta_ii[j] : 0.0;
       any += on ? 1 : 0;
     }
     if (!any) return THR_OK;
     double2 *xg = (double2 *)workspace(380, sizeof(double2) * nsb);
     double2 *t1 = (double2 *)workspace(381, sizeof(double2) * nsb);
     double2 *t2 = (double2 *)workspace(382, sizeof(double2) * nsb);
-    double *drel = (double *)workspace(383, sizeof(double) * 3 * ns_);
+    double *drel = (double *)workspace(383, sizeof(double) * nrel);
     const int nbr = (int)nblk(v5);
     double *rpart = (double *)workspace(384, sizeof(double) * nbr * ns_);
     THR_CHECK(xg && t1 && t2 && drel && rpart, THR_ERR_HIP, "workspace allocation failed");
@@ -640,7 +675,7 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
       if (hrel[k] != 0.0)
         THR_HIP(hipMemcpyAsync(xg + k * nsp, A.x + ((long)k * n + istar[k]) * nsp,
                                sizeof(double2) * nsp, hipMemcpyDeviceToDevice, s));
-    THR_HIP(hipMemcpyAsync(drel, hrel, sizeof(double) * 3 * ns_, hipMemcpyHostToDevice, s));
+    THR_HIP(hipMemcpyAsync(drel, hrel, sizeof(double) * nrel, hipMemcpyHostToDevice, s));
     DslashLaunch a{};
     a.d = d;
     a.nrhs = ns_;
@@ -663,14 +698,14 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
     const long slot = (t - 1) % MSB;
     dim3 g(nbr, ns_);
     if (prec == P32)
-      hipLaunchKernelGGL(k_msf_reliable<P32>, g, dim3(BLK), 0, s, v5, A.b, t2, xg, drel,
-                         (float2 *)ring[P32][slot], nullptr, rpart);
+      hipLaunchKernelGGL(k_msf_reliable<P32>, g, dim3(BLK), 0, s, v5, n, A.b, t2, xg, drel,
+                         (float2 *)ring[P32][slot], nullptr, Pv, rpart);
     else if (prec == P16)
-      hipLaunchKernelGGL(k_msf_reliable<P16>, g, dim3(BLK), 0, s, v5, A.b, t2, xg, drel,
-                         (__half2 *)ring[P16][slot], ringn[slot], rpart);
+      hipLaunchKernelGGL(k_msf_reliable<P16>, g, dim3(BLK), 0, s, v5, n, A.b, t2, xg, drel,
+                         (__half2 *)ring[P16][slot], ringn[slot], Pv, rpart);
     else
-      hipLaunchKernelGGL(k_msf_reliable<P64>, g, dim3(BLK), 0, s, v5, A.b, t2, xg, drel,
-                         (double2 *)ring[P64][slot], nullptr, rpart);
+      hipLaunchKernelGGL(k_msf_reliable<P64>, g, dim3(BLK), 0, s, v5, n, A.b, t2, xg, drel,
+                         (double2 *)ring[P64][slot], nullptr, Pv, rpart);
     hipLaunchKernelGGL(k_msf_reliable_fin, dim3(ns_), dim3(BLK), 0, s, st, drel, rpart, nbr);
     THR_HIP(hipGetLastError());
     n_reliable++;
@@ -801,10 +836,18 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
       return THR_ERR_HIP;
     }
     bool all = true;
-    double worst = 0.0;
+    double worst = 0.0;  // base |r|/|b|
+    double worst_lo = 0.0;  // |r_i|/|b| of the smallest active shift
     for (int k = 0; k < ns_; k++) {
       all = all && hs[k].stop;
-      if (!hs[k].stop) worst = std::max(worst, std::sqrt(hs[k].lambda / hs[k].bnorm2));
+      if (hs[k].stop) continue;
+      const double rb = std::sqrt(hs[k].lambda / hs[k].bnorm2);
+      worst = std::max(worst, rb);
+      const double *ak = A.aden_src ? A.aden_src + (long)k * n : A.aden;
+      int lo = -1;
+      for (int i = 0; i < n; i++)
+        if (hs[k].active[i] && (lo < 0 || ak[i] < ak[lo])) lo = i;
+      if (lo >= 0) worst_lo = std::max(worst_lo, std::fabs(hs[k].zeta_ii[lo]) * rb);
     }
     if (all) break;
     if (t > (long)A.maxcg + 2 * MSB) {
@@ -812,15 +855,19 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
       set_error("multishift: stop flag not raised");
       return THR_ERR_STATE;
     }
-    if (rdelta > 0 && prec != P64 && worst < rdelta * r_ref) {
+    // the trigger follows the smallest shift's own residual: the base
+    // residual of a light-mass operator may hardly fall while the shifted
+    // ones converge through zeta, and the drift of shift i accumulates with
+    // its own residual history
+    if (rdelta > 0 && prec != P64 && worst_lo < rdelta * r_ref) {
       rc = reliable_update();
       if (rc) {
         free_graphs();
         return rc;
       }
-      r_ref = worst;
+      r_ref = worst_lo;
     }
-    if (prec == P64) r_ref = worst;  // drift starts when the storage precision drops
+    if (prec == P64) r_ref = worst_lo;  // drift starts when the storage precision drops
     // lower the Krylov precision as the residual falls
     int want = prec;
     if (A.th32 > 0 && worst < A.th32 && prec == P64) want = P32;
@@ -842,7 +889,7 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
     }
   }
   free_graphs();
-  context_note_reliable(n_reliable);
+  if (A.note_reliable) context_note_reliable(n_reliable);
   if (res) {
     res->reliable = n_reliable;
     res->iterations = (int)t;
@@ -991,6 +1038,7 @@ int msfast_correct(const MsfArgs &A, int max_rounds, double tol_true, double *re
       B.th32 = 1.0;
       B.th16 = 0.0;
       B.start_prec = P32;
+      B.note_reliable = false;
       MsfResult R{};
       TRY_RC(msfast_solve(B, &R, s));
       THR_CHECK(!R.nonfinite, THR_ERR_NONFINITE,
