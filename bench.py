@@ -497,8 +497,8 @@ def bench_serving(rates, seconds, slots):
     return S.rate_sweep(rates, seconds, seed=3, slots=slots, progress=progress)
 
 
-KERNEL_SOURCES = ("thirring-rhmc_amd/csrc/dslash.hip", "thirring-rhmc_amd/csrc/kernels.h",
-                  "thirring-rhmc_amd/csrc/common.h")
+KERNEL_SOURCES = ("thirring-rhmc_amd/csrc/dslash.hip", "thirring-rhmc_amd/csrc/dslash_site.h",
+                  "thirring-rhmc_amd/csrc/kernels.h", "thirring-rhmc_amd/csrc/common.h")
 
 
 def kernel_source_digest():

@@ -63,8 +63,9 @@ extern "C" {
 #define THR_F64 0
 #define THR_F32 1
 #define THR_F16 2 /* half2 per complex + one float scale per 5-D site; the 4 spins
-                     of a site are stored together: [rhs][site][spin] (F64 and F32
-                     store spin planes: [rhs][spin][site]) */
+                     of a site are stored together: [rhs][site][spin].  F32 stores
+                     the spins of a site together too ([rhs][site][spin], 32 bytes
+                     per site); F64 stores spin planes: [rhs][spin][site] */
 
 /* ---- lifetime ------------------------------------------------------------ */
 /* select the device and create the library stream; replaces init_MPI and

@@ -611,34 +611,55 @@ void orc_dirac_op_shifted(double *xout_, const double *xin_, const double *u_, c
 /* ------------------------------------------------------------------------ */
 /* reductions over the interior                                              */
 /* ------------------------------------------------------------------------ */
+/* Each (spin, t) slice is summed by one thread into its own slot and the
+ * slots are added in index order: the result does not depend on the thread
+ * count or on the order in which threads finish (an OpenMP reduction clause
+ * combines the per-thread sums in arrival order, so solver iteration counts
+ * could move by a few between runs). */
+static double sum_slots(const double *part, int n) {
+  double acc = 0.0;
+  for (int i = 0; i < n; i++) acc += part[i];
+  return acc;
+}
+
 static double norm2_int(const zdp *A, const geom *gp) {
   const geom g = *gp;
-  double acc = 0.0;
-#pragma omp parallel for collapse(2) reduction(+ : acc) schedule(static)
+  double *part = malloc(sizeof(double) * 4 * g.lt);
+#pragma omp parallel for collapse(2) schedule(static)
   for (int a = 0; a < 4; a++)
-    for (int t = 1; t <= g.lt; t++)
+    for (int t = 1; t <= g.lt; t++) {
+      double acc = 0.0;
       for (int y = 1; y <= g.ly; y++)
         for (int x = 1; x <= g.lx; x++)
           for (int s = 1; s <= g.ls; s++) {
             zdp v = A[I5(g, s, x, y, t, a)];
             acc += creal(v) * creal(v) + cimag(v) * cimag(v);
           }
-  return acc;
+      part[a * g.lt + t - 1] = acc;
+    }
+  const double r = sum_slots(part, 4 * g.lt);
+  free(part);
+  return r;
 }
 
 static double redot_int(const zdp *A, const zdp *B, const geom *gp) {
   const geom g = *gp;
-  double acc = 0.0;
-#pragma omp parallel for collapse(2) reduction(+ : acc) schedule(static)
+  double *part = malloc(sizeof(double) * 4 * g.lt);
+#pragma omp parallel for collapse(2) schedule(static)
   for (int a = 0; a < 4; a++)
-    for (int t = 1; t <= g.lt; t++)
+    for (int t = 1; t <= g.lt; t++) {
+      double acc = 0.0;
       for (int y = 1; y <= g.ly; y++)
         for (int x = 1; x <= g.lx; x++)
           for (int s = 1; s <= g.ls; s++) {
             long o = I5(g, s, x, y, t, a);
             acc += creal(conj(A[o]) * B[o]);
           }
-  return acc;
+      part[a * g.lt + t - 1] = acc;
+    }
+  const double r = sum_slots(part, 4 * g.lt);
+  free(part);
+  return r;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -801,10 +822,11 @@ void orc_multishift(const double *u_, const int dims[4], float am, int imass, fl
 static double reduce_sp_norm2(const zsp *A, const geom *gp) {
   /* reduce_real_dp5d(abs(r)**2): float32 terms summed in float64 (reductions.F90:6-32) */
   const geom g = *gp;
-  double acc = 0.0;
-#pragma omp parallel for collapse(2) reduction(+ : acc) schedule(static)
+  double *part = malloc(sizeof(double) * 4 * g.lt);
+#pragma omp parallel for collapse(2) schedule(static)
   for (int a = 0; a < 4; a++)
-    for (int t = 1; t <= g.lt; t++)
+    for (int t = 1; t <= g.lt; t++) {
+      double acc = 0.0;
       for (int y = 1; y <= g.ly; y++)
         for (int x = 1; x <= g.lx; x++)
           for (int s = 1; s <= g.ls; s++) {
@@ -812,15 +834,20 @@ static double reduce_sp_norm2(const zsp *A, const geom *gp) {
             float term = crealf(v) * crealf(v) + cimagf(v) * cimagf(v);
             acc += (double)term;
           }
-  return acc;
+      part[a * g.lt + t - 1] = acc;
+    }
+  const double r = sum_slots(part, 4 * g.lt);
+  free(part);
+  return r;
 }
 
 static double reduce_sp_redot(const zsp *A, const zsp *B, const geom *gp) {
   const geom g = *gp;
-  double acc = 0.0;
-#pragma omp parallel for collapse(2) reduction(+ : acc) schedule(static)
+  double *part = malloc(sizeof(double) * 4 * g.lt);
+#pragma omp parallel for collapse(2) schedule(static)
   for (int a = 0; a < 4; a++)
-    for (int t = 1; t <= g.lt; t++)
+    for (int t = 1; t <= g.lt; t++) {
+      double acc = 0.0;
       for (int y = 1; y <= g.ly; y++)
         for (int x = 1; x <= g.lx; x++)
           for (int s = 1; s <= g.ls; s++) {
@@ -828,7 +855,11 @@ static double reduce_sp_redot(const zsp *A, const zsp *B, const geom *gp) {
             float term = crealf(conjf(A[o]) * B[o]);
             acc += (double)term;
           }
-  return acc;
+      part[a * g.lt + t - 1] = acc;
+    }
+  const double r = sum_slots(part, 4 * g.lt);
+  free(part);
+  return r;
 }
 
 void orc_multishift_sp(const double *udp_, const int dims[4], float am, int imass, float am3,

@@ -35,6 +35,16 @@ def check(a, b, tol):
         raise AssertionError(f"relative L2 error {e:.3e} > {tol:.1e}")
 
 
+def site_major(a):
+    """(4 V5, 2) spin planes -> (V5, 4, 2): the FP32 device layout"""
+    return np.ascontiguousarray(a.reshape(4, -1, 2).transpose(1, 0, 2))
+
+
+def spin_planes(a):
+    """(V5, 4, 2) or (4 V5, 2) site-major -> (4 V5, 2) spin planes"""
+    return np.ascontiguousarray(a.reshape(-1, 4, 2).transpose(1, 0, 2).reshape(-1, 2))
+
+
 DIMS = [(4, 4, 4, 8), (6, 4, 8, 4), (3, 5, 2, 12)]
 
 
@@ -247,10 +257,11 @@ def test_device_precisions(G):
     r = D.from_vec(xh[:, 0] + 1j * xh[:, 1], dims)
     o = O.interior(O.dslash(r, u, dims, 0.05, 3)).reshape(-1, order="F")
     check(rc[:, 0] + 1j * rc[:, 1], o, 1e-13)
-    x32 = DeviceArray.from_host(xh.astype(np.float32))
-    out32 = DeviceArray((4 * v5, 2), np.float32)
+    # fp32: the spins of a site together [site][spin]
+    x32 = DeviceArray.from_host(site_major(xh).astype(np.float32))
+    out32 = DeviceArray((v5, 4, 2), np.float32)
     G.dslash_dev(out32, x32, l32, dims, 0.05, imass=3, prec="f32")
-    check(out32.to_host().astype(np.float64), rc, 2e-6)
+    check(spin_planes(out32.to_host()).astype(np.float64), rc, 2e-6)
     # fp16 + per-site scale, spins of a site together [site][spin]: quantise
     # the source the way the kernel stores it
     xs = xh.astype(np.float32).reshape(4, v5, 2).transpose(1, 0, 2)
@@ -299,7 +310,10 @@ def test_dslash_launch_shapes(G, dims):
                 of = O.dslashd if dagger else O.dslash
                 o = O.interior(of(r, u, dims, 0.05, imass)).reshape(-1, order="F")
                 for prec, links, dt in (("f64", l64, np.float64), ("f32", l32, np.float32)):
-                    x = DeviceArray.from_host(xh.astype(dt))
+                    # fp32 fields keep a site's spins together
+                    lay = site_major if prec == "f32" else (lambda a: a)
+                    unlay = spin_planes if prec == "f32" else (lambda a: a)
+                    x = DeviceArray.from_host(np.ascontiguousarray(lay(xh).astype(dt)))
                     first = None
                     for tc in sorted({1, 2, 3, 4, lt}):
                         # every flag set, and occupancy caps (LDS padding) on three of them
@@ -309,7 +323,7 @@ def test_dslash_launch_shapes(G, dims):
                             out = DeviceArray((4 * v5, 2), dt)
                             G.dslash_dev(out, x, links, dims, 0.05, imass=imass, dagger=dagger,
                                          prec=prec)
-                            h = out.to_host()
+                            h = unlay(out.to_host())
                             if first is None:
                                 first = h
                                 check(h[:, 0] + 1j * h[:, 1], o,

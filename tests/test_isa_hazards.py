@@ -111,3 +111,27 @@ def test_f16_dslash_reads_sites_whole(tmp_path):
     assert f16, "no FP16 Dslash kernels found"
     short = {k: v for k, v in f16.items() if v < 7}
     assert not short, list(short.items())[:5]
+
+
+def test_f32_dslash_reads_sites_whole(tmp_path):
+    """FP32 spinors keep the 4 spins of a site in 32 bytes: every FP32 Dslash
+    kernel reads the centre, the two t neighbours and the four x/y neighbours
+    with two 128-bit loads each (14 at least), and stores its output sites
+    with 128-bit stores."""
+    label = re.compile(r"^[0-9a-f]+ <(.*)>:$")
+    loads, stores, cur = {}, {}, None
+    for text in _disassemble(tmp_path):
+        for line in text.splitlines():
+            m = label.match(line)
+            if m:
+                cur = m.group(1)
+                loads[cur] = stores[cur] = 0
+            elif cur is not None and "buffer_load_dwordx4" in line:
+                loads[cur] += 1
+            elif cur is not None and "buffer_store_dwordx4" in line:
+                stores[cur] += 1
+    f32 = [k for k in loads if "8k_dslashILi1E" in k]
+    assert f32, "no FP32 Dslash kernels found"
+    short = [(k, loads[k]) for k in f32 if loads[k] < 14]
+    assert not short, short[:5]
+    assert all(stores[k] >= 2 for k in f32), [(k, stores[k]) for k in f32][:5]

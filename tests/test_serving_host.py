@@ -115,12 +115,13 @@ def test_probe_has_no_side_effects(serving):
 
 
 def test_footprint_counts_solver_scratch(serving):
-    """60 FP64 spinors per source (FP64 and FP32 Krylov storage, the
-    correction batches, source, solution) plus the link fields."""
+    """63 FP64 spinors per source (FP64 and FP32 Krylov storage, the
+    correction and reliable-update batches, source, solution) plus the link
+    fields."""
     S = serving
     r = S.Request(0, 8, 16, 8, 0.3, 0.05, 3, 0)
     spinor = 4 * 8 * 8 * 16 * 8 * 16
-    assert r.footprint() == 3 * 8 * 8 * 16 * 28 + 3 * 60 * spinor + (1 << 20)
+    assert r.footprint() == 3 * 8 * 8 * 16 * 28 + 3 * 63 * spinor + (1 << 20)
 
 
 def test_rate_sweep_sheds_overload(serving):

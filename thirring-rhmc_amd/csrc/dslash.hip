@@ -29,9 +29,12 @@ namespace thr {
 
 namespace {
 
-// Spinor layouts: FP64 and FP32 store spin planes [spin][site]; FP16 stores
-// the 4 spins of a site together [site][spin] (16 bytes, one 128-bit access)
-// with a per-site float scale.
+// Spinor layouts: FP64 stores spin planes [spin][site]; FP32 and FP16 store
+// the 4 spins of a site together [site][spin] (FP32: 32 bytes, two 128-bit
+// accesses; FP16: 16 bytes, one, with a per-site float scale).  A site's FP32
+// spins as spin planes would take four 64-bit accesses: twice the address
+// work per byte of the FP64 planes, which kept the FP32 kernel bound by the
+// texture-address units.
 template <int PREC> struct Acc {
   using C = typename Store<PREC>::C;
   using T = typename Store<PREC>::T;
@@ -48,6 +51,13 @@ template <int PREC> struct Acc {
         const float2 f = __half22float2(__builtin_bit_cast(__half2, w[a]));
         v[a] = {f.x * sc, f.y * sc};
       }
+    } else if constexpr (PREC == P32) {
+      const float4 q0 = *reinterpret_cast<const float4 *>(p + 4 * (long)site);
+      const float4 q1 = *reinterpret_cast<const float4 *>(p + 4 * (long)site + 2);
+      v[0] = {q0.x, q0.y};
+      v[1] = {q0.z, q0.w};
+      v[2] = {q1.x, q1.y};
+      v[3] = {q1.z, q1.w};
     } else {
 #pragma unroll
       for (int a = 0; a < 4; a++) v[a] = ldc(p, a * v5 + site);
@@ -56,6 +66,8 @@ template <int PREC> struct Acc {
   __device__ inline cx<C> load1(unsigned site, unsigned a) const {
     if constexpr (PREC == P16) {
       return ldh(p, 4 * (long)site + a, nrm[site]);
+    } else if constexpr (PREC == P32) {
+      return ldc(p, 4 * (long)site + a);
     } else {
       return ldc(p, a * v5 + site);
     }
@@ -77,6 +89,9 @@ template <int PREC> struct Out {
       nrm[site] = m;
 #pragma unroll
       for (int a = 0; a < 4; a++) p[4 * (long)site + a] = __floats2half2_rn(v[a].r * inv, v[a].i * inv);
+    } else if constexpr (PREC == P32) {
+#pragma unroll
+      for (int a = 0; a < 4; a++) st(p, 4 * (long)site + a, v[a]);
     } else {
 #pragma unroll
       for (int a = 0; a < 4; a++) st(p, a * v5 + site, v[a]);
@@ -87,7 +102,8 @@ template <int PREC> struct Out {
 // Output stores through buffer resources; NT marks them non-temporal (the
 // output is not read again by this kernel, so it should not displace the
 // input lines that neighbouring waves still need from L2).
-// One resource per spin plane, stored with soffset 0: a store of more than
+// FP64: one resource per spin plane, stored with soffset 0 (FP32 and FP16:
+// one resource over the site-major field, soffset 0): a store of more than
 // 64 bits reads its data VGPRs after issue, and the compiler's wait-state
 // check for a following VALU write to those VGPRs skips stores whose soffset
 // is an SGPR (the data can then be overwritten before it is read: measured as
@@ -97,7 +113,7 @@ template <int PREC, bool NT> struct BOut {
   __amdgpu_buffer_rsrc_t rs[4], rn;
   __device__ inline BOut(void *p, float *nrm, unsigned v5) {
     const unsigned plane = v5 * (unsigned)Store<PREC>::bytes;
-    if constexpr (PREC == P16) {  // [site][spin]: one resource over the field
+    if constexpr (PREC != P64) {  // [site][spin]: one resource over the field
       rs[0] = __builtin_amdgcn_make_buffer_rsrc(p, (short)0, (int)(4u * plane), 0x00020000);
       rs[1] = rs[2] = rs[3] = rs[0];
     } else {
@@ -117,12 +133,15 @@ template <int PREC, bool NT> struct BOut {
         __builtin_amdgcn_raw_buffer_store_b128(
             __builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned, make_double2(v[a].r, v[a].i)),
             rs[a], off, 0, aux);
-    } else if constexpr (PREC == P32) {
+    } else if constexpr (PREC == P32) {  // [site][spin]: two 128-bit stores
+      const unsigned o32 = site * 32u;
 #pragma unroll
-      for (int a = 0; a < 4; a++)
-        __builtin_amdgcn_raw_buffer_store_b64(
-            __builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, make_float2(v[a].r, v[a].i)),
-            rs[a], off, 0, aux);
+      for (int h = 0; h < 2; h++) {
+        const float4 q = make_float4(v[2 * h].r, v[2 * h].i, v[2 * h + 1].r, v[2 * h + 1].i);
+        __builtin_amdgcn_raw_buffer_store_b128(
+            __builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned, q), rs[0],
+            o32 + 16u * h, 0, aux);
+      }
     } else {
       float m = 0.f;
 #pragma unroll
@@ -161,9 +180,9 @@ template <int PREC> struct BAcc {
       const double2 v =
           __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(rs, off, a * plane, 0));
       return {v.x, v.y};
-    } else if constexpr (PREC == P32) {
-      const float2 v =
-          __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rs, off, a * plane, 0));
+    } else if constexpr (PREC == P32) {  // [site][spin]
+      const float2 v = __builtin_bit_cast(
+          float2, __builtin_amdgcn_raw_buffer_load_b64(rs, site * 32u + a * 8u, 0, 0));
       return {v.x, v.y};
     } else {  // [site][spin]
       const __half2 h =
@@ -180,7 +199,16 @@ template <int PREC> struct BAcc {
   }
   __device__ inline void load4(unsigned site, cx<C> v[4]) const {
     const float sc = scale(site);
-    if constexpr (PREC == P16) {  // the site's 4 spins in one 128-bit load
+    if constexpr (PREC == P32) {  // the site's 4 spins in two 128-bit loads
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const auto q = __builtin_amdgcn_raw_buffer_load_b128(rs, site * 32u + 16u * h, 0, 0);
+        // the elements as values first (see the FP16 branch)
+        const unsigned e0 = q[0], e1 = q[1], e2 = q[2], e3 = q[3];
+        v[2 * h] = {__uint_as_float(e0), __uint_as_float(e1)};
+        v[2 * h + 1] = {__uint_as_float(e2), __uint_as_float(e3)};
+      }
+    } else if constexpr (PREC == P16) {  // the site's 4 spins in one 128-bit load
       const auto q = __builtin_amdgcn_raw_buffer_load_b128(rs, site * 16u, 0, 0);
 #pragma unroll
       for (int a = 0; a < 4; a++) {

@@ -47,19 +47,20 @@ __device__ inline double fin_sum(const double *partial, int n) {
   return block_sum(v);
 }
 
-// element e = spin * V5 + site of one source's field (FP16 fields store the
-// spins of a site together: [site][spin])
+// one element of a source's field: FP64 stores spin planes (element
+// e = spin * V5 + site), FP32 and FP16 the spins of a site together
+// ([site][spin], FP16 with a per-site scale)
 template <int P> struct VAcc {
   const TT<P> *p;
   const float *nrm;
   long v5;
-  // element e = spin * V5 + site (FP64/FP32 spin planes); FP16 stores the
-  // spins of a site together and is addressed by site and spin
   __device__ inline cx<CC<P>> ld_at(long e, long site, int spin) const {
     if constexpr (P == P16) {
       const float2 f = __half22float2(p[4 * site + spin]);
       const float sc = nrm[site];
       return {f.x * sc, f.y * sc};
+    } else if constexpr (P == P32) {
+      return ldc(p, 4 * site + spin);
     } else {
       return ldc(p, e);
     }
@@ -251,12 +252,13 @@ __global__ __launch_bounds__(BLK) void k_msf_flush(long v5, int n, TT<P> *const 
   const long ns = 4 * v5;
   const long tau = (long)blockIdx.x * BLK + threadIdx.x;
   if (tau >= ns) return;
-  // element e = spin * V5 + site of the FP64 x_i, p_i; FP16 residuals keep a
-  // site's 4 spins together, so there consecutive threads take the 4 spins of
-  // one site (the ring reads stay contiguous, the FP64 planes 16 lanes wide)
+  // element e = spin * V5 + site of the FP64 x_i, p_i; FP32 and FP16
+  // residuals keep a site's 4 spins together, so there consecutive threads
+  // take the 4 spins of one site (the ring reads stay contiguous, the FP64
+  // planes 16 lanes wide)
   const long site = tau >> 2;
   const int spin = (int)(tau & 3);
-  const long e = P == P16 ? spin * v5 + site : tau;
+  const long e = P != P64 ? spin * v5 + site : tau;
   const long off = (long)src * ns;
   cx<double> R[MSB];
 #pragma unroll
@@ -378,7 +380,7 @@ __global__ __launch_bounds__(BLK) void k_msf_reliable(long v5, int n, const doub
     } else if constexpr (P == P32) {
 #pragma unroll
       for (int k = 0; k < 4; k++) {
-        const long e = off + k * v5 + i;
+        const long e = off + 4 * i + k;
         const cx<float> fo = ldc(r, e);
         const cx<float> f{(float)v[k].r, (float)v[k].i};
         st(r, e, f);
@@ -446,7 +448,7 @@ __global__ __launch_bounds__(BLK) void k_conv(long v5, int nsrc, const TT<SP> *a
       const float s = an[(long)src * v5 + i];
       vd[k] = {(double)(f.x * s), (double)(f.y * s)};
     } else {
-      const auto v = ldc(a, off + k * v5 + i);
+      const auto v = ldc(a, off + (SP == P32 ? 4 * i + k : k * v5 + i));
       vd[k] = {(double)v.r, (double)v.i};
     }
     vf[k] = {(float)vd[k].r, (float)vd[k].i};
@@ -461,7 +463,7 @@ __global__ __launch_bounds__(BLK) void k_conv(long v5, int nsrc, const TT<SP> *a
     for (int k = 0; k < 4; k++) b[off + 4 * i + k] = __floats2half2_rn(vf[k].r * inv, vf[k].i * inv);
   } else if constexpr (DP == P32) {
 #pragma unroll
-    for (int k = 0; k < 4; k++) st(b, off + k * v5 + i, vf[k]);
+    for (int k = 0; k < 4; k++) st(b, off + 4 * i + k, vf[k]);
   } else {
 #pragma unroll
     for (int k = 0; k < 4; k++) st(b, off + k * v5 + i, vd[k]);

@@ -452,13 +452,10 @@ __global__ __launch_bounds__(BLK) void k_bcast_copy(long n, int nshift, const TT
   for (int k = 0; k < nshift; k++) dst[(long)k * n + i] = v;
 }
 
+// element-wise FP64 -> FP32 complex conversion (link fields)
 __global__ __launch_bounds__(BLK) void k_d2f(long n, const double2 *a, float2 *b) {
   const long i = (long)blockIdx.x * BLK + threadIdx.x;
   if (i < n) b[i] = make_float2((float)a[i].x, (float)a[i].y);
-}
-__global__ __launch_bounds__(BLK) void k_f2d(long n, const float2 *a, double2 *b) {
-  const long i = (long)blockIdx.x * BLK + threadIdx.x;
-  if (i < n) b[i] = make_double2((double)a[i].x, (double)a[i].y);
 }
 
 inline unsigned nblk(long n) { return (unsigned)((n + BLK - 1) / BLK); }
@@ -760,7 +757,8 @@ int multishift_reference_sp(const OpDesc &op, int n, const double *aden, const d
   if (rc) return rc;
   // u = cmplx(udp), input = cmplx(inputdp) (multishift_module.F90:351-352)
   hipLaunchKernelGGL(k_d2f, dim3(nblk(d.v4() * 3)), dim3(BLK), 0, s, d.v4() * 3, op.u64, u32);
-  hipLaunchKernelGGL(k_d2f, dim3(nblk(ns)), dim3(BLK), 0, s, ns, input, r);
+  // FP32 spinors keep a site's spins together: the conversion reorders too
+  TRY_RC(convert_spinors(d, 1, input, nullptr, P64, r, nullptr, P32, s));
   THR_HIP(hipMemcpyAsync(p, r, sizeof(float2) * ns, hipMemcpyDeviceToDevice, s));
   THR_HIP(hipMemsetAsync(xs, 0, sizeof(float2) * ns * n, s));
   hipLaunchKernelGGL(k_bcast_copy<P32>, dim3(nblk(ns)), dim3(BLK), 0, s, ns, n, r, ps);
@@ -806,7 +804,7 @@ int multishift_reference_sp(const OpDesc &op, int n, const double *aden, const d
   };
   rc = run_batched(iter, stop, s, maxcg);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_f2d, dim3(nblk(ns * n)), dim3(BLK), 0, s, ns * n, xs, out);
+  TRY_RC(convert_spinors(d, n, xs, nullptr, P32, out, nullptr, P64, s));
   MsState hs;
   THR_HIP(hipMemcpyAsync(&hs, st, sizeof(MsState), hipMemcpyDeviceToHost, s));
   THR_HIP(hipStreamSynchronize(s));
