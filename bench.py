@@ -247,9 +247,11 @@ def _plain_leg(G, hr, dims, l64, am, bh, sig, n, nsp, tol, maxcg, stream):
 
 def bench_derivs(G, hr, stream, steps=50, warmup=5):
     """The Shamir fermion force derivs (derivs_module.F90:99-196) on device
-    fields at 16^3 x Ls=48 and 32^3 x Ls=64: mean kernel time from HIP
-    events around each launch on the stream, and GB/s of algorithmic bytes
-    (R and X2 read once, dSdpi read and written once: 2 V5 64 + 2 3 V4 8)."""
+    fields at 16^3 x Ls=48 and 32^3 x Ls=64: time per launch from HIP events
+    around `steps` launches issued back to back on the stream (and the mean
+    of events around each launch, which adds the launch gaps), and GB/s of
+    algorithmic bytes (R and X2 read once, dSdpi read and written once:
+    2 V5 64 + 2 3 V4 8)."""
     out = {}
     rng = np.random.default_rng(11)
     for dims in ((16, 16, 16, 48), (32, 32, 32, 64)):
@@ -270,11 +272,19 @@ def bench_derivs(G, hr, stream, steps=50, warmup=5):
             G.derivs_dev(rs[k % nb], xs[k % nb], 0.37, k & 1, dims, ds, stream=stream)
             ev[k][1].record(stream)
         stream.synchronize()
-        ms = float(np.mean([a.elapsed_ms(b) for a, b in ev]))
+        per_launch = float(np.mean([a.elapsed_ms(b) for a, b in ev]))
+        e0, e1 = hr.Event(), hr.Event()
+        e0.record(stream)
+        for k in range(steps):
+            G.derivs_dev(rs[k % nb], xs[k % nb], 0.37, k & 1, dims, ds, stream=stream)
+        e1.record(stream)
+        stream.synchronize()
+        ms = e0.elapsed_ms(e1) / steps
         nbytes = 2 * v5 * 64 + 2 * 3 * v4 * 8
         gbs = nbytes / (ms * 1e-3) / 1e9
         out["x".join(map(str, dims))] = {"ms": round(ms, 5), "GB_s": round(gbs, 1),
                                          "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4),
+                                         "per_launch_event_ms": round(per_launch, 5),
                                          "algorithmic_bytes": nbytes}
         for a in rs + xs + [ds]:
             a.free()
