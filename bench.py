@@ -251,7 +251,8 @@ def bench_derivs(G, hr, stream, steps=50, warmup=5):
     around `steps` launches issued back to back on the stream (and the mean
     of events around each launch, which adds the launch gaps), and GB/s of
     algorithmic bytes (R and X2 read once, dSdpi read and written once:
-    2 V5 64 + 2 3 V4 8)."""
+    2 V5 64 + 2 3 V4 8); and the force's use of it, the derivs of 25 poles
+    in one launch (thr_derivs_batch_dev: 25 2 V5 64 + 2 3 V4 8 bytes)."""
     out = {}
     rng = np.random.default_rng(11)
     for dims in ((16, 16, 16, 48), (32, 32, 32, 64)):
@@ -282,11 +283,38 @@ def bench_derivs(G, hr, stream, steps=50, warmup=5):
         ms = e0.elapsed_ms(e1) / steps
         nbytes = 2 * v5 * 64 + 2 * 3 * v4 * 8
         gbs = nbytes / (ms * 1e-3) / 1e9
-        out["x".join(map(str, dims))] = {"ms": round(ms, 5), "GB_s": round(gbs, 1),
-                                         "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4),
-                                         "per_launch_event_ms": round(per_launch, 5),
-                                         "algorithmic_bytes": nbytes}
-        for a in rs + xs + [ds]:
+        row = {"ms": round(ms, 5), "GB_s": round(gbs, 1),
+               "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4),
+               "per_launch_event_ms": round(per_launch, 5), "algorithmic_bytes": nbytes}
+        for a in rs + xs:
+            a.free()
+        # the force's use: the derivs of 25 poles (R_k, X2_k) in one launch
+        npair = 25
+        pr, px = [], []
+        for k in range(npair):
+            for lst, seed in ((pr, 1000 + k), (px, 2000 + k)):
+                a = hr.DeviceArray((4 * v5, 2), np.float64)
+                G.gaussian_dev(a, 8 * v5, 1.0, seed, stream=stream)
+                lst.append(a)
+        an = 0.1 + rng.random(npair)
+        fl = [0] * npair
+        G.derivs_batch_dev(pr, px, an, fl, dims, ds, stream=stream)
+        stream.synchronize()
+        e0, e1 = hr.Event(), hr.Event()
+        nrep = 10
+        e0.record(stream)
+        for k in range(nrep):
+            G.derivs_batch_dev(pr, px, an, fl, dims, ds, stream=stream)
+        e1.record(stream)
+        stream.synchronize()
+        bms = e0.elapsed_ms(e1) / nrep
+        bbytes = npair * 2 * v5 * 64 + 2 * 3 * v4 * 8
+        bgbs = bbytes / (bms * 1e-3) / 1e9
+        row["batched_25_poles"] = {"ms": round(bms, 4), "GB_s": round(bgbs, 1),
+                                   "frac_of_peak": round(bgbs / HBM_PEAK_GBS, 4),
+                                   "algorithmic_bytes": bbytes}
+        out["x".join(map(str, dims))] = row
+        for a in pr + px + [ds]:
             a.free()
     return out
 

@@ -334,6 +334,16 @@ int thr_congrad_dev_noalloc(const void *phi, void *x, const void *links64, const
 int thr_derivs_dev(const void *r, const void *x2, double anum, int iflag, const int dims[4],
                    float am, int imass, int kernel, void *dsdpi, void *stream);
 
+/* dSdpi += the Shamir derivs of npair pairs (r[k], x2[k], anum[k], iflag[k])
+ * in one pass (Ls <= 64); r and x2 are host arrays of device spinor
+ * pointers.  No counterpart in the reference, which calls derivs once per
+ * pole of the rational approximation (qmrherm_module.F90:143-181); the sum
+ * equals those calls' up to the order of the additions.  qmrherm iflag 2/3
+ * and force use it for the Shamir kernel. */
+int thr_derivs_batch_dev(const void *const *r, const void *const *x2, const double *anum,
+                         const int *iflag, int npair, const int dims[4], void *dsdpi,
+                         void *stream);
+
 /* reference-faithful multishift on device fields; out [ndiagq][4*V5] complex128 */
 int thr_multishift_dev(const void *links64, const int dims[4], float am, int imass, float am3,
                        int ndiagq, const double *aden, const double *anum, void *out,

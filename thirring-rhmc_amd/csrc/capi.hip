@@ -833,6 +833,29 @@ int thr_derivs_dev(const void *r, const void *x2, double anum, int iflag, const 
                                                  (double *)dsdpi, s));
 }
 
+int thr_derivs_batch_dev(const void *const *r, const void *const *x2, const double *anum,
+                         const int *iflag, int npair, const int dims[4], void *dsdpi,
+                         void *stream) {
+  TRY(check_dims(dims));
+  THR_CHECK(r && x2 && anum && iflag && dsdpi, THR_ERR_ARG, "NULL argument");
+  THR_CHECK(npair >= 1 && npair <= 1024, THR_ERR_ARG, "npair out of range (1..1024)");
+  const Dims d = mkdims(dims);
+  THR_CHECK(d.ls <= 64, THR_ERR_ARG, "batched derivs needs Ls <= 64");
+  std::vector<DerivPair> ph(npair);
+  for (int k = 0; k < npair; k++) {
+    THR_CHECK(r[k] && x2[k], THR_ERR_ARG, "NULL field pointer");
+    THR_CHECK(iflag[k] == 0 || iflag[k] == 1, THR_ERR_ARG, "iflag must be 0 or 1");
+    ph[k] = derivs_pair((const double2 *)r[k], (const double2 *)x2[k], anum[k], iflag[k]);
+  }
+  DerivPair *pd = (DerivPair *)workspace(456, sizeof(DerivPair) * npair);
+  double *part = (double *)workspace(457, sizeof(double) * 3 * d.v4() * npair);
+  THR_CHECK(pd && part, THR_ERR_HIP, "workspace allocation failed");
+  hipStream_t s = pick(stream);
+  // launch_derivs_batch waits for the pair copy, so ph may go out of scope after it
+  return done_if_default(stream,
+                         launch_derivs_batch(d, npair, ph.data(), pd, part, (double *)dsdpi, s));
+}
+
 int thr_multishift_dev(const void *links64, const int dims[4], float am, int imass, float am3,
                        int ndiagq, const double *aden, const double *anum, void *out,
                        const void *input, float res, int maxcg, int use_sp, int kernel,

@@ -107,6 +107,18 @@ int convert_spinors(const Dims &d, int nrhs, const void *src, const float *snrm,
 // u = 1 + i theta with antiperiodic t-links (gauge.F90:12-43), device layout
 int launch_coef(const Dims &d, const float *theta, int ibound, double2 *links, hipStream_t s);
 
+// one (R, X2) pair of a batched derivs with its weights fl32(tzi/2) and
+// +-tzi (tzi = fl32(2 anum), + for iflag 0)
+struct DerivPair {
+  const double2 *r, *x2;
+  double w1, w2;
+};
+DerivPair derivs_pair(const double2 *R, const double2 *X2, double anum, int iflag);
+// dSdpi += the Shamir derivs of every pair, in one pass (Ls <= 64);
+// pairs_dev: device buffer of npair entries (filled from pairs_host); part:
+// device buffer of npair * 3 V4 doubles (the pairs' terms before their sum)
+int launch_derivs_batch(const Dims &d, int npair, const DerivPair *pairs_host,
+                        DerivPair *pairs_dev, double *part, double *dsdpi, hipStream_t s);
 // fermion force (derivs_module.F90:99-196), FP64, accumulates into dsdpi [mu][t][y][x]
 int launch_derivs(const Dims &d, const double2 *R, const double2 *X2, double anum, int iflag,
                   double *dsdpi, hipStream_t s);

@@ -180,6 +180,47 @@ int qmrherm_device(const OpDesc &op, const double2 *phi, double2 *x, float res, 
   // x = anum(0)*Phi only (83)
   hipLaunchKernelGGL(k_lincomb, dim3(nblk(ns)), dim3(BLK), 0, s, ns, 0, anum[0], ad, phi, x1, x);
   THR_HIP(hipGetLastError());
+  if (op.kernel == 0 && d.ls <= 64 && n <= MAXRHS) {
+    // Shamir kernel: M x1(k) (and M Phi0(k)) of every pole in one multi-RHS
+    // Dslash, then every pole's derivs terms in one pass (same terms as the
+    // loop below; dSdpi receives their sum once)
+    double2 *mx = (double2 *)workspace(454, sizeof(double2) * ns * n);
+    double2 *mr = iflag == 3 ? (double2 *)workspace(455, sizeof(double2) * ns * n) : nullptr;
+    DerivPair *pd = (DerivPair *)workspace(456, sizeof(DerivPair) * 2 * n);
+    DerivPair *ph = (DerivPair *)host_workspace(11, sizeof(DerivPair) * 2 * n);
+    double *part = (double *)workspace(457, sizeof(double) * 3 * d.v4() * 2 * n);
+    THR_CHECK(mx && pd && ph && part && (iflag == 2 || mr), THR_ERR_HIP,
+              "workspace allocation failed");
+    DslashLaunch L{};
+    L.d = d;
+    L.nrhs = n;
+    L.prec = P64;
+    L.imass = op.imass;
+    L.am3 = op.am3;
+    for (int k = 0; k < n; k++) L.am[k] = op.am;
+    L.links = op.u64;
+    L.mode = EPI_STORE;
+    L.sc_stride = 1;
+    L.done_stride = 1;
+    L.in = x1;
+    L.out = mx;
+    TRY_RC(launch_dslash(L, s));  // x2 = M x1(k) (143)
+    int np = 0;
+    if (iflag == 2) {
+      for (int k = 0; k < n; k++)
+        ph[np++] = derivs_pair(x1 + (long)k * ns, mx + (long)k * ns, anum[k + 1], 0);
+    } else {
+      L.in = phi0;
+      L.out = mr;
+      TRY_RC(launch_dslash(L, s));  // M R with R = Phi0(k) (159, 166)
+      for (int k = 0; k < n; k++) {
+        const double coeff = -anum[k + 1];
+        ph[np++] = derivs_pair(phi0 + (long)k * ns, mx + (long)k * ns, coeff, 0);
+        ph[np++] = derivs_pair(mr + (long)k * ns, x1 + (long)k * ns, coeff, 1);
+      }
+    }
+    return launch_derivs_batch(d, np, ph, pd, part, dsdpi, s);
+  }
   double2 *x2 = (double2 *)workspace(452, sizeof(double2) * ns);
   THR_CHECK(x2, THR_ERR_HIP, "workspace allocation failed");
   for (int k = 0; k < n; k++) {

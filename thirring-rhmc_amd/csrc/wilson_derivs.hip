@@ -195,6 +195,75 @@ __global__ __launch_bounds__(BLK, 3) void k_derivs_seg(const Dims d, const doubl
   }
 }
 
+// k_derivs_seg over several (R, X2) pairs in one launch, one t step per
+// thread: blockIdx.y is the pair, so the GPU sweeps the pairs in order with
+// every neighbour line shared in L2 by the blocks of the same pair (a thread
+// looping over the pairs instead let neighbouring waves drift onto different
+// pairs and ran at 2.3 TB/s).  Each block writes its links' terms of its pair
+// to part[pair][mu][t][y][x]; k_derivs_sum_pairs adds the pairs in order into
+// dSdpi.  The force sums derivs over the poles of a rational approximation
+// (qmrherm_module.F90:143-181); one launch over all of them reads each
+// pair's fields once like separate calls but fills the GPU on lattices whose
+// single call cannot (25 MB per pair at 16^3 x 48).
+__global__ __launch_bounds__(BLK, 3) void k_derivs_seg_batch(const Dims d,
+                                                          const DerivPair *__restrict__ pairs,
+                                                          unsigned wseg, double *part) {
+  const unsigned lx = d.lx, ly = d.ly, lt = d.lt, ls = d.ls;
+  const unsigned v2 = lx * ly, v3 = v2 * ls, v4 = v2 * lt, v5 = v4 * ls;
+  const unsigned nl = BLK / wseg;           // lines per block
+  const unsigned nb2 = (v2 + nl - 1) / nl;  // blocks per t slice
+  const unsigned t = blockIdx.x / nb2, lb = blockIdx.x - t * nb2;
+  const unsigned line = threadIdx.x / wseg, s = threadIdx.x - line * wseg;
+  const unsigned col = lb * nl + line;  // 2-D site x + lx y
+  const bool valid = s < ls && col < v2 && t < lt;
+  const DerivPair P = pairs[blockIdx.y];
+  unsigned x = 0, y = 0;
+  if (col < v2) {
+    x = col % lx;
+    y = col / lx;
+  }
+  const unsigned xf = (x + 1 == lx) ? 0 : x + 1, yf = (y + 1 == ly) ? 0 : y + 1;
+  const unsigned tf = (t + 1 == lt) ? 0 : t + 1;
+  const unsigned j = s + ls * col + v3 * t, jx = s + ls * (xf + lx * y) + v3 * t;
+  const unsigned jy = s + ls * (x + lx * yf) + v3 * t, jt = s + ls * col + v3 * tf;
+  double v[3] = {0.0, 0.0, 0.0};
+  if (valid) {
+    cx<double> R0[4], X0[4], Rx[4], Xx[4], Ry[4], Xy[4], Rt[4], Xt[4];
+    ld_spinor(P.r, v5, j, R0);
+    ld_spinor(P.x2, v5, j, X0);
+    ld_spinor(P.r, v5, jx, Rx);
+    ld_spinor(P.x2, v5, jx, Xx);
+    ld_spinor(P.r, v5, jy, Ry);
+    ld_spinor(P.x2, v5, jy, Xy);
+    ld_spinor(P.r, v5, jt, Rt);
+    ld_spinor(P.x2, v5, jt, Xt);
+    v[0] = derivs_term<0>(R0, X0, Rx, Xx, P.w1, P.w2);
+    v[1] = derivs_term<1>(R0, X0, Ry, Xy, P.w1, P.w2);
+    v[2] = derivs_term<2>(R0, X0, Rt, Xt, P.w1, P.w2);
+  }
+  // every lane of the wave takes part in the shuffles (idle lanes add 0)
+  for (unsigned off = wseg >> 1; off > 0; off >>= 1) {
+#pragma unroll
+    for (int mu = 0; mu < 3; mu++) v[mu] += __shfl_xor(v[mu], (int)off, 64);
+  }
+  if (s == 0 && col < v2 && t < lt) {
+    double *o = part + (long)blockIdx.y * 3 * v4;
+#pragma unroll
+    for (int mu = 0; mu < 3; mu++) o[(long)mu * v4 + col + v2 * t] = v[mu];
+  }
+}
+
+// dSdpi[e] += sum over the pairs, in pair order, of part[pair][e]
+__global__ __launch_bounds__(BLK) void k_derivs_sum_pairs(long n, int npair,
+                                                          const double *__restrict__ part,
+                                                          double *ds) {
+  const long e = (long)blockIdx.x * BLK + threadIdx.x;
+  if (e >= n) return;
+  double v = 0.0;
+  for (int k = 0; k < npair; k++) v += part[(long)k * n + e];
+  ds[e] += v;
+}
+
 __global__ __launch_bounds__(BLK) void k_derivs_march(const Dims d, const double2 *__restrict__ R,
                                                       const double2 *__restrict__ X2, float tzi,
                                                       int iflag, unsigned nl, unsigned tchunk,
@@ -429,6 +498,35 @@ int launch_dslash_wilson(const Dims &d, int nrhs, int dagger, int imass, float a
     hipLaunchKernelGGL(k_wkd_post, dim3(nb), dim3(BLK), 0, s, d, nrhs, in, tmp, out, z1, z2);
   }
   THR_HIP(hipGetLastError());
+  return THR_OK;
+}
+
+DerivPair derivs_pair(const double2 *R, const double2 *X2, double anum, int iflag) {
+  const float tzi = 2.0f * (float)anum;  // real :: tzi_real = 2*real(anum)
+  return DerivPair{R, X2, (double)(tzi * 0.5f), (double)tzi * (iflag == 0 ? 1.0 : -1.0)};
+}
+
+int launch_derivs_batch(const Dims &d, int npair, const DerivPair *pairs_host,
+                        DerivPair *pairs_dev, double *part, double *dsdpi, hipStream_t s) {
+  THR_CHECK(npair >= 1 && npair <= 65535, THR_ERR_ARG, "derivs batch: 1..65535 pairs");
+  THR_CHECK(d.v5() * 4 < (1L << 31), THR_ERR_SIZE, "derivs: 4*V5 must be below 2^31");
+  THR_CHECK(d.ls <= 64, THR_ERR_ARG, "derivs batch: Ls above 64");
+  THR_CHECK(pairs_dev && part, THR_ERR_ARG, "derivs batch: NULL buffer");
+  const long n = 3 * d.v4();
+  THR_HIP(hipMemcpyAsync(pairs_dev, pairs_host, sizeof(DerivPair) * npair, hipMemcpyHostToDevice,
+                         s));
+  unsigned w = 1;
+  while (w < (unsigned)d.ls) w <<= 1;
+  const unsigned nl = BLK / w;
+  const unsigned v2 = (unsigned)(d.lx * d.ly);
+  const unsigned nb = ((v2 + nl - 1) / nl) * (unsigned)d.lt;
+  hipLaunchKernelGGL(k_derivs_seg_batch, dim3(nb, npair), dim3(BLK), 0, s, d,
+                     (const DerivPair *)pairs_dev, w, part);
+  hipLaunchKernelGGL(k_derivs_sum_pairs, dim3((unsigned)((n + BLK - 1) / BLK)), dim3(BLK), 0, s, n,
+                     npair, (const double *)part, dsdpi);
+  THR_HIP(hipGetLastError());
+  // the host pair array must outlive the copy
+  THR_HIP(hipStreamSynchronize(s));
   return THR_OK;
 }
 

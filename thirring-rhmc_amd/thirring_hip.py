@@ -47,6 +47,7 @@ _SIGS = {
     "thr_noalloc_scratch_bytes": ([], ctypes.c_ulonglong),
     "thr_congrad_dev_noalloc": ([P, P, P, P, F32, F32, I32, F32, I32, IP, IP, P], I32),
     "thr_derivs_dev": ([P, P, F64, I32, P, F32, I32, I32, P, P], I32),
+    "thr_derivs_batch_dev": ([P, P, P, P, I32, P, P, P], I32),
     "thr_measure_shamir_noise": ([P, P, P, P, F32, F32, I32, F32, I32, I32, P, P, IP], I32),
     "thr_measure_wilson_noise": ([P, P, P, P, F32, F32, I32, F32, I32, I32, P, P, IP], I32),
     "thr_multishift": ([P, P, F32, I32, F32, I32, P, P, P, P, F32, I32, I32, I32, IP, P], I32),
@@ -566,6 +567,22 @@ def derivs_dev(r, x2, anum, iflag, dims, dsdpi, am=0.0, imass=3, kernel="shamir"
     _check(lib().thr_derivs_dev(_p(r), _p(x2), float(anum), int(iflag), _dims(dims), am,
                                 int(imass), KERNEL[kernel], _p(dsdpi), _stream(stream)),
            "derivs_dev")
+    return dsdpi
+
+
+def derivs_batch_dev(rs, x2s, anums, iflags, dims, dsdpi, stream=None):
+    """dsdpi += the Shamir derivs of every (rs[k], x2s[k], anums[k], iflags[k])
+    in one pass (Ls <= 64): the per-pole derivs of the force in one launch."""
+    init()
+    n = len(rs)
+    if not (len(x2s) == len(anums) == len(iflags) == n):
+        raise ValueError("rs, x2s, anums and iflags must have the same length")
+    rp = (ctypes.c_void_p * n)(*[_p(a).value for a in rs])
+    xp = (ctypes.c_void_p * n)(*[_p(a).value for a in x2s])
+    an = np.ascontiguousarray(anums, dtype=np.float64)
+    fl = np.ascontiguousarray(iflags, dtype=np.int32)
+    _check(lib().thr_derivs_batch_dev(rp, xp, _p(an), _p(fl), int(n), _dims(dims), _p(dsdpi),
+                                      _stream(stream)), "derivs_batch_dev")
     return dsdpi
 
 
