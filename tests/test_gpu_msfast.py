@@ -134,6 +134,8 @@ def test_reliable_updates_low_precision_from_start(G, th16):
           flush=True)
     assert st0["reliable"] == 0 and st1["reliable"] > 0
     assert st1["switches"][1] == 32
+    if th16 > 0:  # the FP16 blocks (and their reliable updates) did run
+        assert st1["switches"][2] > 32, st1["switches"]
     if th16 == 0.0:
         assert rel1[:, imin].max() <= 1e-10, rel1[:, imin]
         assert rel0[:, imin].max() > 10 * rel1[:, imin].max()

@@ -17,6 +17,7 @@
 // same partials in the same order: identical values everywhere, no broadcast.
 // Storage is FP64 throughout (on such lattices the solve is latency-bound, so
 // lower-precision storage would buy nothing).
+#include <algorithm>
 #include <cstdlib>
 
 #include "dslash_site.h"
@@ -408,7 +409,9 @@ int msfast_persistent(const MsfArgs &A, MsfResult *res, hipStream_t s) {
   // THR_PERSIST_BLOCKS caps the blocks per source (each block then sweeps
   // several site ranges), for tests of the strided sweep
   if (const char *cap = std::getenv("THR_PERSIST_BLOCKS")) want = std::max(1L, std::min(want, atol(cap)));
-  const int bs = (int)std::max(1L, std::min(want, resident / ns_));
+  // at most 128 blocks in all: the barrier's cost grows with the block count
+  // (measured best: 64 blocks at 8^3 x 8, 128 at 12^2 x 24 x 16, 4 sources)
+  const int bs = (int)std::max(1L, std::min({want, resident / ns_, std::max(1L, 128L / ns_)}));
   // every block must be resident at once for the grid barriers
   THR_CHECK((long)bs * ns_ <= resident, THR_ERR_STATE, "persistent grid exceeds residency");
 
