@@ -215,6 +215,39 @@ def test_graph_replay_equals_eager(G):
     assert np.array_equal(outs[1][1], outs[2][1])
 
 
+def test_fused_scalars_bitwise(G):
+    """The Dslash passes deriving omega and gamma from each other's partial
+    sums, with one finalize per 32-iteration block (the default), give
+    bit-identical solutions and counts to a finalize kernel after every pass
+    (THR_FUSED=0), through FP64, FP32 and FP16 storage, reliable updates and
+    the corrections, for sources that stop in different blocks."""
+    import os
+    from hiprt import DeviceArray
+    dims = (4, 4, 6, 8)
+    sig = _remez4()[::2]
+    th, l64, l32, bh, b = _setup(dims, 6, 3)
+    n, nsp = len(sig), bh.shape[1]
+    outs = []
+    for flag in ("0", "1"):
+        os.environ["THR_FUSED"] = flag
+        try:
+            x = DeviceArray((3, n, nsp, 2), np.float64)
+            st = {}
+            it, cgr, rel = G.multishift_fast_dev(l64, l32, dims, [0.01, 0.05, 0.2], 3, sig, b, x,
+                                                 tol=1e-10, maxcg=20000, nsrc=3, th32=1e-3,
+                                                 th16=1e-6, stats=st)
+            print("THR_FUSED", flag, "it", it, "cg_return", list(cgr), "corr",
+                  st["corr_iterations"], "reliable", st["reliable"], flush=True)
+            assert rel.max() <= 1e-10
+            outs.append((it, list(cgr), st, x.to_host()))
+        finally:
+            os.environ.pop("THR_FUSED", None)
+    assert len(set(outs[0][1])) > 1  # the sources stop at different iterations
+    assert outs[0][0] == outs[1][0] and outs[0][1] == outs[1][1]
+    assert outs[0][2] == outs[1][2]
+    assert np.array_equal(outs[0][3], outs[1][3])
+
+
 def test_batched_multi_rhs_workload_24cubed(G):
     """The batched multi-RHS workload at its real shape: 24^3 x Ls=32, one
     link field, 16 Gaussian sources x 25 shifts of x^(-1/4) in one batched

@@ -448,6 +448,24 @@ __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, const Tili
   using T = typename Store<PREC>::T;
   const int rhs = blockIdx.y;
   if (L.done && L.done[(long)rhs * L.done_stride]) return;
+  // the previous pass's partial sums of this right-hand side (the finalize
+  // kernels' loop and tree, so the value is theirs bit for bit)
+  double fs = 0.0;
+  if constexpr (MODE == EPI_CG || MODE == EPI_HREC) {
+    if (L.fsum) {
+      __shared__ double s_fs;
+      const double *pp = L.fsum + (long)rhs * L.fsum_stride;
+      double v = 0.0;
+      for (int k = threadIdx.x; k < L.fsum_n; k += BLK) v += pp[k];
+      v = block_sum(v);
+      if (threadIdx.x == 0) {
+        s_fs = v;
+        if (blockIdx.x == 0) L.frec[(long)rhs * L.sc_stride] = v;
+      }
+      __syncthreads();
+      fs = s_fs;
+    }
+  }
   // 32-bit index arithmetic: 4*V5 < 2^31 is checked on the host
   const unsigned lx = L.d.lx, ly = L.d.ly, lt = L.d.lt, ls = L.d.ls;
   const unsigned v2 = lx * ly, v3 = v2 * ls, v4 = v2 * lt, v5 = v4 * ls;
@@ -507,10 +525,18 @@ __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, const Tili
     // epilogue scalars (device-resident solver state)
     C ea = C(0), eb = C(0);
     if constexpr (MODE == EPI_CG) {
-      ea = (C)L.a_ptr[(long)rhs * L.sc_stride];
+      if (L.fsum) {
+        const double om = -L.fdelta[(long)rhs * L.sc_stride] / fs;  // omega = -delta / alpha
+        ea = (C)(-om);
+      } else {
+        ea = (C)L.a_ptr[(long)rhs * L.sc_stride];
+      }
       if (L.b_ptr) eb = (C)L.b_ptr[(long)rhs * L.sc_stride];
     } else if constexpr (MODE == EPI_HREC) {
-      eb = (C)L.b_ptr[(long)rhs * L.sc_stride];
+      if (L.fsum)
+        eb = (C)(fs / L.fdelta[(long)rhs * L.sc_stride]);  // gamma = lambda / delta
+      else
+        eb = (C)L.b_ptr[(long)rhs * L.sc_stride];
     }
 
     Site<PREC> prv, cur, nxt;
@@ -776,7 +802,15 @@ std::map<TuneKey, DslashConfig> g_tuned;
 
 DslashConfig default_config(const Dims &d) {
   DslashConfig c;
-  c.tchunk = std::min(8, d.lt);
+  // t-chunk 8 (the measured best at 32^3 x 64) unless the lattice is too
+  // small for it to give ~512 blocks per right-hand side: a thread's t
+  // steps run one after another, so a small lattice marched in long chunks
+  // is a few blocks waiting on memory latency (8^3 x 8 at chunk 8: 2 blocks)
+  {
+    const long nb3 = ((long)d.lx * d.ly * d.ls + BLK - 1) / BLK;
+    const long chunks = std::max(1L, (512 + nb3 - 1) / nb3);
+    c.tchunk = (int)std::max(1L, std::min<long>(std::min(8, d.lt), d.lt / chunks));
+  }
   // lane shifts pay when every wave holds whole s-lines; when lines cross
   // waves the edge-lane loads cost as much as they save (measured at Ls = 48)
   c.laneshift = (64 % d.ls) == 0 ? 1 : 0;

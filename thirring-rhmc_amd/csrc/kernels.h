@@ -40,6 +40,18 @@ struct DslashLaunch {
   const double *b_ptr; // EPI_CG: x += b_ptr[rhs*sc_stride] * p  (null: no x update);
                        // EPI_HREC: out = M in + b_ptr[rhs*sc_stride] * out
   int sc_stride;       // stride (in doubles) between the per-rhs scalars
+  // Scalars derived in the kernel instead of read from a_ptr / b_ptr (the
+  // batched multishift's iterations without finalize launches): every block
+  // sums its right-hand side's partial sums of the previous pass,
+  // s = sum_{k < fsum_n} fsum[rhs * fsum_stride + k] (the finalize kernels'
+  // order), and takes the recorded scalars at fdelta[rhs * sc_stride]:
+  //   EPI_CG:   a = fdelta / s            (omega = -delta / alpha, a = -omega)
+  //   EPI_HREC: b = s / fdelta            (gamma = lambda / delta)
+  // Block 0 of each right-hand side stores s at frec[rhs * sc_stride].
+  const double *fsum;  // null: read a_ptr / b_ptr
+  int fsum_n, fsum_stride;
+  const double *fdelta;
+  double *frec;
   double *partial;     // [nrhs][nblocks]
   const int *done;     // optional early-exit flag, done[rhs*done_stride] (device)
   int done_stride;

@@ -15,6 +15,9 @@ struct MsfState {
   int active[MAXSHIFT], touched[MAXSHIFT], cg_returns[MAXSHIFT];
   int n, mini, maxi, jcount, cg_return, maxcg, stop, first_block;
   int nonfinite;  // alpha or lambda came out NaN/Inf: this source stopped there
+  // iterations whose scalars the Dslash passes derive themselves: alpha_j and
+  // lambda_j of the block's iteration j, replayed by the block's finalize
+  double al_rec[MSB], lam_rec[MSB];
 };
 
 struct MsfArgs {

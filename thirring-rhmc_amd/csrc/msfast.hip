@@ -107,18 +107,11 @@ __global__ __launch_bounds__(BLK) void k_msf_init(MsfState *st, int n, const dou
 // alpha = |h|^2, omega, zeta''' and omega_i (multishift_module.F90:118-135);
 // aden holds MAXSHIFT shifts per source.  Thread i updates shift i; every
 // shift's arithmetic is the serial recurrence's, so results do not depend on
-// the thread count.
-__global__ __launch_bounds__(BLK) void k_msf_fin1(MsfState *st, const double *adens,
-                                                  const double *partial, int stride, int nb) {
-  MsfState *S = st + blockIdx.x;
-  if (S->stop) return;
-  const double *aden = adens + (long)blockIdx.x * MAXSHIFT;
-  __shared__ double s_al;
-  const double al = fin_sum(partial + (long)blockIdx.x * stride, nb);
-  if (threadIdx.x == 0) s_al = al;
-  __syncthreads();
+// the thread count.  Returns false (and stops the source) on a non-finite
+// omega.  Block-uniform: every thread of the block calls it.
+__device__ bool fin1_step(MsfState *S, const double *aden, double al) {
   const double os = S->omega, gg = S->gammag;
-  const double om = -S->delta / s_al;
+  const double om = -S->delta / al;
   if (!isfinite(om)) {
     // a non-finite alpha can never converge: stop this source before any of
     // its vectors is updated (its later launches exit on the stop flag)
@@ -127,7 +120,8 @@ __global__ __launch_bounds__(BLK) void k_msf_fin1(MsfState *st, const double *ad
       S->nonfinite = 1;
       S->stop = 1;
     }
-    return;
+    __syncthreads();
+    return false;
   }
   const int i = threadIdx.x;
   if (i >= S->mini - 1 && i < S->maxi && S->active[i]) {
@@ -139,36 +133,34 @@ __global__ __launch_bounds__(BLK) void k_msf_fin1(MsfState *st, const double *ad
   __syncthreads();
   if (threadIdx.x == 0) {
     S->cg_return += 1;
-    S->alpha = s_al;
+    S->alpha = al;
     S->omega_save = os;
     S->omega = om;
     S->minus_omega = -om;
   }
+  __syncthreads();
+  return true;
 }
 
-// lambda, gamma, gamma_i; block coefficients; convergence (136-176, 230-275)
-__global__ __launch_bounds__(BLK) void k_msf_fin2(MsfState *st, const double *partial, int stride,
-                                                  int nb) {
-  MsfState *S = st + blockIdx.x;
-  if (S->stop) return;
-  __shared__ double s_lam;
+// lambda, gamma, gamma_i; block coefficients; convergence (136-176, 230-275).
+// Returns false once the source stops (non-finite lambda, every shift
+// converged, or maxcg).  Block-uniform.
+__device__ bool fin2_step(MsfState *S, double L) {
   __shared__ int s_nmax, s_nmin;
-  const double lam = fin_sum(partial + (long)blockIdx.x * stride, nb);
-  if (threadIdx.x == 0) {
-    s_lam = lam;
-    s_nmax = 0;
-    s_nmin = S->n + 1;
-  }
-  __syncthreads();
-  const double L = s_lam;
   if (!isfinite(L)) {
     __syncthreads();
     if (threadIdx.x == 0) {
       S->nonfinite = 1;
       S->stop = 1;
     }
-    return;
+    __syncthreads();
+    return false;
   }
+  if (threadIdx.x == 0) {
+    s_nmax = 0;
+    s_nmin = S->n + 1;
+  }
+  __syncthreads();
   const double gam = L / S->delta;
   const double omega = S->omega, tol2 = S->tol2;
   const int j = S->jcount, cgr = S->cg_return;
@@ -215,6 +207,8 @@ __global__ __launch_bounds__(BLK) void k_msf_fin2(MsfState *st, const double *pa
     S->zeta_i[i] = S->zeta_ii[i];
     S->zeta_ii[i] = S->zeta_iii[i];
   }
+  __syncthreads();
+  const bool go = !(nmax == 0 || cgr >= S->maxcg);
   if (threadIdx.x == 0) {
     S->lambda = L;
     S->gammag = gam;
@@ -222,7 +216,53 @@ __global__ __launch_bounds__(BLK) void k_msf_fin2(MsfState *st, const double *pa
     S->maxi = nmax;
     S->delta = L;
     S->jcount = j + 1;
-    if (nmax == 0 || cgr >= S->maxcg) S->stop = 1;
+    if (!go) S->stop = 1;
+  }
+  __syncthreads();
+  return go;
+}
+
+__global__ __launch_bounds__(BLK) void k_msf_fin1(MsfState *st, const double *adens,
+                                                  const double *partial, int stride, int nb) {
+  MsfState *S = st + blockIdx.x;
+  if (S->stop) return;
+  __shared__ double s_al;
+  const double al = fin_sum(partial + (long)blockIdx.x * stride, nb);
+  if (threadIdx.x == 0) s_al = al;
+  __syncthreads();
+  fin1_step(S, adens + (long)blockIdx.x * MAXSHIFT, s_al);
+}
+
+__global__ __launch_bounds__(BLK) void k_msf_fin2(MsfState *st, const double *partial, int stride,
+                                                  int nb) {
+  MsfState *S = st + blockIdx.x;
+  if (S->stop) return;
+  __shared__ double s_lam;
+  const double lam = fin_sum(partial + (long)blockIdx.x * stride, nb);
+  if (threadIdx.x == 0) s_lam = lam;
+  __syncthreads();
+  fin2_step(S, s_lam);
+}
+
+// The recurrences of a block's iterations replayed from the scalars the
+// Dslash passes derived and recorded (al_rec, lam_rec; the last lambda is
+// summed here from the last M^dagger pass's partials).  The passes kept
+// iterating to the end of the block; the replay stops where the finalize
+// kernels would have stopped the source, so x_i, p_i and every count are
+// those of the per-iteration finalize.
+__global__ __launch_bounds__(BLK) void k_msf_fin_block(MsfState *st, const double *adens,
+                                                       const double *lpart, int stride, int nb,
+                                                       int nit) {
+  MsfState *S = st + blockIdx.x;
+  if (S->stop) return;
+  __shared__ double s_last;
+  const double last = fin_sum(lpart + (long)blockIdx.x * stride, nb);
+  if (threadIdx.x == 0) s_last = last;
+  __syncthreads();
+  const double *aden = adens + (long)blockIdx.x * MAXSHIFT;
+  for (int j = 0; j < nit; j++) {
+    if (!fin1_step(S, aden, S->al_rec[j])) return;
+    if (!fin2_step(S, j + 1 < nit ? S->lam_rec[j] : s_last)) return;
   }
 }
 
@@ -565,9 +605,13 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
   double *aden = (double *)workspace(365, sizeof(double) * (MAXSHIFT + 1) * ns_);
   double *tolv = aden + (long)MAXSHIFT * ns_;
   double *partial = (double *)workspace(366, sizeof(double) * nb * ns_);
+  // the M^dagger passes' partial sums when the passes derive their own
+  // scalars (the M passes' go to `partial`; a pass reads the other's)
+  double *lpart = (double *)workspace(369, sizeof(double) * nb * ns_);
   void **ringptr = (void **)workspace(367, sizeof(void *) * 3 * MSB + sizeof(float *) * MSB);
   double *sums = (double *)workspace(368, sizeof(double) * ns_);
-  THR_CHECK(Pv && st && aden && partial && ringptr && sums, THR_ERR_HIP, "workspace allocation failed");
+  THR_CHECK(Pv && st && aden && partial && lpart && ringptr && sums, THR_ERR_HIP,
+            "workspace allocation failed");
   {
     const int np = 4 * MSB;
     const size_t nad = (size_t)(MAXSHIFT + 1) * ns_;
@@ -718,6 +762,12 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
   // hipGraph and replayed (THR_GRAPHS=0 disables)
   const char *ge = std::getenv("THR_GRAPHS");
   const bool use_graph = !(ge && ge[0] == '0') && context_graphs();
+  // the Dslash passes derive omega and gamma from each other's partial sums
+  // and record alpha and lambda; one finalize per block replays the shift
+  // recurrences (2 launches per iteration instead of 4; THR_FUSED=0: a
+  // finalize kernel after every pass).  Results are bit-identical.
+  const char *fe = std::getenv("THR_FUSED");
+  const bool fused = !(fe && fe[0] == '0');
   hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
   auto enqueue_block = [&](long t0, int pr) -> int {
     for (int j = 0; j < MSB; j++) {
@@ -742,9 +792,20 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
       } else {
         a.mode = EPI_HREC;
         a.b_ptr = &st->gammag;
+        if (fused && j > 0) {
+          // gamma = lambda_{k-1} / delta_{k-1} from the last M^+ pass's partials
+          a.fsum = lpart;
+          a.fsum_n = nbl[pr];
+          a.fsum_stride = nb;
+          a.fdelta = j >= 2 ? &st->lam_rec[j - 2] : &st->delta;
+          a.frec = &st->lam_rec[j - 1];
+        }
       }
+      if (fused) a.partial = partial;  // alpha partials; lambda ones go to lpart
       TRY_RC(launch_dslash_cfg(a, cfg[pr], s));
-      hipLaunchKernelGGL(k_msf_fin1, dim3(ns_), dim3(BLK), 0, s, st, aden, partial, nb, nbl[pr]);
+      if (!fused)
+        hipLaunchKernelGGL(k_msf_fin1, dim3(ns_), dim3(BLK), 0, s, st, aden, partial, nb,
+                           nbl[pr]);
       // r_new = r + omega M^+ h ; lambda = |r_new|^2
       DslashLaunch b = L;
       b.prec = pr;
@@ -759,10 +820,23 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
       b.r_out_norm = routn;
       b.a_ptr = &st->minus_omega;
       b.b_ptr = nullptr;
+      if (fused) {
+        // omega = -delta_k / alpha_k from the M pass's partials
+        b.fsum = partial;
+        b.fsum_n = nbl[pr];
+        b.fsum_stride = nb;
+        b.fdelta = j >= 1 ? &st->lam_rec[j - 1] : &st->delta;
+        b.frec = &st->al_rec[j];
+        b.partial = lpart;
+      }
       TRY_RC(launch_dslash_cfg(b, cfg[pr], s));
-      hipLaunchKernelGGL(k_msf_fin2, dim3(ns_), dim3(BLK), 0, s, st, partial, nb, nbl[pr]);
+      if (!fused)
+        hipLaunchKernelGGL(k_msf_fin2, dim3(ns_), dim3(BLK), 0, s, st, partial, nb, nbl[pr]);
       THR_HIP(hipGetLastError());
     }
+    if (fused)
+      hipLaunchKernelGGL(k_msf_fin_block, dim3(ns_), dim3(BLK), 0, s, st, aden, lpart, nb, nbl[pr],
+                         MSB);
     // flush the block into x_i, p_i
     dim3 gf(nblk(nsp), ns_);
     void *const *rp = (void *const *)ringptr + pr * MSB;
