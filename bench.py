@@ -142,10 +142,11 @@ def time_dslash(G, hr, dims, links, prec, steps, warmup, nbuf, stream, nrhs=1, a
     return float(b0.elapsed_ms(b1) / steps), float(np.mean(per)), float(t0.elapsed_ms(t1) / steps)
 
 
-def multi_rhs_legs(G, hr, steps, warmup, stream, pinned=None, tuned=()):
+def multi_rhs_legs(G, hr, steps, warmup, stream, pinned=None):
     """FP64 Dslash over several right-hand sides per launch (links read once
     per launch): 4 sources at 32^3 x Ls=64, and 16 sources with masses drawn
-    from {0.005, 0.01, 0.02, 0.05} at 24^3 x Ls=32."""
+    from {0.005, 0.01, 0.02, 0.05} at 24^3 x Ls=32; each with the shape tuned
+    for its batch (the library pins shapes per batch class)."""
     out = {}
     rng = np.random.default_rng(7)
     for dims, nrhs in (((32, 32, 32, 64), 4), ((24, 24, 24, 32), 16)):
@@ -154,9 +155,9 @@ def multi_rhs_legs(G, hr, steps, warmup, stream, pinned=None, tuned=()):
         _, l64, l32 = make_fields(G, hr.DeviceArray, dims, 0.3, 99)
         if pinned:
             G.dslash_set_config(dims, "f64", *pinned)
-        elif dims not in tuned:  # lattices the single-RHS leg tuned keep that shape
+        else:
             G.dslash_autotune(l64, dims, "f64", stream=stream, nrhs=nrhs)
-        tc, fl = G.dslash_get_config(dims, "f64")
+        tc, fl = G.dslash_get_config(dims, "f64", nrhs=nrhs)
         am = rng.choice([0.005, 0.01, 0.02, 0.05], size=nrhs).astype(np.float32)
         nb = max(2, int(np.ceil(4 * 256e6 / (2 * 16 * 4 * v5 * nrhs))))
         progress(f"dslash f64 {dims} nrhs={nrhs}")
@@ -711,18 +712,31 @@ def main():
                        "GB_s": round(dslash_bytes(v4, v5, prec) / (ms * 1e-3) / 1e9, 1)}
     pinned = (args.tchunk, args.shape_flags) if args.tchunk > 0 else None
     multi = None if args.no_multi_rhs else multi_rhs_legs(G, hr, max(20, args.steps // 4),
-                                                           args.warmup, stream, pinned,
-                                                           tuned=(dims,))
+                                                           args.warmup, stream, pinned)
     progress("device copy rate")
     copy_gbs = copy_rate(G, hr, stream)
     solver = None
     if not args.no_solver:
         sd = tuple(int(v) for v in args.solver_lattice.split(","))
         _, s64, s32 = make_fields(G, hr.DeviceArray, sd, 0.3, 4321)
+        # the solver launches every pass over all its sources: pin the shapes
+        # tuned for that batch
+        batch_shapes = {}
+        for prec, lk in (("f64", s64), ("f32", s32), ("f16", s32)):
+            if prec in pins:
+                G.dslash_set_config(sd, prec, *pins[prec])
+            elif args.tchunk > 0:
+                G.dslash_set_config(sd, prec, args.tchunk, args.shape_flags)
+            elif args.solver_sources > 1:
+                progress(f"autotune dslash {prec} nrhs={args.solver_sources}")
+                G.dslash_autotune(lk, sd, prec, stream=stream, nrhs=args.solver_sources)
+            tc, fl = G.dslash_get_config(sd, prec, nrhs=args.solver_sources)
+            batch_shapes[prec] = {"tchunk": tc, "flags": fl}
         solver = bench_multishift(G, hr, sd, s64, s32, args.solver_am, args.solver_sources, stream,
                                   mixed_th32=args.mixed_th32, mixed_th16=args.mixed_th16,
                                   legs_wanted=tuple(args.solver_legs.split(",")))
         solver["lattice"] = list(sd)
+        solver["launch_shapes"] = batch_shapes
         solver["am"] = args.solver_am
     derivs = None if args.no_derivs else bench_derivs(G, hr, stream)
     config4 = None if args.no_config4 else bench_batched_config4(G, hr, stream)

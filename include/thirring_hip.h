@@ -301,12 +301,19 @@ int thr_dslash_dev(void *out, float *out_norm, const void *in, const float *in_n
 int thr_dslash_autotune(const void *links, const int dims[4], int prec, int *tchunk, int *flags,
                         void *stream);
 /* thr_dslash_autotune timing launches over nrhs right-hand sides (1..32), for
- * callers that apply the operator to batches: the pinned shape applies to
- * every nrhs of these dims and precision (no counterpart in the reference) */
+ * callers that apply the operator to batches: nrhs > 1 pins the shape of the
+ * batch class of these dims and precision (launches of 2..32 right-hand
+ * sides), nrhs = 1 the single one (no counterpart in the reference) */
 int thr_dslash_autotune_nrhs(const void *links, const int dims[4], int prec, int nrhs,
                              int *tchunk, int *flags, void *stream);
 int thr_dslash_set_config(const int dims[4], int prec, int tchunk, int flags);
 int thr_dslash_get_config(const int dims[4], int prec, int *tchunk, int *flags);
+/* the shape a launch of nrhs right-hand sides uses: shapes are pinned per
+ * batch class (one right-hand side, or several); thr_dslash_autotune_nrhs
+ * pins the batch class, thr_dslash_autotune the single one,
+ * thr_dslash_set_config both; a batch without a shape of its own takes the
+ * single-RHS one */
+int thr_dslash_get_config_nrhs(const int dims[4], int prec, int nrhs, int *tchunk, int *flags);
 
 /* reference-faithful FP64 CG on device fields (phi, x: 4*V5 complex128) */
 int thr_congrad_dev(const void *phi, void *x, const void *links64, const int dims[4], float res,

@@ -74,7 +74,10 @@ def test_autotune_over_rhs_batches(G):
     G.dslash_dev(out, x, l64, dims, 0.05, imass=3, nrhs=4)
     ref = out.to_host()
     tc, fl = G.dslash_autotune(l64, dims, "f64", nrhs=4)
-    assert (tc, fl) == G.dslash_get_config(dims, "f64")
+    # pinned for batches; the single-RHS shape is untouched
+    assert (tc, fl) == G.dslash_get_config(dims, "f64", nrhs=4)
+    assert (tc, fl) == G.dslash_get_config(dims, "f64", nrhs=32)
+    assert G.dslash_get_config(dims, "f64") == before
     assert 1 <= tc <= lt and 0 <= fl < 256
     G.dslash_dev(out, x, l64, dims, 0.05, imass=3, nrhs=4)
     got = out.to_host()

@@ -217,9 +217,9 @@ def test_graph_replay_equals_eager(G):
 
 def test_fused_scalars_bitwise(G):
     """The Dslash passes deriving omega and gamma from each other's partial
-    sums, with one finalize per 32-iteration block (the default), give
-    bit-identical solutions and counts to a finalize kernel after every pass
-    (THR_FUSED=0), through FP64, FP32 and FP16 storage, reliable updates and
+    sums, with one finalize per 32-iteration block (THR_FUSED=1; the default
+    on launches of few blocks), give bit-identical solutions and counts to a
+    finalize kernel after every pass (THR_FUSED=0), through FP64, FP32 and FP16 storage, reliable updates and
     the corrections, for sources that stop in different blocks."""
     import os
     from hiprt import DeviceArray

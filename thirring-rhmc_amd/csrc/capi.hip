@@ -802,9 +802,14 @@ int thr_dslash_set_config(const int dims[4], int prec, int tchunk, int flags) {
 }
 
 int thr_dslash_get_config(const int dims[4], int prec, int *tchunk, int *flags) {
+  return thr_dslash_get_config_nrhs(dims, prec, 1, tchunk, flags);
+}
+
+int thr_dslash_get_config_nrhs(const int dims[4], int prec, int nrhs, int *tchunk, int *flags) {
   TRY(check_dims(dims));
   THR_CHECK(prec >= 0 && prec <= 2, THR_ERR_ARG, "prec must be THR_F64, THR_F32 or THR_F16");
-  const DslashConfig c = dslash_config(mkdims(dims), prec);
+  THR_CHECK(nrhs >= 1 && nrhs <= MAXRHS, THR_ERR_ARG, "nrhs out of range (1..32)");
+  const DslashConfig c = dslash_config(mkdims(dims), prec, nrhs);
   if (tchunk) *tchunk = c.tchunk;
   if (flags) *flags = shape_flags(c);
   return THR_OK;

@@ -136,6 +136,27 @@ __device__ inline double block_sum(double v) {
   return t;
 }
 
+// sum of n per-block partial sums: thread k adds p[k], p[k + BLK], ... in
+// that order, then block_sum (the result, valid in thread 0, depends only on
+// the values and n).  The loads are issued 8 at a time before they are added,
+// so a long row costs a few memory latencies instead of one per element.
+__device__ inline double partial_sum(const double *p, int n) {
+  constexpr int U = 8;
+  double v = 0.0;
+  for (int k0 = threadIdx.x; k0 < n; k0 += U * BLK) {
+    double a[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int k = k0 + u * BLK;
+      a[u] = k < n ? p[k] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (k0 + u * BLK < n) v += a[u];
+  }
+  return block_sum(v);
+}
+
 }  // namespace thr
 
 // status codes shared with the C ABI

@@ -454,10 +454,7 @@ __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, const Tili
   if constexpr (MODE == EPI_CG || MODE == EPI_HREC) {
     if (L.fsum) {
       __shared__ double s_fs;
-      const double *pp = L.fsum + (long)rhs * L.fsum_stride;
-      double v = 0.0;
-      for (int k = threadIdx.x; k < L.fsum_n; k += BLK) v += pp[k];
-      v = block_sum(v);
+      const double v = partial_sum(L.fsum + (long)rhs * L.fsum_stride, L.fsum_n);
       if (threadIdx.x == 0) {
         s_fs = v;
         if (blockIdx.x == 0) L.frec[(long)rhs * L.sc_stride] = v;
@@ -791,10 +788,13 @@ static bool box_split(const Dims &d, unsigned &px, unsigned &py, unsigned &pt) {
 // _UNIFORM / _LANESHIFT / _BOXES / _LDSX / _NT override the default.
 // ---------------------------------------------------------------------------
 namespace {
+// shapes are pinned per lattice, precision and batch class (one right-hand
+// side, or several: a batch's larger grid favours other shapes)
 struct TuneKey {
-  int lx, ly, lt, ls, prec;
+  int lx, ly, lt, ls, prec, multi;
   bool operator<(const TuneKey &o) const {
-    return std::tie(lx, ly, lt, ls, prec) < std::tie(o.lx, o.ly, o.lt, o.ls, o.prec);
+    return std::tie(lx, ly, lt, ls, prec, multi) <
+           std::tie(o.lx, o.ly, o.lt, o.ls, o.prec, o.multi);
   }
 };
 std::mutex g_tune_mu;
@@ -828,16 +828,20 @@ DslashConfig default_config(const Dims &d) {
 }
 }  // namespace
 
-DslashConfig dslash_config(const Dims &d, int prec) {
+DslashConfig dslash_config(const Dims &d, int prec, int nrhs) {
   std::lock_guard<std::mutex> lk(g_tune_mu);
-  auto it = g_tuned.find(TuneKey{d.lx, d.ly, d.lt, d.ls, prec});
-  if (it != g_tuned.end()) return it->second;
+  // a batch without a shape of its own takes the single-RHS one
+  for (int m = nrhs > 1 ? 1 : 0; m >= 0; m--) {
+    auto it = g_tuned.find(TuneKey{d.lx, d.ly, d.lt, d.ls, prec, m});
+    if (it != g_tuned.end()) return it->second;
+  }
   return default_config(d);
 }
 
-void dslash_set_config(const Dims &d, int prec, DslashConfig c) {
+void dslash_set_config(const Dims &d, int prec, DslashConfig c, int which) {
   std::lock_guard<std::mutex> lk(g_tune_mu);
-  g_tuned[TuneKey{d.lx, d.ly, d.lt, d.ls, prec}] = c;
+  for (int m = 0; m < 2; m++)
+    if (which < 0 || which == m) g_tuned[TuneKey{d.lx, d.ly, d.lt, d.ls, prec, m}] = c;
 }
 
 // block -> lattice mapping of a launch with configuration c; *nblocks
@@ -997,7 +1001,7 @@ int launch_dslash_cfg(const DslashLaunch &L, DslashConfig c, hipStream_t s) {
 }
 
 int launch_dslash(const DslashLaunch &L, hipStream_t s) {
-  return launch_dslash_cfg(L, dslash_config(L.d, L.prec), s);
+  return launch_dslash_cfg(L, dslash_config(L.d, L.prec, L.nrhs), s);
 }
 
 int dslash_autotune(const Dims &d, int prec, const void *links, DslashConfig *best,
@@ -1139,7 +1143,7 @@ int dslash_autotune(const Dims &d, int prec, const void *links, DslashConfig *be
   }
   cleanup();
   if (rc) return rc;
-  dslash_set_config(d, prec, pick);
+  dslash_set_config(d, prec, pick, nrhs > 1 ? 1 : 0);
   if (best) *best = pick;
   return THR_OK;
 }

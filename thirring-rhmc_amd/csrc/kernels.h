@@ -93,10 +93,14 @@ long dslash_blocks(const Dims &d);
 // blocks per right-hand side of a launch with configuration c: the partial
 // entries that launch fills with its sums
 long dslash_grid_blocks(const Dims &d, const DslashConfig &c);
-DslashConfig dslash_config(const Dims &d, int prec);
-void dslash_set_config(const Dims &d, int prec, DslashConfig c);
+// the pinned shape for (d, prec) and a launch of nrhs right-hand sides: one
+// shape per batch class (1, or more); a batch without its own takes the
+// single-RHS shape, and without any the default
+DslashConfig dslash_config(const Dims &d, int prec, int nrhs = 1);
+// which: 0 single-RHS class, 1 batches, -1 both
+void dslash_set_config(const Dims &d, int prec, DslashConfig c, int which = -1);
 // times the launch shapes on scratch fields of nrhs right-hand sides and pins
-// the fastest for (d, prec) (for every nrhs)
+// the fastest for (d, prec) in nrhs's batch class
 int dslash_autotune(const Dims &d, int prec, const void *links, DslashConfig *best,
                     hipStream_t s, int nrhs = 1);
 

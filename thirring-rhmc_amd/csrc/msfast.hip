@@ -41,11 +41,7 @@ inline unsigned nblk(long n) { return (unsigned)((n + BLK - 1) / BLK); }
 template <int P> using TT = typename Store<P>::T;
 template <int P> using CC = typename Store<P>::C;
 
-__device__ inline double fin_sum(const double *partial, int n) {
-  double v = 0.0;
-  for (int k = threadIdx.x; k < n; k += BLK) v += partial[k];
-  return block_sum(v);
-}
+__device__ inline double fin_sum(const double *partial, int n) { return partial_sum(partial, n); }
 
 // one element of a source's field: FP64 stores spin planes (element
 // e = spin * V5 + site), FP32 and FP16 the spins of a site together
@@ -642,7 +638,7 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
   DslashConfig cfg[3];
   int nbl[3];
   for (int pr = 0; pr < 3; pr++) {
-    cfg[pr] = dslash_config(d, pr);
+    cfg[pr] = dslash_config(d, pr, ns_);
     nbl[pr] = (int)dslash_grid_blocks(d, cfg[pr]);
     THR_CHECK(nbl[pr] <= nb, THR_ERR_STATE, "dslash grid exceeds partial stride");
   }
@@ -765,11 +761,20 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
   // the Dslash passes derive omega and gamma from each other's partial sums
   // and record alpha and lambda; one finalize per block replays the shift
   // recurrences (2 launches per iteration instead of 4; THR_FUSED=0: a
-  // finalize kernel after every pass).  Results are bit-identical.
+  // finalize kernel after every pass).  Results are bit-identical.  Every
+  // Dslash block then sums the other pass's partials before its sites, which
+  // costs more than the two saved launches once a pass is many blocks: the
+  // passes fuse only up to FUSE_MAX_BLOCKS blocks (all right-hand sides),
+  // decided per precision (THR_FUSED=1 / 0 forces either way).
+  constexpr long FUSE_MAX_BLOCKS = 2048;
   const char *fe = std::getenv("THR_FUSED");
-  const bool fused = !(fe && fe[0] == '0');
+  const int fuse_env = fe ? (fe[0] == '0' ? 0 : 1) : -1;
+  auto fused_for = [&](int pr) {
+    return fuse_env >= 0 ? fuse_env == 1 : (long)nbl[pr] * ns_ <= FUSE_MAX_BLOCKS;
+  };
   hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
   auto enqueue_block = [&](long t0, int pr) -> int {
+    const bool fused = fused_for(pr);
     for (int j = 0; j < MSB; j++) {
       const long tt = t0 + j;
       void *rin = (tt == 0) ? r0[pr] : ring[pr][(tt - 1) % MSB];

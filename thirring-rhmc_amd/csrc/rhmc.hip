@@ -40,9 +40,7 @@ __global__ __launch_bounds__(BLK) void k_redot(long ns, const double2 *a, const 
 }
 
 __global__ __launch_bounds__(BLK) void k_sum(const double *partial, int n, double *out) {
-  double v = 0.0;
-  for (int k = threadIdx.x; k < n; k += BLK) v += partial[k];
-  const double t = block_sum(v);
+  const double t = partial_sum(partial, n);
   if (threadIdx.x == 0) *out = t;
 }
 

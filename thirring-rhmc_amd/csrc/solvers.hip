@@ -150,11 +150,7 @@ void release_workspace() {
 // ---------------------------------------------------------------------------
 namespace {
 
-__device__ inline double fin_sum(const double *partial, int n) {
-  double v = 0.0;
-  for (int k = threadIdx.x; k < n; k += BLK) v += partial[k];
-  return block_sum(v);
-}
+__device__ inline double fin_sum(const double *partial, int n) { return partial_sum(partial, n); }
 
 template <int P> using TT = typename Store<P>::T;
 template <int P> using CC = typename Store<P>::C;

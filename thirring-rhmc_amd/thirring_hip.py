@@ -67,6 +67,7 @@ _SIGS = {
     "thr_dslash_autotune_nrhs": ([P, P, I32, I32, IP, IP, P], I32),
     "thr_dslash_set_config": ([P, I32, I32, I32], I32),
     "thr_dslash_get_config": ([P, I32, IP, IP], I32),
+    "thr_dslash_get_config_nrhs": ([P, I32, I32, IP, IP], I32),
     "thr_links_to_f32_dev": ([P, P, P, P], I32),
     "thr_device_copy": ([P, P, ctypes.c_ulonglong, P], I32),
     "thr_gaussian_dev": ([P, ctypes.c_ulonglong, I32, F64, ctypes.c_ulonglong, P], I32),
@@ -538,12 +539,13 @@ def dslash_set_config(dims, prec, tchunk, flags):
            "dslash_set_config")
 
 
-def dslash_get_config(dims, prec="f64"):
-    """(tchunk, flags) used for launches with these dims and precision."""
+def dslash_get_config(dims, prec="f64", nrhs=1):
+    """(tchunk, flags) used for launches of nrhs right-hand sides with these
+    dims and precision (shapes are pinned per batch class: 1, or more)."""
     init()
     tc, fl = ctypes.c_int(), ctypes.c_int()
-    _check(lib().thr_dslash_get_config(_dims(dims), PREC[prec], ctypes.byref(tc),
-                                       ctypes.byref(fl)), "dslash_get_config")
+    _check(lib().thr_dslash_get_config_nrhs(_dims(dims), PREC[prec], int(nrhs), ctypes.byref(tc),
+                                            ctypes.byref(fl)), "dslash_get_config")
     return tc.value, fl.value
 
 
