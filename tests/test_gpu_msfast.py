@@ -215,12 +215,14 @@ def test_graph_replay_equals_eager(G):
     assert np.array_equal(outs[1][1], outs[2][1])
 
 
-def test_fused_scalars_bitwise(G):
+@pytest.mark.parametrize("msb", ["32", "64"])
+def test_fused_scalars_bitwise(G, msb):
     """The Dslash passes deriving omega and gamma from each other's partial
-    sums, with one finalize per 32-iteration block (THR_FUSED=1; the default
-    on launches of few blocks), give bit-identical solutions and counts to a
-    finalize kernel after every pass (THR_FUSED=0), through FP64, FP32 and FP16 storage, reliable updates and
-    the corrections, for sources that stop in different blocks."""
+    sums, with one finalize per block (THR_FUSED=1; the default on launches
+    of few blocks), give bit-identical solutions and counts to a finalize
+    kernel after every pass (THR_FUSED=0), through FP64, FP32 and FP16
+    storage, reliable updates and the corrections, for sources that stop in
+    different blocks; with 32- and 64-iteration blocks."""
     import os
     from hiprt import DeviceArray
     dims = (4, 4, 6, 8)
@@ -230,6 +232,7 @@ def test_fused_scalars_bitwise(G):
     outs = []
     for flag in ("0", "1"):
         os.environ["THR_FUSED"] = flag
+        os.environ["THR_MSB"] = msb
         try:
             x = DeviceArray((3, n, nsp, 2), np.float64)
             st = {}
@@ -239,13 +242,51 @@ def test_fused_scalars_bitwise(G):
             print("THR_FUSED", flag, "it", it, "cg_return", list(cgr), "corr",
                   st["corr_iterations"], "reliable", st["reliable"], flush=True)
             assert rel.max() <= 1e-10
+            assert it % int(msb) == 0
             outs.append((it, list(cgr), st, x.to_host()))
         finally:
             os.environ.pop("THR_FUSED", None)
+            os.environ.pop("THR_MSB", None)
     assert len(set(outs[0][1])) > 1  # the sources stop at different iterations
     assert outs[0][0] == outs[1][0] and outs[0][1] == outs[1][1]
     assert outs[0][2] == outs[1][2]
     assert np.array_equal(outs[0][3], outs[1][3])
+
+
+@pytest.mark.parametrize("th32,th16", [(0.0, 0.0), (1e-3, 1e-6)])
+def test_block_length_64(G, th32, th16):
+    """64-iteration blocks (THR_MSB=64: one flush launch per block for FP32
+    and FP16 residuals, two for FP64 ones) against 32-iteration blocks on the
+    same 25-pole, 2-source solve, in FP64 throughout and through FP64 -> FP32
+    -> FP16 with reliable updates and corrections: every pole meets the FP64
+    true residual 1e-10 in both, the solutions agree to 1e-8 relative, and
+    the iteration count and precision switches fall on block boundaries."""
+    import os
+    from hiprt import DeviceArray
+    dims = (4, 4, 6, 8)
+    sig = _remez4()
+    th, l64, l32, bh, b = _setup(dims, 11, 2)
+    n, nsp = len(sig), bh.shape[1]
+    xs = {}
+    for msb in (32, 64):
+        os.environ["THR_MSB"] = str(msb)
+        try:
+            x = DeviceArray((2, n, nsp, 2), np.float64)
+            st = {}
+            it, cgr, rel = G.multishift_fast_dev(l64, l32, dims, [0.02, 0.05], 3, sig, b, x,
+                                                 tol=1e-10, maxcg=20000, nsrc=2, th32=th32,
+                                                 th16=th16, stats=st)
+            print("THR_MSB", msb, "it", it, "cg_return", list(cgr), "switches", st["switches"],
+                  "corr", st["corr_iterations"], flush=True)
+            assert rel.max() <= 1e-10
+            assert it % msb == 0
+            assert all(v < 0 or v % msb == 0 for v in st["switches"])
+            xs[msb] = x.to_host().reshape(2, n, -1)
+        finally:
+            os.environ.pop("THR_MSB", None)
+    num = np.linalg.norm(xs[64] - xs[32], axis=2)
+    den = np.linalg.norm(xs[32], axis=2)
+    assert (num / den).max() <= 1e-8
 
 
 def test_batched_multi_rhs_workload_24cubed(G):

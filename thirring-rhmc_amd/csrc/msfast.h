@@ -4,7 +4,10 @@
 
 namespace thr {
 
-constexpr int MSB = 32;  // iterations per shift-update block
+// iterations per shift-update block: at most MSB; a solve runs blocks of 32
+// or 64 (see msfast_solve)
+constexpr int MSB = 64;
+static_assert(MSB % 16 == 0, "coefficient rows are updated in chunks of 16");
 
 struct MsfState {
   double delta, omega, omega_save, minus_omega, gammag, alpha, lambda, bnorm2, tol2;

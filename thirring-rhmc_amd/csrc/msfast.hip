@@ -168,21 +168,23 @@ __device__ bool fin2_step(MsfState *S, double L) {
     const double gi = gam * zt * om / (S->zeta_ii[i] * omega);
     S->gammas[i] = gi;
     // x_i -= omega_i p_i  with p_i = ca P + sum cb R ; then
-    // p_i = gamma_i p_i + zeta_i''' R_j.  The rows are loaded whole: static
-    // indices keep them in registers and make the loads independent
-    double cbr[MSB], cdr[MSB];
-#pragma unroll
-    for (int q = 0; q < MSB; q++) {
-      cbr[q] = S->cb[i][q];
-      cdr[q] = S->cd[i][q];
-    }
+    // p_i = gamma_i p_i + zeta_i''' R_j.  The rows go in chunks of 16 slots:
+    // static indices keep a chunk in registers and make its loads independent
     S->cc[i] -= om * S->ca[i];
     S->ca[i] *= gi;
+    for (int q0 = 0; q0 < j; q0 += 16) {
+      double cbr[16], cdr[16];
 #pragma unroll
-    for (int q = 0; q < MSB; q++) {
-      if (q < j) {
-        S->cd[i][q] = cdr[q] - om * cbr[q];
-        S->cb[i][q] = cbr[q] * gi;
+      for (int u = 0; u < 16; u++) {
+        cbr[u] = S->cb[i][q0 + u];
+        cdr[u] = S->cd[i][q0 + u];
+      }
+#pragma unroll
+      for (int u = 0; u < 16; u++) {
+        if (q0 + u < j) {
+          S->cd[i][q0 + u] = cdr[u] - om * cbr[u];
+          S->cb[i][q0 + u] = cbr[u] * gi;
+        }
       }
     }
     S->cb[i][j] = zt;
@@ -262,26 +264,40 @@ __global__ __launch_bounds__(BLK) void k_msf_fin_block(MsfState *st, const doubl
   }
 }
 
-// apply the block's coefficients to x_i and p_i (FP64), reading each residual once
-template <int P>
+// a held residual in FP64.  For FP32 values the empty asm makes the copy
+// opaque to the compiler, so the widening stays in the shift loop instead of
+// being hoisted out of it as 64 FP64 pairs (twice the registers)
+__device__ inline cx<double> widen(const cx<double> &v) { return v; }
+__device__ inline cx<double> widen(cx<float> v) {
+  asm volatile("" : "+v"(v.r), "+v"(v.i));
+  return {(double)v.r, (double)v.i};
+}
+
+// apply the block's coefficients of ring slots [q0, q0 + QN) to x_i and p_i
+// (FP64), reading each residual once.  A 64-iteration block takes one launch
+// with FP32/FP16 residuals (held as such, widened per use) and two with FP64
+// ones (q0 = 0, 32: the second adds its slots to the x_i, p_i the first
+// wrote).
+template <int P, int QN>
 __global__ __launch_bounds__(BLK) void k_msf_flush(long v5, int n, TT<P> *const *ring,
                                                    float *const *ring_n, const double2 *b,
                                                    double2 *X, double2 *Pv,
-                                                   const MsfState *sts, int final_flush) {
+                                                   const MsfState *sts, int final_flush,
+                                                   int q0) {
   const int src = blockIdx.y;
   const MsfState *S = sts + src;
   const int jc = S->jcount;
-  if (jc == 0) return;
-  __shared__ double sa[MAXSHIFT], sc[MAXSHIFT], sb[MAXSHIFT][MSB], sd[MAXSHIFT][MSB];
+  if (jc <= q0) return;
+  __shared__ double sa[MAXSHIFT], sc[MAXSHIFT], sb[MAXSHIFT][QN], sd[MAXSHIFT][QN];
   __shared__ int stouch[MAXSHIFT], sact[MAXSHIFT];
   for (int k = threadIdx.x; k < n; k += BLK) {
     sa[k] = S->ca[k];
     sc[k] = S->cc[k];
     stouch[k] = S->touched[k];
     sact[k] = S->active[k] && !S->stop && !final_flush;
-    for (int q = 0; q < MSB; q++) {
-      sb[k][q] = S->cb[k][q];
-      sd[k][q] = S->cd[k][q];
+    for (int q = 0; q < QN; q++) {
+      sb[k][q] = S->cb[k][q0 + q];
+      sd[k][q] = S->cd[k][q0 + q];
     }
   }
   __syncthreads();
@@ -296,35 +312,51 @@ __global__ __launch_bounds__(BLK) void k_msf_flush(long v5, int n, TT<P> *const 
   const int spin = (int)(tau & 3);
   const long e = P != P64 ? spin * v5 + site : tau;
   const long off = (long)src * ns;
-  cx<double> R[MSB];
+  // 32 slots are held in FP64; 64 in their storage precision (the widening
+  // to FP64 is exact, so the sums are the same either way)
+  using RT = std::conditional_t<(QN <= 32), double, CC<P>>;
+  cx<RT> R[QN];
 #pragma unroll
-  for (int q = 0; q < MSB; q++) {
-    if (q < jc) {
-      VAcc<P> a{ring[q] + off, ring_n ? ring_n[q] + (long)src * v5 : nullptr, v5};
+  for (int q = 0; q < QN; q++) {
+    if (q0 + q < jc) {
+      VAcc<P> a{ring[q0 + q] + off, ring_n ? ring_n[q0 + q] + (long)src * v5 : nullptr, v5};
       const cx<CC<P>> v = a.ld_at(e, site, spin);
-      R[q] = {(double)v.r, (double)v.i};
+      R[q] = {(RT)v.r, (RT)v.i};
     } else {
-      R[q] = {0.0, 0.0};
+      R[q] = {RT(0), RT(0)};
     }
   }
+  const bool head = q0 == 0;
   const int first = S->first_block;
-  const cx<double> bv = ldc(b, off + e);
+  const cx<double> bv = head && first ? ldc(b, off + e) : cx<double>{0.0, 0.0};
   for (int i = 0; i < n; i++) {
     if (!stouch[i]) continue;
     const long xo = ((long)src * n + i) * ns + e;
-    const cx<double> base = first ? bv : ldc(Pv, xo);
-    cx<double> xs = first ? cx<double>{0.0, 0.0} : ldc(X, xo);
-    xs = xs + sc[i] * base;
-#pragma unroll
-    for (int q = 0; q < MSB; q++)
-      if (q < jc) xs = xs + sd[i][q] * R[q];
-    st(X, xo, xs);
+    cx<double> xs, ps = {0.0, 0.0};
+    if (head) {
+      const cx<double> base = first ? bv : ldc(Pv, xo);
+      xs = first ? cx<double>{0.0, 0.0} : ldc(X, xo);
+      xs = xs + sc[i] * base;
+      if (sact[i]) ps = sa[i] * base;
+    } else {
+      xs = ldc(X, xo);
+      if (sact[i]) ps = ldc(Pv, xo);
+    }
     if (sact[i]) {
-      cx<double> ps = sa[i] * base;
 #pragma unroll
-      for (int q = 0; q < MSB; q++)
-        if (q < jc) ps = ps + sb[i][q] * R[q];
+      for (int q = 0; q < QN; q++)
+        if (q0 + q < jc) {
+          const cx<double> r = widen(R[q]);
+          xs = xs + sd[i][q] * r;
+          ps = ps + sb[i][q] * r;
+        }
+      st(X, xo, xs);
       st(Pv, xo, ps);
+    } else {
+#pragma unroll
+      for (int q = 0; q < QN; q++)
+        if (q0 + q < jc) xs = xs + sd[i][q] * widen(R[q]);
+      st(X, xo, xs);
     }
   }
 }
@@ -568,12 +600,24 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
   if (msfast_persistent_fits(A)) return msfast_persistent(A, res, s);
   const long v5 = d.v5(), nsp = 4 * v5, nsb = nsp * ns_;
   const int nb = (int)dslash_blocks(d);
+  // iterations per block.  Each flush streams x_i and p_i of every shift the
+  // block touched once (4 field streams per shift, against one read of each
+  // residual), so with many shifts 64-iteration blocks cut the flush traffic
+  // per iteration by up to half.  They double the residual ring and may run
+  // up to 63 iterations past a source's convergence: solves with fewer than
+  // 8 shifts or 2^24 spinor components (all sources) keep 32.  THR_MSB=32 or
+  // 64 forces either.
+  int msb = (n >= 8 && nsb >= (1L << 24)) ? 64 : 32;
+  if (const char *me = std::getenv("THR_MSB")) {
+    const int v = std::atoi(me);
+    if (v == 32 || v == 64) msb = v;
+  }
   // workspace slots of the Krylov buffers per precision pr: ring slot q,
   // h, r0, and the FP16 scales
-  auto w_ring = [](int pr, int q) { return 1000 + pr * MSB + q; };
+  auto w_ring = [](int pr, int q) { return 2000 + pr * MSB + q; };
   auto w_h = [](int pr) { return 1100 + pr; };
   auto w_r0 = [](int pr) { return 1110 + pr; };
-  auto w_ringn = [](int q) { return 1200 + q; };
+  auto w_ringn = [](int q) { return 2200 + q; };
   size_t bytes_p[3] = {16, 8, 4};
   void *ring[3][MSB];
   float *ringn[MSB];
@@ -584,14 +628,15 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
     const bool need = (pr == P64 && A.start_prec == P64) || (pr == P32 && A.th32 > 0) ||
                       (pr == P16 && A.th16 > 0);
     for (int q = 0; q < MSB; q++)
-      ring[pr][q] = need ? workspace(w_ring(pr, q), bytes_p[pr] * nsb) : nullptr;
+      ring[pr][q] = need && q < msb ? workspace(w_ring(pr, q), bytes_p[pr] * nsb) : nullptr;
     hbuf[pr] = need ? workspace(w_h(pr), bytes_p[pr] * nsb) : nullptr;
     r0[pr] = need ? workspace(w_r0(pr), bytes_p[pr] * nsb) : nullptr;
     if (need)
       THR_CHECK(hbuf[pr] && r0[pr], THR_ERR_HIP, "workspace allocation failed");
   }
   if (A.th16 > 0) {
-    for (int q = 0; q < MSB; q++) ringn[q] = (float *)workspace(w_ringn(q), sizeof(float) * v5 * ns_);
+    for (int q = 0; q < MSB; q++)
+      ringn[q] = q < msb ? (float *)workspace(w_ringn(q), sizeof(float) * v5 * ns_) : nullptr;
     hn = (float *)workspace(1300, sizeof(float) * v5 * ns_);
     r0n = (float *)workspace(1301, sizeof(float) * v5 * ns_);
   }
@@ -737,7 +782,7 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
     c.in = t1;
     c.out = t2;
     TRY_RC(launch_dslash(c, s));
-    const long slot = (t - 1) % MSB;
+    const long slot = (t - 1) % msb;
     dim3 g(nbr, ns_);
     if (prec == P32)
       hipLaunchKernelGGL(k_msf_reliable<P32>, g, dim3(BLK), 0, s, v5, n, A.b, t2, xg, drel,
@@ -753,7 +798,7 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
     n_reliable++;
     return THR_OK;
   };
-  // one block = MSB iterations + flush + reset; every block after the first
+  // one block = msb iterations + flush + reset; every block after the first
   // issues identical launches, so it can be captured once per precision as a
   // hipGraph and replayed (THR_GRAPHS=0 disables)
   const char *ge = std::getenv("THR_GRAPHS");
@@ -775,12 +820,12 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
   hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
   auto enqueue_block = [&](long t0, int pr) -> int {
     const bool fused = fused_for(pr);
-    for (int j = 0; j < MSB; j++) {
+    for (int j = 0; j < msb; j++) {
       const long tt = t0 + j;
-      void *rin = (tt == 0) ? r0[pr] : ring[pr][(tt - 1) % MSB];
-      float *rinn = (pr == P16) ? ((tt == 0) ? r0n : ringn[(tt - 1) % MSB]) : nullptr;
-      void *rout = ring[pr][tt % MSB];
-      float *routn = (pr == P16) ? ringn[tt % MSB] : nullptr;
+      void *rin = (tt == 0) ? r0[pr] : ring[pr][(tt - 1) % msb];
+      float *rinn = (pr == P16) ? ((tt == 0) ? r0n : ringn[(tt - 1) % msb]) : nullptr;
+      void *rout = ring[pr][tt % msb];
+      float *routn = (pr == P16) ? ringn[tt % msb] : nullptr;
       // h = M p ; alpha = |h|^2.  p = r + gamma p is never formed: h obeys
       // h_new = M r_new + gamma h_old (M is linear), so the pass reads r and
       // the old h and writes h in place
@@ -841,20 +886,31 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
     }
     if (fused)
       hipLaunchKernelGGL(k_msf_fin_block, dim3(ns_), dim3(BLK), 0, s, st, aden, lpart, nb, nbl[pr],
-                         MSB);
+                         msb);
     // flush the block into x_i, p_i
     dim3 gf(nblk(nsp), ns_);
     void *const *rp = (void *const *)ringptr + pr * MSB;
     float *const *rnp = (float *const *)((void **)ringptr + 3 * MSB);
-    if (pr == P64)
-      hipLaunchKernelGGL(k_msf_flush<P64>, gf, dim3(BLK), 0, s, v5, n, (double2 *const *)rp,
-                         nullptr, A.b, A.x, Pv, st, 0);
-    else if (pr == P32)
-      hipLaunchKernelGGL(k_msf_flush<P32>, gf, dim3(BLK), 0, s, v5, n, (float2 *const *)rp,
-                         nullptr, A.b, A.x, Pv, st, 0);
-    else
-      hipLaunchKernelGGL(k_msf_flush<P16>, gf, dim3(BLK), 0, s, v5, n, (__half2 *const *)rp, rnp,
-                         A.b, A.x, Pv, st, 0);
+    if (pr == P64) {
+      // FP64 residuals: 32 slots per launch
+      for (int q0 = 0; q0 < msb; q0 += 32)
+        hipLaunchKernelGGL((k_msf_flush<P64, 32>), gf, dim3(BLK), 0, s, v5, n,
+                           (double2 *const *)rp, nullptr, A.b, A.x, Pv, st, 0, q0);
+    } else if (pr == P32) {
+      if (msb == 64)
+        hipLaunchKernelGGL((k_msf_flush<P32, 64>), gf, dim3(BLK), 0, s, v5, n,
+                           (float2 *const *)rp, nullptr, A.b, A.x, Pv, st, 0, 0);
+      else
+        hipLaunchKernelGGL((k_msf_flush<P32, 32>), gf, dim3(BLK), 0, s, v5, n,
+                           (float2 *const *)rp, nullptr, A.b, A.x, Pv, st, 0, 0);
+    } else {
+      if (msb == 64)
+        hipLaunchKernelGGL((k_msf_flush<P16, 64>), gf, dim3(BLK), 0, s, v5, n,
+                           (__half2 *const *)rp, rnp, A.b, A.x, Pv, st, 0, 0);
+      else
+        hipLaunchKernelGGL((k_msf_flush<P16, 32>), gf, dim3(BLK), 0, s, v5, n,
+                           (__half2 *const *)rp, rnp, A.b, A.x, Pv, st, 0, 0);
+    }
     hipLaunchKernelGGL(k_msf_reset, dim3(ns_), dim3(MAXSHIFT), 0, s, st);
     THR_HIP(hipGetLastError());
     return THR_OK;
@@ -907,7 +963,7 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
         return rc;
       }
     }
-    t += MSB;
+    t += msb;
     blocks++;
     if (hipMemcpyAsync(hs, st, sizeof(MsfState) * ns_, hipMemcpyDeviceToHost, s) !=
             hipSuccess ||
@@ -931,7 +987,7 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
       if (lo >= 0) worst_lo = std::max(worst_lo, std::fabs(hs[k].zeta_ii[lo]) * rb);
     }
     if (all) break;
-    if (t > (long)A.maxcg + 2 * MSB) {
+    if (t > (long)A.maxcg + 2 * msb) {
       free_graphs();
       set_error("multishift: stop flag not raised");
       return THR_ERR_STATE;
@@ -954,10 +1010,10 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
     if (A.th32 > 0 && worst < A.th32 && prec == P64) want = P32;
     if (A.th16 > 0 && worst < A.th16 && prec != P16) want = P16;
     if (want != prec) {
-      void *rl = ring[prec][(t - 1) % MSB];
-      float *rln = prec == P16 ? ringn[(t - 1) % MSB] : nullptr;
-      int rcc = conv(d, ns_, prec, rl, rln, want, ring[want][(t - 1) % MSB],
-                     want == P16 ? ringn[(t - 1) % MSB] : nullptr, s);
+      void *rl = ring[prec][(t - 1) % msb];
+      float *rln = prec == P16 ? ringn[(t - 1) % msb] : nullptr;
+      int rcc = conv(d, ns_, prec, rl, rln, want, ring[want][(t - 1) % msb],
+                     want == P16 ? ringn[(t - 1) % msb] : nullptr, s);
       if (!rcc)
         rcc = conv(d, ns_, prec, hbuf[prec], prec == P16 ? hn : nullptr, want, hbuf[want],
                    want == P16 ? hn : nullptr, s);
