@@ -285,6 +285,10 @@ int thr_dslash_dev(void *out, float *out_norm, const void *in, const float *in_n
  *   THR_SHAPE_NT_STORES      non-temporal output stores
  *   k << THR_SHAPE_OCC_SHIFT occupancy cap, k = 1..3: at most k + 1 resident
  *                            blocks per CU (LDS padding; 0 = no cap)
+ *   THR_SHAPE_LDS_Y          with THR_SHAPE_LDS_X at Ls = 64, Lx % 4 == 0 and
+ *                            Ly % 2 == 0 (per XCD box with boxes): blocks of
+ *                            4 x by 2 y s-lines (512 threads), the inner y
+ *                            neighbours also through LDS
  * No counterpart in the reference (a tuning knob of this library).
  * thr_dslash_autotune times the candidates (tchunk 1..16 x flags) on scratch
  * fields with the given links (complex128 for F64, complex64 otherwise) and
@@ -298,6 +302,7 @@ int thr_dslash_dev(void *out, float *out_norm, const void *in, const float *in_n
 #define THR_SHAPE_LDS_X 16
 #define THR_SHAPE_NT_STORES 32
 #define THR_SHAPE_OCC_SHIFT 6
+#define THR_SHAPE_LDS_Y 256
 int thr_dslash_autotune(const void *links, const int dims[4], int prec, int *tchunk, int *flags,
                         void *stream);
 /* thr_dslash_autotune timing launches over nrhs right-hand sides (1..32), for

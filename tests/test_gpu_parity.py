@@ -313,7 +313,8 @@ def test_device_precisions(G):
 def test_dslash_launch_shapes(G, dims):
     """Every launch shape (t-chunk marched per thread, XCD remap or XCD boxes,
     scalar links and LDS x exchange when Ls = 64, 32 or 16 (one, two or four
-    s-lines per wave), lane-read walls when Ls = 64; s hops by
+    s-lines per wave), with Ls = 64 also blocks of 4 x by 2 y lines whose
+    inner y neighbours come through LDS, lane-read walls when Ls = 64; s hops by
     lane shifts, with loads at the wave edges when an s-line crosses waves
     (Ls = 12, 48), or by loads; resident blocks capped per CU) gives the same
     operator,
@@ -346,9 +347,11 @@ def test_dslash_launch_shapes(G, dims):
                     x = DeviceArray.from_host(np.ascontiguousarray(lay(xh).astype(dt)))
                     first = None
                     for tc in sorted({1, 2, 3, 4, lt}):
-                        # every flag set, and occupancy caps (LDS padding) on three of them
+                        # every flag set, occupancy caps (LDS padding) on three of them,
+                        # and the 4 x 2-line blocks (y exchange) over the tilings
                         caps = [f | (k << 6) for f in (0, 7, 55) for k in (1, 2, 3)]
-                        for flags in list(range(64)) + caps:
+                        ylds = [f | 256 for f in (22, 23, 30, 54, 55, 62, 63)] + [55 | 256 | 192]
+                        for flags in list(range(64)) + caps + ylds:
                             G.dslash_set_config(dims, prec, min(tc, lt), flags)
                             out = DeviceArray((4 * v5, 2), dt)
                             G.dslash_dev(out, x, links, dims, 0.05, imass=imass, dagger=dagger,

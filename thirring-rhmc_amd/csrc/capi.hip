@@ -755,7 +755,7 @@ static int shape_flags(const DslashConfig &c) {
   return (c.remap ? THR_SHAPE_XCD_REMAP : 0) | (c.uniform_links ? THR_SHAPE_UNIFORM_LINKS : 0) |
          (c.laneshift ? THR_SHAPE_LANESHIFT : 0) | (c.box ? THR_SHAPE_XCD_BOXES : 0) |
          (c.lds_x ? THR_SHAPE_LDS_X : 0) | (c.nt_stores ? THR_SHAPE_NT_STORES : 0) |
-         ((c.occ_cap & 3) << THR_SHAPE_OCC_SHIFT);
+         ((c.occ_cap & 3) << THR_SHAPE_OCC_SHIFT) | (c.lds_y ? THR_SHAPE_LDS_Y : 0);
 }
 
 int thr_dslash_autotune(const void *links, const int dims[4], int prec, int *tchunk, int *flags,
@@ -787,7 +787,7 @@ int thr_dslash_set_config(const int dims[4], int prec, int tchunk, int flags) {
   TRY(check_dims(dims));
   THR_CHECK(prec >= 0 && prec <= 2, THR_ERR_ARG, "prec must be THR_F64, THR_F32 or THR_F16");
   THR_CHECK(tchunk >= 1 && tchunk <= dims[2], THR_ERR_ARG, "tchunk must be in 1..Lt");
-  THR_CHECK((flags & ~255) == 0, THR_ERR_ARG, "unknown shape flags");
+  THR_CHECK((flags & ~511) == 0, THR_ERR_ARG, "unknown shape flags");
   DslashConfig c;
   c.tchunk = tchunk;
   c.remap = (flags & THR_SHAPE_XCD_REMAP) ? 1 : 0;
@@ -797,6 +797,7 @@ int thr_dslash_set_config(const int dims[4], int prec, int tchunk, int flags) {
   c.lds_x = (flags & THR_SHAPE_LDS_X) ? 1 : 0;
   c.nt_stores = (flags & THR_SHAPE_NT_STORES) ? 1 : 0;
   c.occ_cap = (flags >> THR_SHAPE_OCC_SHIFT) & 3;
+  c.lds_y = (flags & THR_SHAPE_LDS_Y) ? 1 : 0;
   dslash_set_config(mkdims(dims), prec, c);
   return THR_OK;
 }

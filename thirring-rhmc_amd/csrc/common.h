@@ -119,28 +119,29 @@ __device__ inline cx<float> ldh(const __half2 *p, long i, float scale) {
 
 constexpr int BLK = 256;  // threads per block for every streaming kernel
 
-// deterministic block reduction: wave shuffles, then a fixed-order LDS sum;
-// the result is valid in thread 0
-__device__ inline double block_sum(double v) {
+// deterministic block reduction over NTH threads: wave shuffles, then a
+// fixed-order LDS sum; the result is valid in thread 0
+template <int NTH>
+__device__ inline double block_sum_n(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  __shared__ double sm[BLK / 64];
+  __shared__ double sm[NTH / 64];
   if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
   __syncthreads();
   double t = 0.0;
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int k = 0; k < BLK / 64; k++) t += sm[k];
+    for (int k = 0; k < NTH / 64; k++) t += sm[k];
   }
   __syncthreads();
   return t;
 }
+__device__ inline double block_sum(double v) { return block_sum_n<BLK>(v); }
 
-// sum of n per-block partial sums: thread k adds p[k], p[k + BLK], ... in
-// that order, then block_sum (the result, valid in thread 0, depends only on
-// the values and n).  The loads are issued 8 at a time before they are added,
+// thread k (< BLK) of a sum of n per-block partial sums: p[k] + p[k + BLK] +
+// ... in that order.  The loads are issued 8 at a time before they are added,
 // so a long row costs a few memory latencies instead of one per element.
-__device__ inline double partial_sum(const double *p, int n) {
+__device__ inline double partial_thread(const double *p, int n) {
   constexpr int U = 8;
   double v = 0.0;
   for (int k0 = threadIdx.x; k0 < n; k0 += U * BLK) {
@@ -154,7 +155,13 @@ __device__ inline double partial_sum(const double *p, int n) {
     for (int u = 0; u < U; u++)
       if (k0 + u * BLK < n) v += a[u];
   }
-  return block_sum(v);
+  return v;
+}
+
+// sum of n per-block partial sums by BLK threads (partial_thread, then
+// block_sum): the result, valid in thread 0, depends only on the values and n
+__device__ inline double partial_sum(const double *p, int n) {
+  return block_sum(partial_thread(p, n));
 }
 
 }  // namespace thr

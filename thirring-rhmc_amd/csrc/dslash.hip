@@ -390,9 +390,9 @@ template <int PREC> __device__ inline Spin<PREC> lane_spin(const Site<PREC> &S, 
 
 // x-neighbour exchange through LDS (one buffer per step parity): FP64/FP32
 // sites as re/im values, FP16 sites as their 4 half2 words and scale
-template <int PREC> struct XBuf {
+template <int PREC, int NW = 4> struct XBuf {
   using C = typename Store<PREC>::C;
-  C (*b)[4][4][64][2];
+  C (*b)[NW][4][64][2];
   __device__ inline void put(unsigned pb, unsigned w, unsigned ln, const Site<PREC> &S) const {
 #pragma unroll
     for (int a = 0; a < 4; a++) {
@@ -407,8 +407,8 @@ template <int PREC> struct XBuf {
     return S;
   }
 };
-template <> struct XBuf<P16> {
-  unsigned (*b)[4][5][64];
+template <int NW> struct XBuf<P16, NW> {
+  unsigned (*b)[NW][5][64];
   __device__ inline void put(unsigned pb, unsigned w, unsigned ln, const Site<P16> &S) const {
 #pragma unroll
     for (int a = 0; a < 4; a++) b[pb][w][a][ln] = S.h[a];
@@ -442,10 +442,15 @@ template <> struct XBuf<P16> {
 //           are fetched across XCDs.
 // Partial sums go to partial[rhs * pstride + blockIdx.x]; block 0 zero-fills
 // [gridDim.x, pstride) so consumers can always sum pstride entries.
-template <int PREC, bool DAG, bool SHUF, int MODE, int UL, bool XL, bool NT>
-__global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, const Tiling TL) {
+template <int PREC, bool DAG, bool SHUF, int MODE, int UL, bool XL, bool NT, bool YL = false>
+__global__ __launch_bounds__(YL ? 2 * BLK : BLK) void k_dslash(const DslashLaunch L,
+                                                               const Tiling TL) {
   using C = typename Store<PREC>::C;
   using T = typename Store<PREC>::T;
+  // YL (Ls = 64 with the LDS x exchange): a block is 8 waves, 4 consecutive
+  // x by 2 consecutive y, wave w at x0 + (w & 3), y0 + (w >> 2)
+  constexpr int NTH = YL ? 2 * BLK : BLK;
+  static_assert(!YL || (UL == 1 && XL), "the y exchange extends the Ls = 64 x exchange");
   const int rhs = blockIdx.y;
   if (L.done && L.done[(long)rhs * L.done_stride]) return;
   // the previous pass's partial sums of this right-hand side (the finalize
@@ -454,7 +459,10 @@ __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, const Tili
   if constexpr (MODE == EPI_CG || MODE == EPI_HREC) {
     if (L.fsum) {
       __shared__ double s_fs;
-      const double v = partial_sum(L.fsum + (long)rhs * L.fsum_stride, L.fsum_n);
+      // the first BLK threads sum as the finalize kernels do (the other
+      // waves add exact zeros after their sums)
+      const double v = block_sum_n<NTH>(
+          threadIdx.x < BLK ? partial_thread(L.fsum + (long)rhs * L.fsum_stride, L.fsum_n) : 0.0);
       if (threadIdx.x == 0) {
         s_fs = v;
         if (blockIdx.x == 0) L.frec[(long)rhs * L.sc_stride] = v;
@@ -472,14 +480,22 @@ __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, const Tili
     const unsigned k = blockIdx.x & 7, l = blockIdx.x >> 3;
     const unsigned rx = k % TL.px, ry = (k / TL.px) % TL.py, rt = k / (TL.px * TL.py);
     const unsigned chunk = l / TL.nb3;
-    const unsigned jr = (l - chunk * TL.nb3) * BLK + threadIdx.x;
     t0 = rt * TL.bt + chunk * TL.tchunk;
     t1 = min(t0 + TL.tchunk, rt * TL.bt + TL.bt);
-    valid = jr < ls * TL.bx * TL.by && chunk * TL.tchunk < TL.bt;
-    s = jr % ls;
-    const unsigned q = jr / ls;
-    x = rx * TL.bx + q % TL.bx;
-    y = ry * TL.by + q / TL.bx;
+    if constexpr (YL) {
+      const unsigned tb = l - chunk * TL.nb3, w = threadIdx.x >> 6;
+      valid = chunk * TL.tchunk < TL.bt;
+      s = threadIdx.x & 63;
+      x = rx * TL.bx + 4 * (tb % TL.ntx) + (w & 3);
+      y = ry * TL.by + 2 * (tb / TL.ntx) + (w >> 2);
+    } else {
+      const unsigned jr = (l - chunk * TL.nb3) * BLK + threadIdx.x;
+      valid = jr < ls * TL.bx * TL.by && chunk * TL.tchunk < TL.bt;
+      s = jr % ls;
+      const unsigned q = jr / ls;
+      x = rx * TL.bx + q % TL.bx;
+      y = ry * TL.by + q / TL.bx;
+    }
     j = s + ls * (x + lx * y);
   } else {
     unsigned b = blockIdx.x;
@@ -489,14 +505,23 @@ __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, const Tili
       if (b < full) b = (b & 7) * per + (b >> 3);
     }
     const unsigned chunk = b / TL.nb3;
-    j = (b - chunk * TL.nb3) * BLK + threadIdx.x;
     t0 = chunk * TL.tchunk;
     t1 = min(t0 + TL.tchunk, lt);
-    valid = j < v3 && t0 < lt;
-    s = j % ls;
-    const unsigned xy = j / ls;
-    x = xy % lx;
-    y = xy / lx;
+    if constexpr (YL) {
+      const unsigned tb = b - chunk * TL.nb3, w = threadIdx.x >> 6;
+      s = threadIdx.x & 63;
+      x = 4 * (tb % TL.ntx) + (w & 3);
+      y = 2 * (tb / TL.ntx) + (w >> 2);
+      j = s + ls * (x + lx * y);
+      valid = t0 < lt;
+    } else {
+      j = (b - chunk * TL.nb3) * BLK + threadIdx.x;
+      valid = j < v3 && t0 < lt;
+      s = j % ls;
+      const unsigned xy = j / ls;
+      x = xy % lx;
+      y = xy / lx;
+    }
   }
   double red = 0.0;
   if (valid) {
@@ -549,15 +574,17 @@ __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, const Tili
       const unsigned l4 = v2 * t;
       cx<C> o[4];
       centre(o, diag, cur);
+      Site<PREC> yf, yb;  // the y neighbours
       if constexpr (XL) {
         // the block's 4 waves are 4 consecutive x at one (y, t) (8 when
         // Ls = 32): the x neighbours inside the block are the centre
         // registers of the next/previous line, exchanged through LDS
         // (double-buffered by step parity, one barrier per step); the outer
         // lines load theirs
-        using XB = std::conditional_t<PREC == P16, unsigned[2][4][5][64], C[2][4][4][64][2]>;
+        constexpr int NW = YL ? 8 : 4;
+        using XB = std::conditional_t<PREC == P16, unsigned[2][NW][5][64], C[2][NW][4][64][2]>;
         __shared__ XB xraw;
-        const XBuf<PREC> xbuf{xraw};
+        const XBuf<PREC, NW> xbuf{xraw};
         const unsigned w = threadIdx.x >> 6, ln = threadIdx.x & 63, pb = (t - t0) & 1;
         xbuf.put(pb, w, ln, cur);
         __syncthreads();
@@ -578,26 +605,36 @@ __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, const Tili
             rb = load_site(in, jxb + v3 * t);
           }
         } else {
-          if (w < 3) {
+          // (with YL the rows are waves 0-3 and 4-7; the y neighbour of
+          // either row in the other comes from LDS, the outer one is loaded)
+          const unsigned wx = w & 3;
+          if (wx < 3) {
             rf = xbuf.get(pb, w + 1, ln);
           } else {
             rf = load_site(in, jxf + v3 * t);
           }
-          if (w > 0) {
+          if (wx > 0) {
             rb = xbuf.get(pb, w - 1, ln);
           } else {
             rb = load_site(in, jxb + v3 * t);
           }
         }
         hop<0, DAG>(o, lk(xy + l4), rf, conj(lk(lxb + l4)), rb);
+        if constexpr (YL) {
+          // after the x hop (its neighbours are dead): the same LDS buffer
+          // holds the other row's centres until the next step's barrier
+          yf = w < 4 ? xbuf.get(pb, w + 4, ln) : load_site(in, jyf + v3 * t);
+          yb = w >= 4 ? xbuf.get(pb, w - 4, ln) : load_site(in, jyb + v3 * t);
+        }
       } else {
         const Site<PREC> rf = load_site(in, jxf + v3 * t), rb = load_site(in, jxb + v3 * t);
         hop<0, DAG>(o, lk(xy + l4), rf, conj(lk(lxb + l4)), rb);
       }
-      {
-        const Site<PREC> rf = load_site(in, jyf + v3 * t), rb = load_site(in, jyb + v3 * t);
-        hop<1, DAG>(o, lk(v4 + xy + l4), rf, conj(lk(v4 + lyb + l4)), rb);
+      if constexpr (!YL) {
+        yf = load_site(in, jyf + v3 * t);
+        yb = load_site(in, jyb + v3 * t);
       }
+      hop<1, DAG>(o, lk(v4 + xy + l4), yf, conj(lk(v4 + lyb + l4)), yb);
       hop<2, DAG>(o, uft, nxt, ubt, prv);
 
       // hops along s (dirac.F90:80-87 for M, 317-324 for M^dagger): spins that
@@ -739,11 +776,11 @@ __global__ __launch_bounds__(BLK) void k_dslash(const DslashLaunch L, const Tili
     }
   }
   if constexpr (MODE != EPI_STORE) {
-    const double tot = block_sum(red);
+    const double tot = block_sum_n<NTH>(red);
     double *P = L.partial + (long)rhs * TL.pstride;
     if (threadIdx.x == 0) P[blockIdx.x] = tot;
     if (blockIdx.x == 0)
-      for (unsigned k = gridDim.x + threadIdx.x; k < TL.pstride; k += BLK) P[k] = 0.0;
+      for (unsigned k = gridDim.x + threadIdx.x; k < TL.pstride; k += NTH) P[k] = 0.0;
   }
 }
 
@@ -822,6 +859,7 @@ DslashConfig default_config(const Dims &d) {
   if (const char *e = std::getenv("THR_DSLASH_LANESHIFT")) c.laneshift = std::atoi(e) ? 1 : 0;
   if (const char *e = std::getenv("THR_DSLASH_BOXES")) c.box = std::atoi(e) ? 1 : 0;
   if (const char *e = std::getenv("THR_DSLASH_LDSX")) c.lds_x = std::atoi(e) ? 1 : 0;
+  if (const char *e = std::getenv("THR_DSLASH_LDSY")) c.lds_y = std::atoi(e) ? 1 : 0;
   if (const char *e = std::getenv("THR_DSLASH_NT")) c.nt_stores = std::atoi(e) ? 1 : 0;
   if (const char *e = std::getenv("THR_DSLASH_OCC")) c.occ_cap = std::max(0, std::min(3, std::atoi(e)));
   return c;
@@ -849,6 +887,9 @@ void dslash_set_config(const Dims &d, int prec, DslashConfig c, int which) {
 static Tiling make_tiling(const Dims &d, const DslashConfig &c, unsigned *nblocks) {
   Tiling TL{};
   TL.pstride = (unsigned)dslash_blocks(d);
+  // 4 x 2-line blocks: the Ls = 64 LDS x exchange (scalar links, lane shifts)
+  // on a slab or box of whole tiles
+  const bool ylc = c.lds_y && c.lds_x && c.uniform_links && c.laneshift && d.ls == 64;
   unsigned px = 1, py = 1, pt = 8;
   if (c.box && box_split(d, px, py, pt)) {
     TL.mode = TILE_BOXES;
@@ -858,12 +899,16 @@ static Tiling make_tiling(const Dims &d, const DslashConfig &c, unsigned *nblock
     TL.by = d.ly / py;
     TL.bt = d.lt / pt;
     TL.tchunk = (unsigned)std::max(1, std::min<int>(c.tchunk, (int)TL.bt));
-    TL.nb3 = ((unsigned)d.ls * TL.bx * TL.by + BLK - 1) / BLK;
+    TL.yl = ylc && TL.bx % 4 == 0 && TL.by % 2 == 0;
+    TL.ntx = TL.bx / 4;
+    TL.nb3 = TL.yl ? TL.ntx * (TL.by / 2) : ((unsigned)d.ls * TL.bx * TL.by + BLK - 1) / BLK;
     *nblocks = 8 * TL.nb3 * ((TL.bt + TL.tchunk - 1) / TL.tchunk);
   } else {
     TL.mode = c.remap ? TILE_REMAP : TILE_LINEAR;
     TL.tchunk = (unsigned)std::max(1, std::min(c.tchunk, d.lt));
-    TL.nb3 = ((unsigned)d.lx * d.ly * d.ls + BLK - 1) / BLK;
+    TL.yl = ylc && d.lx % 4 == 0 && d.ly % 2 == 0;
+    TL.ntx = d.lx / 4;
+    TL.nb3 = TL.yl ? TL.ntx * (d.ly / 2) : ((unsigned)d.lx * d.ly * d.ls + BLK - 1) / BLK;
     *nblocks = TL.nb3 * ((d.lt + TL.tchunk - 1) / TL.tchunk);
   }
   return TL;
@@ -926,7 +971,7 @@ int launch_dslash_cfg(const DslashLaunch &L, DslashConfig c, hipStream_t s) {
   const Tiling TL = make_tiling(L.d, c, &nblocks);
   THR_CHECK(nblocks <= TL.pstride, THR_ERR_ARG, "dslash grid exceeds partial stride");
   dim3 grid(nblocks, (unsigned)L.nrhs);
-  dim3 block(BLK);
+  dim3 block(TL.yl ? 2 * BLK : BLK);
   const bool shuf = c.laneshift != 0;
   // a wave is exactly one s-line when Ls = 64, two when Ls = 32, four when
   // Ls = 16 (blocks are whole waves and every tiling starts a wave at s = 0)
@@ -943,6 +988,7 @@ int launch_dslash_cfg(const DslashLaunch &L, DslashConfig c, hipStream_t s) {
   const bool xl4 = ul4 && c.lds_x && L.d.lx % 16 == 0 &&
                    (TL.mode != TILE_BOXES || TL.bx % 16 == 0);
   const bool nt = c.nt_stores != 0;
+  THR_CHECK(!TL.yl || xl, THR_ERR_STATE, "y exchange without the x exchange");
   THR_CHECK(4 * L.d.v5() * Store<P64>::bytes < (1L << 32), THR_ERR_SIZE,
             "spinor too large for 32-bit buffer offsets");
 #define THR_GO(K)                                                                       \
@@ -952,7 +998,9 @@ int launch_dslash_cfg(const DslashLaunch &L, DslashConfig c, hipStream_t s) {
   } while (0)
 #define THR_LAUNCH_NT(P, D, S, M, NTV)                                 \
   do {                                                                 \
-    if (xl)                                                            \
+    if (TL.yl)                                                         \
+      THR_GO((k_dslash<P, D, true, M, 1, true, NTV, true>));           \
+    else if (xl)                                                       \
       THR_GO((k_dslash<P, D, true, M, 1, true, NTV>));                 \
     else if (ul)                                                       \
       THR_GO((k_dslash<P, D, true, M, 1, false, NTV>));                \
@@ -1092,11 +1140,13 @@ int dslash_autotune(const Dims &d, int prec, const void *links, DslashConfig *be
   tcs.erase(std::unique(tcs.begin(), tcs.end()), tcs.end());
   for (int tc : tcs) {
     if (tc > d.lt || tc < 1) continue;
-    for (int v = 0; v < 64 && !rc; v++) {
+    for (int v = 0; v < 128 && !rc; v++) {
       if ((v & 2) && (!(v & 4) || (d.ls != 64 && d.ls != 32 && d.ls != 16)))
         continue;  // uniform links need Ls = 64, 32 or 16
       if ((v & 16) && !(v & 2)) continue;  // the LDS x exchange needs uniform links
       if ((v & 8) && (v & 1)) continue;            // boxes replace the remap
+      // the y exchange extends the Ls = 64 x exchange (and must apply)
+      if ((v & 64) && (!(v & 16) || d.ls != 64 || d.lx % 4 || d.ly % 2)) continue;
       DslashConfig c;
       c.tchunk = tc;
       c.remap = v & 1;
@@ -1105,6 +1155,11 @@ int dslash_autotune(const Dims &d, int prec, const void *links, DslashConfig *be
       c.box = (v >> 3) & 1;
       c.lds_x = (v >> 4) & 1;
       c.nt_stores = (v >> 5) & 1;
+      c.lds_y = (v >> 6) & 1;
+      if (c.lds_y) {
+        unsigned nbq = 0;
+        if (!make_tiling(d, c, &nbq).yl) continue;  // boxes whose extents do not tile
+      }
       for (int k = 0; k < 2 && !rc; k++) rc = launch(c);  // warm
       if (rc) break;
       float ms = 0.f;

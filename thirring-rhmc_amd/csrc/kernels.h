@@ -71,6 +71,8 @@ struct DslashConfig {
                           // Ls does not divide 64; default on only when it does)
   int box = 1;            // XCD boxes: each XCD owns one box of an (x, y, t) split
   int lds_x = 1;          // with uniform links: inner x neighbours through LDS
+  int lds_y = 0;          // Ls = 64 with the LDS x exchange: blocks of 4 x by 2 y lines
+                          // (512 threads), the inner y neighbours also through LDS
   int nt_stores = 0;      // non-temporal output stores
   int occ_cap = 0;        // 0: no cap; k = 1..3: at most k + 1 blocks per CU (k + 1 waves per
                           // SIMD for 4-wave blocks), by padding each block's LDS
@@ -83,6 +85,7 @@ struct Tiling {
   unsigned tchunk, nb3, pstride;
   unsigned px, py;      // boxes: split of x and y (t split = 8 / (px py))
   unsigned bx, by, bt;  // boxes: box extents
+  unsigned yl, ntx;     // 4 x 2-line blocks (lds_y); tiles per row of the slab or box
 };
 
 int launch_dslash(const DslashLaunch &L, hipStream_t s);
