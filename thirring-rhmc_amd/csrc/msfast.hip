@@ -605,9 +605,9 @@ int msfast_solve(const MsfArgs &A, MsfResult *res, hipStream_t s) {
   // residual), so with many shifts 64-iteration blocks cut the flush traffic
   // per iteration by up to half.  They double the residual ring and may run
   // up to 63 iterations past a source's convergence: solves with fewer than
-  // 8 shifts or 2^24 spinor components (all sources) keep 32.  THR_MSB=32 or
-  // 64 forces either.
-  int msb = (n >= 8 && nsb >= (1L << 24)) ? 64 : 32;
+  // 8 shifts or 2^23 spinor components (all sources; 32^3 x 64 is one
+  // source) keep 32.  THR_MSB=32 or 64 forces either.
+  int msb = (n >= 8 && nsb >= (1L << 23)) ? 64 : 32;
   if (const char *me = std::getenv("THR_MSB")) {
     const int v = std::atoi(me);
     if (v == 32 || v == 64) msb = v;
