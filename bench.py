@@ -651,6 +651,9 @@ def main():
     ap.add_argument("--shapes", default="",
                     help="pin per-precision shapes instead of autotuning: "
                          "prec:tchunk:flags,... (e.g. f64:16:55,f32:2:55,f16:2:39)")
+    ap.add_argument("--solver-shapes", default="",
+                    help="pin the solver lattice's shapes (prec:tchunk:flags,...) instead of "
+                         "autotuning them for the solver's batch")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run N device copies of 1 GiB (scales the HBM counters)")
     args = ap.parse_args()
@@ -722,8 +725,14 @@ def main():
         # the solver launches every pass over all its sources: pin the shapes
         # tuned for that batch
         batch_shapes = {}
+        spins = {}
+        for item in filter(None, args.solver_shapes.split(",")):
+            p, tc, fl = item.split(":")
+            spins[p] = (int(tc), int(fl))
         for prec, lk in (("f64", s64), ("f32", s32), ("f16", s32)):
-            if prec in pins:
+            if prec in spins:
+                G.dslash_set_config(sd, prec, *spins[prec])
+            elif prec in pins:
                 G.dslash_set_config(sd, prec, *pins[prec])
             elif args.tchunk > 0:
                 G.dslash_set_config(sd, prec, args.tchunk, args.shape_flags)
