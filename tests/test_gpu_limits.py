@@ -48,7 +48,14 @@ def test_autotune_config_limits(G):
     dims = (4, 4, 4, 8)
     assert _status(G, G.dslash_set_config, dims, "f64", 0, 0) == 1
     assert _status(G, G.dslash_set_config, dims, "f64", 9, 0) == 1
-    assert _status(G, G.dslash_set_config, dims, "f64", 1, 256) == 1
+    assert _status(G, G.dslash_set_config, dims, "f64", 1, 512) == 1
+    # every defined flag (the y exchange is 256) is accepted and read back
+    saved = G.dslash_get_config(dims, "f64")
+    try:
+        G.dslash_set_config(dims, "f64", 1, 256 | 22)
+        assert G.dslash_get_config(dims, "f64") == (1, 278)
+    finally:
+        G.dslash_set_config(dims, "f64", *saved)
     from hiprt import DeviceArray
     buf = DeviceArray((3 * 4 * 4 * 4, 2), np.float64)
     assert _status(G, G.dslash_autotune, buf, dims, "f64", None, 0) == 1
