@@ -564,6 +564,18 @@ def measured_traffic():
     return rec.get("hbm_bytes_per_launch"), rec.get("source")
 
 
+def traffic_shape():
+    """(tchunk, flags) of the FP64 launch shape the counter summary profiled."""
+    tpath = os.path.join(ROOT, "profiles", "dslash_f64_32x32x32x64_traffic.json")
+    if not os.path.exists(tpath):
+        return None
+    with open(tpath) as fh:
+        kernel = json.load(fh).get("kernel", "")
+    import re
+    m = re.search(r"f64:(\d+):(\d+)", kernel)
+    return (int(m.group(1)), int(m.group(2))) if m else None
+
+
 def copy_rate(G, hr, stream, nbytes=1 << 30, reps=20):
     """Streaming device-to-device copy rate (bytes read + written per second)
     on this GPU, with the library's 16 B/lane non-temporal copy kernel: the
@@ -759,6 +771,10 @@ def main():
     wall = time.time() - t_wall0
 
     traffic, traffic_note = measured_traffic()
+    # the counters were taken at one shape; the timed shape is the autotuner's
+    # pick of this run, which may differ (bytes moved depend on the shape)
+    tshape = traffic_shape()
+    timed_shape = (shapes["f64"]["tchunk"], shapes["f64"]["flags"])
 
     progress("cpu baseline")
     cpu = None if args.no_cpu else cpu_baseline(dims, args.cpu_seconds)
@@ -785,7 +801,10 @@ def main():
                    "event_bracketed_region_ms_per_step": round(region64, 5)},
         "roofline": {"bound": "hbm", "achieved": round(gbs64, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(gbs64 / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "traffic_source": traffic_note},
+                     "traffic_source": traffic_note,
+                     "traffic_shape": ({"tchunk": tshape[0], "flags": tshape[1],
+                                        "same_as_timed": tshape == timed_shape}
+                                       if tshape else None)},
         "hbm_copy_GB_s": round(copy_gbs, 1),
         "frac_of_copy": round(gbs64 / copy_gbs, 4),
         "cpu_baseline": cpu,
