@@ -199,11 +199,15 @@ def test_derivs(G, iflag, dims):
     dg = G.derivs(R, X2, 0.37, iflag, dims, base.copy(order="F"))
     do = O.derivs(R, X2, 0.37, iflag, dims, base.copy(order="F"))
     check(dg - base, do - base, 1e-12)
-@pytest.mark.parametrize("dims", [(4, 4, 6, 8), (3, 5, 4, 12), (4, 4, 4, 64)])
+@pytest.mark.parametrize("dims", [(4, 4, 6, 8), (3, 5, 4, 12), (4, 4, 4, 64), (8, 2, 3, 48)])
 def test_derivs_batch(G, dims):
     """thr_derivs_batch_dev: the derivs of several (R, X2, anum, iflag) pairs
     in one pass equal the oracle's derivs called pair by pair (what the
-    reference's qmrherm does per pole), to 1e-12."""
+    reference's qmrherm does per pole), to 1e-12; where the x+1 neighbours
+    come through LDS (64-lane segments, Lx % 4 == 0) the result agrees with
+    the one with loaded neighbours to the last bits (the compiled code places
+    fused multiply-adds differently)."""
+    import os
     from hiprt import DeviceArray
     lx, ly, lt, ls = dims
     print("case: derivs batch", dims, flush=True)
@@ -227,6 +231,13 @@ def test_derivs_batch(G, dims):
     G.derivs_batch_dev(rd, xd, anums, iflags, dims, ds)
     got = ds.to_host().reshape((lx, ly, lt, 3), order="F")
     check(got - base, do - base, 1e-12)
+    os.environ["THR_DERIVS_LDSX"] = "0"
+    try:
+        ds2 = DeviceArray.from_host(base.reshape(-1, order="F"))
+        G.derivs_batch_dev(rd, xd, anums, iflags, dims, ds2)
+    finally:
+        os.environ.pop("THR_DERIVS_LDSX", None)
+    check(ds2.to_host().reshape((lx, ly, lt, 3), order="F") - base, got - base, 1e-14)
 
 
 @pytest.mark.parametrize("iflag", [0, 1, 2, 3])

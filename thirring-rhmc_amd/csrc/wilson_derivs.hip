@@ -130,6 +130,34 @@ __device__ inline void ld_spinor(const double2 *__restrict__ p, unsigned v5, uns
   for (int a = 0; a < 4; a++) v[a] = ldc(p, (long)a * v5 + site);
 }
 
+// LDS x exchange of the batched derivs: with 64-lane segments (32 < Ls <=
+// 64) and Lx % 4 == 0 a block's 4 waves are 4 consecutive x of one (y, t),
+// so the x+1 neighbour of waves 0-2 is the centre of the next wave: each wave
+// stores its centre (R and X2, 8 complex numbers per lane, 32 KB per block)
+// and reads the next wave's instead of loading it; wave 3 loads its own.
+// 32^3 x 64, 25 poles: 1.74 against 1.82 ms; 16^3 x 48 even.  The same in
+// the single derivs march (two barriers per t step, 156 against 128 VGPRs:
+// 3 waves per SIMD instead of 4) was slower (84 against 78 us at 32^3 x 64).
+struct DXBuf {
+  double2 (*b)[8][64];
+  __device__ inline void put(unsigned w, unsigned ln, const cx<double> R[4],
+                             const cx<double> X[4]) const {
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+      b[w][a][ln] = make_double2(R[a].r, R[a].i);
+      b[w][4 + a][ln] = make_double2(X[a].r, X[a].i);
+    }
+  }
+  __device__ inline void get(unsigned w, unsigned ln, cx<double> R[4], cx<double> X[4]) const {
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+      const double2 r = b[w][a][ln], x = b[w][4 + a][ln];
+      R[a] = {r.x, r.y};
+      X[a] = {x.x, x.y};
+    }
+  }
+};
+
 // Ls <= 64: each s-line takes a segment of W lanes of one wave (W the
 // smallest power of two >= Ls, lanes s >= Ls idle), so the sum over s is a
 // butterfly of wave shuffles inside the segment (a fixed order, no LDS and no
@@ -205,6 +233,7 @@ __global__ __launch_bounds__(BLK, 3) void k_derivs_seg(const Dims d, const doubl
 // (qmrherm_module.F90:143-181); one launch over all of them reads each
 // pair's fields once like separate calls but fills the GPU on lattices whose
 // single call cannot (25 MB per pair at 16^3 x 48).
+template <bool XL>
 __global__ __launch_bounds__(BLK, 3) void k_derivs_seg_batch(const Dims d,
                                                           const DerivPair *__restrict__ pairs,
                                                           unsigned wseg, double *part) {
@@ -227,16 +256,30 @@ __global__ __launch_bounds__(BLK, 3) void k_derivs_seg_batch(const Dims d,
   const unsigned j = s + ls * col + v3 * t, jx = s + ls * (xf + lx * y) + v3 * t;
   const unsigned jy = s + ls * (x + lx * yf) + v3 * t, jt = s + ls * col + v3 * tf;
   double v[3] = {0.0, 0.0, 0.0};
+  const unsigned w = threadIdx.x >> 6, ln = threadIdx.x & 63;
+  cx<double> R0[4], X0[4], Rx[4], Xx[4], Ry[4], Xy[4], Rt[4], Xt[4];
   if (valid) {
-    cx<double> R0[4], X0[4], Rx[4], Xx[4], Ry[4], Xy[4], Rt[4], Xt[4];
     ld_spinor(P.r, v5, j, R0);
     ld_spinor(P.x2, v5, j, X0);
-    ld_spinor(P.r, v5, jx, Rx);
-    ld_spinor(P.x2, v5, jx, Xx);
+    if (!XL || w == 3) {
+      ld_spinor(P.r, v5, jx, Rx);
+      ld_spinor(P.x2, v5, jx, Xx);
+    }
     ld_spinor(P.r, v5, jy, Ry);
     ld_spinor(P.x2, v5, jy, Xy);
     ld_spinor(P.r, v5, jt, Rt);
     ld_spinor(P.x2, v5, jt, Xt);
+  }
+  if constexpr (XL) {
+    // x+1 of waves 0-2 from the next wave's centre (wseg = 64 and full
+    // blocks: every wave takes the barrier)
+    __shared__ double2 xraw[4][8][64];
+    const DXBuf xb{xraw};
+    if (valid) xb.put(w, ln, R0, X0);
+    __syncthreads();
+    if (valid && w < 3) xb.get(w + 1, ln, Rx, Xx);
+  }
+  if (valid) {
     v[0] = derivs_term<0>(R0, X0, Rx, Xx, P.w1, P.w2);
     v[1] = derivs_term<1>(R0, X0, Ry, Xy, P.w1, P.w2);
     v[2] = derivs_term<2>(R0, X0, Rt, Xt, P.w1, P.w2);
@@ -506,6 +549,15 @@ DerivPair derivs_pair(const double2 *R, const double2 *X2, double anum, int ifla
   return DerivPair{R, X2, (double)(tzi * 0.5f), (double)tzi * (iflag == 0 ? 1.0 : -1.0)};
 }
 
+// the batched derivs' LDS x exchange applies with 64-lane segments (32 < Ls
+// <= 64) when a block's 4 lines are 4 consecutive x of one y (Lx % 4 == 0);
+// THR_DERIVS_LDSX=0 switches it off
+static bool derivs_lds_x(const Dims &d, unsigned wseg) {
+  if (const char *e = std::getenv("THR_DERIVS_LDSX"))
+    if (std::atoi(e) == 0) return false;
+  return wseg == 64 && d.lx % 4 == 0;
+}
+
 int launch_derivs_batch(const Dims &d, int npair, const DerivPair *pairs_host,
                         DerivPair *pairs_dev, double *part, double *dsdpi, hipStream_t s) {
   THR_CHECK(npair >= 1 && npair <= 65535, THR_ERR_ARG, "derivs batch: 1..65535 pairs");
@@ -520,8 +572,12 @@ int launch_derivs_batch(const Dims &d, int npair, const DerivPair *pairs_host,
   const unsigned nl = BLK / w;
   const unsigned v2 = (unsigned)(d.lx * d.ly);
   const unsigned nb = ((v2 + nl - 1) / nl) * (unsigned)d.lt;
-  hipLaunchKernelGGL(k_derivs_seg_batch, dim3(nb, npair), dim3(BLK), 0, s, d,
-                     (const DerivPair *)pairs_dev, w, part);
+  if (derivs_lds_x(d, w))
+    hipLaunchKernelGGL(k_derivs_seg_batch<true>, dim3(nb, npair), dim3(BLK), 0, s, d,
+                       (const DerivPair *)pairs_dev, w, part);
+  else
+    hipLaunchKernelGGL(k_derivs_seg_batch<false>, dim3(nb, npair), dim3(BLK), 0, s, d,
+                       (const DerivPair *)pairs_dev, w, part);
   hipLaunchKernelGGL(k_derivs_sum_pairs, dim3((unsigned)((n + BLK - 1) / BLK)), dim3(BLK), 0, s, n,
                      npair, (const double *)part, dsdpi);
   THR_HIP(hipGetLastError());
