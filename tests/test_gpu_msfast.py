@@ -323,6 +323,8 @@ def test_block_length_64(G, th32, th16):
             assert rel.max() <= 1e-10
             assert it % msb == 0
             assert all(v < 0 or v % msb == 0 for v in st["switches"])
+            if th16 > 0:
+                assert st["switches"][2] > 0  # FP16 blocks ran (and were flushed)
             xs[msb] = x.to_host().reshape(2, n, -1)
         finally:
             os.environ.pop("THR_MSB", None)

@@ -316,14 +316,58 @@ __global__ __launch_bounds__(BLK) void k_msf_flush(long v5, int n, TT<P> *const 
   // to FP64 is exact, so the sums are the same either way)
   using RT = std::conditional_t<(QN <= 32), double, CC<P>>;
   cx<RT> R[QN];
+  if constexpr (P == P16 && QN > 32) {
+    // FP16 residuals are 4 bytes per element, so one load per slot and lane
+    // moves few bytes per memory instruction.  Instead the 4 lanes of a site
+    // (consecutive lanes, spin = lane & 3) each load the whole site (16 B)
+    // and scale of one of 4 slots and exchange them through LDS: 2 loads per
+    // 4 slots instead of 8.  The values and their scaling are those of
+    // VAcc<P16>::ld_at.  A wave only exchanges with itself: LDS operations
+    // of a wave run in order, and the wave barriers keep the compiler from
+    // moving them.
+    __shared__ uint4 xh[BLK / 64][64];
+    __shared__ float xn[BLK / 64][64];
+    const unsigned wv = threadIdx.x >> 6, ln = threadIdx.x & 63, g = ln & ~3u;
+    const long so = (long)src * v5 + site;
 #pragma unroll
-  for (int q = 0; q < QN; q++) {
-    if (q0 + q < jc) {
-      VAcc<P> a{ring[q0 + q] + off, ring_n ? ring_n[q0 + q] + (long)src * v5 : nullptr, v5};
-      const cx<CC<P>> v = a.ld_at(e, site, spin);
-      R[q] = {(RT)v.r, (RT)v.i};
-    } else {
-      R[q] = {RT(0), RT(0)};
+    for (int b4 = 0; b4 < QN / 4; b4++) {
+      const int qb = q0 + 4 * b4;  // slots qb .. qb + 3 (< MSB: all allocated)
+      const __half2 *p0 = ring[qb], *p1 = ring[qb + 1], *p2 = ring[qb + 2], *p3 = ring[qb + 3];
+      const float *n0 = ring_n[qb], *n1 = ring_n[qb + 1], *n2 = ring_n[qb + 2], *n3 = ring_n[qb + 3];
+      const __half2 *pk = spin == 0 ? p0 : spin == 1 ? p1 : spin == 2 ? p2 : p3;
+      const float *nk = spin == 0 ? n0 : spin == 1 ? n1 : spin == 2 ? n2 : n3;
+      uint4 raw = make_uint4(0u, 0u, 0u, 0u);
+      float sk = 0.0f;
+      if (qb + spin < jc) {
+        raw = *(const uint4 *)(pk + off + 4 * site);
+        sk = nk[so];
+      }
+      __builtin_amdgcn_wave_barrier();
+      xh[wv][ln] = raw;
+      xn[wv][ln] = sk;
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const uint4 h4 = xh[wv][g + j];
+        const unsigned hw = spin == 0 ? h4.x : spin == 1 ? h4.y : spin == 2 ? h4.z : h4.w;
+        __half2 hv;
+        __builtin_memcpy(&hv, &hw, sizeof(hv));
+        const float2 f = __half22float2(hv);
+        const float sc = xn[wv][g + j];
+        R[4 * b4 + j] = qb + j < jc ? cx<RT>{f.x * sc, f.y * sc} : cx<RT>{RT(0), RT(0)};
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < QN; q++) {
+      if (q0 + q < jc) {
+        VAcc<P> a{ring[q0 + q] + off, ring_n ? ring_n[q0 + q] + (long)src * v5 : nullptr, v5};
+        const cx<CC<P>> v = a.ld_at(e, site, spin);
+        R[q] = {(RT)v.r, (RT)v.i};
+      } else {
+        R[q] = {RT(0), RT(0)};
+      }
     }
   }
   const bool head = q0 == 0;
