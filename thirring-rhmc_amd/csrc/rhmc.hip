@@ -121,14 +121,15 @@ int multishift_mixed(const OpDesc &op, int n, const double *aden, const double2 
   A.x = x1;
   A.tol = 0.75 * (double)res;  // iterated target below the true-residual target
   A.maxcg = max_qmr_iters;
-  // Krylov storage FP32 below |r|/|b| = th32: the looser the target, the
-  // earlier (the FP64 defect correction only has to reach res, and the drift
-  // of the low-precision iterations scales with |r| at the switch).  Measured
-  // at 32^3 x 64: the force at res 1e-8 took 7.8 s at 1e-4, 6.5 s at 1e-2,
-  // 5.9 s at 1e-1; the Hamiltonian at res 1e-10 5.5, 5.1 and 6.9 s.  So
-  // th32 = 1e7 res within [1e-2, 1e-1].  THR_MIXED_TH32 / THR_MIXED_TH16
-  // override the FP32 and FP16 thresholds (0 never).
-  A.th32 = std::min(0.1, std::max(1e-2, 1e7 * (double)res));
+  // Krylov storage FP32 below |r|/|b| = th32.  For a loose target (the
+  // force's res 1e-8) the switch comes early: the FP64 defect correction
+  // only has to reach res (at 32^3 x 64 the force took 7.8 s at th32 = 1e-4,
+  // 6.5 s at 1e-2, 5.9 s at 1e-1, all within the same distance of the FP64
+  // force).  A tight target (the Hamiltonian's res 1e-10) switches at 1e-4:
+  // 1e-2 was 7% faster there but put dH 7e-9 (relative) from the FP64 one
+  // instead of 1e-9.  THR_MIXED_TH32 / THR_MIXED_TH16 override the FP32 and
+  // FP16 thresholds (0 never).
+  A.th32 = (double)res >= 1e-9 ? 0.1 : 1e-4;
   A.th16 = 0.0;
   if (const char *e = std::getenv("THR_MIXED_TH32")) A.th32 = std::atof(e);
   if (const char *e = std::getenv("THR_MIXED_TH16")) A.th16 = std::atof(e);
