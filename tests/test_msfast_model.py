@@ -129,9 +129,11 @@ def _f32(v):
 
 
 def _lowp_solve(m, a, b, sig, reliable, fix_directions, tol=1e-10, maxit=20000, blk=32,
-                rdelta=0.1):
+                rdelta=0.1, th32=1.0):
     """The h-recurrence multishift (h = M p, r_new = r + omega M^T h) with h
-    and r rounded to FP32 every iteration, shift vectors in FP64.  With
+    and r rounded to FP32 every iteration once the storage switched (from
+    the start when th32 >= 1, else at the end of the first block with
+    |r|/|b| < th32), shift vectors in FP64.  With
     `reliable`, at the end of a block where the smallest active shift's
     iterated residual fell by rdelta, the base residual becomes that shift's
     true residual over its zeta, and with `fix_directions` every active p_j
@@ -145,15 +147,20 @@ def _lowp_solve(m, a, b, sig, reliable, fix_directions, tol=1e-10, maxit=20000, 
     omega, gam = 1.0, 0.0
     active = np.ones(ns, bool)
     r_ref = 1.0
+    low = th32 >= 1.0
+
+    def rnd(v):
+        return _f32(v) if low else v
+
     for k in range(maxit):
         mr = m @ r
-        h = _f32(mr if k == 0 else mr + gam * h)
+        h = rnd(mr if k == 0 else mr + gam * h)
         alpha = h @ h
         os_, om = omega, -delta / alpha
         z_iii = np.where(active, z_i * z_ii * os_ / (om * gam * (z_i - z_ii)
                                                      + z_i * os_ * (1 - sig * om)), z_ii)
         om_i = om * z_iii / z_ii
-        rn = _f32(r + om * (m @ h))
+        rn = rnd(r + om * (m @ h))
         lam = rn @ rn
         g = lam / delta
         g_i = g * z_iii * om_i / (z_ii * om)
@@ -166,7 +173,11 @@ def _lowp_solve(m, a, b, sig, reliable, fix_directions, tol=1e-10, maxit=20000, 
         omega, gam, delta, r = om, g, lam, rn
         if not active.any():
             break
-        if reliable and (k + 1) % blk == 0:
+        if not low and (k + 1) % blk == 0 and np.sqrt(lam / b2) < th32:
+            low = True
+            h, r = _f32(h), _f32(r)
+            continue
+        if low and reliable and (k + 1) % blk == 0:
             i = int(np.argmin(np.where(active, sig, np.inf)))
             worst = abs(z_ii[i]) * np.sqrt(lam / b2)
             if worst < rdelta * r_ref:
@@ -178,6 +189,28 @@ def _lowp_solve(m, a, b, sig, reliable, fix_directions, tol=1e-10, maxit=20000, 
                      for i, s in enumerate(sig)])
 
 
+def _spectrum_system(n=300, seed=1):
+    rng = np.random.default_rng(seed)
+    ev = np.exp(rng.uniform(np.log(1e-4), np.log(8.0), n))
+    ev[0] = 1e-4
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    m = (q * np.sqrt(ev)) @ q.T
+    return m, m @ m, rng.standard_normal(n), np.array([1e-5, 1e-4, 1e-3, 1e-2, 1e-1, 1.0])
+
+
+def test_drift_is_set_where_the_precision_drops():
+    """Without reliable updates, a shift's residual gap is the sum of its
+    rounding errors weighted by zeta, so it is set by |r| when the storage
+    drops to FP32: switching two decades later shrinks the smallest shift's
+    gap by more than 10x each time, and shifts that converged before the
+    switch keep FP64 accuracy (the reason the mixed solves run FP64 down to
+    |r|/|b| = 1e-2 and correct the rest)."""
+    m, a, b, sig = _spectrum_system()
+    gaps = {th: _lowp_solve(m, a, b, sig, False, False, th32=th) for th in (1.0, 1e-2, 1e-4)}
+    assert gaps[1.0][0] > 10 * gaps[1e-2][0] > 100 * gaps[1e-4][0]
+    assert gaps[1e-2][2:].max() <= 1e-10  # sigma >= 1e-3 converged in FP64
+
+
 def test_reliable_updates_model():
     """FP32 Krylov storage from the first iteration, 6 shifts on a spectrum
     [1e-4, 8] (condition 8e4): without reliable updates the smallest shift
@@ -185,15 +218,7 @@ def test_reliable_updates_model():
     the shift directions breaks the shifted recurrences (~0.3); with the
     directions moved the smallest shift meets 1e-10, and the heavier ones keep
     the drift they took while converging in FP32 (for the FP64 correction)."""
-    rng = np.random.default_rng(1)
-    n = 300
-    ev = np.exp(rng.uniform(np.log(1e-4), np.log(8.0), n))
-    ev[0] = 1e-4
-    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
-    m = (q * np.sqrt(ev)) @ q.T
-    a = m @ m
-    b = rng.standard_normal(n)
-    sig = np.array([1e-5, 1e-4, 1e-3, 1e-2, 1e-1, 1.0])
+    m, a, b, sig = _spectrum_system()
     plain = _lowp_solve(m, a, b, sig, reliable=False, fix_directions=False)
     broken = _lowp_solve(m, a, b, sig, reliable=True, fix_directions=False)
     fixed = _lowp_solve(m, a, b, sig, reliable=True, fix_directions=True)
