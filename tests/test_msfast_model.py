@@ -129,7 +129,7 @@ def _f32(v):
 
 
 def _lowp_solve(m, a, b, sig, reliable, fix_directions, tol=1e-10, maxit=20000, blk=32,
-                rdelta=0.1, th32=1.0):
+                rdelta=0.1, th32=1.0, return_x=False):
     """The h-recurrence multishift (h = M p, r_new = r + omega M^T h) with h
     and r rounded to FP32 every iteration once the storage switched (from
     the start when th32 >= 1, else at the end of the first block with
@@ -185,8 +185,9 @@ def _lowp_solve(m, a, b, sig, reliable, fix_directions, tol=1e-10, maxit=20000, 
                 if fix_directions:
                     p[active] += z_ii[active, None] * (rnew - r)
                 r, delta, r_ref = rnew, rnew @ rnew, worst
-    return np.array([np.linalg.norm(b - (a + s * np.eye(n)) @ x[i]) / np.sqrt(b2)
-                     for i, s in enumerate(sig)])
+    tr = np.array([np.linalg.norm(b - (a + s * np.eye(n)) @ x[i]) / np.sqrt(b2)
+                   for i, s in enumerate(sig)])
+    return (tr, x) if return_x else tr
 
 
 def _spectrum_system(n=300, seed=1):
@@ -226,3 +227,28 @@ def test_reliable_updates_model():
     assert broken[0] > 1e-2
     assert fixed[0] <= 1e-10
     assert fixed[1:].max() > 1e-8  # the heavier shifts' drift stays
+
+
+def test_defect_correction_model():
+    """The FP64 defect correction after a drifted low-precision solve (FP32
+    storage from the start): for every shift above tol, solve
+    (A + sigma_i) c = r_i with r_i = b - (A + sigma_i) x_i, itself in FP32
+    storage, to tol/2 relative to |b| (tol |b| / (2 |r_i|) relative to
+    |r_i|), and add c.  One round brings every shift to FP64 true residual
+    <= tol: the drift of the correction solve is relative to |r_i|, not
+    |b|."""
+    m, a, b, sig = _spectrum_system()
+    tol = 1e-10
+    tr, x = _lowp_solve(m, a, b, sig, True, True, return_x=True)
+    assert tr.max() > 1e-6  # the heavier shifts drifted
+    bn = np.linalg.norm(b)
+    for i, s in enumerate(sig):
+        if tr[i] <= tol:
+            continue
+        ri = b - (a + s * np.eye(len(b))) @ x[i]
+        tl = 0.5 * tol * bn / np.linalg.norm(ri)
+        _, c = _lowp_solve(m, a, ri, sig[i:i + 1], True, True, tol=tl, return_x=True)
+        x[i] += c[0]
+    final = np.array([np.linalg.norm(b - (a + s * np.eye(len(b))) @ x[i]) / bn
+                      for i, s in enumerate(sig)])
+    assert final.max() <= tol, final
