@@ -67,7 +67,8 @@ class Request:
         """Device bytes the request holds while it runs: theta, links (FP64
         and FP32), sources and solutions, and the batched solver's scratch
         with FP32 Krylov storage below 1e-2 (per source, in FP64 spinors: a
-        ring of 32 residuals plus h and r0 in FP64 (34) and the same in FP32
+        ring of 32 residuals (a one-shift solve keeps 32-iteration blocks)
+        plus h and r0 in FP64 (34) and the same in FP32
         (17), the shift's p (1), the six batches of the FP64 defect
         correction (6) and the three of the reliable updates (3)), plus 1 MiB
         for partial sums and solver state."""
